@@ -1,0 +1,129 @@
+"""ctypes binding of the C ABI in include/espresso_amd_p3m.h.
+
+The shared library is built in-tree by ``espresso_amd/build.py``.  There is no
+fallback: when the library is missing, importing the solver raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libespresso_amd_p3m.so")
+
+P3M_OK = 0
+P3M_ERR_DOMAIN = 1
+P3M_ERR_RUNTIME = 2
+P3M_ERR_INVALID = 3
+P3M_ERR_DEVICE = 4
+P3M_ERR_MEMORY = 5
+
+P3M_FORCES = 1
+P3M_ENERGY = 2
+P3M_PRESSURE = 4
+P3M_DEVICE_POINTERS = 8
+P3M_ASYNC = 16
+
+P3M_PREC_FP64 = 0
+P3M_PREC_FP32 = 1
+
+# every symbol declared in include/espresso_amd_p3m.h
+EXPORTED_SYMBOLS = (
+    "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
+    "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
+    "p3m_debug_charge_mesh", "p3m_debug_indices", "p3m_k_space_error",
+    "p3m_real_space_error", "p3m_pair_force", "p3m_pair_energy",
+    "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end",
+    "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
+    "p3m_last_global_error",
+)
+
+
+class p3m_params(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_int * 3),
+        ("cao", C.c_int),
+        ("alpha", C.c_double),
+        ("r_cut", C.c_double),
+        ("mesh_off", C.c_double * 3),
+        ("epsilon", C.c_double),
+        ("accuracy", C.c_double),
+        ("prefactor", C.c_double),
+        ("box_l", C.c_double * 3),
+        ("skin", C.c_double),
+        ("precision", C.c_int),
+        ("tune", C.c_int),
+        ("timings", C.c_int),
+        ("verbose", C.c_int),
+        ("charge_neutrality_tolerance", C.c_double),
+    ]
+
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises LibraryMissing when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if os.environ.get("ESPRESSO_AMD_NO_TORCH") != "1":
+        # torch wheels bundle their own libamdhip64.so.7; loading it before
+        # this library makes the process share one HIP runtime (the same
+        # SONAME would otherwise bind torch to the system runtime)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            f"{LIB_PATH} is not built; run `python espresso_amd/build.py`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    L.p3m_params_default.argtypes = [C.POINTER(p3m_params)]
+    L.p3m_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.p3m_destroy.argtypes = [vp]
+    L.p3m_set_params.argtypes = [vp, C.POINTER(p3m_params)]
+    L.p3m_get_params.argtypes = [vp, C.POINTER(p3m_params)]
+    L.p3m_is_tuned.argtypes = [vp, C.POINTER(C.c_int)]
+    L.p3m_activate.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]
+    L.p3m_solve.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp, dp]
+    L.p3m_debug_charge_mesh.argtypes = [vp, C.c_int64, vp, vp, vp]
+    L.p3m_debug_indices.argtypes = [vp, C.c_int64, vp, vp, vp]
+    L.p3m_k_space_error.argtypes = [C.c_double, C.POINTER(C.c_int), C.c_int,
+                                    C.c_int, C.c_double, C.c_double, dp, dp]
+    L.p3m_real_space_error.argtypes = [C.c_double, C.c_double, C.c_int,
+                                       C.c_double, C.c_double, dp, dp]
+    L.p3m_pair_force.argtypes = [vp, C.c_double, dp, C.c_double, dp]
+    L.p3m_pair_energy.argtypes = [vp, C.c_double, C.c_double, dp]
+    L.p3m_last_timings.argtypes = [vp, dp, C.c_int]
+    L.p3m_profile_begin.argtypes = [vp, C.c_int]
+    L.p3m_profile_end.argtypes = [vp, dp, C.POINTER(C.c_int)]
+    L.p3m_synchronize.argtypes = [vp]
+    L.p3m_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.p3m_last_error.argtypes = [vp]
+    L.p3m_last_error.restype = C.c_char_p
+    L.p3m_last_global_error.argtypes = []
+    L.p3m_last_global_error.restype = C.c_char_p
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("p3m_last_error", "p3m_last_global_error"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def raise_for(status, ctx=None):
+    """Map a C status code to the Python exception the reference raises."""
+    if status == P3M_OK:
+        return
+    L = load()
+    msg = (L.p3m_last_error(ctx) if ctx else L.p3m_last_global_error())
+    msg = msg.decode() if msg else ""
+    if status in (P3M_ERR_DOMAIN, P3M_ERR_INVALID):
+        raise ValueError(msg)
+    if status == P3M_ERR_MEMORY:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)

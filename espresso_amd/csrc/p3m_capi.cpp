@@ -1,0 +1,477 @@
+// C ABI of the MI355X P3M solver (include/espresso_amd_p3m.h).
+//
+// p3m_context plays the role of ESPResSo's CoulombP3MImpl<FloatType, Arch>
+// (src/core/electrostatics/p3m.impl.hpp:62-184) for one system: it holds the
+// validated parameters, the tuned state and the device solver.  Exceptions
+// never cross the ABI; they become status codes plus a message.
+#include "espresso_amd_p3m.h"
+
+#include "p3m_host.hpp"
+#include "p3m_solver.hpp"
+#include "p3m_tuning.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace espresso_amd;
+
+struct p3m_context {
+  int device = 0;
+  std::unique_ptr<Solver> solver;
+  p3m_params user{};
+  P3MParams params{};
+  bool have_params = false;
+  bool is_tuned = false;
+  bool initialized = false;
+  std::string last_error;
+  // staging for host pointers
+  int64_t cap = 0;
+  double *d_pos = nullptr, *d_q = nullptr, *d_upos = nullptr, *d_force = nullptr;
+};
+
+namespace {
+std::mutex g_err_mutex;
+std::string g_last_error;
+
+void set_global_error(std::string const &msg) {
+  std::lock_guard<std::mutex> lock(g_err_mutex);
+  g_last_error = msg;
+}
+
+template <typename F> int guarded(p3m_context *ctx, F &&f) {
+  try {
+    f();
+    return P3M_OK;
+  } catch (std::domain_error const &e) {
+    if (ctx) ctx->last_error = e.what();
+    set_global_error(e.what());
+    return P3M_ERR_DOMAIN;
+  } catch (std::invalid_argument const &e) {
+    if (ctx) ctx->last_error = e.what();
+    set_global_error(e.what());
+    return P3M_ERR_INVALID;
+  } catch (std::bad_alloc const &e) {
+    if (ctx) ctx->last_error = e.what();
+    set_global_error(e.what());
+    return P3M_ERR_MEMORY;
+  } catch (std::runtime_error const &e) {
+    std::string const msg = e.what();
+    if (ctx) ctx->last_error = msg;
+    set_global_error(msg);
+    return msg.rfind("HIP error", 0) == 0 ? P3M_ERR_DEVICE : P3M_ERR_RUNTIME;
+  } catch (std::exception const &e) {
+    if (ctx) ctx->last_error = e.what();
+    set_global_error(e.what());
+    return P3M_ERR_RUNTIME;
+  } catch (...) {
+    if (ctx) ctx->last_error = "unknown error";
+    set_global_error("unknown error");
+    return P3M_ERR_RUNTIME;
+  }
+}
+
+Vec3d v3(double const *a) { return {a[0], a[1], a[2]}; }
+Vec3i v3i(int const *a) { return {a[0], a[1], a[2]}; }
+
+void free_staging(p3m_context *ctx) {
+  for (double **p : {&ctx->d_pos, &ctx->d_q, &ctx->d_upos, &ctx->d_force}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  ctx->cap = 0;
+}
+
+void ensure_staging(p3m_context *ctx, int64_t n) {
+  if (n <= ctx->cap) return;
+  free_staging(ctx);
+  int64_t const cap = std::max<int64_t>(n, 1024);
+  hip_check(hipMalloc(&ctx->d_pos, sizeof(double) * 3 * cap), "hipMalloc pos");
+  hip_check(hipMalloc(&ctx->d_q, sizeof(double) * cap), "hipMalloc q");
+  hip_check(hipMalloc(&ctx->d_upos, sizeof(double) * 3 * cap), "hipMalloc upos");
+  hip_check(hipMalloc(&ctx->d_force, sizeof(double) * 3 * cap), "hipMalloc force");
+  ctx->cap = cap;
+}
+
+/** Copies host inputs to the staging buffers (unless they already are device
+ *  pointers) and returns the device pointers to use. */
+struct DevArgs {
+  double const *pos, *q, *upos;
+  double *force;
+};
+DevArgs stage_inputs(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                     double const *upos, double *force, unsigned flags) {
+  if (flags & P3M_DEVICE_POINTERS) return {pos, q, upos, force};
+  ensure_staging(ctx, n);
+  hipStream_t st = ctx->solver->stream();
+  DevArgs d{ctx->d_pos, ctx->d_q, nullptr, nullptr};
+  if (n > 0) {
+    hip_check(hipMemcpyAsync(ctx->d_pos, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st),
+              "copy pos");
+    hip_check(hipMemcpyAsync(ctx->d_q, q, sizeof(double) * n, hipMemcpyHostToDevice, st),
+              "copy q");
+    if (upos) {
+      hip_check(hipMemcpyAsync(ctx->d_upos, upos, sizeof(double) * 3 * n,
+                               hipMemcpyHostToDevice, st),
+                "copy upos");
+      d.upos = ctx->d_upos;
+    }
+    if (force) {
+      hip_check(hipMemcpyAsync(ctx->d_force, force, sizeof(double) * 3 * n,
+                               hipMemcpyHostToDevice, st),
+                "copy force");
+      d.force = ctx->d_force;
+    }
+  }
+  return d;
+}
+
+std::vector<double> host_charges(p3m_context *ctx, int64_t n, double const *q, unsigned flags) {
+  std::vector<double> h(static_cast<size_t>(n));
+  if (n == 0) return h;
+  if (flags & P3M_DEVICE_POINTERS) {
+    hip_check(hipMemcpy(h.data(), q, sizeof(double) * n, hipMemcpyDeviceToHost), "copy q");
+  } else {
+    std::memcpy(h.data(), q, sizeof(double) * n);
+  }
+  (void)ctx;
+  return h;
+}
+
+void require_params(p3m_context const *ctx) {
+  if (!ctx->have_params) throw std::runtime_error("P3M: parameters not set");
+  if (!ctx->solver) throw std::runtime_error("P3M: host-only context has no device");
+}
+
+MeshPrecision prec_of(p3m_params const &u) {
+  return u.precision == P3M_PREC_FP32 ? MeshPrecision::FP32 : MeshPrecision::FP64;
+}
+
+/** init(): derive mesh constants, sanity checks, build device state
+ *  (init_cpu_kernels / p3m_gpu_init, p3m.cpp:245-280). */
+void init_solver(p3m_context *ctx) {
+  auto &p = ctx->params;
+  auto const box = v3(ctx->user.box_l);
+  p.cao3 = p.cao * p.cao * p.cao;
+  p.recalc_a_ai_cao_cut(box);
+  sanity_checks_boxl(p, box);
+  p.alpha_L = p.alpha * box[0];
+  p.r_cut_iL = p.r_cut * (1. / box[0]);
+  ctx->solver->configure(p, box, ctx->user.skin, ctx->user.prefactor, prec_of(ctx->user));
+  ctx->initialized = true;
+}
+} // namespace
+
+extern "C" {
+
+int p3m_params_default(p3m_params *p) {
+  if (!p) return P3M_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->mesh[0] = p->mesh[1] = p->mesh[2] = -1;
+  p->cao = -1;
+  p->alpha = -1.;
+  p->r_cut = -1.;
+  p->mesh_off[0] = p->mesh_off[1] = p->mesh_off[2] = -1.;
+  p->epsilon = 0.;
+  p->accuracy = 1e-3;
+  p->prefactor = 0.;
+  p->box_l[0] = p->box_l[1] = p->box_l[2] = 1.;
+  p->skin = 0.4;
+  p->precision = P3M_PREC_FP64;
+  p->tune = 1;
+  p->timings = 10;
+  p->verbose = 1;
+  p->charge_neutrality_tolerance = 2e-12;
+  return P3M_OK;
+}
+
+int p3m_create(int device, p3m_context **out) {
+  if (!out) return P3M_ERR_INVALID;
+  *out = nullptr;
+  return guarded(nullptr, [&] {
+    auto ctx = std::make_unique<p3m_context>();
+    ctx->device = device;
+    if (device != -1) { // device -1: host-only context (validation, estimates)
+      int ndev = 0;
+      hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+      if (device < 0 || device >= ndev)
+        throw std::invalid_argument("P3M: invalid HIP device " + std::to_string(device));
+      ctx->solver = std::make_unique<Solver>(device);
+    }
+    *out = ctx.release();
+  });
+}
+
+int p3m_destroy(p3m_context *ctx) {
+  if (!ctx) return P3M_OK;
+  if (ctx->solver) (void)hipSetDevice(ctx->device);
+  free_staging(ctx);
+  delete ctx;
+  return P3M_OK;
+}
+
+int p3m_set_params(p3m_context *ctx, p3m_params const *p) {
+  if (!ctx || !p) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    // CoulombP3M script interface construction (CoulombP3M.hpp:102-128),
+    // P3MParameters (common.hpp:84-150), CoulombP3MImpl (p3m.impl.hpp:81-92),
+    // Actor::set_prefactor (actor.hpp:48-54)
+    bool const tuning = p->tune != 0;
+    P3MParams params(tuning, p->epsilon, p->r_cut, v3i(p->mesh), v3(p->mesh_off), p->cao,
+                     p->alpha, p->accuracy);
+    if (p->timings <= 0) throw std::domain_error("Parameter 'timings' must be > 0");
+    if (p->prefactor <= 0.) throw std::domain_error("Parameter 'prefactor' must be > 0");
+    for (int d = 0; d < 3; ++d)
+      if (!(p->box_l[d] > 0.)) throw std::domain_error("box_l must be > 0");
+    ctx->is_tuned = not params.tuning;
+    params.tuning = false;
+    ctx->params = params;
+    ctx->user = *p;
+    ctx->have_params = true;
+    ctx->initialized = false;
+  });
+}
+
+int p3m_get_params(p3m_context const *ctx, p3m_params *out) {
+  if (!ctx || !out) return P3M_ERR_INVALID;
+  *out = ctx->user;
+  auto const &p = ctx->params;
+  for (int d = 0; d < 3; ++d) {
+    out->mesh[d] = p.mesh[d];
+    out->mesh_off[d] = p.mesh_off[d];
+  }
+  out->cao = p.cao;
+  out->alpha = p.alpha;
+  out->r_cut = p.r_cut;
+  out->accuracy = p.accuracy;
+  out->epsilon = p.epsilon;
+  out->tune = ctx->is_tuned ? 0 : 1;
+  return P3M_OK;
+}
+
+int p3m_is_tuned(p3m_context const *ctx, int *out) {
+  if (!ctx || !out) return P3M_ERR_INVALID;
+  *out = ctx->is_tuned ? 1 : 0;
+  return P3M_OK;
+}
+
+int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                 unsigned flags) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    auto const box = v3(ctx->user.box_l);
+    // sanity checks before tuning (p3m.impl.hpp:124-139, p3m.hpp:67-73):
+    // the k-space cutoff is not known yet, only the epsilon/cubic rule and
+    // charge neutrality apply
+    if (ctx->params.epsilon != P3M_EPSILON_METALLIC) {
+      if (box[0] != box[1] or box[1] != box[2] or
+          (ctx->params.mesh[0] != -1 and (ctx->params.mesh[0] != ctx->params.mesh[1] or
+                                          ctx->params.mesh[1] != ctx->params.mesh[2])))
+        throw std::runtime_error("CoulombP3M: non-metallic epsilon requires cubic box");
+    }
+    auto const charges = host_charges(ctx, n, q, flags);
+    check_charge_neutrality(charges, ctx->user.charge_neutrality_tolerance);
+    // tune (p3m.cpp:757-791)
+    auto &p = ctx->params;
+    if (p.alpha_L == 0. and p.alpha != 0.) p.alpha_L = p.alpha * box[0];
+    if (p.r_cut_iL == 0. and p.r_cut != 0.) p.r_cut_iL = p.r_cut * (1. / box[0]);
+    if (!ctx->is_tuned) {
+      TuningInput in;
+      in.box_l = box;
+      in.skin = ctx->user.skin;
+      in.prefactor = ctx->user.prefactor;
+      in.timings = ctx->user.timings;
+      in.verbose = ctx->user.verbose != 0;
+      in.name = "CoulombP3M";
+      for (double const qq : charges)
+        if (qq != 0.) {
+          in.sum_qpart++;
+          in.sum_q2 += qq * qq;
+        }
+      DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, flags);
+      ensure_staging(ctx, n);
+      double *scratch_force = ctx->d_force; // forces are computed but discarded
+      TuningTimer timer = [&](Vec3i const &mesh, int cao, double r_cut_iL, double alpha_L,
+                              int timings) -> double {
+        P3MParams trial = p;
+        trial.mesh = mesh;
+        trial.cao = cao;
+        trial.alpha_L = alpha_L;
+        trial.alpha = alpha_L * (1. / box[0]);
+        trial.r_cut_iL = r_cut_iL;
+        trial.r_cut = r_cut_iL * box[0];
+        trial.recalc_a_ai_cao_cut(box);
+        try {
+          sanity_checks_boxl(trial, box);
+          ctx->solver->configure(trial, box, ctx->user.skin, ctx->user.prefactor,
+                                 prec_of(ctx->user));
+        } catch (std::runtime_error const &) {
+          return -1.;
+        }
+        SolveResult r;
+        ctx->solver->solve(n, d.pos, d.q, nullptr, nullptr, P3M_ENERGY, &r); // warm-up
+        double total = 0.;
+        for (int t = 0; t < timings; ++t) {
+          ctx->solver->solve(n, d.pos, d.q, nullptr, scratch_force, P3M_FORCES | P3M_ENERGY,
+                             &r);
+          total += ctx->solver->last_ms[0];
+        }
+        return total / timings + model_real_space_ms(in.sum_qpart, trial.r_cut, box);
+      };
+      tune_p3m(p, in, timer);
+      ctx->is_tuned = true;
+    }
+    init_solver(ctx);
+  });
+}
+
+int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+              double const *pos_unfolded, double *force, unsigned flags, double *energy,
+              double *virial) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->initialized) init_solver(ctx);
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    if ((flags & P3M_FORCES) && !force && n > 0)
+      throw std::invalid_argument("P3M: force array required");
+    bool const async = (flags & P3M_ASYNC) != 0;
+    if (async && !(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
+    DevArgs const d = stage_inputs(ctx, n, pos, q, pos_unfolded, force, flags);
+    SolveResult r;
+    ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
+                       &r, !async);
+    if (async) {
+      if (energy) *energy = 0.;
+      if (virial)
+        for (int k = 0; k < 9; ++k) virial[k] = 0.;
+      return;
+    }
+    if ((flags & P3M_FORCES) && !(flags & P3M_DEVICE_POINTERS) && n > 0) {
+      hip_check(hipMemcpy(force, d.force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+                "copy force");
+    }
+    if (energy) *energy = (flags & P3M_ENERGY) ? r.energy : 0.;
+    if (virial)
+      for (int k = 0; k < 9; ++k) virial[k] = (flags & P3M_PRESSURE) ? r.virial[k] : 0.;
+  });
+}
+
+int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                          double *mesh_out) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->initialized) init_solver(ctx);
+    DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);
+    std::vector<double> out;
+    ctx->solver->debug_charge_mesh(n, d.pos, d.q, out);
+    std::memcpy(mesh_out, out.data(), sizeof(double) * out.size());
+  });
+}
+
+int p3m_debug_indices(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                      int32_t *nmp) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->initialized) init_solver(ctx);
+    DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);
+    std::vector<int32_t> out;
+    ctx->solver->debug_indices(n, d.pos, d.q, out);
+    std::memcpy(nmp, out.data(), sizeof(int32_t) * out.size());
+  });
+}
+
+int p3m_k_space_error(double prefactor, int const mesh[3], int cao, int n_c_part,
+                      double sum_q2, double alpha_L, double const box_l[3], double *out) {
+  if (!out) return P3M_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    *out = k_space_error(prefactor, v3i(mesh), cao, n_c_part, sum_q2, alpha_L, v3(box_l));
+  });
+}
+
+int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double sum_q2,
+                         double alpha_L, double const box_l[3], double *out) {
+  if (!out) return P3M_ERR_INVALID;
+  return guarded(nullptr, [&] {
+    *out = real_space_error(prefactor, r_cut_iL, n_c_part, sum_q2, alpha_L, v3(box_l));
+  });
+}
+
+int p3m_pair_force(p3m_context const *ctx, double q1q2, double const d[3], double dist,
+                   double out[3]) {
+  if (!ctx || !out) return P3M_ERR_INVALID;
+  auto const f = pair_force(ctx->user.prefactor, ctx->params.alpha, ctx->params.r_cut, q1q2,
+                            v3(d), dist);
+  out[0] = f[0];
+  out[1] = f[1];
+  out[2] = f[2];
+  return P3M_OK;
+}
+
+int p3m_pair_energy(p3m_context const *ctx, double q1q2, double dist, double *out) {
+  if (!ctx || !out) return P3M_ERR_INVALID;
+  *out = pair_energy(ctx->user.prefactor, ctx->params.alpha, ctx->params.r_cut, q1q2, dist);
+  return P3M_OK;
+}
+
+int p3m_last_timings(p3m_context const *ctx, double *ms, int n) {
+  if (!ctx || !ms || !ctx->solver) return P3M_ERR_INVALID;
+  for (int i = 0; i < n && i < 8; ++i) ms[i] = ctx->solver->last_ms[i];
+  return P3M_OK;
+}
+
+int p3m_profile_begin(p3m_context *ctx, int max_solves) {
+  if (!ctx || !ctx->solver) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] { ctx->solver->profile_begin(max_solves); });
+}
+
+int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves) {
+  if (!ctx || !ctx->solver || !stage_ms) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    int const k = ctx->solver->profile_end(stage_ms);
+    if (nsolves) *nsolves = k;
+  });
+}
+
+int p3m_synchronize(p3m_context *ctx) {
+  if (!ctx || !ctx->solver) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    hip_check(hipStreamSynchronize(ctx->solver->stream()), "hipStreamSynchronize");
+  });
+}
+
+int p3m_get_stream(p3m_context const *ctx, void **stream) {
+  if (!ctx || !stream || !ctx->solver) return P3M_ERR_INVALID;
+  *stream = static_cast<void *>(ctx->solver->stream());
+  return P3M_OK;
+}
+
+char const *p3m_last_error(p3m_context const *ctx) {
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+char const *p3m_last_global_error(void) {
+  std::lock_guard<std::mutex> lock(g_err_mutex);
+  static thread_local std::string copy;
+  copy = g_last_error;
+  return copy.c_str();
+}
+
+} // extern "C"

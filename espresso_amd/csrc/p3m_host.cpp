@@ -1,0 +1,290 @@
+// Host-side P3M parameter handling; see p3m_host.hpp for the reference map.
+#include "p3m_host.hpp"
+
+#include <algorithm>
+#include <iomanip>
+#include <limits>
+#include <numbers>
+#include <sstream>
+
+namespace espresso_amd {
+
+P3MParams::P3MParams(bool tuning_, double epsilon_, double r_cut_, Vec3i const &mesh_,
+                     Vec3d const &mesh_off_, int cao_, double alpha_, double accuracy_)
+    : tuning{tuning_}, mesh{mesh_}, mesh_off{mesh_off_}, cao{cao_}, accuracy{accuracy_},
+      epsilon{epsilon_}, alpha{alpha_}, r_cut{r_cut_} {
+  // common.hpp:94-150
+  constexpr double value_to_tune = -1.;
+  if (epsilon < 0.) throw std::domain_error("Parameter 'epsilon' must be >= 0");
+  if (accuracy <= 0.) throw std::domain_error("Parameter 'accuracy' must be > 0");
+  if (r_cut <= 0.) {
+    if (tuning and r_cut == value_to_tune)
+      r_cut = 0.;
+    else
+      throw std::domain_error("Parameter 'r_cut' must be > 0");
+  }
+  if (alpha <= 0.) {
+    if (tuning and alpha == value_to_tune)
+      alpha = 0.;
+    else
+      throw std::domain_error("Parameter 'alpha' must be > 0");
+  }
+  bool const all_pos = mesh[0] >= 1 and mesh[1] >= 1 and mesh[2] >= 1;
+  bool const first_only = mesh[0] >= 1 and mesh[1] == -1 and mesh[2] == -1;
+  bool const all_tune = tuning and mesh[0] == -1 and mesh[1] == -1 and mesh[2] == -1;
+  if (not(all_pos or first_only or all_tune))
+    throw std::domain_error("Parameter 'mesh' must be > 0");
+  bool const off_ok = mesh_off[0] >= 0. and mesh_off[1] >= 0. and mesh_off[2] >= 0. and
+                      mesh_off[0] <= 1. and mesh_off[1] <= 1. and mesh_off[2] <= 1.;
+  if (not off_ok) {
+    if (mesh_off[0] == -1. and mesh_off[1] == -1. and mesh_off[2] == -1.)
+      mesh_off = {P3M_MESHOFF, P3M_MESHOFF, P3M_MESHOFF};
+    else
+      throw std::domain_error("Parameter 'mesh_off' must be >= 0 and <= 1");
+  }
+  if ((cao < 1 or cao > 7) and (not tuning or cao != -1))
+    throw std::domain_error("Parameter 'cao' must be >= 1 and <= 7");
+  // Vector3i::broadcast(cao) > mesh is a lexicographic-free "all greater"
+  // comparison in the reference (Utils::Vector operator>): every component.
+  if (not tuning and (cao > mesh[0] and cao > mesh[1] and cao > mesh[2]))
+    throw std::domain_error("Parameter 'cao' cannot be larger than 'mesh'");
+}
+
+void P3MParams::recalc_a_ai_cao_cut(Vec3d const &box_l) {
+  for (int d = 0; d < 3; ++d) {
+    ai[d] = static_cast<double>(mesh[d]) / box_l[d];
+    a[d] = 1. / ai[d];
+    cao_cut[d] = (static_cast<double>(cao) / 2.) * a[d];
+  }
+}
+
+void LocalMesh::calc_local_ca_mesh(P3MParams const &params, Vec3d const &box_l, double skin,
+                                   double space_layer) {
+  // common.cpp:34-106 with my_left = 0 and my_right = box_l (one rank).
+  Vec3d inner_ld_gp, inner_ur_gp, outer_ld_gp, outer_ur_gp;
+  for (int i = 0; i < 3; ++i) {
+    double const full_skin = (params.cao_cut[i] + skin) + (i == 2 ? space_layer : 0.);
+    double const inner_ld_pos = 0.;
+    double const inner_ur_pos = box_l[i];
+    double const outer_ur_pos = inner_ur_pos + full_skin;
+    double const outer_ld_pos = inner_ld_pos - full_skin;
+    inner_ld_gp[i] = inner_ld_pos * params.ai[i] - params.mesh_off[i];
+    inner_ur_gp[i] = inner_ur_pos * params.ai[i] - params.mesh_off[i];
+    outer_ld_gp[i] = outer_ld_pos * params.ai[i] - params.mesh_off[i];
+    outer_ur_gp[i] = outer_ur_pos * params.ai[i] - params.mesh_off[i];
+  }
+  Vec3i ind;
+  for (int i = 0; i < 3; ++i) in_ld[i] = static_cast<int>(std::ceil(inner_ld_gp[i]));
+  for (int i = 0; i < 3; ++i) in_ur[i] = static_cast<int>(std::floor(inner_ur_gp[i]));
+  for (int i = 0; i < 3; ++i) {
+    if (inner_ur_gp[i] - in_ur[i] < ROUND_ERROR_PREC) in_ur[i]--;
+    if (inner_ld_gp[i] - in_ld[i] + 1. < ROUND_ERROR_PREC) in_ld[i]--;
+  }
+  for (int i = 0; i < 3; ++i) inner[i] = in_ur[i] - in_ld[i] + 1;
+  for (int i = 0; i < 3; ++i) ld_ind[i] = static_cast<int>(std::ceil(outer_ld_gp[i]));
+  for (int i = 0; i < 3; ++i) margin[i * 2] = in_ld[i] - ld_ind[i];
+  for (int i = 0; i < 3; ++i) ind[i] = static_cast<int>(std::floor(outer_ur_gp[i]));
+  for (int i = 0; i < 3; ++i)
+    if (outer_ur_gp[i] - ind[i] == 0.) ind[i]--;
+  for (int i = 0; i < 3; ++i) margin[i * 2 + 1] = ind[i] - in_ur[i];
+  size = 1;
+  for (int i = 0; i < 3; ++i) {
+    dim[i] = ind[i] - ld_ind[i] + 1;
+    size *= dim[i];
+  }
+  for (int i = 0; i < 3; ++i) in_ld[i] = margin[i * 2];
+  for (int i = 0; i < 3; ++i) in_ur[i] = margin[i * 2] + inner[i];
+  q_2_off = dim[2] - params.cao;
+  q_21_off = dim[2] * (dim[1] - params.cao);
+  recalc_ld_pos(params);
+}
+
+void LocalMesh::recalc_ld_pos(P3MParams const &params) {
+  for (int i = 0; i < 3; ++i) ld_pos[i] = (ld_ind[i] + params.mesh_off[i]) * params.a[i];
+}
+
+std::vector<int> calc_meshift(int n, bool zero_out_midpoint) {
+  std::vector<int> ret(static_cast<std::size_t>(std::max(n, 0)), 0);
+  for (int j = 1; j <= n / 2; ++j) {
+    ret[j] = j;
+    ret[n - j] = -j;
+  }
+  if (zero_out_midpoint and n > 0) ret[n / 2] = 0;
+  return ret;
+}
+
+double sinc(double x) {
+  double const pix = std::numbers::pi * x;
+  if (std::fabs(x) > 0.1) return std::sin(pix) / pix;
+  constexpr double c0 = 1., c2 = -1. / 6., c4 = 1. / 120., c6 = -1. / 5040.,
+                   c8 = 1. / 362880.;
+  double const pix2 = pix * pix;
+  return c0 + pix2 * (c2 + pix2 * (c4 + pix2 * (c6 + pix2 * c8)));
+}
+
+double analytic_cotangent_sum(int cao, int n, double mesh_i) {
+  // Hockney & Eastwood eq. 7-66 as even polynomials in c = cos^2(pi n/M).
+  double const theta = static_cast<double>(n) * mesh_i * std::numbers::pi;
+  double const c = std::cos(theta) * std::cos(theta);
+  switch (cao) {
+  case 1: return 1.;
+  case 2: return (1. + c * 2.) / 3.;
+  case 3: return (2. + c * (11. + c * 2.)) / 15.;
+  case 4: return (17. + c * (180. + c * (114. + c * 4.))) / 315.;
+  case 5: return (62. + c * (1072. + c * (1452. + c * (247. + c * 2.)))) / 2835.;
+  case 6:
+    return (1382. + c * (35396. + c * (83021. + c * (34096. + c * (2026. + c * 4.))))) /
+           155925.;
+  case 7:
+    return (21844. +
+            c * (776661. +
+                 c * (2801040. + c * (2123860. + c * (349500. + c * (8166. + c * 4.)))))) /
+           6081075.;
+  default: throw std::logic_error("Invalid value cao=" + std::to_string(cao));
+  }
+}
+
+double real_space_error(double pref, double r_cut_iL, int n_c_part, double sum_q2,
+                        double alpha_L, Vec3d const &box_l) {
+  double const volume = box_l[0] * box_l[1] * box_l[2];
+  return (2. * pref * sum_q2 * std::exp(-(r_cut_iL * alpha_L) * (r_cut_iL * alpha_L))) /
+         std::sqrt(static_cast<double>(n_c_part) * r_cut_iL * box_l[0] * volume);
+}
+
+double k_space_error(double pref, Vec3i const &mesh, int cao, int n_c_part, double sum_q2,
+                     double alpha_L, Vec3d const &box_l) {
+  // p3m.cpp:187-243 with P3M_BRILLOUIN = 0: the aliasing sums reduce to the
+  // single image nm = n (p3m.cpp:164-185).
+  Vec3d const mesh_i{1. / mesh[0], 1. / mesh[1], 1. / mesh[2]};
+  double const alpha_L_i = 1. / alpha_L;
+  double const factor1 = (std::numbers::pi * alpha_L_i) * (std::numbers::pi * alpha_L_i);
+  Vec3i const stop{mesh[0] / 2, mesh[1] / 2, mesh[2] / 2};
+  // per-axis tables of the cotangent sums and sinc factors
+  std::array<std::vector<double>, 3> cs_tab, sinc_tab;
+  for (int d = 0; d < 3; ++d) {
+    for (int n = -stop[d]; n < stop[d]; ++n) {
+      cs_tab[d].push_back(analytic_cotangent_sum(cao, n, mesh_i[d]));
+      sinc_tab[d].push_back(sinc(n * mesh_i[d]));
+    }
+  }
+  double he_q = 0.;
+  for (int x = -stop[0]; x < stop[0]; ++x)
+    for (int y = -stop[1]; y < stop[1]; ++y)
+      for (int z = -stop[2]; z < stop[2]; ++z) {
+        if (x == 0 and y == 0 and z == 0) continue;
+        int const n2 = x * x + y * y + z * z;
+        double const cs = cs_tab[0][x + stop[0]] * cs_tab[1][y + stop[1]] * cs_tab[2][z + stop[2]];
+        double const ex = std::exp(-factor1 * n2);
+        double const energy = std::pow(
+            sinc_tab[0][x + stop[0]] * sinc_tab[1][y + stop[1]] * sinc_tab[2][z + stop[2]],
+            2 * cao);
+        double const alias1 = ex * ex / n2;
+        double const alias2 = energy * ex * n2 / n2;
+        double const d = alias1 - (alias2 / cs) * (alias2 / cs) / n2;
+        if (d > 0. and std::fabs(d / alias1) > ROUND_ERROR_PREC) he_q += d;
+      }
+  return 2. * pref * sum_q2 * std::sqrt(he_q / static_cast<double>(n_c_part)) /
+         (box_l[1] * box_l[2]);
+}
+
+Accuracy calculate_accuracy(double pref, Vec3i const &mesh, int cao, double r_cut_iL,
+                            int n_c_part, double sum_q2, double accuracy, Vec3d const &box_l) {
+  double alpha_L;
+  double rs_err = real_space_error(pref, r_cut_iL, n_c_part, sum_q2, 0., box_l);
+  if (std::numbers::sqrt2 * rs_err > accuracy)
+    alpha_L = std::sqrt(std::log(std::numbers::sqrt2 * rs_err / accuracy)) / r_cut_iL;
+  else
+    alpha_L = 0.1;
+  rs_err = real_space_error(pref, r_cut_iL, n_c_part, sum_q2, alpha_L, box_l);
+  double const ks_err = k_space_error(pref, mesh, cao, n_c_part, sum_q2, alpha_L, box_l);
+  return {std::hypot(rs_err, ks_err), rs_err, ks_err, alpha_L};
+}
+
+static double as_erfc_part(double d) {
+  // Abramowitz & Stegun 7.1.26 (AS_erfc_part.hpp:29-40)
+  constexpr double a1 = 0.254829592, a2 = -0.284496736, a3 = 1.421413741,
+                   a4 = -1.453152027, a5 = 1.061405429, p = 0.3275911;
+  double const t = 1.0 / (1.0 + p * d);
+  return t * (a1 + t * (a2 + t * (a3 + t * (a4 + t * a5))));
+}
+
+Vec3d pair_force(double prefactor, double alpha, double r_cut, double q1q2, Vec3d const &d,
+                 double dist) {
+  if (q1q2 == 0. or dist >= r_cut or dist <= 0.) return {0., 0., 0.};
+  double const adist = alpha * dist;
+  double const exp_adist_sq = std::exp(-adist * adist);
+  double const dist_sq = dist * dist;
+  double const two_a_sqrt_pi_i = 2. * alpha * std::numbers::inv_sqrtpi;
+  double const erfc_part_ri = as_erfc_part(adist) / dist;
+  double const fac = exp_adist_sq * (erfc_part_ri + two_a_sqrt_pi_i) / dist_sq;
+  double const s = fac * prefactor * q1q2;
+  return {s * d[0], s * d[1], s * d[2]};
+}
+
+double pair_energy(double prefactor, double alpha, double r_cut, double q1q2, double dist) {
+  if (q1q2 == 0. or dist >= r_cut or dist <= 0.) return 0.;
+  double const adist = alpha * dist;
+  double const erfc_part_ri = as_erfc_part(adist) / dist;
+  return prefactor * q1q2 * erfc_part_ri * std::exp(-adist * adist);
+}
+
+void sanity_checks_boxl(P3MParams const &params, Vec3d const &box_l) {
+  for (int i = 0; i < 3; ++i) {
+    if (params.cao_cut[i] >= 0.5 * box_l[i]) {
+      std::stringstream msg;
+      msg << "P3M_init: k-space cutoff " << params.cao_cut[i]
+          << " is larger than half of box dimension " << box_l[i];
+      throw std::runtime_error(msg.str());
+    }
+    if (params.cao_cut[i] >= box_l[i]) {
+      std::stringstream msg;
+      msg << "P3M_init: k-space cutoff " << params.cao_cut[i]
+          << " is larger than local box dimension " << box_l[i];
+      throw std::runtime_error(msg.str());
+    }
+  }
+  if (params.epsilon != P3M_EPSILON_METALLIC) {
+    if (box_l[0] != box_l[1] or box_l[1] != box_l[2] or params.mesh[0] != params.mesh[1] or
+        params.mesh[1] != params.mesh[2])
+      throw std::runtime_error("CoulombP3M: non-metallic epsilon requires cubic box");
+  }
+}
+
+double charge_excess_ratio(std::vector<double> const &charges) {
+  // Kahan-compensated sum of the nonzero charges over the smallest |q|.
+  double sum = 0., comp = 0.;
+  double q_min = std::numeric_limits<double>::infinity();
+  for (double const q : charges) {
+    if (q != 0.) {
+      double const y = q - comp;
+      double const t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
+      q_min = std::min(q_min, std::fabs(q));
+    }
+  }
+  return std::fabs(sum) / q_min;
+}
+
+void check_charge_neutrality(std::vector<double> const &charges, double relative_tolerance) {
+  if (relative_tolerance == -1.) return;
+  double const excess_ratio = charge_excess_ratio(charges);
+  if (excess_ratio >= relative_tolerance) {
+    std::ostringstream serializer;
+    serializer << std::scientific << std::setprecision(4) << excess_ratio;
+    throw std::runtime_error(
+        "The system is not charge neutral. Please neutralize the system "
+        "before adding a new actor by adding the corresponding counterions "
+        "to the system. Alternatively you can turn off the electroneutrality "
+        "check by supplying check_neutrality=False when creating the actor. "
+        "In this case you may be simulating a non-neutral system which will "
+        "affect physical observables like e.g. the pressure, the chemical "
+        "potentials of charged species or potential energies of the system. "
+        "Since simulations of non charge neutral systems are special please "
+        "make sure you know what you are doing. The relative charge excess "
+        "was " +
+        serializer.str());
+  }
+}
+
+} // namespace espresso_amd

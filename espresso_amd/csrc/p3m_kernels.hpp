@@ -1,0 +1,359 @@
+// Device building blocks shared by the P3M kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace espresso_amd::dev {
+
+template <typename T> struct CxOf;
+template <> struct CxOf<double> { using type = double2; };
+template <> struct CxOf<float> { using type = float2; };
+template <typename T> using cx_t = typename CxOf<T>::type;
+
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+/** multiply by -i (forward) or +i (inverse) */
+template <typename C> __device__ __forceinline__ C cmul_mi(C a, bool inv) {
+  return inv ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// ---------------------------------------------------------------------------
+// Cardinal B-spline weights (Hockney & Eastwood 5-189).  For order P and
+// stencil point i the weight at offset x in [-1/2, 1/2] is M_P(i + 1/2 - x),
+// M_P(u) = 1/(P-1)! sum_{j<=floor(u)} (-1)^j C(P,j) (u-j)^(P-1).  The integer
+// numerators and denominators are derived here at compile time and reduced by
+// their gcd, which yields the tabulated polynomials of
+// src/utils/include/utils/math/bspline.hpp:30-177.
+// ---------------------------------------------------------------------------
+template <int P> struct BsplineTable {
+  double num[P][P];
+  double den[P];
+};
+
+constexpr long long cx_binom(int n, int k) {
+  if (k < 0 or k > n) return 0;
+  long long r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+constexpr long long cx_ipow(long long b, int e) {
+  long long r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+constexpr long long cx_gcd(long long a, long long b) {
+  if (a < 0) a = -a;
+  if (b < 0) b = -b;
+  while (b != 0) {
+    long long const t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+template <int P> constexpr BsplineTable<P> make_bspline_table() {
+  BsplineTable<P> t{};
+  long long fact = 1;
+  for (int k = 2; k <= P - 1; ++k) fact *= k;
+  for (int i = 0; i < P; ++i) {
+    long long c[P] = {};
+    for (int j = 0; j <= i; ++j) {
+      long long const sgn = (j % 2) ? -1 : 1;
+      long long const a = 2 * (i - j) + 1;
+      for (int m = 0; m < P; ++m)
+        c[m] += sgn * cx_binom(P, j) * cx_binom(P - 1, m) * cx_ipow(a, P - 1 - m) *
+                cx_ipow(-2, m);
+    }
+    long long const den = cx_ipow(2, P - 1) * fact;
+    long long g = den;
+    for (int m = 0; m < P; ++m) g = cx_gcd(g, c[m]);
+    for (int m = 0; m < P; ++m) t.num[i][m] = static_cast<double>(c[m] / g);
+    t.den[i] = static_cast<double>(den / g);
+  }
+  return t;
+}
+
+/** w[i] = B-spline weight of stencil point i at offset x (FP64 Horner,
+ *  integer-coefficient numerator divided by its denominator). */
+template <int P> __device__ __forceinline__ void bspline_weights(double x, double *w) {
+  constexpr BsplineTable<P> T = make_bspline_table<P>();
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    double acc = T.num[i][P - 1];
+#pragma unroll
+    for (int m = P - 2; m >= 0; --m) acc = T.num[i][m] + x * acc;
+    w[i] = acc / T.den[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bit-exact mesh coordinate of a particle (interpolation.hpp:144-160):
+//   t = ((x - ld_pos) * ai) - pos_shift,  nmp = (int) t,  dist = (t - nmp) - 0.5
+// evaluated with every operation rounded separately, as on the reference's
+// x86-64 build.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ca_coordinate(double x, double ld_pos, double ai,
+                                                double pos_shift) {
+#pragma clang fp contract(off)
+  double const u = x - ld_pos;
+  double const v = u * ai;
+  return v - pos_shift;
+}
+__device__ __forceinline__ double ca_distance(double t, int nmp) {
+#pragma clang fp contract(off)
+  double const u = t - static_cast<double>(nmp);
+  return u - 0.5;
+}
+
+/** Geometry of one system shared by the particle kernels. */
+struct DevGeo {
+  int M[3];      // mesh
+  int Kz;        // Mz/2 + 1
+  int dim[3];    // CA mesh dims
+  int marg[3];   // CA-local index of the FFT origin (left margin)
+  int B[3];      // brick dims (first stencil points per brick)
+  int nb[3];     // bricks per dim
+  double ld_pos[3];
+  double ai[3];
+  double box_l[3];
+  double pos_shift;
+};
+
+/** Mesh coordinates of one particle for stencil order P: CA-local first
+ *  stencil point nmp, its periodic FFT-mesh index gi and the offsets dist.
+ *  Positions whose stencil would leave the CA mesh (further outside the box
+ *  than the Verlet skin, which the reference never sees) are folded into
+ *  the box first.  Returns false for non-finite positions. */
+template <int P>
+__device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double const x[3],
+                                                     int nmp[3], int gi[3], double dist[3]) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double t = ca_coordinate(x[d], g.ld_pos[d], g.ai[d], g.pos_shift);
+    if (!(t >= 0. && t < static_cast<double>(g.dim[d] - P + 1))) {
+      double xf = x[d] - g.box_l[d] * floor(x[d] / g.box_l[d]);
+      if (xf >= g.box_l[d]) xf -= g.box_l[d];
+      t = ca_coordinate(xf, g.ld_pos[d], g.ai[d], g.pos_shift);
+      if (!(t >= 0. && t < static_cast<double>(g.dim[d] - P + 1))) {
+        ok = false;
+        t = 0.;
+      }
+    }
+    int const n = static_cast<int>(t);
+    nmp[d] = n;
+    dist[d] = ca_distance(t, n);
+    int gg = n - g.marg[d];
+    gg %= g.M[d];
+    if (gg < 0) gg += g.M[d];
+    gi[d] = gg;
+  }
+  return ok;
+}
+
+/** i k_d for FFT index j of a dimension of size M with the Nyquist entry set
+ *  to zero (d_op = calc_meshift(mesh, true), data_struct.hpp:68-70). */
+__device__ __forceinline__ int dop_shift(int j, int M) {
+  if (2 * j == M) return 0;
+  return (2 * j < M) ? j : j - M;
+}
+/** calc_meshift(mesh, false) entry (common.hpp:248-264). */
+__device__ __forceinline__ int plain_shift(int j, int M) { return (2 * j < M) ? j : j - M; }
+
+// ---------------------------------------------------------------------------
+// wave64 / block reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+/** Sum K values over the block (blockDim.x multiple of 64, <= 1024); the
+ *  result is valid in thread 0.  `scratch` needs K * 16 doubles. */
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *scratch) {
+  int const lane = threadIdx.x & 63;
+  int const wid = threadIdx.x >> 6;
+  int const nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) scratch[k * 16 + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.;
+      for (int w = 0; w < nw; ++w) s += scratch[k * 16 + w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// LDS-resident batched 1D FFT (Stockham autosort, mixed radix).
+//   stage with radix R, current span Ns, butterfly j in [0, n/R):
+//     v_r   = in[j + r n/R] * w_n^(r (j mod Ns) n/(Ns R))
+//     V     = DFT_R(v)
+//     out[(j / Ns) Ns R + (j mod Ns) + r Ns] = V_r
+// Twiddles come from an LDS table tw[m] = exp(-2 pi i m / n) (conjugated for
+// the inverse).  Radices 2, 4, 8 have closed butterflies, 3, 5, 6, 7 use a
+// register DFT with table roots, larger primes one output per work item.
+// Unnormalised in both directions, as FFTW in the reference (fft.cpp:732-769).
+// ---------------------------------------------------------------------------
+struct FftPlanDev {
+  int n;
+  int nstages;
+  int radix[16];
+};
+
+template <typename C> __device__ __forceinline__ C tw_get(C const *tw, int m, bool inv) {
+  C w = tw[m];
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+template <typename C, int R>
+__device__ __forceinline__ void dft_reg(C (&v)[R], C const *tw, int n, bool inv) {
+  if constexpr (R == 2) {
+    C const a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C const s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    C const s13 = cadd(v[1], v[3]), d13 = cmul_mi(csub(v[1], v[3]), inv);
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+  } else if constexpr (R == 8) {
+    using T = decltype(v[0].x);
+    T const h = T(0.70710678118654752440084436210484903928);
+    C a[4], b[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      a[r] = cadd(v[r], v[r + 4]);
+      b[r] = csub(v[r], v[r + 4]);
+    }
+    // b_r *= w_8^r  (forward w_8 = exp(-i pi/4))
+    b[1] = inv ? C{h * (b[1].x - b[1].y), h * (b[1].x + b[1].y)}
+               : C{h * (b[1].x + b[1].y), h * (b[1].y - b[1].x)};
+    b[2] = cmul_mi(b[2], inv);
+    b[3] = inv ? C{-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y)}
+               : C{h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y)};
+    C const as02 = cadd(a[0], a[2]), ad02 = csub(a[0], a[2]);
+    C const as13 = cadd(a[1], a[3]), ad13 = cmul_mi(csub(a[1], a[3]), inv);
+    C const bs02 = cadd(b[0], b[2]), bd02 = csub(b[0], b[2]);
+    C const bs13 = cadd(b[1], b[3]), bd13 = cmul_mi(csub(b[1], b[3]), inv);
+    v[0] = cadd(as02, as13);
+    v[4] = csub(as02, as13);
+    v[2] = cadd(ad02, ad13);
+    v[6] = csub(ad02, ad13);
+    v[1] = cadd(bs02, bs13);
+    v[5] = csub(bs02, bs13);
+    v[3] = cadd(bd02, bd13);
+    v[7] = csub(bd02, bd13);
+  } else {
+    int const step = n / R;
+    C out[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      C s = v[0];
+#pragma unroll
+      for (int r = 1; r < R; ++r) s = cadd(s, cmul(v[r], tw_get(tw, ((r * k) % R) * step, inv)));
+      out[k] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = out[k];
+  }
+}
+
+template <typename C, int R>
+__device__ __forceinline__ void stockham_bfly(C const *src, C *dst, int j, int nb, int Ns, int n,
+                                              C const *tw, bool inv) {
+  C v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+  int const jm = j % Ns;
+  if (Ns > 1) {
+    int const step = (n / (Ns * R)) * jm;
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_get(tw, r * step, inv));
+  }
+  dft_reg<C, R>(v, tw, n, inv);
+  int const idxD = (j / Ns) * Ns * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
+}
+
+/** Transform `nlines` lines of length plan.n held in LDS (line l starts at
+ *  l * ls).  Returns the buffer holding the result (buf0 or buf1).  Must be
+ *  called by the whole block; ends with a barrier. */
+template <typename C>
+__device__ C *lds_fft(C *buf0, C *buf1, int nlines, int ls, FftPlanDev const &plan, C const *tw,
+                      bool inv) {
+  int const n = plan.n;
+  C *in = buf0;
+  C *out = buf1;
+  int Ns = 1;
+  for (int s = 0; s < plan.nstages; ++s) {
+    int const R = plan.radix[s];
+    if (R <= 8) {
+      int const nb = n / R;
+      int const items = nlines * nb;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const l = it / nb;
+        int const j = it - l * nb;
+        C const *src = in + l * ls;
+        C *dst = out + l * ls;
+        switch (R) {
+        case 2: stockham_bfly<C, 2>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 3: stockham_bfly<C, 3>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 4: stockham_bfly<C, 4>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 5: stockham_bfly<C, 5>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 6: stockham_bfly<C, 6>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 7: stockham_bfly<C, 7>(src, dst, j, nb, Ns, n, tw, inv); break;
+        default: stockham_bfly<C, 8>(src, dst, j, nb, Ns, n, tw, inv); break;
+        }
+      }
+    } else {
+      // generic radix: one output element per item
+      int const nb = n / R;
+      int const items = nlines * n;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const l = it / n;
+        int const e = it - l * n;
+        int const j = e / R;  // butterfly
+        int const k = e - j * R;
+        int const jm = j % Ns;
+        C const *src = in + l * ls;
+        C *dst = out + l * ls;
+        int const base = (n / (Ns * R)) * (jm + k * Ns);
+        C s = src[j];
+        for (int r = 1; r < R; ++r) {
+          int const m = static_cast<int>((static_cast<long long>(r) * base) % n);
+          s = cadd(s, cmul(src[j + r * nb], tw_get(tw, m, inv)));
+        }
+        dst[(j / Ns) * Ns * R + jm + k * Ns] = s;
+      }
+    }
+    __syncthreads();
+    C *t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  return in;
+}
+
+} // namespace espresso_amd::dev

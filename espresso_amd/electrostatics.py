@@ -1,0 +1,413 @@
+"""P3M Coulomb actors backed by the MI355X solver.
+
+Mirrors ``espressomd.electrostatics.P3M`` / ``P3MGPU``
+(src/python/espressomd/electrostatics.py:129-284): the same keyword
+arguments, defaults, validation messages and read-only parameters
+(src/script_interface/electrostatics/CoulombP3M.hpp:66-128).  The actor
+computes the long-range (k-space) part only; particles are passed as arrays:
+
+    p3m = P3M(prefactor=1., accuracy=1e-4, mesh=32, cao=5, r_cut=3.,
+              alpha=1.07465, tune=False)
+    p3m.activate(pos, q, box_l=[L, L, L])
+    f = np.zeros_like(pos); p3m.add_long_range_forces(pos, q, f)
+    e = p3m.long_range_energy(pos, q)
+
+Arrays may be numpy arrays (host, copied per call) or torch tensors on a
+HIP device (used in place, no PCIe traffic).
+"""
+import ctypes as C
+import numbers
+
+import numpy as np
+
+from . import _lib
+
+_KEYS = ("prefactor", "accuracy", "cao", "r_cut", "alpha", "mesh", "epsilon",
+         "mesh_off", "check_neutrality", "check_complex_residuals", "tune",
+         "timings", "verbose", "single_precision")
+
+
+def _is_int(x):
+    return isinstance(x, (numbers.Integral, np.integer)) and not isinstance(x, bool)
+
+
+def _is_float(x):
+    return isinstance(x, (numbers.Real, np.floating)) and not isinstance(x, bool)
+
+
+class _Arrays:
+    """Device or host view of pos/q/force for one call."""
+
+    def __init__(self, pos, q, force=None, pos_unfolded=None):
+        self.keep = []
+        self.device = _is_device_tensor(pos)
+        if self.device:
+            import torch
+            for t, name in ((q, "q"), (force, "force"),
+                            (pos_unfolded, "pos_unfolded")):
+                if t is not None and not _is_device_tensor(t):
+                    raise TypeError(f"'{name}' must be a device tensor too")
+            pos = pos.contiguous().to(torch.float64)
+            q = q.contiguous().to(torch.float64)
+            self.keep += [pos, q]
+            self.n = int(q.shape[0])
+            self.pos = pos.data_ptr()
+            self.q = q.data_ptr()
+            self.force = None
+            if force is not None:
+                if force.dtype != torch.float64 or not force.is_contiguous():
+                    raise TypeError("'force' must be a contiguous float64 tensor")
+                self.force = force.data_ptr()
+            self.upos = None
+            if pos_unfolded is not None:
+                up = pos_unfolded.contiguous().to(torch.float64)
+                self.keep.append(up)
+                self.upos = up.data_ptr()
+        else:
+            pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64)
+                                       .reshape(-1, 3))
+            q = np.ascontiguousarray(np.asarray(q, dtype=np.float64).reshape(-1))
+            if pos.shape[0] != q.shape[0]:
+                raise ValueError("pos and q must describe the same particles")
+            self.keep += [pos, q]
+            self.n = int(q.shape[0])
+            self.pos = pos.ctypes.data
+            self.q = q.ctypes.data
+            self.force = None
+            if force is not None:
+                if not (isinstance(force, np.ndarray) and force.dtype == np.float64
+                        and force.flags.c_contiguous and force.size == 3 * self.n):
+                    raise TypeError(
+                        "'force' must be a C-contiguous float64 array of shape (n, 3)")
+                self.force = force.ctypes.data
+            self.upos = None
+            if pos_unfolded is not None:
+                up = np.ascontiguousarray(np.asarray(pos_unfolded, np.float64)
+                                          .reshape(-1, 3))
+                self.keep.append(up)
+                self.upos = up.ctypes.data
+
+    @property
+    def flags(self):
+        return _lib.P3M_DEVICE_POINTERS if self.device else 0
+
+
+def _is_device_tensor(x):
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class _P3MBase:
+    """Common parameter handling of the P3M actors
+    (electrostatics.py:129-176, CoulombP3M.hpp:102-128)."""
+
+    _name = "P3M"
+    _gpu = False
+
+    def required_keys(self):
+        return {"prefactor", "accuracy"}
+
+    def default_params(self):
+        return {"cao": -1,
+                "r_cut": -1.,
+                "alpha": -1.,
+                "mesh": [-1, -1, -1],
+                "epsilon": 0.,
+                "mesh_off": [-1., -1., -1.],
+                "prefactor": 0.,
+                "check_neutrality": True,
+                "check_complex_residuals": True,
+                "tune": True,
+                "timings": 10,
+                "verbose": True}
+
+    def valid_keys(self):
+        return set(_KEYS)
+
+    def __init__(self, device=0, **kwargs):
+        L = _lib.load()
+        unknown = set(kwargs) - self.valid_keys()
+        if unknown:
+            raise ValueError(
+                f"Only the following keys can be given as keyword arguments: "
+                f"{sorted(self.valid_keys())}, got {sorted(kwargs)} "
+                f"(unknown {sorted(unknown)})")
+        missing = self.required_keys() - set(kwargs)
+        if missing:
+            raise ValueError(
+                f"At least the following keys have to be given as keyword "
+                f"arguments: {sorted(self.required_keys())}, got "
+                f"{sorted(kwargs)} (missing {sorted(missing)})")
+        params = self.default_params()
+        params.update(kwargs)
+        self.validate_params(params)
+        self._params = params
+        self._ctx = C.c_void_p()
+        _lib.raise_for(L.p3m_create(int(device), C.byref(self._ctx)))
+        self._device = int(device)
+        self._activated = False
+        self._box_l = None
+        self._build_cparams(box_l=(1., 1., 1.), skin=0.4)
+        _lib.raise_for(L.p3m_set_params(self._ctx, C.byref(self._cp)),
+                       self._ctx)
+
+    def validate_params(self, params):
+        """electrostatics.py:148-176 plus the script-interface GPU check
+        (CoulombP3M.hpp:110-113)."""
+        if _is_int(params["mesh"]):
+            params["mesh"] = 3 * [params["mesh"]]
+        mesh = params["mesh"]
+        if not (hasattr(mesh, "__len__") and len(mesh) == 3
+                and all(_is_int(m) for m in mesh)):
+            raise ValueError(
+                "Parameter 'mesh' has to be an integer or integer list of length 3")
+        params["mesh"] = [int(m) for m in mesh]
+        if any(m % 2 != 0 and m != -1 for m in params["mesh"]):
+            raise ValueError(
+                "P3M requires an even number of mesh points in all directions")
+        if isinstance(params["epsilon"], str) and params["epsilon"] == "metallic":
+            params["epsilon"] = 0.0
+        if not _is_float(params["epsilon"]):
+            raise ValueError(
+                "Parameter 'epsilon' has to be a float or 'metallic'")
+        mo = params["mesh_off"]
+        if not (hasattr(mo, "__len__") and len(mo) == 3
+                and all(_is_float(m) for m in mo)):
+            raise ValueError(
+                "Parameter 'mesh_off' has to be a (3,) array_like of values "
+                "between 0.0 and 1.0")
+        if not _is_int(params["timings"]):
+            raise TypeError("Parameter 'timings' has to be an integer")
+        if not isinstance(params["tune"], (bool, np.bool_)):
+            raise TypeError("Parameter 'tune' has to be a boolean")
+        if self._gpu and not params["single_precision"]:
+            raise ValueError("P3M GPU only implemented in single-precision mode")
+
+    def _build_cparams(self, box_l, skin):
+        L = _lib.load()
+        p = self._params
+        cp = _lib.p3m_params()
+        L.p3m_params_default(C.byref(cp))
+        for d in range(3):
+            cp.mesh[d] = int(p["mesh"][d])
+            cp.mesh_off[d] = float(p["mesh_off"][d])
+            cp.box_l[d] = float(box_l[d])
+        cp.cao = int(p["cao"])
+        cp.alpha = float(p["alpha"])
+        cp.r_cut = float(p["r_cut"])
+        cp.epsilon = float(p["epsilon"])
+        cp.accuracy = float(p["accuracy"])
+        cp.prefactor = float(p["prefactor"])
+        cp.skin = float(skin)
+        cp.precision = (_lib.P3M_PREC_FP32 if p["single_precision"]
+                        else _lib.P3M_PREC_FP64)
+        cp.tune = 1 if p["tune"] else 0
+        cp.timings = int(p["timings"])
+        cp.verbose = 1 if p["verbose"] else 0
+        cp.charge_neutrality_tolerance = 2e-12 if p["check_neutrality"] else -1.
+        self._cp = cp
+
+    # -- activation ---------------------------------------------------------
+    def activate(self, pos, q, box_l, skin=0.4):
+        """Sanity checks, tuning and init on the given particles
+        (CoulombP3MImpl::on_activation, p3m.impl.hpp:124-139)."""
+        L = _lib.load()
+        if not self._activated or tuple(box_l) != self._box_l:
+            self._build_cparams(box_l=box_l, skin=skin)
+            _lib.raise_for(L.p3m_set_params(self._ctx, C.byref(self._cp)),
+                           self._ctx)
+        a = _Arrays(pos, q)
+        _lib.raise_for(L.p3m_activate(self._ctx, a.n, a.pos, a.q, a.flags),
+                       self._ctx)
+        self._activated = True
+        self._box_l = tuple(float(x) for x in box_l)
+        return self
+
+    def _require_active(self):
+        if not self._activated:
+            raise RuntimeError("P3M actor is not activated")
+
+    # -- CoulombP3M long-range interface ------------------------------------
+    def add_long_range_forces(self, pos, q, force, pos_unfolded=None,
+                              sync=True):
+        """force -= k-space force (p3m.impl.hpp:145-154); in place.
+
+        With device tensors and ``sync=False`` the solve is only enqueued on
+        the solver's stream (call ``synchronize()`` before reading)."""
+        self._require_active()
+        a = _Arrays(pos, q, force, pos_unfolded)
+        flags = _lib.P3M_FORCES | a.flags
+        if not sync:
+            if not a.device:
+                raise ValueError("sync=False needs device tensors")
+            flags |= _lib.P3M_ASYNC
+        _lib.raise_for(_lib.load().p3m_solve(
+            self._ctx, a.n, a.pos, a.q, a.upos, a.force, flags, None, None),
+            self._ctx)
+        return force
+
+    def synchronize(self):
+        _lib.raise_for(_lib.load().p3m_synchronize(self._ctx), self._ctx)
+
+    def profile_begin(self, max_solves):
+        _lib.raise_for(_lib.load().p3m_profile_begin(self._ctx, int(max_solves)),
+                       self._ctx)
+
+    def profile_end(self):
+        """(number of profiled solves, summed stage times in ms)."""
+        out = (C.c_double * 7)()
+        n = C.c_int(0)
+        _lib.raise_for(_lib.load().p3m_profile_end(self._ctx, out, C.byref(n)),
+                       self._ctx)
+        keys = ("total", "bin", "spread", "fft_fwd", "convolve", "fft_bwd",
+                "gather")
+        return n.value, dict(zip(keys, out[:]))
+
+    def long_range_energy(self, pos, q, pos_unfolded=None):
+        """k-space energy incl. self/net-charge/dipole terms (p3m.cpp:551-589)."""
+        self._require_active()
+        a = _Arrays(pos, q, None, pos_unfolded)
+        e = C.c_double(0.)
+        _lib.raise_for(_lib.load().p3m_solve(
+            self._ctx, a.n, a.pos, a.q, a.upos, None,
+            _lib.P3M_ENERGY | a.flags, C.byref(e), None), self._ctx)
+        return e.value
+
+    def long_range_pressure(self, pos, q):
+        """k-space virial tensor as returned by long_range_pressure
+        (p3m.cpp:393-449), i.e. before the division by the volume."""
+        self._require_active()
+        a = _Arrays(pos, q)
+        v = (C.c_double * 9)()
+        _lib.raise_for(_lib.load().p3m_solve(
+            self._ctx, a.n, a.pos, a.q, None, None,
+            _lib.P3M_PRESSURE | a.flags, None, v), self._ctx)
+        return np.array(v[:]).reshape(3, 3)
+
+    def solve(self, pos, q, force=None, energy=True, pressure=False,
+              pos_unfolded=None):
+        """Forces, energy and virial from one charge assignment and FFT."""
+        self._require_active()
+        a = _Arrays(pos, q, force, pos_unfolded)
+        flags = a.flags
+        if force is not None:
+            flags |= _lib.P3M_FORCES
+        if energy:
+            flags |= _lib.P3M_ENERGY
+        if pressure:
+            flags |= _lib.P3M_PRESSURE
+        e = C.c_double(0.)
+        v = (C.c_double * 9)()
+        _lib.raise_for(_lib.load().p3m_solve(
+            self._ctx, a.n, a.pos, a.q, a.upos, a.force, flags, C.byref(e), v),
+            self._ctx)
+        return e.value, np.array(v[:]).reshape(3, 3)
+
+    def pair_force(self, q1q2, d, dist):
+        out = (C.c_double * 3)()
+        dd = (C.c_double * 3)(*[float(x) for x in d])
+        _lib.raise_for(_lib.load().p3m_pair_force(self._ctx, q1q2, dd, dist, out),
+                       self._ctx)
+        return np.array(out[:])
+
+    def pair_energy(self, q1q2, dist):
+        out = C.c_double(0.)
+        _lib.raise_for(_lib.load().p3m_pair_energy(self._ctx, q1q2, dist,
+                                                   C.byref(out)), self._ctx)
+        return out.value
+
+    # -- stage outputs for parity tests -------------------------------------
+    def charge_mesh(self, pos, q):
+        self._require_active()
+        a = _Arrays(pos, q)
+        if a.device:
+            raise TypeError("charge_mesh takes host arrays")
+        m = self.get_params()["mesh"]
+        out = np.zeros(int(np.prod(m)))
+        _lib.raise_for(_lib.load().p3m_debug_charge_mesh(
+            self._ctx, a.n, a.pos, a.q, out.ctypes.data), self._ctx)
+        return out.reshape(tuple(m))
+
+    def mesh_indices(self, pos, q):
+        self._require_active()
+        a = _Arrays(pos, q)
+        out = np.zeros((a.n, 3), np.int32)
+        _lib.raise_for(_lib.load().p3m_debug_indices(
+            self._ctx, a.n, a.pos, a.q, out.ctypes.data), self._ctx)
+        return out
+
+    def last_timings(self):
+        out = (C.c_double * 8)()
+        _lib.load().p3m_last_timings(self._ctx, out, 8)
+        keys = ("total", "bin", "spread", "fft_fwd", "convolve", "fft_bwd",
+                "gather")
+        return dict(zip(keys, out[:7]))
+
+    # -- read-only parameters (CoulombP3M.hpp:66-100) -----------------------
+    def get_params(self):
+        cp = _lib.p3m_params()
+        _lib.raise_for(_lib.load().p3m_get_params(self._ctx, C.byref(cp)),
+                       self._ctx)
+        tuned = C.c_int(0)
+        _lib.load().p3m_is_tuned(self._ctx, C.byref(tuned))
+        box = self._box_l or tuple(cp.box_l)
+        mesh = [cp.mesh[d] for d in range(3)]
+        a = [box[d] / mesh[d] if mesh[d] > 0 else 0. for d in range(3)]
+        return {
+            "prefactor": cp.prefactor,
+            "accuracy": cp.accuracy,
+            "mesh": mesh,
+            "mesh_off": [cp.mesh_off[d] for d in range(3)],
+            "cao": cp.cao,
+            "alpha": cp.alpha,
+            "r_cut": cp.r_cut,
+            "alpha_L": cp.alpha * box[0],
+            "r_cut_iL": cp.r_cut / box[0],
+            "a": a,
+            "epsilon": cp.epsilon,
+            "is_tuned": bool(tuned.value),
+            "single_precision": bool(self._params["single_precision"]),
+            "timings": self._params["timings"],
+            "tune": self._params["tune"],
+            "verbose": self._params["verbose"],
+            "check_neutrality": self._params["check_neutrality"],
+            "check_complex_residuals": self._params["check_complex_residuals"],
+        }
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        params = self.get_params()
+        if name in params:
+            return params[name]
+        raise AttributeError(name)
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            try:
+                _lib.load().p3m_destroy(ctx)
+            except Exception:  # pragma: no cover
+                pass
+            self._ctx = None
+
+
+class P3M(_P3MBase):
+    """P3M electrostatics solver (FP64 by default, FP32 mesh with
+    ``single_precision=True``)."""
+
+    def default_params(self):
+        return {"single_precision": False, **super().default_params()}
+
+
+class P3MGPU(_P3MBase):
+    """P3M electrostatics solver with GPU support (FP32 mesh, FP64
+    reductions and force accumulation)."""
+
+    _gpu = True
+
+    def default_params(self):
+        return {"single_precision": True, **super().default_params()}

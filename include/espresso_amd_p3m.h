@@ -1,0 +1,170 @@
+/*
+ * espresso_amd_p3m.h -- C ABI of the MI355X P3M k-space Coulomb solver.
+ *
+ * Drop-in boundary for the long-range part of ESPResSo's P3M Coulomb actor.
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the ESPResSo source tree):
+ *   src/core/electrostatics/p3m.hpp            struct CoulombP3M
+ *   src/core/electrostatics/p3m.impl.hpp       CoulombP3MImpl / new_p3m_handle
+ *   src/core/electrostatics/p3m_gpu_cuda.cuh   p3m_gpu_init / _add_farfield_force
+ *   src/core/electrostatics/p3m_gpu_error.hpp  p3m_k_space_error_gpu
+ *   src/core/p3m/common.hpp                    P3MParameters
+ *
+ * Conventions
+ *  - Every function returns an int status (P3M_OK = 0); nothing throws across
+ *    the ABI.  The message of the last failure on a context is returned by
+ *    p3m_last_error(); context-less calls use p3m_last_global_error().
+ *  - Status codes map to the reference's exception types:
+ *      P3M_ERR_DOMAIN    -> std::domain_error      (parameter validation)
+ *      P3M_ERR_RUNTIME   -> std::runtime_error     (sanity checks, tuning)
+ *      P3M_ERR_INVALID   -> std::invalid_argument
+ *      P3M_ERR_DEVICE    -> HIP runtime failure    (cuda_fatal_error analogue)
+ *  - Positions are AoS [n][3] doubles, charges [n] doubles, forces [n][3]
+ *    doubles accumulated in place (force -= ...), as the reference does on
+ *    particles.  With P3M_DEVICE_POINTERS set, the three arrays are device
+ *    pointers and nothing crosses PCIe; otherwise they are host pointers.
+ *  - A context owns its device buffers and one HIP stream.  Calls on one
+ *    context are serialised by the caller (one context per host thread).
+ */
+#ifndef ESPRESSO_AMD_P3M_H
+#define ESPRESSO_AMD_P3M_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define P3M_OK 0
+#define P3M_ERR_DOMAIN 1
+#define P3M_ERR_RUNTIME 2
+#define P3M_ERR_INVALID 3
+#define P3M_ERR_DEVICE 4
+#define P3M_ERR_MEMORY 5
+
+/* solve flags */
+#define P3M_FORCES 1u
+#define P3M_ENERGY 2u
+#define P3M_PRESSURE 4u
+#define P3M_DEVICE_POINTERS 8u
+#define P3M_ASYNC 16u /* enqueue only: no host sync, no energy/virial output;
+                         requires P3M_DEVICE_POINTERS */
+
+/* mesh precision */
+#define P3M_PREC_FP64 0 /* plain FP64 solver (drop-in for P3M, double) */
+#define P3M_PREC_FP32 1 /* FP32 mesh, FP64 weights/reductions/forces    */
+
+typedef struct p3m_context p3m_context;
+
+/* P3M parameters.  Field meaning and the -1 "to be tuned" sentinels follow
+ * P3MParameters (src/core/p3m/common.hpp:59-150) and the Python actor
+ * defaults (src/python/espressomd/electrostatics.py:134-146). */
+typedef struct {
+  int mesh[3];        /* mesh points per direction, {-1,-1,-1} = tune      */
+  int cao;            /* charge assignment order 1..7, -1 = tune           */
+  double alpha;       /* Ewald splitting parameter, -1 = tune              */
+  double r_cut;       /* real-space cutoff, -1 = tune                      */
+  double mesh_off[3]; /* first mesh point offset in [0,1], -1 = 0.5        */
+  double epsilon;     /* surrounding dielectric, 0 = metallic              */
+  double accuracy;    /* target rms force error                            */
+  double prefactor;   /* electrostatics prefactor (Bjerrum length * kT)    */
+  double box_l[3];    /* box lengths                                       */
+  double skin;        /* Verlet skin: fixes the charge-assignment mesh     */
+  int precision;      /* P3M_PREC_FP64 or P3M_PREC_FP32                    */
+  int tune;           /* nonzero: parameters still to be tuned             */
+  int timings;        /* tuning repetitions per trial (> 0)                */
+  int verbose;        /* tuning log to stdout                              */
+  double charge_neutrality_tolerance; /* -1 disables the check (actor.hpp:38) */
+} p3m_params;
+
+/* Fill defaults of the P3M actor (electrostatics.py:134-146). */
+int p3m_params_default(p3m_params *p);
+
+/* Create/destroy a solver context on HIP device `device` (device -1 gives a
+ * host-only context: parameter validation and error estimates, no solves).
+ * Replaces new_p3m_handle<...> (p3m.impl.hpp:186-197) and the lazily created
+ * P3MGpuParams owner (p3m_gpu_cuda.cu:116-139). */
+int p3m_create(int device, p3m_context **out);
+int p3m_destroy(p3m_context *ctx);
+
+/* Validate and store parameters: the P3MParameters constructor
+ * (common.hpp:84-150) and CoulombP3MImpl constructor (p3m.impl.hpp:81-92),
+ * with identical messages.  Does not build meshes. */
+int p3m_set_params(p3m_context *ctx, const p3m_params *p);
+
+/* Current (possibly tuned) parameters; mirrors the read-only getters of the
+ * script interface (src/script_interface/electrostatics/CoulombP3M.hpp:66-100). */
+int p3m_get_params(const p3m_context *ctx, p3m_params *out);
+int p3m_is_tuned(const p3m_context *ctx, int *out);
+
+/* Activation: sanity checks, tuning if requested, then init.
+ * Replaces CoulombP3MImpl::on_activation / tune / init (p3m.impl.hpp:124-139,
+ * p3m.cpp:757-791, 245-280, p3m_gpu_cuda.cu:549-667).  Particles are needed
+ * to count charges (count_charged_particles, p3m.cpp:97-115) and for the
+ * neutrality check (coulomb.cpp:286-325); pointers follow `flags`. */
+int p3m_activate(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                 unsigned flags);
+
+/* Long-range solve.  P3M_FORCES: force[i] -= q_i E(r_i) * prefactor, the
+ * k-space part of add_long_range_forces (p3m.cpp:452-549); P3M_ENERGY:
+ * long_range_energy (p3m.cpp:551-589) into *energy; P3M_PRESSURE: the
+ * k-space virial tensor of long_range_pressure (p3m.cpp:393-449) into
+ * virial[9] (row-major, not yet divided by the volume, as the reference
+ * returns it).  `pos_unfolded` (may be NULL = pos) feeds the dipole term
+ * when epsilon is not metallic (p3m.cpp:375-386). */
+int p3m_solve(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+              const double *pos_unfolded, double *force, unsigned flags, double *energy,
+              double *virial);
+
+/* Stage outputs for parity checks (device-side meshes copied to host):
+ * charge mesh after the periodic wrap, [Mx][My][Mz] real. */
+int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                          double *mesh_out);
+/* Per-particle global mesh index of the first stencil point (bit-exact
+ * counterpart of p3m_calculate_interpolation_weights, interpolation.hpp:138-178),
+ * -1 for uncharged particles; `nmp` gets the CA-mesh-local point [n][3]. */
+int p3m_debug_indices(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                      int32_t *nmp);
+
+/* k-space rms force error estimate (p3m.cpp:187-243; GPU twin
+ * p3m_k_space_error_gpu, p3m_gpu_error.hpp:31-33). */
+int p3m_k_space_error(double prefactor, const int mesh[3], int cao, int n_c_part,
+                      double sum_q2, double alpha_L, const double box_l[3], double *out);
+/* real-space rms force error (Kolafa-Perram, p3m.cpp:145-160). */
+int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double sum_q2,
+                         double alpha_L, const double box_l[3], double *out);
+
+/* Real-space pair terms CoulombP3M::pair_force / pair_energy (p3m.hpp:146-180). */
+int p3m_pair_force(const p3m_context *ctx, double q1q2, const double d[3], double dist,
+                   double out[3]);
+int p3m_pair_energy(const p3m_context *ctx, double q1q2, double dist, double *out);
+
+/* Timing of the last solve on the device (ms), from HIP events on the
+ * context stream: [0] total, then per stage (bin, spread, fft_fwd,
+ * convolve, fft_bwd, gather). */
+int p3m_last_timings(const p3m_context *ctx, double *ms, int n);
+
+/* Profiling of asynchronous solves: per-stage HIP events of the next
+ * `max_solves` solves are recorded into a ring on the context stream with no
+ * host synchronisation; p3m_profile_end synchronises and returns the number
+ * of profiled solves and the summed stage times in ms (7 values, order as in
+ * p3m_last_timings). */
+int p3m_profile_begin(p3m_context *ctx, int max_solves);
+int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves);
+
+/* Block until all work enqueued on the context stream has finished. */
+int p3m_synchronize(p3m_context *ctx);
+
+/* Device stream of the context (hipStream_t) for callers that enqueue
+ * around the solver. */
+int p3m_get_stream(const p3m_context *ctx, void **stream);
+
+const char *p3m_last_error(const p3m_context *ctx);
+const char *p3m_last_global_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESPRESSO_AMD_P3M_H */

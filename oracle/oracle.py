@@ -1,0 +1,303 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes view of the FP64 CPU oracle.
+
+The oracle (``p3m_oracle.c``) restates ESPResSo's ``CoulombP3MImpl<double,
+Arch::CPU>`` on one rank (src/core/electrostatics/p3m.cpp:97-591) plus the
+closed-form error estimates (p3m.cpp:145-243), the real-space pair terms
+(p3m.hpp:146-180) and an independent FP64 Ewald sum.  Only tests/,
+``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may use
+this module, and only to check or time the reference algorithm; the product
+path never calls it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libp3m_oracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile the oracle in place (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "libp3m_oracle.so"],
+                   check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    L.p3mo_sinc.restype = C.c_double
+    L.p3mo_sinc.argtypes = [C.c_double]
+    L.p3mo_cotangent_sum.restype = C.c_double
+    L.p3mo_cotangent_sum.argtypes = [C.c_int, C.c_int, C.c_double]
+    L.p3mo_as_erfc_part.restype = C.c_double
+    L.p3mo_as_erfc_part.argtypes = [C.c_double]
+    L.p3mo_bspline.restype = C.c_double
+    L.p3mo_bspline.argtypes = [C.c_int, C.c_int, C.c_double]
+    L.p3mo_bspline_coeffs.restype = None
+    L.p3mo_bspline_coeffs.argtypes = [C.c_int, C.c_int, _lp,
+                                      C.POINTER(C.c_int64)]
+    L.p3mo_meshift.restype = None
+    L.p3mo_meshift.argtypes = [C.c_int, C.c_int, _ip]
+    L.p3mo_fft3d.restype = None
+    L.p3mo_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int]
+    L.p3mo_new.restype = C.c_void_p
+    L.p3mo_new.argtypes = [_ip, C.c_int, C.c_double, C.c_double, _dp,
+                           C.c_double, _dp, C.c_double, C.c_double]
+    L.p3mo_free.restype = None
+    L.p3mo_free.argtypes = [C.c_void_p]
+    L.p3mo_geometry.restype = None
+    L.p3mo_geometry.argtypes = [C.c_void_p, _ip, _dp]
+    L.p3mo_weights.restype = C.c_int
+    L.p3mo_weights.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int), _ip,
+                               _dp]
+    L.p3mo_count_charged.restype = None
+    L.p3mo_count_charged.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.p3mo_add_long_range_forces.restype = C.c_int
+    L.p3mo_add_long_range_forces.argtypes = [C.c_void_p, C.c_int, _dp, _dp,
+                                             C.c_void_p, _dp]
+    L.p3mo_long_range_energy.restype = C.c_double
+    L.p3mo_long_range_energy.argtypes = [C.c_void_p, C.c_int, _dp, _dp,
+                                         C.c_void_p]
+    L.p3mo_long_range_pressure.restype = C.c_int
+    L.p3mo_long_range_pressure.argtypes = [C.c_void_p, C.c_int, _dp, _dp,
+                                           _dp]
+    L.p3mo_charge_mesh.restype = C.c_int
+    L.p3mo_charge_mesh.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+    L.p3mo_fields.restype = C.c_int
+    L.p3mo_fields.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.p3mo_influence.restype = None
+    L.p3mo_influence.argtypes = [C.c_void_p, _dp, _dp]
+    L.p3mo_real_space_error.restype = C.c_double
+    L.p3mo_real_space_error.argtypes = [C.c_double, C.c_double, C.c_int,
+                                        C.c_double, C.c_double, _dp]
+    L.p3mo_k_space_error.restype = C.c_double
+    L.p3mo_k_space_error.argtypes = [C.c_double, _ip, C.c_int, C.c_int,
+                                     C.c_double, C.c_double, _dp]
+    L.p3mo_calculate_accuracy.restype = None
+    L.p3mo_calculate_accuracy.argtypes = [C.c_double, _ip, C.c_int,
+                                          C.c_double, C.c_int, C.c_double,
+                                          C.c_double, _dp, _dp]
+    L.p3mo_tune_r_cut.restype = C.c_int
+    L.p3mo_tune_r_cut.argtypes = [C.c_double, _ip, C.c_int, C.c_int,
+                                  C.c_double, C.c_double, _dp, C.c_double,
+                                  C.c_double, C.c_double, _dp]
+    L.p3mo_real_space.restype = C.c_double
+    L.p3mo_real_space.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
+                                  C.c_double, C.c_double, _dp, _dp]
+    L.p3mo_ewald_kspace.restype = C.c_double
+    L.p3mo_ewald_kspace.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
+                                    C.c_double, C.c_double, C.c_int, _dp]
+    _lib = L
+    return L
+
+
+def _v3(x, dtype=np.float64):
+    a = np.asarray(x, dtype=dtype)
+    if a.ndim == 0:
+        a = np.full(3, a, dtype=dtype)
+    return np.ascontiguousarray(a)
+
+
+def _pos(pos):
+    return np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+
+
+class OracleP3M:
+    """FP64 CPU P3M (k-space part) at fixed parameters."""
+
+    def __init__(self, box_l, mesh, cao, alpha, r_cut, prefactor,
+                 epsilon=0.0, mesh_off=(-1., -1., -1.), skin=0.4):
+        L = lib()
+        self.box_l = _v3(box_l)
+        self.mesh = _v3(mesh, np.int32)
+        self.cao = int(cao)
+        self.alpha = float(alpha)
+        self.r_cut = float(r_cut)
+        self.prefactor = float(prefactor)
+        self.epsilon = float(epsilon)
+        self.skin = float(skin)
+        self._h = L.p3mo_new(self.mesh, self.cao, self.alpha, self.r_cut,
+                             _v3(mesh_off), self.epsilon, self.box_l,
+                             self.prefactor, self.skin)
+        if not self._h:
+            raise RuntimeError("oracle: invalid CA-mesh geometry")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().p3mo_free(self._h)
+            self._h = None
+
+    def geometry(self):
+        ints = np.zeros(32, np.int32)
+        dbls = np.zeros(16, np.float64)
+        lib().p3mo_geometry(self._h, ints, dbls)
+        return dict(dim=ints[0:3].copy(), size=int(ints[3]),
+                    ld_ind=ints[4:7].copy(), inner=ints[7:10].copy(),
+                    in_ld=ints[10:13].copy(), in_ur=ints[13:16].copy(),
+                    margin=ints[16:22].copy(), q_2_off=int(ints[22]),
+                    q_21_off=int(ints[23]), ld_pos=dbls[0:3].copy(),
+                    a=dbls[3:6].copy(), ai=dbls[6:9].copy(),
+                    cao_cut=dbls[9:12].copy())
+
+    def weights(self, pos):
+        """CA-mesh linear index, nearest mesh point and weights (3, cao)."""
+        ind = C.c_int(0)
+        nmp = np.zeros(3, np.int32)
+        w = np.zeros(3 * self.cao)
+        rc = lib().p3mo_weights(self._h, _v3(pos), C.byref(ind), nmp, w)
+        if rc:
+            raise ValueError("position outside the charge-assignment mesh")
+        return ind.value, nmp, w.reshape(3, self.cao)
+
+    def forces(self, pos, q, pos_unfolded=None):
+        pos = _pos(pos)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        f = np.zeros_like(pos)
+        up = None
+        if pos_unfolded is not None:
+            up_arr = _pos(pos_unfolded)
+            up = up_arr.ctypes.data
+        rc = lib().p3mo_add_long_range_forces(self._h, len(q), pos, q, up, f)
+        if rc:
+            raise ValueError("position outside the charge-assignment mesh")
+        return f
+
+    def energy(self, pos, q, pos_unfolded=None):
+        pos = _pos(pos)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        up = None
+        if pos_unfolded is not None:
+            up_arr = _pos(pos_unfolded)
+            up = up_arr.ctypes.data
+        return lib().p3mo_long_range_energy(self._h, len(q), pos, q, up)
+
+    def pressure(self, pos, q):
+        out = np.zeros(9)
+        rc = lib().p3mo_long_range_pressure(
+            self._h, len(q), _pos(pos),
+            np.ascontiguousarray(q, dtype=np.float64), out)
+        if rc:
+            raise ValueError("position outside the charge-assignment mesh")
+        return out.reshape(3, 3)
+
+    def charge_mesh(self, pos, q):
+        out = np.zeros(int(np.prod(self.mesh)))
+        rc = lib().p3mo_charge_mesh(self._h, len(q), _pos(pos),
+                                    np.ascontiguousarray(q, np.float64), out)
+        if rc:
+            raise ValueError("position outside the charge-assignment mesh")
+        return out.reshape(tuple(self.mesh))
+
+    def fields(self, pos, q):
+        m = tuple(self.mesh)
+        outs = [np.zeros(int(np.prod(m))) for _ in range(3)]
+        rc = lib().p3mo_fields(self._h, len(q), _pos(pos),
+                               np.ascontiguousarray(q, np.float64), *outs)
+        if rc:
+            raise ValueError("position outside the charge-assignment mesh")
+        return [o.reshape(m) for o in outs]
+
+    def influence(self):
+        m = int(np.prod(self.mesh))
+        gf = np.zeros(m)
+        ge = np.zeros(m)
+        lib().p3mo_influence(self._h, gf, ge)
+        return gf.reshape(tuple(self.mesh)), ge.reshape(tuple(self.mesh))
+
+
+def sinc(x):
+    return lib().p3mo_sinc(float(x))
+
+
+def bspline(order, i, x):
+    return lib().p3mo_bspline(int(order), int(i), float(x))
+
+
+def bspline_coeffs(order, i):
+    num = np.zeros(order, np.int64)
+    den = C.c_int64(0)
+    lib().p3mo_bspline_coeffs(order, i, num, C.byref(den))
+    return num, den.value
+
+
+def meshift(n, zero_out_midpoint=False):
+    out = np.zeros(n, np.int32)
+    lib().p3mo_meshift(n, int(bool(zero_out_midpoint)), out)
+    return out
+
+
+def fft3d(a, sign):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    buf = a.view(np.float64).reshape(-1).copy()
+    lib().p3mo_fft3d(a.shape[0], a.shape[1], a.shape[2], buf, int(sign))
+    return buf.view(np.complex128).reshape(a.shape)
+
+
+def cotangent_sum(cao, n, mesh_i):
+    return lib().p3mo_cotangent_sum(cao, n, mesh_i)
+
+
+def real_space_error(pref, r_cut_iL, n_c_part, sum_q2, alpha_L, box_l):
+    return lib().p3mo_real_space_error(pref, r_cut_iL, n_c_part, sum_q2,
+                                       alpha_L, _v3(box_l))
+
+
+def k_space_error(pref, mesh, cao, n_c_part, sum_q2, alpha_L, box_l):
+    return lib().p3mo_k_space_error(pref, _v3(mesh, np.int32), cao, n_c_part,
+                                    sum_q2, alpha_L, _v3(box_l))
+
+
+def calculate_accuracy(pref, mesh, cao, r_cut_iL, n_c_part, sum_q2,
+                       accuracy, box_l):
+    out = np.zeros(4)
+    lib().p3mo_calculate_accuracy(pref, _v3(mesh, np.int32), cao, r_cut_iL,
+                                  n_c_part, sum_q2, accuracy, _v3(box_l), out)
+    return tuple(out)
+
+
+def tune_r_cut(pref, mesh, cao, n_c_part, sum_q2, accuracy, box_l,
+               skin=0.4):
+    """Bisection of TuningAlgorithm::get_mc_time for fixed (mesh, cao).
+
+    Returns (r_cut_iL, alpha_L, accuracy) or None."""
+    box_l = _v3(box_l)
+    r_max = (min(box_l) / 2. - skin) / box_l[0]
+    out = np.zeros(3)
+    rc = lib().p3mo_tune_r_cut(pref, _v3(mesh, np.int32), cao, n_c_part,
+                               sum_q2, accuracy, box_l, skin, 0.0, r_max, out)
+    if rc:
+        return None
+    return tuple(out)
+
+
+def real_space(pos, q, box_l, prefactor, alpha, r_cut, virial=False):
+    pos = _pos(pos)
+    f = np.zeros_like(pos)
+    vir = np.zeros(9)
+    e = lib().p3mo_real_space(len(q), pos, np.ascontiguousarray(q, np.float64),
+                              _v3(box_l), prefactor, alpha, r_cut, f, vir)
+    if virial:
+        return e, f, vir.reshape(3, 3)
+    return e, f
+
+
+def ewald_kspace(pos, q, box_l, prefactor, alpha, kmax, n_out=None):
+    """FP64 Ewald reciprocal-space energy and forces on the first n_out."""
+    pos = _pos(pos)
+    n = len(q)
+    n_out = n if n_out is None else int(n_out)
+    f = np.zeros((n_out, 3))
+    e = lib().p3mo_ewald_kspace(n, pos, np.ascontiguousarray(q, np.float64),
+                                _v3(box_l), prefactor, alpha, kmax, n_out, f)
+    return e, f
