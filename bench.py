@@ -83,6 +83,11 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 timed loop (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 256 MiB device copy first (reference "
+                         "dispatch for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -110,6 +115,12 @@ def main():
           f"accuracy={prm['accuracy']:.3e}", flush=True)
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     tp = torch.tensor(pos, device=dev)
     tq = torch.tensor(q, device=dev)
     tf = torch.zeros((n, 3), dtype=torch.float64, device=dev)

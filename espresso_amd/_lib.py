@@ -7,7 +7,10 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libespresso_amd_p3m.so")
+# ESPRESSO_AMD_LIB selects another build of the same library (e.g. a variant
+# built for a measurement); the default is the in-tree build.
+LIB_PATH = os.environ.get("ESPRESSO_AMD_LIB",
+                          os.path.join(HERE, "lib", "libespresso_amd_p3m.so"))
 
 P3M_OK = 0
 P3M_ERR_DOMAIN = 1

@@ -120,11 +120,62 @@ struct DevGeo {
   int marg[3];   // CA-local index of the FFT origin (left margin)
   int B[3];      // brick dims (first stencil points per brick)
   int nb[3];     // bricks per dim
+  int T[3];      // tile dims B + cao - 1
+  unsigned long long tz_magic, tyz_magic; // floor(2^32 / d) + 1 for d = T[2], T[1]*T[2]
   double ld_pos[3];
   double ai[3];
   double box_l[3];
   double pos_shift;
 };
+
+/** floor(e / d) for 0 <= e < 2^16 and 1 <= d < 2^16 from the magic number
+ *  floor(2^32 / d) + 1 (exact in that range). */
+__device__ __forceinline__ int magic_div(int e, unsigned long long magic) {
+  return static_cast<int>((static_cast<unsigned long long>(e) * magic) >> 32);
+}
+
+/** Periodic mesh index of tile row/column v = o + a >= 0 (tiles may be
+ *  wider than the mesh for tiny meshes, so wrap as often as needed). */
+__device__ __forceinline__ int wrap_once(int v, int M) {
+  while (v >= M) v -= M;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// DPP lane exchanges on FP64 values (two 32-bit moves per value).
+// ---------------------------------------------------------------------------
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+  int const lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  int const hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  int const lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  int const hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+/** Sum over aligned segments of S lanes (S in {1, 4, 16, 32, 64}); the
+ *  result is valid in the first lane of each segment.  Must be executed by
+ *  all 64 lanes (lanes outside the work contribute zeros). */
+template <int S> __device__ __forceinline__ double segment_total(double v) {
+  if constexpr (S >= 4) {
+    v += dpp_f64<0xB1>(v); // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v); // quad_perm [2,3,0,1]
+  }
+  if constexpr (S >= 16) {
+    v += dpp_f64<0x141>(v); // row_half_mirror
+    v += dpp_f64<0x140>(v); // row_mirror: every lane holds its row sum
+  }
+  if constexpr (S == 32) {
+    double const a = readlane_f64(v, 0) + readlane_f64(v, 16);
+    double const b = readlane_f64(v, 32) + readlane_f64(v, 48);
+    v = ((threadIdx.x & 63) < 32) ? a : b;
+  } else if constexpr (S == 64) {
+    v = (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+  }
+  return v;
+}
 
 /** Mesh coordinates of one particle for stencil order P: CA-local first
  *  stencil point nmp, its periodic FFT-mesh index gi and the offsets dist.

@@ -39,7 +39,8 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 64; // particles staged per brick pass
+constexpr int kPG = 512;    // block size of the particle-mesh kernels
+constexpr int kChunk = 64;  // particles staged per brick pass
 
 // ---------------------------------------------------------------------------
 // small utility kernels
@@ -219,6 +220,29 @@ struct ChunkSlots {
   int *lo;    // [kChunk] tile index of the first stencil point
 };
 
+/** Lane segments for stencil order P: a particle occupies S = next power of
+ *  two >= P^2 lanes of a wave64 (ppw particles per wave); lane r < P^2 of a
+ *  segment handles stencil column (j, k) = (r / P, r % P). */
+template <int P> struct Seg {
+  static constexpr int S = (P * P <= 1) ? 1 : (P * P <= 4) ? 4 : (P * P <= 16) ? 16
+                           : (P * P <= 32) ? 32 : 64;
+  static constexpr int ppw = 64 / S;
+};
+template <int P> struct LaneSeg {
+  int wave, nwaves, sub, r, j, k;
+  bool active;
+  __device__ LaneSeg() {
+    int const lane = threadIdx.x & 63;
+    wave = threadIdx.x >> 6;
+    nwaves = blockDim.x >> 6;
+    sub = lane / Seg<P>::S;
+    r = lane - sub * Seg<P>::S;
+    active = r < P * P;
+    j = active ? r / P : 0;
+    k = active ? r - (r / P) * P : 0;
+  }
+};
+
 template <int P>
 __device__ __forceinline__ void stage_chunk(DevGeo const &g, double4 const *spos, int c0, int cnt,
                                             int o0, int o1, int o2, int TY, int TZ,
@@ -240,54 +264,59 @@ __device__ __forceinline__ void stage_chunk(DevGeo const &g, double4 const *spos
 }
 
 template <int P, typename MT>
-__global__ __launch_bounds__(kBlock) void k_spread(DevGeo g, int const *__restrict__ offset,
-                                                   double4 const *__restrict__ spos,
-                                                   MT *__restrict__ rho) {
+__global__ __launch_bounds__(kPG) void k_spread(DevGeo g, int const *__restrict__ offset,
+                                                double4 const *__restrict__ spos,
+                                                MT *__restrict__ rho) {
+  // The tile accumulates in FP64 for both mesh precisions: LDS FP64 atomics
+  // (ds_add_f64) are several times faster than FP32 ones (ds_add_f32) for
+  // this scattered pattern on gfx950, and the sum is rounded once.
   extern __shared__ double smem_d[];
-  int const TX = g.B[0] + P - 1, TY = g.B[1] + P - 1, TZ = g.B[2] + P - 1;
-  int const tile_n = TX * TY * TZ;
+  int const TY = g.T[1], TZ = g.T[2];
+  int const TYZ = TY * TZ;
+  int const tile_n = g.T[0] * TYZ;
   ChunkSlots s;
   s.w = smem_d;
   s.qq = s.w + kChunk * 3 * P;
   s.lo = reinterpret_cast<int *>(s.qq + kChunk);
-  MT *tile = reinterpret_cast<MT *>(s.lo + kChunk);
+  double *tile = reinterpret_cast<double *>(s.lo + kChunk);
 
   int const b = blockIdx.x;
   int const bz = b % g.nb[2], by = (b / g.nb[2]) % g.nb[1], bx = b / (g.nb[2] * g.nb[1]);
   int const o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
   int const beg = offset[b], end = offset[b + 1];
   if (beg == end) return;
-  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = MT(0);
+  LaneSeg<P> const seg;
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
     __syncthreads();
     stage_chunk<P>(g, spos, c0, cnt, o0, o1, o2, TY, TZ, s);
     __syncthreads();
-    int const items = cnt * P * P;
-    for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      int const p = it / (P * P);
-      int const r = it - p * (P * P);
-      int const i = r / P;
-      int const j = r - i * P;
-      double const *w = s.w + p * 3 * P;
-      double const wxy = w[i] * w[P + j];
-      double const qv = s.qq[p];
-      MT *dst = tile + s.lo[p] + (i * TY + j) * TZ;
+    // one lane segment per particle, lanes over the (j, k) plane of the
+    // stencil, loop over i: every instruction touches one compact patch
+    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
+    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
+      int const p = grp * Seg<P>::ppw + seg.sub;
+      if (seg.active && p < cnt) {
+        double const *w = s.w + p * 3 * P;
+        double const wyzq = (w[P + seg.j] * w[2 * P + seg.k]) * s.qq[p];
+        double *dst = tile + s.lo[p] + seg.j * TZ + seg.k;
 #pragma unroll
-      for (int k = 0; k < P; ++k) atomicAdd(dst + k, static_cast<MT>((wxy * w[2 * P + k]) * qv));
+        for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, w[i] * wyzq);
+      }
     }
   }
   __syncthreads();
   int const M0 = g.M[0], M1 = g.M[1], M2 = g.M[2];
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
-    MT const v = tile[e];
-    if (v != MT(0)) {
-      int const a = e / (TY * TZ);
-      int const rr = e - a * (TY * TZ);
-      int const bb = rr / TZ;
+    double const v = tile[e];
+    if (v != 0.) {
+      int const a = magic_div(e, g.tyz_magic);
+      int const rr = e - a * TYZ;
+      int const bb = magic_div(rr, g.tz_magic);
       int const c = rr - bb * TZ;
-      int const x = (o0 + a) % M0, y = (o1 + bb) % M1, z = (o2 + c) % M2;
-      unsafeAtomicAdd(rho + ((int64_t)x * M1 + y) * M2 + z, v);
+      int const x = wrap_once(o0 + a, M0), y = wrap_once(o1 + bb, M1), z = wrap_once(o2 + c, M2);
+      unsafeAtomicAdd(rho + ((int64_t)x * M1 + y) * M2 + z, static_cast<MT>(v));
     }
   }
 }
@@ -556,23 +585,44 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(cx_t<MT> const *__restrict
 
 // ---------------------------------------------------------------------------
 // force back-interpolation (AssignForces<cao>, p3m.cpp:344-373): one block
-// per brick, the brick's field tiles in LDS, partial sums per stencil row,
-// FP64 accumulation per particle, F -= q * prefactor/V * E.
+// per brick with the brick's field tiles in LDS.  A lane segment per particle
+// covers the (j, k) plane of the stencil and loops over i; the segment is
+// reduced with a fixed xor butterfly (deterministic, no atomics) and its
+// first lane applies F -= q * prefactor/V * E in FP64.
 // ---------------------------------------------------------------------------
-template <int P, typename MT>
-__global__ __launch_bounds__(kBlock) void k_gather(
+/** Lane segments of the gather: S = next power of two >= P lanes per
+ *  particle, lane i < P of a segment sweeps the (j, k) plane of stencil
+ *  layer i. */
+template <int P> struct GSeg {
+  static constexpr int S = (P <= 1) ? 1 : (P <= 2) ? 2 : (P <= 4) ? 4 : 8;
+  static constexpr int ppw = 64 / S;
+};
+
+/** Sum over aligned segments of S in {1, 2, 4, 8} lanes with DPP; valid in
+ *  every lane of the segment.  Executed by all lanes of the wave. */
+template <int S> __device__ __forceinline__ double small_segment_total(double v) {
+  if constexpr (S >= 2) v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+  if constexpr (S >= 4) v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+  if constexpr (S >= 8) v += dpp_f64<0x141>(v); // row_half_mirror
+  return v;
+}
+
+template <int P, int BB, typename MT>
+__global__ __launch_bounds__(kPG) void k_gather(
     DevGeo g, int const *__restrict__ offset, double4 const *__restrict__ spos,
     int const *__restrict__ sidx, MT const *__restrict__ Ex, MT const *__restrict__ Ey,
     MT const *__restrict__ Ez, double *__restrict__ force, double force_prefac,
     double const *__restrict__ sums, double dip_pref) {
+  constexpr int T = BB + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  constexpr int S = GSeg<P>::S;
+  constexpr int ppw = GSeg<P>::ppw;
   extern __shared__ double smem_d[];
-  int const TX = g.B[0] + P - 1, TY = g.B[1] + P - 1, TZ = g.B[2] + P - 1;
-  int const tile_n = TX * TY * TZ;
   ChunkSlots s;
   s.w = smem_d;
   s.qq = s.w + kChunk * 3 * P;
-  double *acc = s.qq + kChunk; // [kChunk][3]
-  s.lo = reinterpret_cast<int *>(acc + 3 * kChunk);
+  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
   MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
   MT *ty = tx + tile_n;
   MT *tz = ty + tile_n;
@@ -581,14 +631,15 @@ __global__ __launch_bounds__(kBlock) void k_gather(
   int const beg = offset[b], end = offset[b + 1];
   if (beg == end) return;
   int const bz = b % g.nb[2], by = (b / g.nb[2]) % g.nb[1], bx = b / (g.nb[2] * g.nb[1]);
-  int const o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
+  int const o0 = bx * BB, o1 = by * BB, o2 = bz * BB;
   int const M0 = g.M[0], M1 = g.M[1], M2 = g.M[2];
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
-    int const a = e / (TY * TZ);
-    int const rr = e - a * (TY * TZ);
-    int const bb = rr / TZ;
-    int const c = rr - bb * TZ;
-    int64_t const gidx = ((int64_t)((o0 + a) % M0) * M1 + (o1 + bb) % M1) * M2 + (o2 + c) % M2;
+    int const a = e / TYZ;
+    int const rr = e - a * TYZ;
+    int const bb = rr / T;
+    int const c = rr - bb * T;
+    int64_t const gidx =
+        ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + bb, M1)) * M2 + wrap_once(o2 + c, M2);
     tx[e] = Ex[gidx];
     ty[e] = Ey[gidx];
     tz[e] = Ez[gidx];
@@ -599,43 +650,138 @@ __global__ __launch_bounds__(kBlock) void k_gather(
     dm[1] = dip_pref * sums[4];
     dm[2] = dip_pref * sums[5];
   }
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const li = lane - sub * S;
+  bool const active = li < P;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
     __syncthreads();
-    stage_chunk<P>(g, spos, c0, cnt, o0, o1, o2, TY, TZ, s);
-    for (int e = threadIdx.x; e < 3 * kChunk; e += blockDim.x) acc[e] = 0.;
+    stage_chunk<P>(g, spos, c0, cnt, o0, o1, o2, T, T, s);
     __syncthreads();
-    int const items = cnt * P * P;
-    for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      int const p = it / (P * P);
-      int const r = it - p * (P * P);
-      int const i = r / P;
-      int const j = r - i * P;
-      double const *w = s.w + p * 3 * P;
-      double const wxy = w[i] * w[P + j];
-      int const base = s.lo[p] + (i * TY + j) * TZ;
+    int const ngroups = (cnt + ppw - 1) / ppw;
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+      int const p = grp * ppw + sub;
+      bool const valid = active && p < cnt;
       double fx = 0., fy = 0., fz = 0.;
+      if (valid) {
+        double const *w = s.w + p * 3 * P;
+        double wz[P];
 #pragma unroll
-      for (int k = 0; k < P; ++k) {
-        double const ww = wxy * w[2 * P + k];
-        fx += ww * static_cast<double>(tx[base + k]);
-        fy += ww * static_cast<double>(ty[base + k]);
-        fz += ww * static_cast<double>(tz[base + k]);
+        for (int k = 0; k < P; ++k) wz[k] = w[2 * P + k];
+        double const wxi = w[li];
+        int const base = s.lo[p] + li * TYZ;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          double const wj = wxi * w[P + j];
+          int const row = base + j * T;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            double const ww = wj * wz[k];
+            fx += ww * static_cast<double>(tx[row + k]);
+            fy += ww * static_cast<double>(ty[row + k]);
+            fz += ww * static_cast<double>(tz[row + k]);
+          }
+        }
       }
-      atomicAdd(acc + 3 * p + 0, fx);
-      atomicAdd(acc + 3 * p + 1, fy);
-      atomicAdd(acc + 3 * p + 2, fz);
+      fx = small_segment_total<S>(fx);
+      fy = small_segment_total<S>(fy);
+      fz = small_segment_total<S>(fz);
+      if (li == 0 && p < cnt) {
+        double const qv = s.qq[p];
+        double const pf = qv * force_prefac;
+        int64_t const idx = sidx[c0 + p];
+        double *f = force + 3 * idx;
+        f[0] = (f[0] - pf * fx) - qv * dm[0];
+        f[1] = (f[1] - pf * fy) - qv * dm[1];
+        f[2] = (f[2] - pf * fz) - qv * dm[2];
+      }
     }
+  }
+}
+
+/** Gather with one lane segment per particle over the (j, k) plane of the
+ *  stencil (every LDS instruction serves one particle, so reads are free of
+ *  inter-particle bank conflicts; the segment is reduced with DPP and lane
+ *  reads).  Preferred for FP64 fields. */
+template <int P, int BB, typename MT>
+__global__ __launch_bounds__(kPG) void k_gather_plane(
+    DevGeo g, int const *__restrict__ offset, double4 const *__restrict__ spos,
+    int const *__restrict__ sidx, MT const *__restrict__ Ex, MT const *__restrict__ Ey,
+    MT const *__restrict__ Ez, double *__restrict__ force, double force_prefac,
+    double const *__restrict__ sums, double dip_pref) {
+  constexpr int T = BB + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + kChunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
+  MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
+  MT *ty = tx + tile_n;
+  MT *tz = ty + tile_n;
+
+  int const b = blockIdx.x;
+  int const beg = offset[b], end = offset[b + 1];
+  if (beg == end) return;
+  int const bz = b % g.nb[2], by = (b / g.nb[2]) % g.nb[1], bx = b / (g.nb[2] * g.nb[1]);
+  int const o0 = bx * BB, o1 = by * BB, o2 = bz * BB;
+  int const M0 = g.M[0], M1 = g.M[1], M2 = g.M[2];
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
+    int const a = e / TYZ;
+    int const rr = e - a * TYZ;
+    int const bb = rr / T;
+    int const c = rr - bb * T;
+    int64_t const gidx =
+        ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + bb, M1)) * M2 + wrap_once(o2 + c, M2);
+    tx[e] = Ex[gidx];
+    ty[e] = Ey[gidx];
+    tz[e] = Ez[gidx];
+  }
+  double dm[3] = {0., 0., 0.};
+  if (dip_pref != 0.) {
+    dm[0] = dip_pref * sums[3];
+    dm[1] = dip_pref * sums[4];
+    dm[2] = dip_pref * sums[5];
+  }
+  LaneSeg<P> const seg;
+  for (int c0 = beg; c0 < end; c0 += kChunk) {
+    int const cnt = min(kChunk, end - c0);
     __syncthreads();
-    if (threadIdx.x < cnt) {
-      int const t = threadIdx.x;
-      double const qv = s.qq[t];
-      double const pf = qv * force_prefac;
-      int64_t const idx = sidx[c0 + t];
-      double *f = force + 3 * idx;
-      f[0] = (f[0] - pf * acc[3 * t + 0]) - qv * dm[0];
-      f[1] = (f[1] - pf * acc[3 * t + 1]) - qv * dm[1];
-      f[2] = (f[2] - pf * acc[3 * t + 2]) - qv * dm[2];
+    stage_chunk<P>(g, spos, c0, cnt, o0, o1, o2, T, T, s);
+    __syncthreads();
+    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
+    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
+      int const p = grp * Seg<P>::ppw + seg.sub;
+      bool const valid = seg.active && p < cnt;
+      double fx = 0., fy = 0., fz = 0.;
+      if (valid) {
+        double const *w = s.w + p * 3 * P;
+        double const wyz = w[P + seg.j] * w[2 * P + seg.k];
+        int const base = s.lo[p] + seg.j * T + seg.k;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+          double const ww = w[i] * wyz;
+          fx += ww * static_cast<double>(tx[base + i * TYZ]);
+          fy += ww * static_cast<double>(ty[base + i * TYZ]);
+          fz += ww * static_cast<double>(tz[base + i * TYZ]);
+        }
+      }
+      fx = segment_total<Seg<P>::S>(fx);
+      fy = segment_total<Seg<P>::S>(fy);
+      fz = segment_total<Seg<P>::S>(fz);
+      if (seg.r == 0 && p < cnt) {
+        double const qv = s.qq[p];
+        double const pf = qv * force_prefac;
+        int64_t const idx = sidx[c0 + p];
+        double *f = force + 3 * idx;
+        f[0] = (f[0] - pf * fx) - qv * dm[0];
+        f[1] = (f[1] - pf * fy) - qv * dm[1];
+        f[2] = (f[2] - pf * fz) - qv * dm[2];
+      }
     }
   }
 }
@@ -702,6 +848,8 @@ struct Solver::Impl {
   bool profiling = false;
   // launch shapes
   int nlp_z = 1, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
+  // gather mapping: plane segments (FP64 fields) or layer segments (FP32)
+  bool gather_plane = true;
   size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_yi = 0, lds_spread = 0, lds_gather = 0;
 
   void free_mesh() {
@@ -740,7 +888,9 @@ Solver::~Solver() {
 }
 
 void Solver::set_brick(int bx, int by, int bz) {
-  m_brick = {std::max(1, bx), std::max(1, by), std::max(1, bz)};
+  if (bx != 8 || by != 8 || bz != 8)
+    throw std::invalid_argument("P3M: only 8x8x8 bricks are built into this library");
+  m_brick = {bx, by, bz};
   if (m_configured) configure(m->params, m->box_l, m->skin, m->prefactor, m->prec);
 }
 
@@ -798,13 +948,18 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
     g.M[d] = params.mesh[d];
     g.dim[d] = m_lm.dim[d];
     g.marg[d] = m_lm.margin[2 * d];
-    g.B[d] = std::min(m_brick[d], params.mesh[d]);
+    g.B[d] = m_brick[d];
     g.nb[d] = (params.mesh[d] + g.B[d] - 1) / g.B[d];
     g.ld_pos[d] = m_lm.ld_pos[d];
     g.ai[d] = m->params.ai[d];
     g.box_l[d] = box_l[d];
   }
   g.Kz = params.mesh[2] / 2 + 1;
+  for (int d = 0; d < 3; ++d) g.T[d] = g.B[d] + cao - 1;
+  if (g.T[0] * g.T[1] * g.T[2] >= 65536)
+    throw std::runtime_error("P3M: brick tile too large");
+  g.tz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[2]) + 1;
+  g.tyz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[1] * g.T[2]) + 1;
   g.pos_shift = std::floor((cao - 1) / 2.0) - (cao % 2) / 2.0;
   m->nbricks = g.nb[0] * g.nb[1] * g.nb[2];
 
@@ -848,12 +1003,13 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
   m->lds_yi = lds_lines(g.M[1], 3 * m->ncol_yi);
   int const tile_n = (g.B[0] + cao - 1) * (g.B[1] + cao - 1) * (g.B[2] + cao - 1);
   size_t const chunk_bytes = kChunk * (3 * cao + 1) * sizeof(double) + kChunk * sizeof(int);
-  m->lds_spread = chunk_bytes + tile_n * msz;
-  m->lds_gather = chunk_bytes + 3 * kChunk * sizeof(double) + 3 * tile_n * msz;
+  m->lds_spread = chunk_bytes + tile_n * sizeof(double);
+  m->lds_gather = chunk_bytes + 3 * tile_n * msz;
   for (size_t b : {m->lds_z, m->lds_y, m->lds_x, m->lds_yi, m->lds_spread, m->lds_gather})
     if (b > lds_limit())
       throw std::runtime_error("P3M: mesh or brick too large for the LDS budget");
   m->conv_blocks = ((g.Kz + m->ncol_x - 1) / m->ncol_x) * g.M[1];
+  m->gather_plane = (precision == MeshPrecision::FP64);
   hip_check(hipMalloc(&m->conv_part, sizeof(double) * 8 * m->conv_blocks), "hipMalloc conv");
 
   auto set_attrs = [&](auto tag) {
@@ -865,7 +1021,8 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
     allow_lds(k_fft_y_inv<MT>, m->lds_yi);
     launch_cao(cao, [&](auto P) {
       allow_lds(k_spread<P.value, MT>, m->lds_spread);
-      allow_lds(k_gather<P.value, MT>, m->lds_gather);
+      allow_lds(k_gather<P.value, 8, MT>, m->lds_gather);
+      allow_lds(k_gather_plane<P.value, 8, MT>, m->lds_gather);
     });
   };
   if (precision == MeshPrecision::FP64)
@@ -936,7 +1093,7 @@ void run_solve(Solver::Impl *m, hipStream_t st, int64_t n, double const *pos, do
   k_fill<MT><<<grid_for(R, kBlock, 8192), kBlock, 0, st>>>(static_cast<MT *>(m->rho), MT(0), R);
   (void)hipEventRecord(ev[1], st);
   launch_cao(cao, [&](auto P) {
-    k_spread<P.value, MT><<<m->nbricks, kBlock, m->lds_spread, st>>>(g, m->offset, m->spos,
+    k_spread<P.value, MT><<<m->nbricks, kPG, m->lds_spread, st>>>(g, m->offset, m->spos,
                                                                      static_cast<MT *>(m->rho));
   });
   (void)hipEventRecord(ev[2], st);
@@ -993,10 +1150,16 @@ void run_solve(Solver::Impl *m, hipStream_t st, int64_t n, double const *pos, do
       dip_pref = m->prefactor *
                  (4. * std::numbers::pi / volume / (2. * m->params.epsilon + 1.));
     launch_cao(cao, [&](auto P) {
-      k_gather<P.value, MT><<<m->nbricks, kBlock, m->lds_gather, st>>>(
-          g, m->offset, m->spos, m->sidx, static_cast<MT const *>(m->rho),
-          static_cast<MT const *>(m->Ey), static_cast<MT const *>(m->Ez), force, force_prefac,
-          m->sums, dip_pref);
+      if (m->gather_plane)
+        k_gather_plane<P.value, 8, MT><<<m->nbricks, kPG, m->lds_gather, st>>>(
+            g, m->offset, m->spos, m->sidx, static_cast<MT const *>(m->rho),
+            static_cast<MT const *>(m->Ey), static_cast<MT const *>(m->Ez), force, force_prefac,
+            m->sums, dip_pref);
+      else
+        k_gather<P.value, 8, MT><<<m->nbricks, kPG, m->lds_gather, st>>>(
+            g, m->offset, m->spos, m->sidx, static_cast<MT const *>(m->rho),
+            static_cast<MT const *>(m->Ey), static_cast<MT const *>(m->Ez), force, force_prefac,
+            m->sums, dip_pref);
     });
   }
   (void)hipEventRecord(ev[6], st);
