@@ -148,6 +148,8 @@ std::vector<double> host_charges(p3m_context *ctx, int64_t n, double const *q, u
 
 void require_params(p3m_context const *ctx) {
   if (!ctx->have_params) throw std::runtime_error("P3M: parameters not set");
+}
+void require_device(p3m_context const *ctx) {
   if (!ctx->solver) throw std::runtime_error("P3M: host-only context has no device");
 }
 
@@ -165,6 +167,7 @@ void init_solver(p3m_context *ctx) {
   sanity_checks_boxl(p, box);
   p.alpha_L = p.alpha * box[0];
   p.r_cut_iL = p.r_cut * (1. / box[0]);
+  require_device(ctx);
   ctx->solver->configure(p, box, ctx->user.skin, ctx->user.prefactor, prec_of(ctx->user));
   ctx->initialized = true;
 }
@@ -268,7 +271,7 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
-    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (ctx->solver) hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     auto const box = v3(ctx->user.box_l);
     // sanity checks before tuning (p3m.impl.hpp:124-139, p3m.hpp:67-73):
     // the k-space cutoff is not known yet, only the epsilon/cubic rule and
@@ -298,6 +301,9 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
           in.sum_qpart++;
           in.sum_q2 += qq * qq;
         }
+      if (in.sum_qpart == 0)
+        throw std::runtime_error("CoulombP3M: no charged particles in the system");
+      require_device(ctx);
       DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, flags);
       ensure_staging(ctx, n);
       double *scratch_force = ctx->d_force; // forces are computed but discarded
@@ -341,6 +347,7 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
+    require_device(ctx);
     if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (!ctx->initialized) init_solver(ctx);
@@ -375,6 +382,7 @@ int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, double const *pos, double
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
+    require_device(ctx);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (!ctx->initialized) init_solver(ctx);
     DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);
@@ -389,6 +397,7 @@ int p3m_debug_indices(p3m_context *ctx, int64_t n, double const *pos, double con
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
+    require_device(ctx);
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     if (!ctx->initialized) init_solver(ctx);
     DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);

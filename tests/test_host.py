@@ -1,0 +1,137 @@
+"""Host-side tests of the C ABI library (no GPU): the library loads and
+exports every entry point of include/espresso_amd_p3m.h, parameter
+validation and activation errors carry the reference's messages
+(src/core/p3m/common.hpp:94-150, electrostatics.py:148-176,
+coulomb_interface.py, p3m_tuning_exceptions.py, long_range_actors.py), and
+the host error estimates and pair terms agree with the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from espresso_amd import _lib
+from espresso_amd.electrostatics import P3M, P3MGPU
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+VALID = dict(prefactor=1., accuracy=1e-3, mesh=16, cao=5, r_cut=2.0, alpha=1.0,
+             tune=False)
+
+
+def test_library_exports_header_symbols():
+    header = open(os.path.join(ROOT, "include", "espresso_amd_p3m.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(p3m_\w+)\s*\(",
+                              header, re.M))
+    assert declared == set(_lib.EXPORTED_SYMBOLS)
+    L = _lib.load()
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+def host(**kw):
+    return P3M(device=-1, **kw)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(epsilon=-1.), "Parameter 'epsilon' must be >= 0"),
+    (dict(accuracy=0.), "Parameter 'accuracy' must be > 0"),
+    (dict(r_cut=-2.), "Parameter 'r_cut' must be > 0"),
+    (dict(alpha=0.), "Parameter 'alpha' must be > 0"),
+    (dict(mesh=0), "Parameter 'mesh' must be > 0"),
+    (dict(mesh_off=[2., 0.5, 0.5]), "Parameter 'mesh_off' must be >= 0 and <= 1"),
+    (dict(cao=8), "Parameter 'cao' must be >= 1 and <= 7"),
+    (dict(cao=7, mesh=6), "Parameter 'cao' cannot be larger than 'mesh'"),
+    (dict(timings=0), "Parameter 'timings' must be > 0"),
+    (dict(prefactor=-1.), "Parameter 'prefactor' must be > 0"),
+    (dict(mesh=[16, 15, 16]),
+     "P3M requires an even number of mesh points in all directions"),
+])
+def test_parameter_validation_messages(kw, msg):
+    params = dict(VALID)
+    params.update(kw)
+    with pytest.raises(ValueError, match=re.escape(msg)):
+        host(**params)
+
+
+def test_tuning_sentinels_accepted():
+    # -1 means "to be tuned" when tune=True (common.hpp:104-118)
+    p = host(prefactor=1., accuracy=1e-3)
+    assert p.get_params()["is_tuned"] is False
+    with pytest.raises(ValueError, match="Parameter 'r_cut' must be > 0"):
+        host(prefactor=1., accuracy=1e-3, tune=False, mesh=16, cao=5, alpha=1.)
+
+
+def test_gpu_actor_requires_single_precision():
+    with pytest.raises(ValueError, match="P3M GPU only implemented in single-precision mode"):
+        P3MGPU(device=-1, single_precision=False, **VALID)
+    assert P3MGPU(device=-1, **VALID).get_params()["single_precision"] is True
+
+
+def test_missing_required_keys():
+    with pytest.raises(ValueError, match="prefactor"):
+        P3M(device=-1, accuracy=1e-3)
+
+
+def _two(box=10.):
+    pos = np.array([[1., 1., 1.], [box / 2, box / 2, box / 2]])
+    return pos, np.array([-1., 1.])
+
+
+def test_activation_errors():
+    pos, q = _two()
+    # charge neutrality (coulomb.cpp:286-325)
+    with pytest.raises(RuntimeError, match="The system is not charge neutral"):
+        host(**VALID).activate(pos, np.array([1., 1.]), box_l=[10.] * 3)
+    host(check_neutrality=False, **VALID)  # accepted
+    # non-cubic box with non-metallic epsilon (p3m.cpp:812-818)
+    with pytest.raises(RuntimeError,
+                       match="CoulombP3M: non-metallic epsilon requires cubic box"):
+        host(**dict(VALID, epsilon=1.)).activate(pos, q, box_l=[10., 10., 20.])
+    # no charges while tuning (p3m.cpp:768-771)
+    with pytest.raises(RuntimeError,
+                       match="CoulombP3M: no charged particles in the system"):
+        host(prefactor=2., accuracy=1e-2).activate(pos, np.zeros(2), box_l=[10.] * 3)
+    # k-space cutoff larger than half the box (p3m.cpp:795-801,
+    # long_range_actors.py:226-236)
+    with pytest.raises(RuntimeError,
+                       match="k-space cutoff .+ is larger than half of box dimension"):
+        host(prefactor=1., accuracy=1., r_cut=1., alpha=1., cao=6, mesh=6,
+             tune=False).activate(pos, q, box_l=[20.] * 3)
+    # a valid host-only activation stops at the device
+    with pytest.raises(RuntimeError, match="host-only context has no device"):
+        host(**VALID).activate(pos, q, box_l=[10.] * 3)
+
+
+@pytest.mark.parametrize("mesh,cao", [([16, 16, 16], 3), ([32, 24, 20], 5),
+                                      ([48, 48, 48], 7)])
+def test_k_space_error_matches_oracle(mesh, cao):
+    L = _lib.load()
+    box = [11., 9., 10.]
+    out = C.c_double()
+    m = (C.c_int * 3)(*mesh)
+    b = (C.c_double * 3)(*box)
+    for alpha_L in (5., 10., 20.):
+        _lib.raise_for(L.p3m_k_space_error(1.3, m, cao, 500, 480., alpha_L, b,
+                                           C.byref(out)))
+        ref = O.k_space_error(1.3, mesh, cao, 500, 480., alpha_L, box)
+        assert abs(out.value - ref) <= 1e-12 * ref
+        _lib.raise_for(L.p3m_real_space_error(1.3, 0.2, 500, 480., alpha_L, b,
+                                              C.byref(out)))
+        ref = O.real_space_error(1.3, 0.2, 500, 480., alpha_L, box)
+        assert abs(out.value - ref) <= 1e-14 * ref
+
+
+def test_pair_terms_match_oracle():
+    p = host(**dict(VALID, prefactor=1.7, r_cut=3.0, alpha=0.8))
+    pos = np.array([[1., 1., 1.], [2.3, 1.4, 0.2]])
+    q = np.array([1.5, -0.5])
+    d = pos[0] - pos[1]
+    dist = float(np.linalg.norm(d))
+    e_ref, f_ref = O.real_space(pos, q, [10.] * 3, 1.7, 0.8, 3.0)
+    np.testing.assert_allclose(p.pair_force(q[0] * q[1], d, dist), f_ref[0],
+                               rtol=1e-14)
+    assert abs(p.pair_energy(q[0] * q[1], dist) - e_ref) <= 1e-14 * abs(e_ref)
+    assert p.pair_energy(1., 3.5) == 0.
+    assert np.all(p.pair_force(1., [0., 0., 3.5], 3.5) == 0.)
