@@ -37,7 +37,15 @@ EXPORTED_SYMBOLS = (
     "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
+    "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
+    "p3m_batch_particle_offset", "p3m_batch_system_bytes", "p3m_batch_solve",
+    "p3m_batch_graph_solve", "p3m_batch_set_launch_shape",
+    "p3m_batch_synchronize", "p3m_batch_get_stream", "p3m_batch_profile_begin",
+    "p3m_batch_profile_end", "p3m_batch_last_error", "p3m_tune_model",
 )
+
+_STRING_RESULT = ("p3m_last_error", "p3m_last_global_error",
+                  "p3m_batch_last_error")
 
 
 class p3m_params(C.Structure):
@@ -111,19 +119,50 @@ def load():
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []
     L.p3m_last_global_error.restype = C.c_char_p
+    ip = C.POINTER(C.c_int)
+    L.p3m_batch_create.argtypes = [C.c_int, C.c_int, C.POINTER(vp)]
+    L.p3m_batch_destroy.argtypes = [vp]
+    L.p3m_batch_add_system.argtypes = [vp, C.POINTER(p3m_params), C.c_int64, ip]
+    L.p3m_batch_particle_offset.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    L.p3m_batch_system_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
+    L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
+    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int]
+    L.p3m_batch_synchronize.argtypes = [vp]
+    L.p3m_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.p3m_batch_profile_begin.argtypes = [vp, C.c_int]
+    L.p3m_batch_profile_end.argtypes = [vp, dp, ip]
+    L.p3m_batch_last_error.argtypes = [vp]
+    L.p3m_batch_last_error.restype = C.c_char_p
+    L.p3m_tune_model.argtypes = [C.POINTER(p3m_params), C.c_int, C.c_double,
+                                 C.c_int, C.c_int, C.c_int, C.c_int]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("p3m_last_error", "p3m_last_global_error"):
+        if name not in _STRING_RESULT:
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
 
 
-def raise_for(status, ctx=None):
+def wait_for_torch(stream_handle):
+    """Order the solver stream after the work torch has queued so far on its
+    current stream (the producer of the device tensors handed over)."""
+    import torch
+    if stream_handle:
+        torch.cuda.ExternalStream(stream_handle).wait_stream(
+            torch.cuda.current_stream())
+
+
+def raise_for(status, ctx=None, batch=None):
     """Map a C status code to the Python exception the reference raises."""
     if status == P3M_OK:
         return
     L = load()
-    msg = (L.p3m_last_error(ctx) if ctx else L.p3m_last_global_error())
+    if batch:
+        msg = L.p3m_batch_last_error(batch)
+    elif ctx:
+        msg = L.p3m_last_error(ctx)
+    else:
+        msg = L.p3m_last_global_error()
     msg = msg.decode() if msg else ""
     if status in (P3M_ERR_DOMAIN, P3M_ERR_INVALID):
         raise ValueError(msg)

@@ -1,0 +1,216 @@
+"""Many independent P3M systems solved per launch.
+
+The reference solves one system per process (one ``CoulombP3M`` actor per
+``System``).  ``P3MBatch`` applies the same k-space algorithm to many systems
+at once: each system has its own box, mesh, charge-assignment order and
+splitting parameter, keeps its meshes and influence function resident in
+device memory, and is solved together with any other selected systems in one
+launch per pipeline stage.  Particle data of all systems is concatenated in
+the order the systems were added and must live on the device (torch tensors)::
+
+    b = P3MBatch(single_precision=True)
+    ids = [b.add_system(box_l=[L, L, L], n=n, prefactor=1., mesh=[32] * 3,
+                        cao=5, alpha=a, r_cut=rc) for ...]
+    pos = torch.empty((b.total_particles, 3), dtype=torch.float64,
+                      device="cuda")
+    ...
+    energies, virials = b.solve(pos, q, force, energy=True)
+
+``tune_model`` picks (mesh, cao, r_cut, alpha) for one system with the
+reference's tuning algorithm (src/core/electrostatics/p3m.cpp:594-791) using
+a cost model instead of timed integration steps, so that hundreds of systems
+can be tuned without device runs.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _device_ptr(t, name, dtype_size, numel):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError(f"'{name}' must be a device tensor")
+    if t.dtype != torch.float64 or not t.is_contiguous():
+        raise TypeError(f"'{name}' must be a contiguous float64 tensor")
+    if t.numel() != numel:
+        raise ValueError(f"'{name}' has {t.numel()} values, expected {numel}")
+    return t.data_ptr()
+
+
+class P3MBatch:
+    """Batched k-space solver (FP32 mesh by default, FP64 with
+    ``single_precision=False``)."""
+
+    def __init__(self, device=0, single_precision=True):
+        L = _lib.load()
+        self._b = C.c_void_p()
+        prec = _lib.P3M_PREC_FP32 if single_precision else _lib.P3M_PREC_FP64
+        _lib.raise_for(L.p3m_batch_create(int(device), prec, C.byref(self._b)))
+        self._counts = []
+        self.single_precision = bool(single_precision)
+
+    def _check(self, status):
+        _lib.raise_for(status, batch=self._b)
+
+    def add_system(self, box_l, n, prefactor, mesh, cao, alpha, r_cut,
+                   accuracy=1e-4, epsilon=0., mesh_off=(-1., -1., -1.),
+                   skin=0.4):
+        """Add a system with fixed parameters and ``n`` particles; returns
+        its id.  Validation follows P3MParameters (common.hpp:94-150) and the
+        post-tuning sanity checks (p3m.cpp:710-750)."""
+        L = _lib.load()
+        cp = _lib.p3m_params()
+        L.p3m_params_default(C.byref(cp))
+        if isinstance(mesh, (int, np.integer)):
+            mesh = [int(mesh)] * 3
+        for d in range(3):
+            cp.mesh[d] = int(mesh[d])
+            cp.box_l[d] = float(box_l[d])
+            cp.mesh_off[d] = float(mesh_off[d])
+        cp.cao = int(cao)
+        cp.alpha = float(alpha)
+        cp.r_cut = float(r_cut)
+        cp.accuracy = float(accuracy)
+        cp.epsilon = 0. if epsilon == "metallic" else float(epsilon)
+        cp.prefactor = float(prefactor)
+        cp.skin = float(skin)
+        cp.tune = 0
+        sid = C.c_int(-1)
+        self._check(L.p3m_batch_add_system(self._b, C.byref(cp), int(n),
+                                           C.byref(sid)))
+        self._counts.append(int(n))
+        return sid.value
+
+    @property
+    def num_systems(self):
+        return len(self._counts)
+
+    @property
+    def total_particles(self):
+        return int(sum(self._counts))
+
+    def particle_count(self, sys_id):
+        return self._counts[sys_id]
+
+    def offset(self, sys_id):
+        off = C.c_int64(0)
+        self._check(_lib.load().p3m_batch_particle_offset(self._b, int(sys_id),
+                                                          C.byref(off)))
+        return off.value
+
+    def system_bytes(self, sys_id):
+        """Resident device bytes of one system (meshes, influence function)."""
+        out = C.c_int64(0)
+        self._check(_lib.load().p3m_batch_system_bytes(self._b, int(sys_id),
+                                                       C.byref(out)))
+        return out.value
+
+    def _selection(self, systems):
+        if systems is None:
+            systems = range(self.num_systems)
+        ids = [int(s) for s in systems]
+        return len(ids), (C.c_int * max(len(ids), 1))(*ids)
+
+    def _arrays(self, pos, q, force):
+        n = self.total_particles
+        p = _device_ptr(pos, "pos", 8, 3 * n)
+        qq = _device_ptr(q, "q", 8, n)
+        f = None if force is None else _device_ptr(force, "force", 8, 3 * n)
+        _lib.wait_for_torch(self.stream())
+        return p, qq, f
+
+    def solve(self, pos, q, force=None, systems=None, energy=False,
+              pressure=False, sync=True):
+        """Solve the selected systems (all by default).  ``force`` is
+        accumulated in place (F -= k-space force).  Returns per-system
+        energies [nsel] and virials [nsel, 3, 3] (before division by the
+        volume), zeros for quantities not requested; ``sync=False`` only
+        enqueues and returns None."""
+        p, qq, f = self._arrays(pos, q, force)
+        nsel, ids = self._selection(systems)
+        flags = 0
+        if force is not None:
+            flags |= _lib.P3M_FORCES
+        if energy:
+            flags |= _lib.P3M_ENERGY
+        if pressure:
+            flags |= _lib.P3M_PRESSURE
+        if not sync:
+            flags |= _lib.P3M_ASYNC
+            self._check(_lib.load().p3m_batch_solve(
+                self._b, nsel, ids, p, qq, f, flags, None, None))
+            return None
+        e = (C.c_double * max(nsel, 1))()
+        v = (C.c_double * max(9 * nsel, 9))()
+        self._check(_lib.load().p3m_batch_solve(self._b, nsel, ids, p, qq, f,
+                                                flags, e, v))
+        return (np.array(e[:nsel]),
+                np.array(v[:9 * nsel]).reshape(nsel, 3, 3))
+
+    def graph_solve(self, pos, q, force, systems=None):
+        """Force solve replayed from a captured hipGraph (enqueue only)."""
+        p, qq, f = self._arrays(pos, q, force)
+        nsel, ids = self._selection(systems)
+        self._check(_lib.load().p3m_batch_graph_solve(self._b, nsel, ids, p,
+                                                      qq, f))
+
+    def set_launch_shape(self, gather_plane=True, ncol=8):
+        self._check(_lib.load().p3m_batch_set_launch_shape(
+            self._b, 1 if gather_plane else 0, int(ncol)))
+
+    def synchronize(self):
+        self._check(_lib.load().p3m_batch_synchronize(self._b))
+
+    def stream(self):
+        s = C.c_void_p()
+        self._check(_lib.load().p3m_batch_get_stream(self._b, C.byref(s)))
+        return s.value
+
+    def profile_begin(self, max_solves):
+        self._check(_lib.load().p3m_batch_profile_begin(self._b,
+                                                        int(max_solves)))
+
+    def profile_end(self):
+        """(number of profiled solves, summed stage times in ms)."""
+        out = (C.c_double * 7)()
+        n = C.c_int(0)
+        self._check(_lib.load().p3m_batch_profile_end(self._b, out,
+                                                      C.byref(n)))
+        keys = ("total", "bin", "spread", "fft_fwd", "convolve", "fft_bwd",
+                "gather")
+        return n.value, dict(zip(keys, out[:]))
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b:
+            try:
+                _lib.load().p3m_batch_destroy(b)
+            except Exception:  # pragma: no cover
+                pass
+            self._b = None
+
+
+def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
+               mesh_range=(1, 1 << 20), cao_range=(1, 7), skin=0.4,
+               epsilon=0., verbose=False):
+    """Tuned parameters of one system from the reference's tuning algorithm
+    with a modelled cost per trial.  Returns a dict with mesh, cao, alpha,
+    r_cut and the estimated accuracy; raises RuntimeError("CoulombP3M: failed
+    to reach requested accuracy") like the reference."""
+    L = _lib.load()
+    cp = _lib.p3m_params()
+    L.p3m_params_default(C.byref(cp))
+    for d in range(3):
+        cp.box_l[d] = float(box_l[d])
+    cp.prefactor = float(prefactor)
+    cp.accuracy = float(accuracy)
+    cp.epsilon = float(epsilon)
+    cp.skin = float(skin)
+    cp.verbose = 1 if verbose else 0
+    _lib.raise_for(L.p3m_tune_model(C.byref(cp), int(n_charged), float(sum_q2),
+                                    int(mesh_range[0]), int(mesh_range[1]),
+                                    int(cao_range[0]), int(cao_range[1])))
+    return {"mesh": [cp.mesh[d] for d in range(3)], "cao": cp.cao,
+            "alpha": cp.alpha, "r_cut": cp.r_cut, "accuracy": cp.accuracy}

@@ -1,0 +1,1689 @@
+// Descriptor-driven batched P3M solver; see p3m_batch.hpp.
+#include "p3m_batch.hpp"
+#include "p3m_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+
+namespace espresso_amd {
+
+void hip_check(hipError_t err, char const *what) {
+  if (err != hipSuccess)
+    throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(err));
+}
+
+FftPlan1D make_fft_plan(int n) {
+  if (n < 1) throw std::runtime_error("FFT length must be >= 1");
+  FftPlan1D p;
+  p.n = n;
+  int m = n;
+  auto push = [&p](int r) {
+    if (p.nstages >= FftPlan1D::max_stages) throw std::runtime_error("FFT plan too long");
+    p.radix[p.nstages++] = r;
+  };
+  while (m % 8 == 0) { push(8); m /= 8; }
+  while (m % 4 == 0) { push(4); m /= 4; }
+  while (m % 2 == 0) { push(2); m /= 2; }
+  for (int f : {3, 5, 7}) while (m % f == 0) { push(f); m /= f; }
+  for (int f = 11; m > 1 && f * f <= m; f += 2) while (m % f == 0) { push(f); m /= f; }
+  if (m > 1) push(m);
+  return p;
+}
+
+namespace {
+using namespace dev;
+
+constexpr int kBlock = 256;  // block size of the mesh and particle-list kernels
+constexpr int kPG = 512;     // block size of the particle-mesh kernels
+constexpr int kChunk = 64;   // particles staged per brick pass
+constexpr int kBrick = 8;    // first-stencil points per brick edge
+constexpr int kFill = 1024;  // mesh elements zeroed per block
+
+/** Per-system state read by every kernel. */
+struct SysDesc {
+  DevGeo g;
+  int cao;
+  int n;
+  long long part_off;
+  int brick_off; // into the brick counters
+  int off_off;   // into the brick offsets (brick_off + system index)
+  int nbricks;
+  void *rho, *Ey, *Ez, *khat, *A, *B, *G;
+  int tw_off[3];
+  FftPlanDev plan[3];
+  double wv[3];
+  double hais;
+  double force_prefac;
+  double dip_pref;
+};
+
+/** slot i of the selection with wt[i] <= b < wt[i+1] */
+__device__ __forceinline__ int find_slot(int const *__restrict__ wt, int nsel, int b) {
+  int lo = 0, hi = nsel;
+  while (hi - lo > 1) {
+    int const mid = (lo + hi) >> 1;
+    if (wt[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct Slot {
+  int slot, s, lb;
+};
+__device__ __forceinline__ Slot locate(int const *wt, int const *sel, int nsel) {
+  Slot r;
+  r.slot = find_slot(wt, nsel, blockIdx.x);
+  r.s = sel[r.slot];
+  r.lb = blockIdx.x - wt[r.slot];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// influence function on the half spectrum [Mx][My][Kz] (FP64 arithmetic):
+// grid_influence_function<T, S=1, m=0> (influence_function.hpp:110-150) with
+// G_opt (influence_function.hpp:53-89): G = 4 pi exp(-k^2/4a^2) / (k^2 U^2),
+// U^2 = prod_d sinc(k_d h_d / 2pi)^(2 cao), zero where every index is a
+// multiple of M/2 and where k^2/4a^2 >= 30.
+// ---------------------------------------------------------------------------
+__device__ double dev_sinc(double x) {
+  double const pix = std::numbers::pi * x;
+  if (fabs(x) > 0.1) return sin(pix) / pix;
+  double const c2 = -1. / 6., c4 = 1. / 120., c6 = -1. / 5040., c8 = 1. / 362880.;
+  double const pix2 = pix * pix;
+  return 1. + pix2 * (c2 + pix2 * (c4 + pix2 * (c6 + pix2 * c8)));
+}
+
+template <typename MT>
+__global__ void k_influence(MT *G, int Mx, int My, int Mz, int Kz, double wx, double wy,
+                            double wz, double hx, double hy, double hz, double alpha, int cao) {
+  int64_t const total = (int64_t)Mx * My * Kz;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int const kz = static_cast<int>(e % Kz);
+    int const y = static_cast<int>((e / Kz) % My);
+    int const x = static_cast<int>(e / ((int64_t)Kz * My));
+    double g = 0.;
+    int const hmx = Mx / 2, hmy = My / 2, hmz = Mz / 2;
+    bool const zero = (hmx == 0 || x % hmx == 0) && (hmy == 0 || y % hmy == 0) &&
+                      (hmz == 0 || kz % hmz == 0);
+    if (!zero) {
+      double const k0 = plain_shift(x, Mx) * wx;
+      double const k1 = plain_shift(y, My) * wy;
+      double const k2v = plain_shift(kz, Mz) * wz;
+      double const k2 = k0 * k0 + k1 * k1 + k2v * k2v;
+      if (k2 != 0.) {
+        double const f0 = dev_sinc(k0 * (1. / ((2. * std::numbers::pi) / hx)));
+        double const f1 = dev_sinc(k1 * (1. / ((2. * std::numbers::pi) / hy)));
+        double const f2 = dev_sinc(k2v * (1. / ((2. * std::numbers::pi) / hz)));
+        double const U2 = pow(f0 * f1 * f2, 2 * cao);
+        double const ex = (1. / (2. * alpha)) * (1. / (2. * alpha)) * k2;
+        double numerator = 0.;
+        if (ex < 30.) {
+          double const f3 = exp(-ex) * (4. * std::numbers::pi / k2);
+          numerator = U2 * f3 * k2;
+        }
+        g = numerator / (k2 * (U2 * U2));
+      }
+    }
+    G[e] = static_cast<MT>(g);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// particle binning (one block per 256 particles of a system)
+// ---------------------------------------------------------------------------
+__global__ void k_zero_counts(SysDesc const *__restrict__ sys, int const *__restrict__ sel,
+                              int nsel, int *counts) {
+  int const s = sel[blockIdx.x];
+  SysDesc const &d = sys[s];
+  for (int b = threadIdx.x; b < d.nbricks; b += blockDim.x) counts[d.brick_off + b] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
+                                                int const *__restrict__ sel,
+                                                int const *__restrict__ wt, int nsel,
+                                                ParticleArrays pa, int *counts, int *p_brick,
+                                                int *p_rank, int *p_nmp, double *partials,
+                                                int *err_flag) {
+  __shared__ double red[8 * 16];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
+  double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+  if (li < d.n) {
+    int64_t const i = d.part_off + li;
+    double const qi = pa.q[i];
+    int b = -1;
+    if (qi != 0.) {
+      double const x[3] = {pa.pos[3 * i], pa.pos[3 * i + 1], pa.pos[3 * i + 2]};
+      int nmp[3], gi[3];
+      double dist[3];
+      if (!particle_coordinates_rt(d.g, d.cao, x, nmp, gi, dist)) atomicOr(err_flag, 1);
+      b = ((gi[0] / kBrick) * d.g.nb[1] + gi[1] / kBrick) * d.g.nb[2] + gi[2] / kBrick;
+      p_rank[i] = atomicAdd(&counts[d.brick_off + b], 1);
+      if (p_nmp) {
+        p_nmp[3 * li] = nmp[0];
+        p_nmp[3 * li + 1] = nmp[1];
+        p_nmp[3 * li + 2] = nmp[2];
+      }
+      double const *u = pa.upos ? pa.upos + 3 * i : x;
+      v[0] = 1.;
+      v[1] = qi;
+      v[2] = qi * qi;
+      v[3] = qi * u[0];
+      v[4] = qi * u[1];
+      v[5] = qi * u[2];
+    } else if (p_nmp) {
+      p_nmp[3 * li] = p_nmp[3 * li + 1] = p_nmp[3 * li + 2] = -1;
+    }
+    p_brick[i] = b;
+  }
+  block_sum<7>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 7; ++k) partials[blockIdx.x * 8 + k] = v[k];
+}
+
+/** per-system deterministic reduction of block partials: one block per
+ *  selected system, sums[s*16 + base + k] = sum over the system's blocks */
+__global__ void k_reduce_sys(int const *__restrict__ sel, int const *__restrict__ wt, int nsel,
+                             double const *part, int nk, int base, double *sums) {
+  __shared__ double red[16 * 8];
+  int const slot = blockIdx.x;
+  int const s = sel[slot];
+  double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  for (int b = wt[slot] + threadIdx.x; b < wt[slot + 1]; b += blockDim.x)
+    for (int k = 0; k < nk; ++k) v[k] += part[b * 8 + k];
+  block_sum<8>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nk; ++k) sums[s * 16 + base + k] = v[k];
+  (void)nsel;
+}
+
+/** exclusive scan of one system's brick counts (one block per system) */
+__global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
+                                               int const *__restrict__ sel, int const *counts,
+                                               int *offsets) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  SysDesc const &d = sys[sel[blockIdx.x]];
+  int const nb = d.nbricks;
+  int const *count = counts + d.brick_off;
+  int *offset = offsets + d.off_off;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  int const lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < nb; base += blockDim.x) {
+    int const i = base + threadIdx.x;
+    int const c = (i < nb) ? count[i] : 0;
+    int s = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int const t = __shfl_up(s, off, 64);
+      if (lane >= off) s += t;
+    }
+    if (lane == 63) wsum[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        int const t = wsum[w];
+        wsum[w] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+    if (i < nb) offset[i] = carry + wsum[wid] + s - c;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += wsum[wid] + s;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) offset[nb] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ sys,
+                                                    int const *__restrict__ sel,
+                                                    int const *__restrict__ wt, int nsel,
+                                                    ParticleArrays pa, int const *p_brick,
+                                                    int const *p_rank, int const *offsets,
+                                                    double4 *spos, int *sidx) {
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
+  if (li >= d.n) return;
+  int64_t const i = d.part_off + li;
+  int const b = p_brick[i];
+  if (b < 0) return;
+  int64_t const slot = d.part_off + offsets[d.off_off + b] + p_rank[i];
+  spos[slot] = double4{pa.pos[3 * i], pa.pos[3 * i + 1], pa.pos[3 * i + 2], pa.q[i]};
+  sidx[slot] = static_cast<int>(li);
+}
+
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fill_rho(SysDesc const *__restrict__ sys,
+                                                     int const *__restrict__ sel,
+                                                     int const *__restrict__ wt, int nsel) {
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int64_t const R = (int64_t)d.g.M[0] * d.g.M[1] * d.g.M[2];
+  MT *rho = static_cast<MT *>(d.rho);
+  int64_t const beg = (int64_t)L.lb * kFill;
+  int64_t const end = min(beg + kFill, R);
+  for (int64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rho[e] = MT(0);
+}
+
+// ---------------------------------------------------------------------------
+// charge assignment into LDS brick tiles (AssignCharge<cao>, p3m.cpp:283-330;
+// p3m_interpolate, interpolation.hpp:191-208).  One block per brick of
+// 8x8x8 first-stencil points: the tile holds the brick plus cao-1 planes on
+// the high side; contributions are added with LDS FP64 atomics (for both
+// mesh precisions: ds_add_f64 is several times faster than ds_add_f32 for
+// this scattered pattern on gfx950, and the sum is rounded once) and the
+// tile is flushed into the periodic mesh with device-scope atomics.
+// ---------------------------------------------------------------------------
+struct ChunkSlots {
+  double *w;  // [kChunk][3P]
+  double *qq; // [kChunk]
+  int *lo;    // [kChunk] tile index of the first stencil point
+};
+
+template <int P>
+__device__ __forceinline__ void stage_chunk(DevGeo const &g, double4 const *spos, int64_t c0,
+                                            int cnt, int o0, int o1, int o2, int TY, int TZ,
+                                            ChunkSlots s) {
+  int const t = threadIdx.x;
+  if (t < cnt) {
+    double4 const p = spos[c0 + t];
+    double const x[3] = {p.x, p.y, p.z};
+    int nmp[3], gi[3];
+    double dist[3];
+    particle_coordinates<P>(g, x, nmp, gi, dist);
+    double *w = s.w + t * 3 * P;
+    bspline_weights<P>(dist[0], w);
+    bspline_weights<P>(dist[1], w + P);
+    bspline_weights<P>(dist[2], w + 2 * P);
+    s.qq[t] = p.w;
+    s.lo[t] = ((gi[0] - o0) * TY + (gi[1] - o1)) * TZ + (gi[2] - o2);
+  }
+}
+
+/** Lane segments over the (j, k) stencil plane: S = next power of two >= P^2
+ *  lanes per particle, ppw particles per wave64. */
+template <int P> struct Seg {
+  static constexpr int S = (P * P <= 1) ? 1 : (P * P <= 4) ? 4 : (P * P <= 16) ? 16
+                           : (P * P <= 32) ? 32 : 64;
+  static constexpr int ppw = 64 / S;
+};
+template <int P> struct LaneSeg {
+  int wave, nwaves, sub, r, j, k;
+  bool active;
+  __device__ LaneSeg() {
+    int const lane = threadIdx.x & 63;
+    wave = threadIdx.x >> 6;
+    nwaves = blockDim.x >> 6;
+    sub = lane / Seg<P>::S;
+    r = lane - sub * Seg<P>::S;
+    active = r < P * P;
+    j = active ? r / P : 0;
+    k = active ? r - (r / P) * P : 0;
+  }
+};
+
+template <int P> __device__ __forceinline__ void brick_origin(SysDesc const &d, int b, int &o0,
+                                                              int &o1, int &o2) {
+  int const bz = b % d.g.nb[2], by = (b / d.g.nb[2]) % d.g.nb[1], bx = b / (d.g.nb[2] * d.g.nb[1]);
+  o0 = bx * kBrick;
+  o1 = by * kBrick;
+  o2 = bz * kBrick;
+}
+
+template <int P, typename MT>
+__global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
+                                                int const *__restrict__ sel,
+                                                int const *__restrict__ wt, int nsel,
+                                                int const *__restrict__ offsets,
+                                                double4 const *__restrict__ spos) {
+  constexpr int T = kBrick + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + kChunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
+  double *tile = reinterpret_cast<double *>(s.lo + kChunk);
+
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int const b = L.lb;
+  int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
+  if (beg == end) return;
+  int o0, o1, o2;
+  brick_origin<P>(d, b, o0, o1, o2);
+  double4 const *sp = spos + d.part_off;
+  LaneSeg<P> const seg;
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
+  for (int c0 = beg; c0 < end; c0 += kChunk) {
+    int const cnt = min(kChunk, end - c0);
+    __syncthreads();
+    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
+    __syncthreads();
+    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
+    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
+      int const p = grp * Seg<P>::ppw + seg.sub;
+      if (seg.active && p < cnt) {
+        double const *w = s.w + p * 3 * P;
+        double const wyzq = (w[P + seg.j] * w[2 * P + seg.k]) * s.qq[p];
+        double *dst = tile + s.lo[p] + seg.j * T + seg.k;
+#pragma unroll
+        for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, w[i] * wyzq);
+      }
+    }
+  }
+  __syncthreads();
+  int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
+  MT *rho = static_cast<MT *>(d.rho);
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
+    double const v = tile[e];
+    if (v != 0.) {
+      int const a = e / TYZ;
+      int const rr = e - a * TYZ;
+      int const bb = rr / T;
+      int const c = rr - bb * T;
+      int const x = wrap_once(o0 + a, M0), y = wrap_once(o1 + bb, M1), z = wrap_once(o2 + c, M2);
+      unsafeAtomicAdd(rho + ((int64_t)x * M1 + y) * M2 + z, static_cast<MT>(v));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT passes.  Real mesh [Mx][My][Mz]; half spectrum [Mx][My][Kz].
+// ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ void load_twiddles(C *tw_lds, C const *tw, int n) {
+  for (int m = threadIdx.x; m < n; m += blockDim.x) tw_lds[m] = tw[m];
+}
+
+/** forward r2c along z: two real rows per complex line
+ *  (X1 = (Z_k + conj Z_{n-k})/2, X2 = (Z_k - conj Z_{n-k})/2i) */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict__ sys,
+                                                      int const *__restrict__ sel,
+                                                      int const *__restrict__ wt, int nsel,
+                                                      cx_t<MT> const *__restrict__ tw, int nlp) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  FftPlanDev const plan = d.plan[2];
+  int const n = plan.n;
+  int const ls = n + 1;
+  int const rows = d.g.M[0] * d.g.M[1];
+  int const Kz = d.g.Kz;
+  MT const *rho = static_cast<MT const *>(d.rho);
+  C *khat = static_cast<C *>(d.khat);
+  C *twl = reinterpret_cast<C *>(smem_d);
+  C *b0 = twl + n;
+  C *b1 = b0 + nlp * ls;
+  load_twiddles(twl, tw + d.tw_off[2], n);
+  int const pair0 = L.lb * nlp;
+  for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
+    int const l = e / n, j = e - l * n;
+    int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
+    MT const re = (r0 < rows) ? rho[r0 * n + j] : MT(0);
+    MT const im = (r1 < rows) ? rho[r1 * n + j] : MT(0);
+    b0[l * ls + j] = C{re, im};
+  }
+  __syncthreads();
+  C *res = lds_fft(b0, b1, nlp, ls, plan, twl, false);
+  for (int e = threadIdx.x; e < nlp * Kz; e += blockDim.x) {
+    int const l = e / Kz, k = e - l * Kz;
+    int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
+    C const Z = res[l * ls + k];
+    C const Zc = cconj(res[l * ls + (n - k) % n]);
+    C const sm = cadd(Z, Zc), df = csub(Z, Zc);
+    if (r0 < rows) khat[r0 * Kz + k] = C{MT(0.5) * sm.x, MT(0.5) * sm.y};
+    if (r1 < rows) khat[r1 * Kz + k] = C{MT(0.5) * df.y, MT(-0.5) * df.x};
+  }
+}
+
+/** forward c2c along y for a tile of kz columns at fixed x (in place) */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_y(SysDesc const *__restrict__ sys,
+                                                  int const *__restrict__ sel,
+                                                  int const *__restrict__ wt, int nsel,
+                                                  cx_t<MT> const *__restrict__ tw, int ncol) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  FftPlanDev const plan = d.plan[1];
+  int const n = plan.n;
+  int const ls = n + 1;
+  int const Kz = d.g.Kz;
+  int const ncb = (Kz + ncol - 1) / ncol;
+  int const cb = L.lb % ncb;
+  int64_t const x = L.lb / ncb;
+  int const My = d.g.M[1];
+  C *khat = static_cast<C *>(d.khat);
+  C *twl = reinterpret_cast<C *>(smem_d);
+  C *b0 = twl + n;
+  C *b1 = b0 + ncol * ls;
+  load_twiddles(twl, tw + d.tw_off[1], n);
+  int const c0 = cb * ncol;
+  int const nc = min(ncol, Kz - c0);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) b0[c * ls + j] = khat[(x * My + j) * Kz + c0 + c];
+  }
+  __syncthreads();
+  C *res = lds_fft(b0, b1, nc, ls, plan, twl, false);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) khat[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
+  }
+}
+
+/** forward c2c along x, convolution with the influence function, energy and
+ *  virial partial sums (p3m.cpp:393-449, 551-567), ik along x and inverse c2c
+ *  along x (p3m.cpp:491-522): A = IFFT_x(G rho^), B = IFFT_x(i k_x G rho^). */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict__ sys,
+                                                       int const *__restrict__ sel,
+                                                       int const *__restrict__ wt, int nsel,
+                                                       cx_t<MT> const *__restrict__ tw, int ncol,
+                                                       int want_fields, double *partials) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  __shared__ double red[8 * 16];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  FftPlanDev const plan = d.plan[0];
+  int const n = plan.n; // == Mx
+  int const ls = n + 1;
+  int const Kz = d.g.Kz;
+  int const Mx = d.g.M[0], My = d.g.M[1], Mz = d.g.M[2];
+  int const ncb = (Kz + ncol - 1) / ncol;
+  int const cb = L.lb % ncb;
+  int const y = L.lb / ncb;
+  C const *khat = static_cast<C const *>(d.khat);
+  MT const *G = static_cast<MT const *>(d.G);
+  C *Aout = static_cast<C *>(d.A);
+  C *Bout = static_cast<C *>(d.B);
+  C *twl = reinterpret_cast<C *>(smem_d);
+  C *X = twl + n;            // 2*ncol lines
+  C *Y = X + 2 * ncol * ls;  // 2*ncol lines
+  load_twiddles(twl, tw + d.tw_off[0], n);
+  int const c0 = cb * ncol;
+  int const nc = min(ncol, Kz - c0);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) X[c * ls + j] = khat[((int64_t)j * My + y) * Kz + c0 + c];
+  }
+  __syncthreads();
+  C *res = lds_fft(X, Y, nc, ls, plan, twl, false);
+  C *oth = (res == X) ? Y : X;
+  double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  double const wvx = d.wv[0], wvy = d.wv[1], wvz = d.wv[2];
+  double const ky = dop_shift(y, My) * wvy;
+  for (int e = threadIdx.x; e < n * nc; e += blockDim.x) {
+    int const c = e / n, j = e - c * n;
+    int const kz = c0 + c;
+    C const rh = res[c * ls + j];
+    MT const gv = G[((int64_t)j * My + y) * Kz + kz];
+    double const wgt = (kz == 0 || 2 * kz == Mz) ? 1. : 2.;
+    double const ek = static_cast<double>(gv) *
+                      (static_cast<double>(rh.x) * rh.x + static_cast<double>(rh.y) * rh.y);
+    double const kx = dop_shift(j, Mx) * wvx;
+    double const kzv = dop_shift(kz, Mz) * wvz;
+    double const norm_sq = kx * kx + ky * ky + kzv * kzv;
+    acc[0] += wgt * ek;
+    if (norm_sq != 0.) {
+      double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
+      acc[1] += pref * kx * kx;
+      acc[2] += pref * kx * ky;
+      acc[3] += pref * kx * kzv;
+      acc[4] += pref * ky * ky;
+      acc[5] += pref * ky * kzv;
+      acc[6] += pref * kzv * kzv;
+      acc[7] += wgt * ek;
+    }
+    if (want_fields) {
+      C const phi = C{gv * rh.x, gv * rh.y};
+      MT const kxf = static_cast<MT>(dop_shift(j, Mx)) * static_cast<MT>(wvx);
+      oth[c * ls + j] = phi;                                   // A
+      oth[(ncol + c) * ls + j] = C{-kxf * phi.y, kxf * phi.x}; // B = i kx phi
+    }
+  }
+  block_sum<8>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; ++k) partials[blockIdx.x * 8 + k] = acc[k];
+  if (!want_fields) return;
+  C *ires = lds_fft(oth, res, 2 * ncol, ls, plan, twl, true);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) {
+      int64_t const gidx = ((int64_t)j * My + y) * Kz + c0 + c;
+      Aout[gidx] = ires[c * ls + j];
+      Bout[gidx] = ires[(ncol + c) * ls + j];
+    }
+  }
+}
+
+/** inverse c2c along y of the x-transformed fields, in place:
+ *  A <- IFFT_y(A) (z field before i k_z), B <- IFFT_y(B) (x field),
+ *  khat <- IFFT_y(i k_y A) (y field). */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict__ sys,
+                                                      int const *__restrict__ sel,
+                                                      int const *__restrict__ wt, int nsel,
+                                                      cx_t<MT> const *__restrict__ tw, int ncol) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  FftPlanDev const plan = d.plan[1];
+  int const n = plan.n; // == My
+  int const ls = n + 1;
+  int const Kz = d.g.Kz;
+  int const My = d.g.M[1];
+  int const ncb = (Kz + ncol - 1) / ncol;
+  int const cb = L.lb % ncb;
+  int64_t const x = L.lb / ncb;
+  C *A = static_cast<C *>(d.A);
+  C *B = static_cast<C *>(d.B);
+  C *Fy = static_cast<C *>(d.khat);
+  double const wvy = d.wv[1];
+  C *twl = reinterpret_cast<C *>(smem_d);
+  C *S = twl + n;
+  C *W0 = S + ncol * ls;
+  C *W1 = W0 + ncol * ls;
+  load_twiddles(twl, tw + d.tw_off[1], n);
+  int const c0 = cb * ncol;
+  int const nc = min(ncol, Kz - c0);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) {
+      C const v = A[(x * My + j) * Kz + c0 + c];
+      S[c * ls + j] = v;
+      W0[c * ls + j] = v;
+    }
+  }
+  __syncthreads();
+  C *res = lds_fft(W0, W1, nc, ls, plan, twl, true);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) A[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * nc; e += blockDim.x) {
+    int const c = e / n, j = e - c * n;
+    MT const kyf = static_cast<MT>(dop_shift(j, My)) * static_cast<MT>(wvy);
+    C const v = S[c * ls + j];
+    W0[c * ls + j] = C{-kyf * v.y, kyf * v.x};
+  }
+  __syncthreads();
+  res = lds_fft(W0, W1, nc, ls, plan, twl, true);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) Fy[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) W0[c * ls + j] = B[(x * My + j) * Kz + c0 + c];
+  }
+  __syncthreads();
+  res = lds_fft(W0, W1, nc, ls, plan, twl, true);
+  for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
+    int const j = e / ncol, c = e - j * ncol;
+    if (c < nc) B[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
+  }
+}
+
+/** inverse c2r along z of the three field components (blockIdx.y: 0 = x
+ *  from B into rho, 1 = y from khat into Ey, 2 = z from A times i k_z into
+ *  Ez); two half-spectrum rows per complex line. */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict__ sys,
+                                                      int const *__restrict__ sel,
+                                                      int const *__restrict__ wt, int nsel,
+                                                      cx_t<MT> const *__restrict__ tw, int nlp) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int const comp = blockIdx.y;
+  FftPlanDev const plan = d.plan[2];
+  int const n = plan.n; // == Mz
+  int const ls = n + 1;
+  int const rows = d.g.M[0] * d.g.M[1];
+  int const Kz = d.g.Kz;
+  C const *S = static_cast<C const *>(comp == 0 ? d.B : comp == 1 ? d.khat : d.A);
+  MT *E = static_cast<MT *>(comp == 0 ? d.rho : comp == 1 ? d.Ey : d.Ez);
+  bool const mul_ikz = (comp == 2);
+  double const wvz = d.wv[2];
+  C *twl = reinterpret_cast<C *>(smem_d);
+  C *b0 = twl + n;
+  C *b1 = b0 + nlp * ls;
+  load_twiddles(twl, tw + d.tw_off[2], n);
+  int const pair0 = L.lb * nlp;
+  for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
+    int const l = e / n, k = e - l * n;
+    int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
+    bool const upper = (k >= Kz);
+    int const ks = upper ? n - k : k;
+    C x1 = (r0 < rows) ? S[r0 * Kz + ks] : C{MT(0), MT(0)};
+    C x2 = (r1 < rows) ? S[r1 * Kz + ks] : C{MT(0), MT(0)};
+    if (mul_ikz) {
+      MT const kz = static_cast<MT>(dop_shift(ks, n)) * static_cast<MT>(wvz);
+      x1 = C{-kz * x1.y, kz * x1.x};
+      x2 = C{-kz * x2.y, kz * x2.x};
+    }
+    if (ks == 0 || 2 * ks == n) {
+      x1.y = MT(0);
+      x2.y = MT(0);
+    }
+    if (upper) {
+      x1 = cconj(x1);
+      x2 = cconj(x2);
+    }
+    b0[l * ls + k] = C{x1.x - x2.y, x1.y + x2.x};
+  }
+  __syncthreads();
+  C *res = lds_fft(b0, b1, nlp, ls, plan, twl, true);
+  for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
+    int const l = e / n, j = e - l * n;
+    int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
+    C const v = res[l * ls + j];
+    if (r0 < rows) E[r0 * n + j] = v.x;
+    if (r1 < rows) E[r1 * n + j] = v.y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// force back-interpolation (AssignForces<cao>, p3m.cpp:344-373): one block
+// per brick with the brick's three field tiles in LDS; F -= q prefactor/V E
+// accumulated in FP64 per particle (plus the dipole term for non-metallic
+// epsilon, p3m.cpp:538-547).
+// ---------------------------------------------------------------------------
+template <int P, typename MT>
+__device__ __forceinline__ void load_field_tiles(SysDesc const &d, int o0, int o1, int o2, MT *tx,
+                                                 MT *ty, MT *tz) {
+  constexpr int T = kBrick + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
+  MT const *Ex = static_cast<MT const *>(d.rho);
+  MT const *Ey = static_cast<MT const *>(d.Ey);
+  MT const *Ez = static_cast<MT const *>(d.Ez);
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
+    int const a = e / TYZ;
+    int const rr = e - a * TYZ;
+    int const bb = rr / T;
+    int const c = rr - bb * T;
+    int64_t const gidx =
+        ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + bb, M1)) * M2 + wrap_once(o2 + c, M2);
+    tx[e] = Ex[gidx];
+    ty[e] = Ey[gidx];
+    tz[e] = Ez[gidx];
+  }
+}
+
+__device__ __forceinline__ void apply_force(double *force, int64_t idx, double qv,
+                                            double force_prefac, double fx, double fy, double fz,
+                                            double const dm[3]) {
+  double const pf = qv * force_prefac;
+  double *f = force + 3 * idx;
+  f[0] = (f[0] - pf * fx) - qv * dm[0];
+  f[1] = (f[1] - pf * fy) - qv * dm[1];
+  f[2] = (f[2] - pf * fz) - qv * dm[2];
+}
+
+/** Plane mapping: one lane segment per particle over the (j, k) plane, loop
+ *  over i; LDS reads of an instruction serve one particle (no inter-particle
+ *  bank conflicts); 64-lane DPP + lane-read reduction.  Used for FP64. */
+template <int P, typename MT>
+__global__ __launch_bounds__(kPG) void k_gather_plane(
+    SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
+    int nsel, int const *__restrict__ offsets, double4 const *__restrict__ spos,
+    int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
+  constexpr int T = kBrick + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + kChunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
+  MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
+  MT *ty = tx + tile_n;
+  MT *tz = ty + tile_n;
+
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int const b = L.lb;
+  int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
+  if (beg == end) return;
+  int o0, o1, o2;
+  brick_origin<P>(d, b, o0, o1, o2);
+  load_field_tiles<P, MT>(d, o0, o1, o2, tx, ty, tz);
+  double dm[3] = {0., 0., 0.};
+  if (d.dip_pref != 0.)
+    for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
+  double4 const *sp = spos + d.part_off;
+  int const *si = sidx + d.part_off;
+  double *fbase = force + 3 * d.part_off;
+  LaneSeg<P> const seg;
+  for (int c0 = beg; c0 < end; c0 += kChunk) {
+    int const cnt = min(kChunk, end - c0);
+    __syncthreads();
+    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
+    __syncthreads();
+    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
+    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
+      int const p = grp * Seg<P>::ppw + seg.sub;
+      bool const valid = seg.active && p < cnt;
+      double fx = 0., fy = 0., fz = 0.;
+      if (valid) {
+        double const *w = s.w + p * 3 * P;
+        double const wyz = w[P + seg.j] * w[2 * P + seg.k];
+        int const base = s.lo[p] + seg.j * T + seg.k;
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+          double const ww = w[i] * wyz;
+          fx += ww * static_cast<double>(tx[base + i * TYZ]);
+          fy += ww * static_cast<double>(ty[base + i * TYZ]);
+          fz += ww * static_cast<double>(tz[base + i * TYZ]);
+        }
+      }
+      fx = segment_total<Seg<P>::S>(fx);
+      fy = segment_total<Seg<P>::S>(fy);
+      fz = segment_total<Seg<P>::S>(fz);
+      if (seg.r == 0 && p < cnt)
+        apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
+    }
+  }
+}
+
+/** Layer mapping: S = next power of two >= P lanes per particle, lane i sweeps
+ *  the (j, k) plane of stencil layer i; cheap 3-step DPP reduction, several
+ *  particles per instruction.  Used for FP32 fields. */
+template <int P> struct GSeg {
+  static constexpr int S = (P <= 1) ? 1 : (P <= 2) ? 2 : (P <= 4) ? 4 : 8;
+  static constexpr int ppw = 64 / S;
+};
+template <int S> __device__ __forceinline__ double small_segment_total(double v) {
+  if constexpr (S >= 2) v += dpp_f64<0xB1>(v);  // quad_perm [1,0,3,2]
+  if constexpr (S >= 4) v += dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+  if constexpr (S >= 8) v += dpp_f64<0x141>(v); // row_half_mirror
+  return v;
+}
+
+template <int P, typename MT>
+__global__ __launch_bounds__(kPG) void k_gather_layer(
+    SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
+    int nsel, int const *__restrict__ offsets, double4 const *__restrict__ spos,
+    int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
+  constexpr int T = kBrick + P - 1;
+  constexpr int TYZ = T * T;
+  constexpr int tile_n = T * TYZ;
+  constexpr int S = GSeg<P>::S;
+  constexpr int ppw = GSeg<P>::ppw;
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + kChunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
+  MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
+  MT *ty = tx + tile_n;
+  MT *tz = ty + tile_n;
+
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int const b = L.lb;
+  int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
+  if (beg == end) return;
+  int o0, o1, o2;
+  brick_origin<P>(d, b, o0, o1, o2);
+  load_field_tiles<P, MT>(d, o0, o1, o2, tx, ty, tz);
+  double dm[3] = {0., 0., 0.};
+  if (d.dip_pref != 0.)
+    for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
+  double4 const *sp = spos + d.part_off;
+  int const *si = sidx + d.part_off;
+  double *fbase = force + 3 * d.part_off;
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const li = lane - sub * S;
+  bool const active = li < P;
+  for (int c0 = beg; c0 < end; c0 += kChunk) {
+    int const cnt = min(kChunk, end - c0);
+    __syncthreads();
+    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
+    __syncthreads();
+    int const ngroups = (cnt + ppw - 1) / ppw;
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+      int const p = grp * ppw + sub;
+      bool const valid = active && p < cnt;
+      double fx = 0., fy = 0., fz = 0.;
+      if (valid) {
+        double const *w = s.w + p * 3 * P;
+        double wz[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) wz[k] = w[2 * P + k];
+        double const wxi = w[li];
+        int const base = s.lo[p] + li * TYZ;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          double const wj = wxi * w[P + j];
+          int const row = base + j * T;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            double const ww = wj * wz[k];
+            fx += ww * static_cast<double>(tx[row + k]);
+            fy += ww * static_cast<double>(ty[row + k]);
+            fz += ww * static_cast<double>(tz[row + k]);
+          }
+        }
+      }
+      fx = small_segment_total<S>(fx);
+      fy = small_segment_total<S>(fy);
+      fz = small_segment_total<S>(fz);
+      if (li == 0 && p < cnt)
+        apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
+    }
+  }
+}
+
+template <typename F> void dispatch_cao(int cao, F &&f) {
+  switch (cao) {
+  case 1: f(std::integral_constant<int, 1>{}); break;
+  case 2: f(std::integral_constant<int, 2>{}); break;
+  case 3: f(std::integral_constant<int, 3>{}); break;
+  case 4: f(std::integral_constant<int, 4>{}); break;
+  case 5: f(std::integral_constant<int, 5>{}); break;
+  case 6: f(std::integral_constant<int, 6>{}); break;
+  case 7: f(std::integral_constant<int, 7>{}); break;
+  default: throw std::runtime_error("cao must be in [1, 7]");
+  }
+}
+
+__global__ void k_fill_int(int *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+template <typename T> void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+constexpr int kLdsBytes = 160 * 1024; // LDS per workgroup on gfx950
+
+/** Raise the dynamic LDS limit of a kernel to what its static LDS leaves. */
+template <typename Kern> void allow_lds(Kern k) {
+  hipFuncAttributes attr{};
+  hip_check(hipFuncGetAttributes(&attr, reinterpret_cast<void const *>(k)),
+            "hipFuncGetAttributes");
+  int const dyn = kLdsBytes - static_cast<int>(attr.sharedSizeBytes);
+  hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn),
+            "hipFuncSetAttribute");
+}
+
+size_t chunk_bytes(int cao) {
+  return kChunk * (3 * cao + 1) * sizeof(double) + kChunk * sizeof(int);
+}
+int tile_points(int cao) {
+  int const t = kBrick + cao - 1;
+  return t * t * t;
+}
+
+/** lines per block such that twiddles + lines_per_unit * units lines fit */
+int pick_lines(int n, size_t csz, int lines_per_unit, int want, size_t budget) {
+  size_t const line = static_cast<size_t>(n + 1) * csz;
+  size_t const twb = static_cast<size_t>(n) * csz;
+  int u = want;
+  while (u > 1 && twb + static_cast<size_t>(lines_per_unit) * u * line > budget) --u;
+  return u;
+}
+} // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct HostSys {
+  P3MParams params;
+  Vec3d box_l{};
+  double skin = 0.;
+  double prefactor = 0.;
+  LocalMesh lm;
+  int64_t n = 0;
+  int64_t part_off = 0;
+  int brick_off = 0;
+  int nbricks = 0;
+  size_t bytes = 0;
+  SysDesc desc{};
+};
+
+/** Work tables of one selection of systems. */
+struct Plan {
+  std::vector<int> sel;
+  int nsel = 0;
+  int *dbuf = nullptr; // device copy of all tables below
+  int *d_sel = nullptr, *wt_bin = nullptr, *wt_fill = nullptr, *wt_z = nullptr, *wt_y = nullptr,
+      *wt_x = nullptr, *wt_yi = nullptr;
+  int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_y = 0, blocks_x = 0, blocks_yi = 0;
+  struct Group {
+    int cao;
+    int nsel;
+    int *d_sel, *wt;
+    int blocks;
+  };
+  std::vector<Group> groups;
+  int nlp_z = 1, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
+  size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_yi = 0;
+  bool any_dipole = false;
+  // graph
+  hipGraphExec_t graph = nullptr;
+  ParticleArrays graph_pa{};
+  unsigned graph_flags = 0;
+  int64_t version = -1;
+};
+
+struct BatchSolver::Impl {
+  std::vector<HostSys> sys;
+  SysDesc *d_sys = nullptr;
+  size_t d_sys_cap = 0;
+  bool descs_dirty = true;
+  int64_t version = 0; // bumped when systems or particle counts change
+  // twiddle tables by (size, precision)
+  std::map<int, int> tw_offset; // size -> offset (complex elements)
+  void *d_tw = nullptr;
+  size_t tw_len = 0;
+  // brick counters / offsets / particle scratch
+  int *counts = nullptr, *offsets = nullptr;
+  int counts_cap = 0, offsets_cap = 0;
+  int64_t cap_n = 0;
+  int *p_brick = nullptr, *p_rank = nullptr, *sidx = nullptr;
+  double4 *spos = nullptr;
+  double *bin_part = nullptr, *conv_part = nullptr;
+  int bin_cap = 0, conv_cap = 0;
+  double *sums = nullptr;
+  int sums_cap = 0;
+  int *err_flag = nullptr;
+  double *h_sums = nullptr;
+  int h_sums_cap = 0;
+  hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> ring;
+  int ring_used = 0;
+  bool profiling = false;
+  Plan plan;
+  bool kernels_ready = false;
+};
+
+BatchSolver::BatchSolver(int device, MeshPrecision precision)
+    : m(new Impl), m_device(device), m_prec(precision) {
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : m->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipMalloc(&m->err_flag, sizeof(int)), "hipMalloc err");
+  m_shape.gather_plane = (precision == MeshPrecision::FP64);
+  m_shape.ncol = 8;
+}
+
+namespace {
+void free_plan(Plan &p) {
+  if (p.graph) (void)hipGraphExecDestroy(p.graph);
+  p.graph = nullptr;
+  dfree(p.dbuf);
+  p = Plan{};
+}
+} // namespace
+
+BatchSolver::~BatchSolver() {
+  (void)hipSetDevice(m_device);
+  if (m_stream) (void)hipStreamSynchronize(m_stream);
+  free_plan(m->plan);
+  for (auto &h : m->sys) {
+    void *ptrs[] = {h.desc.rho, h.desc.Ey, h.desc.Ez, h.desc.khat, h.desc.A, h.desc.B, h.desc.G};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+  dfree(m->d_sys);
+  dfree(m->d_tw);
+  dfree(m->counts);
+  dfree(m->offsets);
+  dfree(m->p_brick);
+  dfree(m->p_rank);
+  dfree(m->sidx);
+  dfree(m->spos);
+  dfree(m->bin_part);
+  dfree(m->conv_part);
+  dfree(m->sums);
+  dfree(m->err_flag);
+  if (m->h_sums) (void)hipHostFree(m->h_sums);
+  for (auto &e : m->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : m->ring) (void)hipEventDestroy(e);
+  if (m_stream) (void)hipStreamDestroy(m_stream);
+  delete m;
+}
+
+int BatchSolver::num_systems() const { return static_cast<int>(m->sys.size()); }
+int64_t BatchSolver::particle_offset(int s) const { return m->sys.at(s).part_off; }
+int64_t BatchSolver::particle_count(int s) const { return m->sys.at(s).n; }
+int64_t BatchSolver::total_particles() const {
+  return m->sys.empty() ? 0 : m->sys.back().part_off + m->sys.back().n;
+}
+size_t BatchSolver::system_bytes(int s) const { return m->sys.at(s).bytes; }
+LocalMesh const &BatchSolver::local_mesh(int s) const { return m->sys.at(s).lm; }
+
+void BatchSolver::set_launch_shape(LaunchShape const &s) {
+  m_shape = s;
+  m_shape.ncol = std::max(1, std::min(64, s.ncol));
+  hip_check(hipStreamSynchronize(m_stream), "sync");
+  free_plan(m->plan);
+}
+
+namespace {
+template <typename MT> void append_twiddles(BatchSolver::Impl *m, int n, hipStream_t st) {
+  using C = cx_t<MT>;
+  if (m->tw_offset.count(n)) return;
+  size_t const old = m->tw_len;
+  std::vector<C> h(old + n);
+  if (old) {
+    hip_check(hipMemcpy(h.data(), m->d_tw, sizeof(C) * old, hipMemcpyDeviceToHost), "copy tw");
+  }
+  for (int k = 0; k < n; ++k) {
+    long double const ang = -2.0L * std::numbers::pi_v<long double> * k / n;
+    h[old + k] = C{static_cast<MT>(std::cos(ang)), static_cast<MT>(std::sin(ang))};
+  }
+  hip_check(hipStreamSynchronize(st), "sync");
+  void *nt = nullptr;
+  hip_check(hipMalloc(&nt, sizeof(C) * h.size()), "hipMalloc twiddles");
+  hip_check(hipMemcpy(nt, h.data(), sizeof(C) * h.size(), hipMemcpyHostToDevice), "copy tw");
+  dfree(m->d_tw);
+  m->d_tw = nt;
+  m->tw_offset[n] = static_cast<int>(old);
+  m->tw_len = h.size();
+}
+
+template <typename MT> void set_kernel_attrs() {
+  allow_lds(k_fft_z_r2c<MT>);
+  allow_lds(k_fft_z_c2r<MT>);
+  allow_lds(k_fft_y<MT>);
+  allow_lds(k_fft_x_conv<MT>);
+  allow_lds(k_fft_y_inv<MT>);
+  for (int cao = 1; cao <= 7; ++cao)
+    dispatch_cao(cao, [&](auto P) {
+      allow_lds(k_spread<P.value, MT>);
+      allow_lds(k_gather_plane<P.value, MT>);
+      allow_lds(k_gather_layer<P.value, MT>);
+    });
+}
+
+void recompute_offsets(BatchSolver::Impl *m) {
+  int64_t off = 0;
+  int boff = 0;
+  for (size_t s = 0; s < m->sys.size(); ++s) {
+    auto &h = m->sys[s];
+    h.part_off = off;
+    off += h.n;
+    h.brick_off = boff;
+    boff += h.nbricks;
+    h.desc.n = static_cast<int>(h.n);
+    h.desc.part_off = h.part_off;
+    h.desc.brick_off = h.brick_off;
+    h.desc.off_off = h.brick_off + static_cast<int>(s);
+    h.desc.nbricks = h.nbricks;
+  }
+  m->descs_dirty = true;
+  m->version++;
+}
+} // namespace
+
+int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, double skin,
+                            double prefactor, int64_t n_particles) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  if (!m->kernels_ready) {
+    if (m_prec == MeshPrecision::FP64)
+      set_kernel_attrs<double>();
+    else
+      set_kernel_attrs<float>();
+    m->kernels_ready = true;
+  }
+  HostSys h;
+  h.params = params_in;
+  h.box_l = box_l;
+  h.skin = skin;
+  h.prefactor = prefactor;
+  h.n = n_particles;
+  h.params.recalc_a_ai_cao_cut(box_l);
+  h.lm.calc_local_ca_mesh(h.params, box_l, skin);
+  for (int d = 0; d < 3; ++d)
+    if (h.lm.inner[d] != h.params.mesh[d])
+      throw std::runtime_error("P3M: charge-assignment mesh does not match the FFT mesh");
+  int const cao = h.params.cao;
+  if (cao < 1 || cao > 7) throw std::runtime_error("cao must be in [1, 7]");
+  SysDesc &d = h.desc;
+  DevGeo &g = d.g;
+  for (int k = 0; k < 3; ++k) {
+    g.M[k] = h.params.mesh[k];
+    g.dim[k] = h.lm.dim[k];
+    g.marg[k] = h.lm.margin[2 * k];
+    g.B[k] = kBrick;
+    g.nb[k] = (g.M[k] + kBrick - 1) / kBrick;
+    g.T[k] = kBrick + cao - 1;
+    g.ld_pos[k] = h.lm.ld_pos[k];
+    g.ai[k] = h.params.ai[k];
+    g.box_l[k] = box_l[k];
+  }
+  g.Kz = g.M[2] / 2 + 1;
+  g.tz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[2]) + 1;
+  g.tyz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[1] * g.T[2]) + 1;
+  g.pos_shift = std::floor((cao - 1) / 2.0) - (cao % 2) / 2.0;
+  d.cao = cao;
+  h.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+  double const two_pi = 2. * std::numbers::pi;
+  double const volume = box_l[0] * box_l[1] * box_l[2];
+  for (int k = 0; k < 3; ++k) d.wv[k] = two_pi * (1. / box_l[k]);
+  d.hais = (1. / 2. / h.params.alpha) * (1. / 2. / h.params.alpha);
+  d.force_prefac = prefactor / volume;
+  d.dip_pref = (h.params.epsilon != P3M_EPSILON_METALLIC)
+                   ? prefactor * (4. * std::numbers::pi / volume / (2. * h.params.epsilon + 1.))
+                   : 0.;
+  for (int k = 0; k < 3; ++k) {
+    FftPlan1D const p = make_fft_plan(g.M[k]);
+    d.plan[k].n = p.n;
+    d.plan[k].nstages = p.nstages;
+    for (int st = 0; st < 16; ++st) d.plan[k].radix[st] = p.radix[st];
+    if (m_prec == MeshPrecision::FP64)
+      append_twiddles<double>(m, g.M[k], m_stream);
+    else
+      append_twiddles<float>(m, g.M[k], m_stream);
+  }
+  size_t const msz = (m_prec == MeshPrecision::FP64) ? 8 : 4;
+  int64_t const R = (int64_t)g.M[0] * g.M[1] * g.M[2];
+  int64_t const K = (int64_t)g.M[0] * g.M[1] * g.Kz;
+  hip_check(hipMalloc(&d.rho, R * msz), "hipMalloc rho");
+  hip_check(hipMalloc(&d.Ey, R * msz), "hipMalloc Ey");
+  hip_check(hipMalloc(&d.Ez, R * msz), "hipMalloc Ez");
+  hip_check(hipMalloc(&d.khat, K * 2 * msz), "hipMalloc khat");
+  hip_check(hipMalloc(&d.A, K * 2 * msz), "hipMalloc A");
+  hip_check(hipMalloc(&d.B, K * 2 * msz), "hipMalloc B");
+  hip_check(hipMalloc(&d.G, K * msz), "hipMalloc G");
+  h.bytes = static_cast<size_t>(3 * R * msz + 3 * K * 2 * msz + K * msz);
+  int const gg = static_cast<int>(std::min<int64_t>((K + 255) / 256, 16384));
+  if (m_prec == MeshPrecision::FP64)
+    k_influence<double><<<gg, 256, 0, m_stream>>>(
+        static_cast<double *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
+        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, cao);
+  else
+    k_influence<float><<<gg, 256, 0, m_stream>>>(
+        static_cast<float *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
+        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, cao);
+  hip_check(hipGetLastError(), "k_influence");
+  hip_check(hipStreamSynchronize(m_stream), "sync influence");
+  m->sys.push_back(h);
+  recompute_offsets(m);
+  // twiddle offsets of all systems (the table may have been reallocated)
+  for (auto &hs : m->sys)
+    for (int k = 0; k < 3; ++k) hs.desc.tw_off[k] = m->tw_offset.at(hs.desc.g.M[k]);
+  free_plan(m->plan);
+  return static_cast<int>(m->sys.size()) - 1;
+}
+
+void BatchSolver::set_particle_count(int s, int64_t n) {
+  if (m->sys.at(s).n == n) return;
+  hip_check(hipStreamSynchronize(m_stream), "sync");
+  m->sys.at(s).n = n;
+  recompute_offsets(m);
+  free_plan(m->plan);
+}
+
+namespace {
+void upload_descs(BatchSolver::Impl *m, hipStream_t st) {
+  if (!m->descs_dirty) return;
+  size_t const need = m->sys.size();
+  if (need > m->d_sys_cap) {
+    dfree(m->d_sys);
+    hip_check(hipMalloc(&m->d_sys, sizeof(SysDesc) * need), "hipMalloc sys");
+    m->d_sys_cap = need;
+  }
+  std::vector<SysDesc> h(need);
+  for (size_t s = 0; s < need; ++s) h[s] = m->sys[s].desc;
+  hip_check(hipMemcpyAsync(m->d_sys, h.data(), sizeof(SysDesc) * need, hipMemcpyHostToDevice, st),
+            "copy sys");
+  hip_check(hipStreamSynchronize(st), "sync sys");
+  m->descs_dirty = false;
+}
+
+void ensure_scratch(BatchSolver::Impl *m, hipStream_t st) {
+  int64_t const ntot = m->sys.empty() ? 0 : m->sys.back().part_off + m->sys.back().n;
+  int nb_need = 0;
+  for (auto const &h : m->sys) nb_need += h.nbricks;
+  bool const grow = ntot > m->cap_n || nb_need > m->counts_cap ||
+                    nb_need + static_cast<int>(m->sys.size()) > m->offsets_cap ||
+                    static_cast<int>(m->sys.size()) * 16 > m->sums_cap;
+  if (grow) hip_check(hipStreamSynchronize(st), "sync before realloc"); // queued work may use them
+  if (ntot > m->cap_n) {
+    dfree(m->p_brick);
+    dfree(m->p_rank);
+    dfree(m->sidx);
+    dfree(m->spos);
+    int64_t const cap = std::max<int64_t>(ntot + ntot / 4, 1024);
+    hip_check(hipMalloc(&m->p_brick, sizeof(int) * cap), "hipMalloc p_brick");
+    hip_check(hipMalloc(&m->p_rank, sizeof(int) * cap), "hipMalloc p_rank");
+    hip_check(hipMalloc(&m->sidx, sizeof(int) * cap), "hipMalloc sidx");
+    hip_check(hipMalloc(&m->spos, sizeof(double4) * cap), "hipMalloc spos");
+    m->cap_n = cap;
+  }
+  int nb = 0;
+  for (auto const &h : m->sys) nb += h.nbricks;
+  if (nb > m->counts_cap) {
+    dfree(m->counts);
+    hip_check(hipMalloc(&m->counts, sizeof(int) * std::max(nb, 1)), "hipMalloc counts");
+    m->counts_cap = nb;
+  }
+  int const no = nb + static_cast<int>(m->sys.size());
+  if (no > m->offsets_cap) {
+    dfree(m->offsets);
+    hip_check(hipMalloc(&m->offsets, sizeof(int) * std::max(no, 1)), "hipMalloc offsets");
+    m->offsets_cap = no;
+  }
+  int const ns = static_cast<int>(m->sys.size()) * 16;
+  if (ns > m->sums_cap) {
+    dfree(m->sums);
+    hip_check(hipMalloc(&m->sums, sizeof(double) * std::max(ns, 16)), "hipMalloc sums");
+    if (m->h_sums) (void)hipHostFree(m->h_sums);
+    hip_check(hipHostMalloc(&m->h_sums, sizeof(double) * (std::max(ns, 16) + 8)),
+              "hipHostMalloc sums");
+    m->sums_cap = ns;
+  }
+}
+
+void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision prec,
+                LaunchShape const &shape, hipStream_t st) {
+  Plan &p = m->plan;
+  if (p.version == m->version && p.sel == sel && p.dbuf) return;
+  hip_check(hipStreamSynchronize(st), "sync before plan rebuild"); // queued work reads the tables
+  free_plan(p);
+  p.sel = sel;
+  p.nsel = static_cast<int>(sel.size());
+  size_t const csz = (prec == MeshPrecision::FP64) ? 16 : 8;
+  int max_mx = 1, max_my = 1, max_mz = 1;
+  for (int s : sel) {
+    auto const &g = m->sys.at(s).desc.g;
+    max_mx = std::max(max_mx, g.M[0]);
+    max_my = std::max(max_my, g.M[1]);
+    max_mz = std::max(max_mz, g.M[2]);
+    if (m->sys[s].desc.dip_pref != 0.) p.any_dipole = true;
+  }
+  size_t const budget = 64 * 1024;
+  p.nlp_z = pick_lines(max_mz, csz, 2, 16, budget);
+  p.ncol_y = pick_lines(max_my, csz, 2, shape.ncol, budget);
+  p.ncol_x = pick_lines(max_mx, csz, 4, shape.ncol, budget);
+  p.ncol_yi = pick_lines(max_my, csz, 3, shape.ncol, budget);
+  auto lds_lines = [csz](int n, int lines) {
+    return static_cast<size_t>(n) * csz + static_cast<size_t>(lines) * (n + 1) * csz;
+  };
+  p.lds_z = lds_lines(max_mz, 2 * p.nlp_z);
+  p.lds_y = lds_lines(max_my, 2 * p.ncol_y);
+  p.lds_x = lds_lines(max_mx, 4 * p.ncol_x);
+  p.lds_yi = lds_lines(max_my, 3 * p.ncol_yi);
+  for (size_t b : {p.lds_z, p.lds_y, p.lds_x, p.lds_yi})
+    if (b > 96 * 1024) throw std::runtime_error("P3M: mesh too large for the LDS budget");
+
+  // tables: sel, then prefix arrays (each nsel+1), then per-cao groups
+  std::vector<int> host;
+  auto push_prefix = [&](auto blocks_of) {
+    int const start = static_cast<int>(host.size());
+    int acc = 0;
+    host.push_back(0);
+    for (int s : sel) {
+      acc += blocks_of(m->sys[s]);
+      host.push_back(acc);
+    }
+    return std::make_pair(start, acc);
+  };
+  host.insert(host.end(), sel.begin(), sel.end());
+  auto const bin = push_prefix([](HostSys const &h) {
+    return static_cast<int>((h.n + kBlock - 1) / kBlock);
+  });
+  auto const fill = push_prefix([](HostSys const &h) {
+    int64_t const R = (int64_t)h.desc.g.M[0] * h.desc.g.M[1] * h.desc.g.M[2];
+    return static_cast<int>((R + kFill - 1) / kFill);
+  });
+  int const nlp = p.nlp_z, ncy = p.ncol_y, ncx = p.ncol_x, ncyi = p.ncol_yi;
+  auto const fz = push_prefix([nlp](HostSys const &h) {
+    int const pairs = (h.desc.g.M[0] * h.desc.g.M[1] + 1) / 2;
+    return (pairs + nlp - 1) / nlp;
+  });
+  auto const fy = push_prefix([ncy](HostSys const &h) {
+    return ((h.desc.g.Kz + ncy - 1) / ncy) * h.desc.g.M[0];
+  });
+  auto const fx = push_prefix([ncx](HostSys const &h) {
+    return ((h.desc.g.Kz + ncx - 1) / ncx) * h.desc.g.M[1];
+  });
+  auto const fyi = push_prefix([ncyi](HostSys const &h) {
+    return ((h.desc.g.Kz + ncyi - 1) / ncyi) * h.desc.g.M[0];
+  });
+  struct GH {
+    int cao, sel_start, nsel, wt_start, blocks;
+  };
+  std::vector<GH> gh;
+  for (int cao = 1; cao <= 7; ++cao) {
+    std::vector<int> gs;
+    for (int s : sel)
+      if (m->sys[s].desc.cao == cao) gs.push_back(s);
+    if (gs.empty()) continue;
+    GH g{cao, static_cast<int>(host.size()), static_cast<int>(gs.size()), 0, 0};
+    host.insert(host.end(), gs.begin(), gs.end());
+    g.wt_start = static_cast<int>(host.size());
+    int acc = 0;
+    host.push_back(0);
+    for (int s : gs) {
+      acc += m->sys[s].nbricks;
+      host.push_back(acc);
+    }
+    g.blocks = acc;
+    gh.push_back(g);
+  }
+  hip_check(hipMalloc(&p.dbuf, sizeof(int) * std::max<size_t>(host.size(), 1)), "hipMalloc plan");
+  hip_check(hipMemcpy(p.dbuf, host.data(), sizeof(int) * host.size(), hipMemcpyHostToDevice),
+            "copy plan");
+  p.d_sel = p.dbuf;
+  p.wt_bin = p.dbuf + bin.first;
+  p.blocks_bin = bin.second;
+  p.wt_fill = p.dbuf + fill.first;
+  p.blocks_fill = fill.second;
+  p.wt_z = p.dbuf + fz.first;
+  p.blocks_z = fz.second;
+  p.wt_y = p.dbuf + fy.first;
+  p.blocks_y = fy.second;
+  p.wt_x = p.dbuf + fx.first;
+  p.blocks_x = fx.second;
+  p.wt_yi = p.dbuf + fyi.first;
+  p.blocks_yi = fyi.second;
+  for (auto const &g : gh)
+    p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks});
+  p.version = m->version;
+  if (p.blocks_bin > m->bin_cap) {
+    dfree(m->bin_part); // the stream is idle (synchronised above)
+    hip_check(hipMalloc(&m->bin_part, sizeof(double) * 8 * std::max(p.blocks_bin, 1)),
+              "hipMalloc bin partials");
+    m->bin_cap = p.blocks_bin;
+  }
+  if (p.blocks_x > m->conv_cap) {
+    dfree(m->conv_part);
+    hip_check(hipMalloc(&m->conv_part, sizeof(double) * 8 * std::max(p.blocks_x, 1)),
+              "hipMalloc conv partials");
+    m->conv_cap = p.blocks_x;
+  }
+}
+
+template <typename MT>
+void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, unsigned flags,
+             LaunchShape const &shape, hipEvent_t *ev, int *p_nmp, bool stop_after_spread) {
+  using C = cx_t<MT>;
+  Plan &p = m->plan;
+  bool const want_force = (flags & 1u) && pa.force;
+  int const ns = p.nsel;
+  C const *tw = static_cast<C const *>(m->d_tw);
+  SysDesc const *sys = m->d_sys;
+  if (ev) (void)hipEventRecord(ev[0], st);
+  k_zero_counts<<<ns, kBlock, 0, st>>>(sys, p.d_sel, ns, m->counts);
+  if (p.blocks_bin > 0) {
+    k_bin<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->counts, m->p_brick,
+                                            m->p_rank, p_nmp, m->bin_part, m->err_flag);
+  }
+  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, m->bin_part, 7, 0, m->sums);
+  k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, m->counts, m->offsets);
+  if (p.blocks_bin > 0)
+    k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->p_brick,
+                                                m->p_rank, m->offsets, m->spos, m->sidx);
+  k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
+  if (ev) (void)hipEventRecord(ev[1], st);
+  for (auto const &g : p.groups) {
+    dispatch_cao(g.cao, [&](auto P) {
+      size_t const lds = chunk_bytes(P.value) + tile_points(P.value) * sizeof(double);
+      k_spread<P.value, MT><<<g.blocks, kPG, lds, st>>>(sys, g.d_sel, g.wt, g.nsel, m->offsets,
+                                                          m->spos);
+    });
+  }
+  if (ev) (void)hipEventRecord(ev[2], st);
+  if (stop_after_spread) return;
+  k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
+  k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);
+  if (ev) (void)hipEventRecord(ev[3], st);
+  k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
+                                                        want_force ? 1 : 0, m->conv_part);
+  if (ev) (void)hipEventRecord(ev[4], st);
+  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, m->conv_part, 8, 8, m->sums);
+  if (want_force) {
+    k_fft_y_inv<MT><<<p.blocks_yi, kBlock, p.lds_yi, st>>>(sys, p.d_sel, p.wt_yi, ns, tw,
+                                                            p.ncol_yi);
+    dim3 const gz(p.blocks_z, 3);
+    k_fft_z_c2r<MT><<<gz, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
+  }
+  if (ev) (void)hipEventRecord(ev[5], st);
+  if (want_force) {
+    for (auto const &g : p.groups) {
+      dispatch_cao(g.cao, [&](auto P) {
+        size_t const lds = chunk_bytes(P.value) + 3 * tile_points(P.value) * sizeof(MT);
+        if (shape.gather_plane)
+          k_gather_plane<P.value, MT><<<g.blocks, kPG, lds, st>>>(
+              sys, g.d_sel, g.wt, g.nsel, m->offsets, m->spos, m->sidx, pa.force, m->sums);
+        else
+          k_gather_layer<P.value, MT><<<g.blocks, kPG, lds, st>>>(
+              sys, g.d_sel, g.wt, g.nsel, m->offsets, m->spos, m->sidx, pa.force, m->sums);
+      });
+    }
+  }
+  if (ev) (void)hipEventRecord(ev[6], st);
+}
+
+void finalize_results(BatchSolver::Impl *m, std::vector<int> const &sel,
+                      std::vector<SolveResult> *out) {
+  out->assign(sel.size(), SolveResult{});
+  for (size_t i = 0; i < sel.size(); ++i) {
+    int const s = sel[i];
+    HostSys const &hs = m->sys[s];
+    double const *h = m->h_sums + 16 * s;
+    SolveResult &r = (*out)[i];
+    r.n_q = h[0];
+    r.sum_q = h[1];
+    r.sum_q2 = h[2];
+    for (int d = 0; d < 3; ++d) r.dipole[d] = h[3 + d];
+    // energy (p3m.cpp:551-589)
+    double const volume = hs.box_l[0] * hs.box_l[1] * hs.box_l[2];
+    double const alpha = hs.params.alpha;
+    double energy = h[8] / (2. * volume);
+    energy -= r.sum_q2 * alpha * std::numbers::inv_sqrtpi;
+    energy -= (r.sum_q * r.sum_q) * std::numbers::pi / (2. * volume * alpha * alpha);
+    if (hs.params.epsilon != P3M_EPSILON_METALLIC) {
+      double const pref = 4. * std::numbers::pi / volume / (2. * hs.params.epsilon + 1.);
+      energy += pref * (r.dipole[0] * r.dipole[0] + r.dipole[1] * r.dipole[1] +
+                        r.dipole[2] * r.dipole[2]);
+    }
+    r.energy = (r.n_q > 0.) ? energy * hs.prefactor : 0.;
+    // virial tensor (p3m.cpp:393-449)
+    double const v[9] = {h[9] + h[15], h[10], h[11], h[10], h[12] + h[15], h[13], h[11], h[13],
+                         h[14] + h[15]};
+    for (int k = 0; k < 9; ++k) r.virial[k] = v[k] * hs.prefactor / (2. * volume);
+  }
+}
+} // namespace
+
+void BatchSolver::solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags,
+                        bool sync, std::vector<SolveResult> *out) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  if (sel.empty()) {
+    if (out) out->clear();
+    return;
+  }
+  for (int s : sel)
+    if (s < 0 || s >= num_systems()) throw std::invalid_argument("P3M: invalid system id");
+  ensure_scratch(m, m_stream);
+  upload_descs(m, m_stream);
+  build_plan(m, sel, m_prec, m_shape, m_stream);
+  hipEvent_t *ev = m->ev;
+  if (m->profiling && static_cast<size_t>(m->ring_used + 1) * 8 <= m->ring.size())
+    ev = &m->ring[8 * static_cast<size_t>(m->ring_used++)];
+  k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+  if (m_prec == MeshPrecision::FP64)
+    enqueue<double>(m, m_stream, pa, flags, m_shape, ev, nullptr, false);
+  else
+    enqueue<float>(m, m_stream, pa, flags, m_shape, ev, nullptr, false);
+  hip_check(hipGetLastError(), "P3M solve launch");
+  if (!sync) return;
+  int const ns = num_systems() * 16;
+  hip_check(hipMemcpyAsync(m->h_sums, m->sums, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                           m_stream),
+            "copy sums");
+  hip_check(hipMemcpyAsync(m->h_sums + ns, m->err_flag, sizeof(int), hipMemcpyDeviceToHost,
+                           m_stream),
+            "copy err");
+  hip_check(hipStreamSynchronize(m_stream), "P3M solve");
+  int err = 0;
+  std::memcpy(&err, m->h_sums + ns, sizeof(int));
+  if (err) throw std::runtime_error("P3M: non-finite particle position");
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, ev[0], ev[6]);
+  last_ms[0] = t;
+  for (int st = 1; st <= 6; ++st) {
+    (void)hipEventElapsedTime(&t, ev[st - 1], ev[st]);
+    last_ms[st] = t;
+  }
+  if (out) finalize_results(m, sel, out);
+}
+
+void BatchSolver::graph_solve(std::vector<int> const &sel, ParticleArrays const &pa,
+                              unsigned flags) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  if (sel.empty()) return;
+  ensure_scratch(m, m_stream);
+  upload_descs(m, m_stream);
+  build_plan(m, sel, m_prec, m_shape, m_stream);
+  Plan &p = m->plan;
+  bool const same = p.graph && p.graph_flags == flags && p.graph_pa.pos == pa.pos &&
+                    p.graph_pa.q == pa.q && p.graph_pa.upos == pa.upos &&
+                    p.graph_pa.force == pa.force;
+  if (!same) {
+    if (p.graph) (void)hipGraphExecDestroy(p.graph);
+    p.graph = nullptr;
+    hipGraph_t graph;
+    hip_check(hipStreamBeginCapture(m_stream, hipStreamCaptureModeRelaxed), "begin capture");
+    k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+    if (m_prec == MeshPrecision::FP64)
+      enqueue<double>(m, m_stream, pa, flags, m_shape, nullptr, nullptr, false);
+    else
+      enqueue<float>(m, m_stream, pa, flags, m_shape, nullptr, nullptr, false);
+    hip_check(hipStreamEndCapture(m_stream, &graph), "end capture");
+    hip_check(hipGraphInstantiate(&p.graph, graph, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(graph);
+    p.graph_pa = pa;
+    p.graph_flags = flags;
+  }
+  hip_check(hipGraphLaunch(p.graph, m_stream), "graph launch");
+}
+
+void BatchSolver::profile_begin(int max_solves) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  hip_check(hipStreamSynchronize(m_stream), "sync");
+  size_t const need = static_cast<size_t>(std::max(max_solves, 1)) * 8;
+  while (m->ring.size() < need) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    m->ring.push_back(e);
+  }
+  m->ring_used = 0;
+  m->profiling = true;
+}
+
+int BatchSolver::profile_end(double *stage_ms) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  hip_check(hipStreamSynchronize(m_stream), "sync");
+  for (int k = 0; k < 7; ++k) stage_ms[k] = 0.;
+  for (int i = 0; i < m->ring_used; ++i) {
+    hipEvent_t *ev = &m->ring[8 * static_cast<size_t>(i)];
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, ev[0], ev[6]);
+    stage_ms[0] += t;
+    for (int s = 1; s <= 6; ++s) {
+      (void)hipEventElapsedTime(&t, ev[s - 1], ev[s]);
+      stage_ms[s] += t;
+    }
+  }
+  m->profiling = false;
+  int const n = m->ring_used;
+  m->ring_used = 0;
+  return n;
+}
+
+void BatchSolver::debug_charge_mesh(int s, ParticleArrays const &pa, std::vector<double> &out) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  ensure_scratch(m, m_stream);
+  upload_descs(m, m_stream);
+  build_plan(m, {s}, m_prec, m_shape, m_stream);
+  k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+  if (m_prec == MeshPrecision::FP64)
+    enqueue<double>(m, m_stream, pa, 0, m_shape, m->ev, nullptr, true);
+  else
+    enqueue<float>(m, m_stream, pa, 0, m_shape, m->ev, nullptr, true);
+  hip_check(hipGetLastError(), "P3M spread launch");
+  auto const &g = m->sys[s].desc.g;
+  int64_t const R = (int64_t)g.M[0] * g.M[1] * g.M[2];
+  out.resize(R);
+  if (m_prec == MeshPrecision::FP64) {
+    hip_check(hipMemcpyAsync(out.data(), m->sys[s].desc.rho, R * sizeof(double),
+                             hipMemcpyDeviceToHost, m_stream),
+              "copy rho");
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+  } else {
+    std::vector<float> tmp(R);
+    hip_check(hipMemcpyAsync(tmp.data(), m->sys[s].desc.rho, R * sizeof(float),
+                             hipMemcpyDeviceToHost, m_stream),
+              "copy rho");
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+    for (int64_t i = 0; i < R; ++i) out[i] = tmp[i];
+  }
+}
+
+void BatchSolver::debug_indices(int s, ParticleArrays const &pa, std::vector<int32_t> &out) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  ensure_scratch(m, m_stream);
+  upload_descs(m, m_stream);
+  build_plan(m, {s}, m_prec, m_shape, m_stream);
+  int64_t const n = m->sys[s].n;
+  int *d_nmp = nullptr;
+  hip_check(hipMalloc(&d_nmp, sizeof(int) * 3 * std::max<int64_t>(n, 1)), "hipMalloc nmp");
+  try {
+    k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+    if (m_prec == MeshPrecision::FP64)
+      enqueue<double>(m, m_stream, pa, 0, m_shape, m->ev, d_nmp, true);
+    else
+      enqueue<float>(m, m_stream, pa, 0, m_shape, m->ev, d_nmp, true);
+    hip_check(hipGetLastError(), "P3M bin launch");
+    out.resize(3 * n);
+    hip_check(hipMemcpyAsync(out.data(), d_nmp, sizeof(int) * 3 * n, hipMemcpyDeviceToHost,
+                             m_stream),
+              "copy nmp");
+    hip_check(hipStreamSynchronize(m_stream), "sync");
+  } catch (...) {
+    (void)hipFree(d_nmp);
+    throw;
+  }
+  (void)hipFree(d_nmp);
+}
+
+} // namespace espresso_amd

@@ -1,0 +1,120 @@
+// Descriptor-driven P3M k-space solver for many systems per launch (MI355X).
+//
+// Every kernel of the pipeline (bin, scan, scatter, spread, r2c/c2c/c2r LDS
+// FFT passes with the fused influence/ik convolution, gather, per-system
+// reductions) runs over a selection of systems in one launch: a work table
+// (prefix sums of blocks per selected system) maps each workgroup to its
+// system, whose mesh, stencil order, FFT plan and buffers come from a
+// per-system descriptor in device memory.  Spread and gather are compiled
+// per charge-assignment order and launched once per order present.
+//
+// A single system is a batch of one: the plain FP64 solver and the batched
+// FP32-mesh solver use the same kernels.
+#pragma once
+
+#include "p3m_host.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace espresso_amd {
+
+enum class MeshPrecision : int { FP64 = 0, FP32 = 1 };
+
+/** Radix plan of one 1D transform length. */
+struct FftPlan1D {
+  static constexpr int max_stages = 16;
+  int n = 0;
+  int nstages = 0;
+  int radix[max_stages] = {};
+};
+FftPlan1D make_fft_plan(int n);
+
+struct SolveResult {
+  double energy = 0.;    // k-space energy incl. self/net/dipole terms * prefactor
+  double virial[9] = {}; // k-space virial tensor (p3m.cpp:393-449 return value)
+  double n_q = 0.;       // number of charged particles
+  double sum_q = 0.;
+  double sum_q2 = 0.;
+  double dipole[3] = {};
+};
+
+/** Device particle arrays of all systems, concatenated in system order
+ *  (positions AoS [n][3], charges [n], forces [n][3]; upos may be null). */
+struct ParticleArrays {
+  double const *pos = nullptr;
+  double const *q = nullptr;
+  double const *upos = nullptr;
+  double *force = nullptr;
+};
+
+/** Launch shapes picked by the autotuner for a class of systems. */
+struct LaunchShape {
+  bool gather_plane = true; // plane-mapped gather (else layer-mapped)
+  int ncol = 8;             // kz columns per block in the y/x FFT passes
+};
+
+class BatchSolver {
+public:
+  BatchSolver(int device, MeshPrecision precision);
+  ~BatchSolver();
+  BatchSolver(BatchSolver const &) = delete;
+  BatchSolver &operator=(BatchSolver const &) = delete;
+
+  /** Add a system with tuned parameters; allocates its resident meshes and
+   *  computes its influence function.  Returns the system id. */
+  int add_system(P3MParams const &params, Vec3d const &box_l, double skin, double prefactor,
+                 int64_t n_particles);
+  int num_systems() const;
+  /** Change the particle count of system s (offsets of later systems move). */
+  void set_particle_count(int s, int64_t n);
+  int64_t particle_offset(int s) const;
+  int64_t particle_count(int s) const;
+  int64_t total_particles() const;
+  /** Bytes of resident device memory held for system s (meshes, G). */
+  size_t system_bytes(int s) const;
+  LocalMesh const &local_mesh(int s) const;
+
+  /** Solve the selected systems.  Forces are accumulated in place into
+   *  pa.force (F -= ...) with P3M_FORCES.  With sync, per-system results
+   *  are returned in `out` (indexed like `sel`). */
+  void solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags, bool sync,
+             std::vector<SolveResult> *out);
+
+  /** Copy the charge mesh of system s after spreading ([Mx][My][Mz]). */
+  void debug_charge_mesh(int s, ParticleArrays const &pa, std::vector<double> &out);
+  /** CA-local first stencil point of every particle of system s. */
+  void debug_indices(int s, ParticleArrays const &pa, std::vector<int32_t> &out);
+
+  /** Per-stage event ring for asynchronous solves (see Solver). */
+  void profile_begin(int max_solves);
+  int profile_end(double *stage_ms);
+  float last_ms[8] = {};
+
+  /** Capture the solve of `sel` (forces, no readback) in a hipGraph and
+   *  replay it; the graph is rebuilt when the selection changes. */
+  void graph_solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags);
+
+  void set_launch_shape(LaunchShape const &s);
+  LaunchShape launch_shape() const { return m_shape; }
+  hipStream_t stream() const { return m_stream; }
+  int device() const { return m_device; }
+  MeshPrecision precision() const { return m_prec; }
+
+  struct Impl;
+
+private:
+  Impl *m;
+  int m_device;
+  MeshPrecision m_prec;
+  hipStream_t m_stream = nullptr;
+  LaunchShape m_shape;
+};
+
+void hip_check(hipError_t err, char const *what);
+
+} // namespace espresso_amd

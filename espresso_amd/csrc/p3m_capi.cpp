@@ -7,6 +7,7 @@
 #include "espresso_amd_p3m.h"
 
 #include "p3m_host.hpp"
+#include "p3m_batch.hpp"
 #include "p3m_solver.hpp"
 #include "p3m_tuning.hpp"
 
@@ -38,6 +39,12 @@ struct p3m_context {
   double *d_pos = nullptr, *d_q = nullptr, *d_upos = nullptr, *d_force = nullptr;
 };
 
+struct p3m_batch {
+  int device = 0;
+  std::unique_ptr<BatchSolver> solver;
+  std::string last_error;
+};
+
 namespace {
 std::mutex g_err_mutex;
 std::string g_last_error;
@@ -47,36 +54,33 @@ void set_global_error(std::string const &msg) {
   g_last_error = msg;
 }
 
-template <typename F> int guarded(p3m_context *ctx, F &&f) {
+template <typename F> int guarded(std::string *err, F &&f) {
+  auto fail = [&](std::string const &msg, int code) {
+    if (err) *err = msg;
+    set_global_error(msg);
+    return code;
+  };
   try {
     f();
     return P3M_OK;
   } catch (std::domain_error const &e) {
-    if (ctx) ctx->last_error = e.what();
-    set_global_error(e.what());
-    return P3M_ERR_DOMAIN;
+    return fail(e.what(), P3M_ERR_DOMAIN);
   } catch (std::invalid_argument const &e) {
-    if (ctx) ctx->last_error = e.what();
-    set_global_error(e.what());
-    return P3M_ERR_INVALID;
+    return fail(e.what(), P3M_ERR_INVALID);
   } catch (std::bad_alloc const &e) {
-    if (ctx) ctx->last_error = e.what();
-    set_global_error(e.what());
-    return P3M_ERR_MEMORY;
+    return fail(e.what(), P3M_ERR_MEMORY);
   } catch (std::runtime_error const &e) {
     std::string const msg = e.what();
-    if (ctx) ctx->last_error = msg;
-    set_global_error(msg);
-    return msg.rfind("HIP error", 0) == 0 ? P3M_ERR_DEVICE : P3M_ERR_RUNTIME;
+    return fail(msg, msg.rfind("HIP error", 0) == 0 ? P3M_ERR_DEVICE : P3M_ERR_RUNTIME);
   } catch (std::exception const &e) {
-    if (ctx) ctx->last_error = e.what();
-    set_global_error(e.what());
-    return P3M_ERR_RUNTIME;
+    return fail(e.what(), P3M_ERR_RUNTIME);
   } catch (...) {
-    if (ctx) ctx->last_error = "unknown error";
-    set_global_error("unknown error");
-    return P3M_ERR_RUNTIME;
+    return fail("unknown error", P3M_ERR_RUNTIME);
   }
+}
+
+template <typename F> int guarded(p3m_context *ctx, F &&f) {
+  return guarded(ctx ? &ctx->last_error : nullptr, std::forward<F>(f));
 }
 
 Vec3d v3(double const *a) { return {a[0], a[1], a[2]}; }
@@ -199,7 +203,7 @@ int p3m_params_default(p3m_params *p) {
 int p3m_create(int device, p3m_context **out) {
   if (!out) return P3M_ERR_INVALID;
   *out = nullptr;
-  return guarded(nullptr, [&] {
+  return guarded(static_cast<std::string *>(nullptr), [&] {
     auto ctx = std::make_unique<p3m_context>();
     ctx->device = device;
     if (device != -1) { // device -1: host-only context (validation, estimates)
@@ -410,7 +414,7 @@ int p3m_debug_indices(p3m_context *ctx, int64_t n, double const *pos, double con
 int p3m_k_space_error(double prefactor, int const mesh[3], int cao, int n_c_part,
                       double sum_q2, double alpha_L, double const box_l[3], double *out) {
   if (!out) return P3M_ERR_INVALID;
-  return guarded(nullptr, [&] {
+  return guarded(static_cast<std::string *>(nullptr), [&] {
     *out = k_space_error(prefactor, v3i(mesh), cao, n_c_part, sum_q2, alpha_L, v3(box_l));
   });
 }
@@ -418,7 +422,7 @@ int p3m_k_space_error(double prefactor, int const mesh[3], int cao, int n_c_part
 int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double sum_q2,
                          double alpha_L, double const box_l[3], double *out) {
   if (!out) return P3M_ERR_INVALID;
-  return guarded(nullptr, [&] {
+  return guarded(static_cast<std::string *>(nullptr), [&] {
     *out = real_space_error(prefactor, r_cut_iL, n_c_part, sum_q2, alpha_L, v3(box_l));
   });
 }
@@ -481,6 +485,230 @@ char const *p3m_last_global_error(void) {
   static thread_local std::string copy;
   copy = g_last_error;
   return copy.c_str();
+}
+
+} // extern "C"
+
+/* ---------------------------------------------------------------- batch */
+
+namespace {
+/** Fixed (already tuned) parameters of one batch system, validated like
+ *  p3m_set_params followed by the post-tuning sanity checks. */
+P3MParams fixed_params(p3m_params const *p) {
+  P3MParams params(false, p->epsilon, p->r_cut, v3i(p->mesh), v3(p->mesh_off), p->cao,
+                   p->alpha, p->accuracy);
+  if (p->prefactor <= 0.) throw std::domain_error("Parameter 'prefactor' must be > 0");
+  for (int d = 0; d < 3; ++d) {
+    if (!(p->box_l[d] > 0.)) throw std::domain_error("box_l must be > 0");
+    if (params.mesh[d] < 1)
+      throw std::domain_error("P3M batch: every system needs a fixed mesh");
+  }
+  auto const box = v3(p->box_l);
+  params.cao3 = params.cao * params.cao * params.cao;
+  params.recalc_a_ai_cao_cut(box);
+  sanity_checks_boxl(params, box);
+  params.alpha_L = params.alpha * box[0];
+  params.r_cut_iL = params.r_cut * (1. / box[0]);
+  return params;
+}
+
+std::vector<int> selection(p3m_batch const *b, int nsel, int const *ids) {
+  if (nsel < 0) throw std::invalid_argument("P3M batch: negative selection size");
+  if (nsel > 0 && !ids) throw std::invalid_argument("P3M batch: null system id list");
+  std::vector<int> sel(ids, ids + nsel);
+  int const ns = b->solver->num_systems();
+  std::vector<char> seen(static_cast<size_t>(ns), 0);
+  for (int s : sel) {
+    if (s < 0 || s >= ns)
+      throw std::invalid_argument("P3M batch: invalid system id " + std::to_string(s));
+    if (seen[s]) throw std::invalid_argument("P3M batch: system selected twice");
+    seen[s] = 1;
+  }
+  return sel;
+}
+} // namespace
+
+extern "C" {
+
+int p3m_batch_create(int device, int precision, p3m_batch **out) {
+  if (!out) return P3M_ERR_INVALID;
+  *out = nullptr;
+  return guarded(static_cast<std::string *>(nullptr), [&] {
+    if (precision != P3M_PREC_FP64 && precision != P3M_PREC_FP32)
+      throw std::invalid_argument("P3M batch: unknown precision");
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    if (device < 0 || device >= ndev)
+      throw std::invalid_argument("P3M: invalid HIP device " + std::to_string(device));
+    auto b = std::make_unique<p3m_batch>();
+    b->device = device;
+    b->solver = std::make_unique<BatchSolver>(
+        device, precision == P3M_PREC_FP32 ? MeshPrecision::FP32 : MeshPrecision::FP64);
+    *out = b.release();
+  });
+}
+
+int p3m_batch_destroy(p3m_batch *b) {
+  if (!b) return P3M_OK;
+  (void)hipSetDevice(b->device);
+  delete b;
+  return P3M_OK;
+}
+
+int p3m_batch_add_system(p3m_batch *b, p3m_params const *p, int64_t n, int *sys_id) {
+  if (!b || !p) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    P3MParams const params = fixed_params(p);
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    int const id = b->solver->add_system(params, v3(p->box_l), p->skin, p->prefactor, n);
+    if (sys_id) *sys_id = id;
+  });
+}
+
+int p3m_batch_particle_offset(p3m_batch const *b, int sys_id, int64_t *offset) {
+  if (!b || !offset || sys_id < 0 || sys_id >= b->solver->num_systems())
+    return P3M_ERR_INVALID;
+  *offset = b->solver->particle_offset(sys_id);
+  return P3M_OK;
+}
+
+int p3m_batch_system_bytes(p3m_batch const *b, int sys_id, int64_t *bytes) {
+  if (!b || !bytes || sys_id < 0 || sys_id >= b->solver->num_systems())
+    return P3M_ERR_INVALID;
+  *bytes = static_cast<int64_t>(b->solver->system_bytes(sys_id));
+  return P3M_OK;
+}
+
+int p3m_batch_solve(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
+                    double const *q, double *force, unsigned flags, double *energy,
+                    double *virial) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    auto const sel = selection(b, nsel, sys_ids);
+    bool const async = (flags & P3M_ASYNC) != 0;
+    if ((flags & P3M_FORCES) && !force && b->solver->total_particles() > 0)
+      throw std::invalid_argument("P3M: force array required");
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    ParticleArrays const pa{pos, q, nullptr, (flags & P3M_FORCES) ? force : nullptr};
+    std::vector<SolveResult> res;
+    b->solver->solve(sel, pa, flags, !async, async ? nullptr : &res);
+    for (int i = 0; i < nsel; ++i) {
+      bool const have = !async && static_cast<size_t>(i) < res.size();
+      if (energy) energy[i] = (have && (flags & P3M_ENERGY)) ? res[i].energy : 0.;
+      if (virial)
+        for (int k = 0; k < 9; ++k)
+          virial[9 * i + k] = (have && (flags & P3M_PRESSURE)) ? res[i].virial[k] : 0.;
+    }
+  });
+}
+
+int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
+                          double const *q, double *force) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    auto const sel = selection(b, nsel, sys_ids);
+    if (!force && b->solver->total_particles() > 0)
+      throw std::invalid_argument("P3M: force array required");
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    b->solver->graph_solve(sel, ParticleArrays{pos, q, nullptr, force}, P3M_FORCES);
+  });
+}
+
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    if (ncol < 1 || ncol > 64 || (ncol & (ncol - 1)) != 0)
+      throw std::invalid_argument("P3M batch: ncol must be a power of two in [1, 64]");
+    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol});
+  });
+}
+
+int p3m_batch_synchronize(p3m_batch *b) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    hip_check(hipStreamSynchronize(b->solver->stream()), "hipStreamSynchronize");
+  });
+}
+
+int p3m_batch_get_stream(p3m_batch const *b, void **stream) {
+  if (!b || !stream) return P3M_ERR_INVALID;
+  *stream = static_cast<void *>(b->solver->stream());
+  return P3M_OK;
+}
+
+int p3m_batch_profile_begin(p3m_batch *b, int max_solves) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] { b->solver->profile_begin(max_solves); });
+}
+
+int p3m_batch_profile_end(p3m_batch *b, double *stage_ms, int *nsolves) {
+  if (!b || !stage_ms) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    int const k = b->solver->profile_end(stage_ms);
+    if (nsolves) *nsolves = k;
+  });
+}
+
+char const *p3m_batch_last_error(p3m_batch const *b) {
+  return b ? b->last_error.c_str() : "null batch";
+}
+
+int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
+                   int mesh_max, int cao_min, int cao_max) {
+  if (!inout) return P3M_ERR_INVALID;
+  return guarded(static_cast<std::string *>(nullptr), [&] {
+    P3MParams params(true, inout->epsilon, inout->r_cut, v3i(inout->mesh), v3(inout->mesh_off),
+                     inout->cao, inout->alpha, inout->accuracy);
+    if (inout->prefactor <= 0.) throw std::domain_error("Parameter 'prefactor' must be > 0");
+    auto const box = v3(inout->box_l);
+    for (int d = 0; d < 3; ++d)
+      if (!(box[d] > 0.)) throw std::domain_error("box_l must be > 0");
+    if (n_charged <= 0)
+      throw std::runtime_error("CoulombP3M: no charged particles in the system");
+    if (mesh_min < 1 || mesh_max < mesh_min || cao_min < 1 || cao_max > 7 || cao_max < cao_min)
+      throw std::invalid_argument("P3M: invalid tuning ranges");
+    TuningInput in;
+    in.box_l = box;
+    in.skin = inout->skin;
+    in.prefactor = inout->prefactor;
+    in.sum_qpart = n_charged;
+    in.sum_q2 = sum_q2;
+    in.timings = 1;
+    in.verbose = inout->verbose != 0;
+    in.mesh_min = mesh_min;
+    in.mesh_max = mesh_max;
+    in.cao_min = cao_min;
+    in.cao_max = cao_max;
+    P3MParams const base = params;
+    TuningTimer timer = [&](Vec3i const &mesh, int cao, double r_cut_iL, double alpha_L,
+                            int) -> double {
+      P3MParams trial = base;
+      trial.mesh = mesh;
+      trial.cao = cao;
+      trial.alpha_L = alpha_L;
+      trial.alpha = alpha_L * (1. / box[0]);
+      trial.r_cut_iL = r_cut_iL;
+      trial.r_cut = r_cut_iL * box[0];
+      trial.recalc_a_ai_cao_cut(box);
+      try {
+        sanity_checks_boxl(trial, box);
+      } catch (std::runtime_error const &) {
+        return -1.;
+      }
+      return model_step_ms(mesh, cao, trial.r_cut, n_charged, box);
+    };
+    tune_p3m(params, in, timer);
+    for (int d = 0; d < 3; ++d) {
+      inout->mesh[d] = params.mesh[d];
+      inout->mesh_off[d] = params.mesh_off[d];
+    }
+    inout->cao = params.cao;
+    inout->alpha = params.alpha;
+    inout->r_cut = params.r_cut;
+    inout->accuracy = params.accuracy;
+    inout->tune = 0;
+  });
 }
 
 } // extern "C"

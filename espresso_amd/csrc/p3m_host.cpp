@@ -154,35 +154,39 @@ double real_space_error(double pref, double r_cut_iL, int n_c_part, double sum_q
 double k_space_error(double pref, Vec3i const &mesh, int cao, int n_c_part, double sum_q2,
                      double alpha_L, Vec3d const &box_l) {
   // p3m.cpp:187-243 with P3M_BRILLOUIN = 0: the aliasing sums reduce to the
-  // single image nm = n (p3m.cpp:164-185).
+  // single image nm = n (p3m.cpp:164-185).  The summand is even in every
+  // component of n, so the sum over n in [-M/2, M/2)^3 runs over one octant
+  // k in [0, M/2]^3 with multiplicity 1 for k = 0 and k = M/2 and 2 otherwise.
   Vec3d const mesh_i{1. / mesh[0], 1. / mesh[1], 1. / mesh[2]};
   double const alpha_L_i = 1. / alpha_L;
   double const factor1 = (std::numbers::pi * alpha_L_i) * (std::numbers::pi * alpha_L_i);
   Vec3i const stop{mesh[0] / 2, mesh[1] / 2, mesh[2] / 2};
-  // per-axis tables of the cotangent sums and sinc factors
-  std::array<std::vector<double>, 3> cs_tab, sinc_tab;
+  std::array<std::vector<double>, 3> cs_tab, sinc_tab, mult;
   for (int d = 0; d < 3; ++d) {
-    for (int n = -stop[d]; n < stop[d]; ++n) {
+    for (int n = 0; n <= stop[d]; ++n) {
       cs_tab[d].push_back(analytic_cotangent_sum(cao, n, mesh_i[d]));
       sinc_tab[d].push_back(sinc(n * mesh_i[d]));
+      mult[d].push_back((n == 0 or n == stop[d]) ? 1. : 2.);
     }
   }
   double he_q = 0.;
-  for (int x = -stop[0]; x < stop[0]; ++x)
-    for (int y = -stop[1]; y < stop[1]; ++y)
-      for (int z = -stop[2]; z < stop[2]; ++z) {
+  for (int x = 0; x <= stop[0]; ++x)
+    for (int y = 0; y <= stop[1]; ++y) {
+      double const w_xy = mult[0][x] * mult[1][y];
+      double const cs_xy = cs_tab[0][x] * cs_tab[1][y];
+      double const f_xy = sinc_tab[0][x] * sinc_tab[1][y];
+      for (int z = 0; z <= stop[2]; ++z) {
         if (x == 0 and y == 0 and z == 0) continue;
         int const n2 = x * x + y * y + z * z;
-        double const cs = cs_tab[0][x + stop[0]] * cs_tab[1][y + stop[1]] * cs_tab[2][z + stop[2]];
+        double const cs = cs_xy * cs_tab[2][z];
         double const ex = std::exp(-factor1 * n2);
-        double const energy = std::pow(
-            sinc_tab[0][x + stop[0]] * sinc_tab[1][y + stop[1]] * sinc_tab[2][z + stop[2]],
-            2 * cao);
+        double const energy = std::pow(f_xy * sinc_tab[2][z], 2 * cao);
         double const alias1 = ex * ex / n2;
         double const alias2 = energy * ex * n2 / n2;
         double const d = alias1 - (alias2 / cs) * (alias2 / cs) / n2;
-        if (d > 0. and std::fabs(d / alias1) > ROUND_ERROR_PREC) he_q += d;
+        if (d > 0. and std::fabs(d / alias1) > ROUND_ERROR_PREC) he_q += w_xy * mult[2][z] * d;
       }
+    }
   return 2. * pref * sum_q2 * std::sqrt(he_q / static_cast<double>(n_c_part)) /
          (box_l[1] * box_l[2]);
 }

@@ -182,9 +182,8 @@ template <int S> __device__ __forceinline__ double segment_total(double v) {
  *  Positions whose stencil would leave the CA mesh (further outside the box
  *  than the Verlet skin, which the reference never sees) are folded into
  *  the box first.  Returns false for non-finite positions. */
-template <int P>
-__device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double const x[3],
-                                                     int nmp[3], int gi[3], double dist[3]) {
+__device__ __forceinline__ bool particle_coordinates_rt(DevGeo const &g, int P, double const x[3],
+                                                        int nmp[3], int gi[3], double dist[3]) {
   bool ok = true;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -207,6 +206,11 @@ __device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double con
     gi[d] = gg;
   }
   return ok;
+}
+template <int P>
+__device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double const x[3],
+                                                     int nmp[3], int gi[3], double dist[3]) {
+  return particle_coordinates_rt(g, P, x, nmp, gi, dist);
 }
 
 /** i k_d for FFT index j of a dimension of size M with the Nyquist entry set

@@ -43,6 +43,11 @@ public:
           std::min(max_npart_per_dim, std::cbrt(static_cast<double>(m_in.sum_qpart)));
       m_density_min = min_npart_per_dim / normalized_box_dim;
       m_density_max = max_npart_per_dim / normalized_box_dim;
+      // optional mesh range (densities of the smallest box dimension)
+      double const lmin = std::min({L[0], L[1], L[2]});
+      double const lmax = std::max({L[0], L[1], L[2]});
+      m_density_min = std::max(m_density_min, (m_in.mesh_min - 0.5) / lmin);
+      m_density_max = std::min(m_density_max, (m_in.mesh_max + 0.5) / lmax);
       m_tune_mesh = true;
     } else {
       double const mesh_density = static_cast<double>(m_p.mesh[0]) * (1. / L[0]);
@@ -73,9 +78,9 @@ public:
   void determine_cao_limits(int initial_cao) {
     // TuningAlgorithm.cpp:76-86
     if (m_p.cao == -1) {
-      cao_min = 1;
-      cao_max = 7;
-      cao_best = initial_cao;
+      cao_min = std::max(1, m_in.cao_min);
+      cao_max = std::min(7, m_in.cao_max);
+      cao_best = std::clamp(initial_cao, cao_min, cao_max);
     } else {
       cao_min = cao_max = cao_best = m_p.cao;
     }
@@ -282,6 +287,15 @@ private:
   std::size_t m_n_trials = 0;
 };
 } // namespace
+
+double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l) {
+  // device k-space solve: stencil work per charge and FFT work per mesh
+  // point, with unit costs measured for the FP32-mesh kernels on MI355X, plus
+  // the modelled real-space loop
+  double const R = static_cast<double>(mesh[0]) * mesh[1] * mesh[2];
+  double const stencil = static_cast<double>(n_q) * cao * cao * cao;
+  return 1.3e-9 * stencil + 1.2e-7 * R + 0.01 + model_real_space_ms(n_q, r_cut, box_l);
+}
 
 double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l) {
   // pairs within r_cut for a uniform system, at 1 ns per pair (a tuned CPU

@@ -24,7 +24,14 @@ struct TuningInput {
   int timings = 10;
   bool verbose = false;
   std::string name = "CoulombP3M";
+  // optional restriction of the explored meshes and orders
+  int mesh_min = 1, mesh_max = 1 << 20;
+  int cao_min = 1, cao_max = 7;
 };
+
+/** Modelled cost (ms) of one k-space solve on the device plus the real-space
+ *  pair loop, for tuning without timing runs. */
+double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l);
 
 /** Returns the cost of one step in ms for a candidate (mesh, cao, r_cut_iL,
  *  alpha_L), or a negative value when the candidate cannot be timed. */

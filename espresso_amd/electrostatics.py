@@ -91,6 +91,13 @@ class _Arrays:
     def flags(self):
         return _lib.P3M_DEVICE_POINTERS if self.device else 0
 
+    def order_after_torch(self, ctx):
+        """Device tensors: the solver stream waits for torch's stream."""
+        if self.device:
+            s = C.c_void_p()
+            _lib.raise_for(_lib.load().p3m_get_stream(ctx, C.byref(s)), ctx)
+            _lib.wait_for_torch(s.value)
+
 
 def _is_device_tensor(x):
     try:
@@ -220,6 +227,7 @@ class _P3MBase:
             _lib.raise_for(L.p3m_set_params(self._ctx, C.byref(self._cp)),
                            self._ctx)
         a = _Arrays(pos, q)
+        a.order_after_torch(self._ctx)
         _lib.raise_for(L.p3m_activate(self._ctx, a.n, a.pos, a.q, a.flags),
                        self._ctx)
         self._activated = True
@@ -239,6 +247,7 @@ class _P3MBase:
         the solver's stream (call ``synchronize()`` before reading)."""
         self._require_active()
         a = _Arrays(pos, q, force, pos_unfolded)
+        a.order_after_torch(self._ctx)
         flags = _lib.P3M_FORCES | a.flags
         if not sync:
             if not a.device:
@@ -270,6 +279,7 @@ class _P3MBase:
         """k-space energy incl. self/net-charge/dipole terms (p3m.cpp:551-589)."""
         self._require_active()
         a = _Arrays(pos, q, None, pos_unfolded)
+        a.order_after_torch(self._ctx)
         e = C.c_double(0.)
         _lib.raise_for(_lib.load().p3m_solve(
             self._ctx, a.n, a.pos, a.q, a.upos, None,
@@ -281,6 +291,7 @@ class _P3MBase:
         (p3m.cpp:393-449), i.e. before the division by the volume."""
         self._require_active()
         a = _Arrays(pos, q)
+        a.order_after_torch(self._ctx)
         v = (C.c_double * 9)()
         _lib.raise_for(_lib.load().p3m_solve(
             self._ctx, a.n, a.pos, a.q, None, None,
@@ -292,6 +303,7 @@ class _P3MBase:
         """Forces, energy and virial from one charge assignment and FFT."""
         self._require_active()
         a = _Arrays(pos, q, force, pos_unfolded)
+        a.order_after_torch(self._ctx)
         flags = a.flags
         if force is not None:
             flags |= _lib.P3M_FORCES
@@ -323,6 +335,7 @@ class _P3MBase:
     def charge_mesh(self, pos, q):
         self._require_active()
         a = _Arrays(pos, q)
+        a.order_after_torch(self._ctx)
         if a.device:
             raise TypeError("charge_mesh takes host arrays")
         m = self.get_params()["mesh"]
@@ -334,6 +347,7 @@ class _P3MBase:
     def mesh_indices(self, pos, q):
         self._require_active()
         a = _Arrays(pos, q)
+        a.order_after_torch(self._ctx)
         out = np.zeros((a.n, 3), np.int32)
         _lib.raise_for(_lib.load().p3m_debug_indices(
             self._ctx, a.n, a.pos, a.q, out.ctypes.data), self._ctx)

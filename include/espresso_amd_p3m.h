@@ -163,6 +163,47 @@ int p3m_get_stream(const p3m_context *ctx, void **stream);
 const char *p3m_last_error(const p3m_context *ctx);
 const char *p3m_last_global_error(void);
 
+/* ------------------------------------------------------------------------
+ * Batched solver: many independent systems per launch.  No counterpart in
+ * the reference, which solves one system per process; it is the same
+ * k-space algorithm as p3m_solve applied to every selected system.  All
+ * particle arrays are device pointers to the systems' particles concatenated
+ * in the order the systems were added (offsets from
+ * p3m_batch_particle_offset).
+ * ------------------------------------------------------------------------ */
+typedef struct p3m_batch p3m_batch;
+
+int p3m_batch_create(int device, int precision, p3m_batch **out);
+int p3m_batch_destroy(p3m_batch *b);
+/* Add a system with fixed (tuned) parameters and n particles; validation and
+ * sanity checks as in p3m_set_params / activation. */
+int p3m_batch_add_system(p3m_batch *b, const p3m_params *p, int64_t n, int *sys_id);
+int p3m_batch_particle_offset(const p3m_batch *b, int sys_id, int64_t *offset);
+int p3m_batch_system_bytes(const p3m_batch *b, int sys_id, int64_t *bytes);
+/* Solve the selected systems: flags P3M_FORCES / P3M_ENERGY / P3M_PRESSURE,
+ * P3M_ASYNC (enqueue only).  energy[nsel] and virial[9*nsel] may be NULL. */
+int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
+                    const double *q, double *force, unsigned flags, double *energy,
+                    double *virial);
+/* Force solve of the selection replayed from a captured hipGraph (captured on
+ * first use, re-captured when the selection or arrays change); enqueue only. */
+int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
+                          const double *q, double *force);
+/* Launch shape: gather mapping (1 = plane, 0 = layer) and FFT columns/block. */
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol);
+int p3m_batch_synchronize(p3m_batch *b);
+int p3m_batch_get_stream(const p3m_batch *b, void **stream);
+int p3m_batch_profile_begin(p3m_batch *b, int max_solves);
+int p3m_batch_profile_end(p3m_batch *b, double *stage_ms, int *nsolves);
+const char *p3m_batch_last_error(const p3m_batch *b);
+
+/* Parameter tuning without a device: the reference's tuning algorithm
+ * (p3m.cpp:594-791, TuningAlgorithm.cpp:57-328) with a modelled cost in place
+ * of timed integration steps, meshes restricted to [mesh_min, mesh_max] and
+ * orders to [cao_min, cao_max].  Fills mesh, cao, r_cut, alpha, accuracy. */
+int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
+                   int mesh_max, int cao_min, int cao_max);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,3 +135,32 @@ def test_pair_terms_match_oracle():
     assert abs(p.pair_energy(q[0] * q[1], dist) - e_ref) <= 1e-14 * abs(e_ref)
     assert p.pair_energy(1., 3.5) == 0.
     assert np.all(p.pair_force(1., [0., 0., 3.5], 3.5) == 0.)
+
+
+def test_tune_model_reaches_accuracy_within_ranges():
+    # reference tuning algorithm (p3m.cpp:594-791) with a cost model: the
+    # result must satisfy the requested accuracy by the oracle's estimate
+    from espresso_amd import tune_model
+    rng = np.random.default_rng(3)
+    n = 2000
+    q = np.where(rng.random(n) < 0.5, 1., -1.)
+    box = [15., 15., 15.]
+    for acc, mr, cr in ((1e-4, (16, 48), (3, 7)), (1e-3, (8, 24), (2, 5))):
+        t = tune_model(box, n, float(np.sum(q * q)), 1.0, acc,
+                       mesh_range=mr, cao_range=cr)
+        assert mr[0] <= t["mesh"][0] <= mr[1] and cr[0] <= t["cao"] <= cr[1]
+        assert t["mesh"][0] == t["mesh"][1] == t["mesh"][2]
+        ref = O.calculate_accuracy(1.0, t["mesh"], t["cao"], t["r_cut"] / box[0],
+                                   n, float(np.sum(q * q)), acc, box)
+        assert ref[0] <= acc
+        assert abs(ref[0] - t["accuracy"]) <= 1e-9 * ref[0]
+        assert abs(ref[3] - t["alpha"] * box[0]) <= 1e-9 * ref[3]
+
+
+def test_tune_model_failure_message():
+    from espresso_amd import tune_model
+    with pytest.raises(RuntimeError, match="failed to reach requested accuracy"):
+        tune_model([10.] * 3, 100, 100., 1.0, 1e-9, mesh_range=(4, 8),
+                   cao_range=(1, 2))
+    with pytest.raises(RuntimeError, match="no charged particles"):
+        tune_model([10.] * 3, 0, 0., 1.0, 1e-3)
