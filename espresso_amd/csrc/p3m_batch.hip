@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   extern __shared__ double smem_d[];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  FftPlanDev const plan = d.plan[2];
+  FftPlanDev const &plan = d.plan[2];
   int const n = plan.n;
   int const ls = n + 1;
   int const rows = d.g.M[0] * d.g.M[1];
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SysDesc const *__restrict__ sy
   extern __shared__ double smem_d[];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  FftPlanDev const plan = d.plan[1];
+  FftPlanDev const &plan = d.plan[1];
   int const n = plan.n;
   int const ls = n + 1;
   int const Kz = d.g.Kz;
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   __shared__ double red[8 * 16];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  FftPlanDev const plan = d.plan[0];
+  FftPlanDev const &plan = d.plan[0];
   int const n = plan.n; // == Mx
   int const ls = n + 1;
   int const Kz = d.g.Kz;
@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   extern __shared__ double smem_d[];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  FftPlanDev const plan = d.plan[1];
+  FftPlanDev const &plan = d.plan[1];
   int const n = plan.n; // == My
   int const ls = n + 1;
   int const Kz = d.g.Kz;
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int const comp = blockIdx.y;
-  FftPlanDev const plan = d.plan[2];
+  FftPlanDev const &plan = d.plan[2];
   int const n = plan.n; // == Mz
   int const ls = n + 1;
   int const rows = d.g.M[0] * d.g.M[1];
@@ -813,8 +813,9 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
 }
 
 /** Layer mapping: S = next power of two >= P lanes per particle, lane i sweeps
- *  the (j, k) plane of stencil layer i; cheap 3-step DPP reduction, several
- *  particles per instruction.  Used for FP32 fields. */
+ *  the (j, k) plane of stencil layer i in the mesh precision; the lanes of a
+ *  particle are summed in FP64 with a 3-step DPP reduction.  Several particles
+ *  per instruction.  Used for FP32 fields. */
 template <int P> struct GSeg {
   static constexpr int S = (P <= 1) ? 1 : (P <= 2) ? 2 : (P <= 4) ? 4 : 8;
   static constexpr int ppw = 64 / S;
@@ -874,30 +875,32 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
     for (int grp = wave; grp < ngroups; grp += nwaves) {
       int const p = grp * ppw + sub;
       bool const valid = active && p < cnt;
-      double fx = 0., fy = 0., fz = 0.;
+      // per-lane stencil layer in the mesh precision MT; the cross-lane sum
+      // and the force update are FP64
+      MT ax = MT(0), ay = MT(0), az = MT(0);
       if (valid) {
         double const *w = s.w + p * 3 * P;
-        double wz[P];
+        MT wz[P];
 #pragma unroll
-        for (int k = 0; k < P; ++k) wz[k] = w[2 * P + k];
+        for (int k = 0; k < P; ++k) wz[k] = static_cast<MT>(w[2 * P + k]);
         double const wxi = w[li];
         int const base = s.lo[p] + li * TYZ;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          double const wj = wxi * w[P + j];
+          MT const wj = static_cast<MT>(wxi * w[P + j]);
           int const row = base + j * T;
 #pragma unroll
           for (int k = 0; k < P; ++k) {
-            double const ww = wj * wz[k];
-            fx += ww * static_cast<double>(tx[row + k]);
-            fy += ww * static_cast<double>(ty[row + k]);
-            fz += ww * static_cast<double>(tz[row + k]);
+            MT const ww = wj * wz[k];
+            ax += ww * tx[row + k];
+            ay += ww * ty[row + k];
+            az += ww * tz[row + k];
           }
         }
       }
-      fx = small_segment_total<S>(fx);
-      fy = small_segment_total<S>(fy);
-      fz = small_segment_total<S>(fz);
+      double const fx = small_segment_total<S>(static_cast<double>(ax));
+      double const fy = small_segment_total<S>(static_cast<double>(ay));
+      double const fz = small_segment_total<S>(static_cast<double>(az));
       if (li == 0 && p < cnt)
         apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
     }

@@ -355,14 +355,18 @@ __device__ __forceinline__ void stockham_bfly(C const *src, C *dst, int j, int n
  *  l * ls).  Returns the buffer holding the result (buf0 or buf1).  Must be
  *  called by the whole block; ends with a barrier. */
 template <typename C>
-__device__ C *lds_fft(C *buf0, C *buf1, int nlines, int ls, FftPlanDev const &plan, C const *tw,
+__device__ __forceinline__ C *lds_fft(C *buf0, C *buf1, int nlines, int ls, FftPlanDev const &plan,
+                                      C const *tw,
                       bool inv) {
-  int const n = plan.n;
+  // plan fields are block-uniform: keep them in scalar registers so that the
+  // radix dispatch below is a scalar branch
+  int const n = __builtin_amdgcn_readfirstlane(plan.n);
+  int const nstages = __builtin_amdgcn_readfirstlane(plan.nstages);
   C *in = buf0;
   C *out = buf1;
   int Ns = 1;
-  for (int s = 0; s < plan.nstages; ++s) {
-    int const R = plan.radix[s];
+  for (int s = 0; s < nstages; ++s) {
+    int const R = __builtin_amdgcn_readfirstlane(plan.radix[s]);
     if (R <= 8) {
       int const nb = n / R;
       int const items = nlines * nb;
