@@ -30,7 +30,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_c2_fp64.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "hbm_traffic_c2_n1e6_mesh128_cao7.json")
+# kernel behind each single-kernel stage of the FP64 solve at cao 7
+STAGE_KERNEL = {"spread": "k_spread<7, double>",
+                "convolve": "k_fft_x_conv<double>",
+                "gather": "k_gather_plane<7, double>"}
 
 
 def make_system(n=1_000_000, seed=1002):
@@ -143,10 +148,11 @@ def main():
     t_dom = single[dom] * 1e-3
     achieved = ab[dom] / t_dom / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if os.path.exists(TRAFFIC_FILE) and args.n == 1_000_000:
         with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh).get(dom)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
+            traffic = json.load(fh)["bytes_per_dispatch"].get(STAGE_KERNEL[dom])
+    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom],
+                "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": ab[dom],
