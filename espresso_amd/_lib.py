@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = (
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
     "p3m_batch_particle_offset", "p3m_batch_system_bytes", "p3m_batch_solve",
-    "p3m_batch_graph_solve", "p3m_batch_set_launch_shape",
+    "p3m_batch_graph_solve", "p3m_batch_set_launch_shape", "p3m_batch_autotune",
     "p3m_batch_synchronize", "p3m_batch_get_stream", "p3m_batch_profile_begin",
     "p3m_batch_profile_end", "p3m_batch_last_error", "p3m_tune_model",
 )
@@ -127,7 +127,9 @@ def load():
     L.p3m_batch_system_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
-    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int]
+    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, ip,
+                                     ip, ip, dp, dp]
     L.p3m_batch_synchronize.argtypes = [vp]
     L.p3m_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_batch_profile_begin.argtypes = [vp, C.c_int]
@@ -135,7 +137,8 @@ def load():
     L.p3m_batch_last_error.argtypes = [vp]
     L.p3m_batch_last_error.restype = C.c_char_p
     L.p3m_tune_model.argtypes = [C.POINTER(p3m_params), C.c_int, C.c_double,
-                                 C.c_int, C.c_int, C.c_int, C.c_int]
+                                 C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_double]
     for name in EXPORTED_SYMBOLS:
         if name not in _STRING_RESULT:
             getattr(L, name).restype = C.c_int

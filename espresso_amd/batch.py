@@ -156,9 +156,25 @@ class P3MBatch:
         self._check(_lib.load().p3m_batch_graph_solve(self._b, nsel, ids, p,
                                                       qq, f))
 
-    def set_launch_shape(self, gather_plane=True, ncol=8):
+    def set_launch_shape(self, gather_plane=True, ncol=8, zlines=16):
         self._check(_lib.load().p3m_batch_set_launch_shape(
-            self._b, 1 if gather_plane else 0, int(ncol)))
+            self._b, 1 if gather_plane else 0, int(ncol), int(zlines)))
+
+    def autotune(self, pos, q, scratch_force, systems=None, reps=3):
+        """Pick the fastest launch shape (gather mapping, FFT columns, z
+        lines) for force solves of the selection by timing every candidate;
+        forces accumulate into ``scratch_force``.  Returns the chosen shape
+        and the mean solve times before and after."""
+        p, qq, f = self._arrays(pos, q, scratch_force)
+        nsel, ids = self._selection(systems)
+        gp, nc, zl = C.c_int(), C.c_int(), C.c_int()
+        best, default = C.c_double(), C.c_double()
+        self._check(_lib.load().p3m_batch_autotune(
+            self._b, nsel, ids, p, qq, f, int(reps), C.byref(gp), C.byref(nc),
+            C.byref(zl), C.byref(best), C.byref(default)))
+        return {"gather_plane": bool(gp.value), "ncol": nc.value,
+                "zlines": zl.value, "best_ms": best.value,
+                "default_ms": default.value}
 
     def synchronize(self):
         self._check(_lib.load().p3m_batch_synchronize(self._b))
@@ -194,11 +210,12 @@ class P3MBatch:
 
 def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
                mesh_range=(1, 1 << 20), cao_range=(1, 7), skin=0.4,
-               epsilon=0., verbose=False):
+               epsilon=0., verbose=False, pair_ns=1.0):
     """Tuned parameters of one system from the reference's tuning algorithm
-    with a modelled cost per trial.  Returns a dict with mesh, cao, alpha,
-    r_cut and the estimated accuracy; raises RuntimeError("CoulombP3M: failed
-    to reach requested accuracy") like the reference."""
+    with a modelled cost per trial (device k-space stage rates plus a
+    real-space loop at ``pair_ns`` ns per pair).  Returns a dict with mesh,
+    cao, alpha, r_cut and the estimated accuracy; raises RuntimeError
+    ("CoulombP3M: failed to reach requested accuracy") like the reference."""
     L = _lib.load()
     cp = _lib.p3m_params()
     L.p3m_params_default(C.byref(cp))
@@ -211,6 +228,7 @@ def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
     cp.verbose = 1 if verbose else 0
     _lib.raise_for(L.p3m_tune_model(C.byref(cp), int(n_charged), float(sum_q2),
                                     int(mesh_range[0]), int(mesh_range[1]),
-                                    int(cao_range[0]), int(cao_range[1])))
+                                    int(cao_range[0]), int(cao_range[1]),
+                                    float(pair_ns)))
     return {"mesh": [cp.mesh[d] for d in range(3)], "cao": cp.cao,
             "alpha": cp.alpha, "r_cut": cp.r_cut, "accuracy": cp.accuracy}

@@ -1093,8 +1093,53 @@ LocalMesh const &BatchSolver::local_mesh(int s) const { return m->sys.at(s).lm; 
 void BatchSolver::set_launch_shape(LaunchShape const &s) {
   m_shape = s;
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
+  m_shape.zlines = std::max(1, std::min(64, s.zlines));
   hip_check(hipStreamSynchronize(m_stream), "sync");
   free_plan(m->plan);
+}
+
+AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays const &pa,
+                                     int reps) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  reps = std::max(reps, 1);
+  hipEvent_t t0, t1;
+  hip_check(hipEventCreate(&t0), "hipEventCreate");
+  hip_check(hipEventCreate(&t1), "hipEventCreate");
+  auto time_shape = [&](LaunchShape const &shape) {
+    set_launch_shape(shape);
+    solve(sel, pa, 1u, false, nullptr); // builds the plan, warms caches
+    hip_check(hipEventRecord(t0, m_stream), "hipEventRecord");
+    for (int r = 0; r < reps; ++r) solve(sel, pa, 1u, false, nullptr);
+    hip_check(hipEventRecord(t1, m_stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+    return static_cast<double>(ms) / reps;
+  };
+  AutotuneResult res;
+  LaunchShape const start = m_shape;
+  res.default_ms = time_shape(start);
+  res.best = start;
+  res.best_ms = res.default_ms;
+  res.candidates = 1;
+  for (bool plane : {true, false})
+    for (int ncol : {2, 4, 8, 16})
+      for (int zl : {4, 8, 16}) {
+        LaunchShape const c{plane, ncol, zl};
+        if (c.gather_plane == start.gather_plane && c.ncol == start.ncol &&
+            c.zlines == start.zlines)
+          continue;
+        double const ms = time_shape(c);
+        ++res.candidates;
+        if (ms < res.best_ms) {
+          res.best_ms = ms;
+          res.best = c;
+        }
+      }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  set_launch_shape(res.best);
+  return res;
 }
 
 namespace {
@@ -1332,7 +1377,7 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
     if (m->sys[s].desc.dip_pref != 0.) p.any_dipole = true;
   }
   size_t const budget = 64 * 1024;
-  p.nlp_z = pick_lines(max_mz, csz, 2, 16, budget);
+  p.nlp_z = pick_lines(max_mz, csz, 2, shape.zlines, budget);
   p.ncol_y = pick_lines(max_my, csz, 2, shape.ncol, budget);
   p.ncol_x = pick_lines(max_mx, csz, 4, shape.ncol, budget);
   p.ncol_yi = pick_lines(max_my, csz, 3, shape.ncol, budget);

@@ -56,6 +56,14 @@ struct ParticleArrays {
 struct LaunchShape {
   bool gather_plane = true; // plane-mapped gather (else layer-mapped)
   int ncol = 8;             // kz columns per block in the y/x FFT passes
+  int zlines = 16;          // pairs of real z lines per block in the z passes
+};
+
+struct AutotuneResult {
+  LaunchShape best;
+  double best_ms = 0.;     // mean force-solve time of the best shape
+  double default_ms = 0.;  // same for the shape in use before tuning
+  int candidates = 0;
 };
 
 class BatchSolver {
@@ -100,6 +108,10 @@ public:
   void graph_solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags);
 
   void set_launch_shape(LaunchShape const &s);
+  /** Time force solves of `sel` for every candidate launch shape (gather
+   *  mapping x FFT columns x z lines), `reps` solves each, and keep the
+   *  fastest.  Forces are accumulated into pa.force on every solve. */
+  AutotuneResult autotune(std::vector<int> const &sel, ParticleArrays const &pa, int reps);
   LaunchShape launch_shape() const { return m_shape; }
   hipStream_t stream() const { return m_stream; }
   int device() const { return m_device; }

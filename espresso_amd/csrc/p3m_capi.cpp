@@ -615,12 +615,34 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double con
   });
 }
 
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol) {
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
-    if (ncol < 1 || ncol > 64 || (ncol & (ncol - 1)) != 0)
-      throw std::invalid_argument("P3M batch: ncol must be a power of two in [1, 64]");
-    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol});
+    for (int v : {ncol, zlines})
+      if (v < 1 || v > 64 || (v & (v - 1)) != 0)
+        throw std::invalid_argument(
+            "P3M batch: ncol and zlines must be powers of two in [1, 64]");
+    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol, zlines});
+  });
+}
+
+int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
+                       double const *q, double *force, int reps, int *gather_plane,
+                       int *ncol, int *zlines, double *best_ms, double *default_ms) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    auto const sel = selection(b, nsel, sys_ids);
+    if (!force && b->solver->total_particles() > 0)
+      throw std::invalid_argument("P3M: force array required");
+    if (sel.empty()) throw std::invalid_argument("P3M batch: empty selection");
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    AutotuneResult const r =
+        b->solver->autotune(sel, ParticleArrays{pos, q, nullptr, force}, reps);
+    if (gather_plane) *gather_plane = r.best.gather_plane ? 1 : 0;
+    if (ncol) *ncol = r.best.ncol;
+    if (zlines) *zlines = r.best.zlines;
+    if (best_ms) *best_ms = r.best_ms;
+    if (default_ms) *default_ms = r.default_ms;
   });
 }
 
@@ -655,7 +677,7 @@ char const *p3m_batch_last_error(p3m_batch const *b) {
 }
 
 int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
-                   int mesh_max, int cao_min, int cao_max) {
+                   int mesh_max, int cao_min, int cao_max, double pair_ns) {
   if (!inout) return P3M_ERR_INVALID;
   return guarded(static_cast<std::string *>(nullptr), [&] {
     P3MParams params(true, inout->epsilon, inout->r_cut, v3i(inout->mesh), v3(inout->mesh_off),
@@ -696,7 +718,7 @@ int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min
       } catch (std::runtime_error const &) {
         return -1.;
       }
-      return model_step_ms(mesh, cao, trial.r_cut, n_charged, box);
+      return model_step_ms(mesh, cao, trial.r_cut, n_charged, box, pair_ns > 0. ? pair_ns : 1.);
     };
     tune_p3m(params, in, timer);
     for (int d = 0; d < 3; ++d) {

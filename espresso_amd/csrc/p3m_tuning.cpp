@@ -288,23 +288,24 @@ private:
 };
 } // namespace
 
-double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l) {
-  // device k-space solve: stencil work per charge and FFT work per mesh
-  // point, with unit costs measured for the FP32-mesh kernels on MI355X, plus
-  // the modelled real-space loop
+double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l,
+                     double pair_ns) {
+  // device k-space cost from the measured stage rates of the solver: about
+  // 1.3 ns per particle stencil point (spread + gather) and 0.12 ns per mesh
+  // point (FFTs and convolution), plus a fixed launch cost
   double const R = static_cast<double>(mesh[0]) * mesh[1] * mesh[2];
   double const stencil = static_cast<double>(n_q) * cao * cao * cao;
-  return 1.3e-9 * stencil + 1.2e-7 * R + 0.01 + model_real_space_ms(n_q, r_cut, box_l);
+  return 1.3e-9 * stencil + 1.2e-7 * R + 0.01 + model_real_space_ms(n_q, r_cut, box_l, pair_ns);
 }
 
-double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l) {
-  // pairs within r_cut for a uniform system, at 1 ns per pair (a tuned CPU
-  // short-range loop); only used to rank candidate parameter sets
+double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l, double pair_ns) {
+  // pairs within r_cut for a uniform system; only used to rank candidate
+  // parameter sets
   double const volume = box_l[0] * box_l[1] * box_l[2];
   double const density = n_q / volume;
   double const pairs =
       0.5 * n_q * density * (4. / 3.) * std::numbers::pi * r_cut * r_cut * r_cut;
-  return pairs * 1e-6;
+  return pairs * pair_ns * 1e-6;
 }
 
 void tune_p3m(P3MParams &params, TuningInput const &in, TuningTimer const &timer) {

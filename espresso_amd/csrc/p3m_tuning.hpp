@@ -30,8 +30,10 @@ struct TuningInput {
 };
 
 /** Modelled cost (ms) of one k-space solve on the device plus the real-space
- *  pair loop, for tuning without timing runs. */
-double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l);
+ *  pair loop at `pair_ns` nanoseconds per pair, for tuning without timing
+ *  runs. */
+double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l,
+                     double pair_ns = 1.);
 
 /** Returns the cost of one step in ms for a candidate (mesh, cao, r_cut_iL,
  *  alpha_L), or a negative value when the candidate cannot be timed. */
@@ -43,7 +45,8 @@ using TuningTimer = std::function<double(Vec3i const &mesh, int cao, double r_cu
  *  requested accuracy") like TuningAlgorithm::tune (TuningAlgorithm.hpp:154-175). */
 void tune_p3m(P3MParams &params, TuningInput const &in, TuningTimer const &timer);
 
-/** Modelled real-space cost (ms) of the short-range loop for N_q charges. */
-double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l);
+/** Modelled real-space cost (ms) of the short-range loop for N_q charges at
+ *  `pair_ns` nanoseconds per pair within r_cut. */
+double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l, double pair_ns = 1.);
 
 } // namespace espresso_amd

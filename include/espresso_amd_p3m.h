@@ -189,8 +189,17 @@ int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *po
  * first use, re-captured when the selection or arrays change); enqueue only. */
 int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                           const double *q, double *force);
-/* Launch shape: gather mapping (1 = plane, 0 = layer) and FFT columns/block. */
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol);
+/* Launch shape: gather mapping (1 = plane, 0 = layer), kz columns per block
+ * of the y/x FFT passes and pairs of real lines per block of the z passes
+ * (powers of two in [1, 64]). */
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines);
+/* Autotuner: time `reps` force solves of the selection for every candidate
+ * launch shape and keep the fastest; returns the chosen shape and the mean
+ * solve time before (default_ms) and after (best_ms).  `force` is scratch:
+ * forces accumulate into it on every timed solve. */
+int p3m_batch_autotune(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
+                       const double *q, double *force, int reps, int *gather_plane,
+                       int *ncol, int *zlines, double *best_ms, double *default_ms);
 int p3m_batch_synchronize(p3m_batch *b);
 int p3m_batch_get_stream(const p3m_batch *b, void **stream);
 int p3m_batch_profile_begin(p3m_batch *b, int max_solves);
@@ -199,10 +208,12 @@ const char *p3m_batch_last_error(const p3m_batch *b);
 
 /* Parameter tuning without a device: the reference's tuning algorithm
  * (p3m.cpp:594-791, TuningAlgorithm.cpp:57-328) with a modelled cost in place
- * of timed integration steps, meshes restricted to [mesh_min, mesh_max] and
- * orders to [cao_min, cao_max].  Fills mesh, cao, r_cut, alpha, accuracy. */
+ * of timed integration steps (device k-space stage rates plus a real-space
+ * loop at pair_ns nanoseconds per pair; pair_ns <= 0 selects 1 ns), meshes
+ * restricted to [mesh_min, mesh_max] and orders to [cao_min, cao_max].
+ * Fills mesh, cao, r_cut, alpha, accuracy. */
 int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
-                   int mesh_max, int cao_min, int cao_max);
+                   int mesh_max, int cao_min, int cao_max, double pair_ns);
 
 #ifdef __cplusplus
 }

@@ -199,3 +199,31 @@ def test_batch_argument_errors():
         # k-space cutoff larger than the box (p3m.cpp:710-750)
         b.add_system(box_l=[4.] * 3, n=10, prefactor=1., mesh=6, cao=6,
                      alpha=1., r_cut=1.)
+
+
+def test_launch_shapes_and_autotune_keep_results():
+    import torch
+    systems = _systems(seed=31)
+    b = _batch(systems, single=True)
+    pos, q = _device_arrays(systems)
+    ref = torch.zeros_like(pos)
+    b.solve(pos, q, ref)
+    ref_np = ref.cpu().numpy()
+    f = torch.zeros_like(pos)
+    for plane in (True, False):
+        for ncol, zl in ((1, 1), (2, 4), (16, 16), (64, 64)):
+            b.set_launch_shape(plane, ncol, zl)
+            f.zero_()
+            torch.cuda.synchronize()
+            b.solve(pos, q, f)
+            assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (plane, ncol, zl)
+    with pytest.raises(ValueError, match="powers of two"):
+        b.set_launch_shape(True, 3, 4)
+    scratch = torch.zeros_like(pos)
+    shape = b.autotune(pos, q, scratch, reps=1)
+    assert shape["ncol"] in (2, 4, 8, 16) and shape["zlines"] in (4, 8, 16)
+    assert 0. < shape["best_ms"] <= shape["default_ms"]
+    f.zero_()
+    torch.cuda.synchronize()
+    b.solve(pos, q, f)
+    assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5
