@@ -127,9 +127,10 @@ def load():
     L.p3m_batch_system_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
-    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int,
+                                             C.c_int]
     L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, ip,
-                                     ip, ip, dp, dp]
+                                     ip, ip, ip, dp, dp]
     L.p3m_batch_synchronize.argtypes = [vp]
     L.p3m_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_batch_profile_begin.argtypes = [vp, C.c_int]

@@ -156,9 +156,14 @@ class P3MBatch:
         self._check(_lib.load().p3m_batch_graph_solve(self._b, nsel, ids, p,
                                                       qq, f))
 
-    def set_launch_shape(self, gather_plane=True, ncol=8, zlines=16):
+    def set_launch_shape(self, gather_plane=True, ncol=8, zlines=16,
+                         fused_lds=96 * 1024):
+        """Gather mapping, FFT columns per block (split y/x passes), z-line
+        pairs per block, and the LDS bytes up to which a system's x-y planes
+        are transformed by the fused y/x pass (0: never)."""
         self._check(_lib.load().p3m_batch_set_launch_shape(
-            self._b, 1 if gather_plane else 0, int(ncol), int(zlines)))
+            self._b, 1 if gather_plane else 0, int(ncol), int(zlines),
+            int(fused_lds)))
 
     def autotune(self, pos, q, scratch_force, systems=None, reps=3):
         """Pick the fastest launch shape (gather mapping, FFT columns, z
@@ -167,14 +172,14 @@ class P3MBatch:
         and the mean solve times before and after."""
         p, qq, f = self._arrays(pos, q, scratch_force)
         nsel, ids = self._selection(systems)
-        gp, nc, zl = C.c_int(), C.c_int(), C.c_int()
+        gp, nc, zl, fl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         best, default = C.c_double(), C.c_double()
         self._check(_lib.load().p3m_batch_autotune(
             self._b, nsel, ids, p, qq, f, int(reps), C.byref(gp), C.byref(nc),
-            C.byref(zl), C.byref(best), C.byref(default)))
+            C.byref(zl), C.byref(fl), C.byref(best), C.byref(default)))
         return {"gather_plane": bool(gp.value), "ncol": nc.value,
-                "zlines": zl.value, "best_ms": best.value,
-                "default_ms": default.value}
+                "zlines": zl.value, "fused_lds": fl.value,
+                "best_ms": best.value, "default_ms": default.value}
 
     def synchronize(self):
         self._check(_lib.load().p3m_batch_synchronize(self._b))

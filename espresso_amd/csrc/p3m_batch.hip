@@ -25,9 +25,13 @@ FftPlan1D make_fft_plan(int n) {
     if (p.nstages >= FftPlan1D::max_stages) throw std::runtime_error("FFT plan too long");
     p.radix[p.nstages++] = r;
   };
+  // few stages: radix 8 first, a leftover factor 2 merged with a 3 into a
+  // radix-6 register DFT (each stage is one LDS round trip and barrier)
   while (m % 8 == 0) { push(8); m /= 8; }
-  while (m % 4 == 0) { push(4); m /= 4; }
-  while (m % 2 == 0) { push(2); m /= 2; }
+  if (m % 4 == 0) { push(4); m /= 4; }
+  if (m % 2 == 0) {
+    if (m % 3 == 0) { push(6); m /= 6; } else { push(2); m /= 2; }
+  }
   for (int f : {3, 5, 7}) while (m % f == 0) { push(f); m /= f; }
   for (int f = 11; m > 1 && f * f <= m; f += 2) while (m % f == 0) { push(f); m /= f; }
   if (m > 1) push(m);
@@ -54,6 +58,7 @@ struct SysDesc {
   int nbricks;
   void *rho, *Ey, *Ez, *khat, *A, *B, *G;
   int tw_off[3];
+  int kzmajor; // 1: half spectra stored [Kz][Mx][My] and the fused y/x pass
   FftPlanDev plan[3];
   double wv[3];
   double hais;
@@ -102,13 +107,21 @@ __device__ double dev_sinc(double x) {
 
 template <typename MT>
 __global__ void k_influence(MT *G, int Mx, int My, int Mz, int Kz, double wx, double wy,
-                            double wz, double hx, double hy, double hz, double alpha, int cao) {
+                            double wz, double hx, double hy, double hz, double alpha, int cao,
+                            int kzmajor) {
   int64_t const total = (int64_t)Mx * My * Kz;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * blockDim.x) {
-    int const kz = static_cast<int>(e % Kz);
-    int const y = static_cast<int>((e / Kz) % My);
-    int const x = static_cast<int>(e / ((int64_t)Kz * My));
+    int kz, y, x;
+    if (kzmajor) { // [Kz][Mx][My]
+      y = static_cast<int>(e % My);
+      x = static_cast<int>((e / My) % Mx);
+      kz = static_cast<int>(e / ((int64_t)My * Mx));
+    } else { // [Mx][My][Kz]
+      kz = static_cast<int>(e % Kz);
+      y = static_cast<int>((e / Kz) % My);
+      x = static_cast<int>(e / ((int64_t)Kz * My));
+    }
     double g = 0.;
     int const hmx = Mx / 2, hmy = My / 2, hmz = Mz / 2;
     bool const zero = (hmx == 0 || x % hmx == 0) && (hmy == 0 || y % hmy == 0) &&
@@ -193,10 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
 /** per-system deterministic reduction of block partials: one block per
  *  selected system, sums[s*16 + base + k] = sum over the system's blocks */
 __global__ void k_reduce_sys(int const *__restrict__ sel, int const *__restrict__ wt, int nsel,
-                             double const *part, int nk, int base, double *sums) {
+                             double const *part, int nk, int base, double *sums,
+                             int skip_empty) {
   __shared__ double red[16 * 8];
   int const slot = blockIdx.x;
   int const s = sel[slot];
+  if (skip_empty && wt[slot + 1] == wt[slot]) return; // system handled by another pass
   double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   for (int b = wt[slot] + threadIdx.x; b < wt[slot + 1]; b += blockDim.x)
     for (int k = 0; k < nk; ++k) v[k] += part[b * 8 + k];
@@ -426,6 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   int const ls = n + 1;
   int const rows = d.g.M[0] * d.g.M[1];
   int const Kz = d.g.Kz;
+  FastDiv const fn = make_fastdiv(n), fnl = make_fastdiv(nlp), fkz = make_fastdiv(Kz);
   MT const *rho = static_cast<MT const *>(d.rho);
   C *khat = static_cast<C *>(d.khat);
   C *twl = reinterpret_cast<C *>(smem_d);
@@ -434,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   load_twiddles(twl, tw + d.tw_off[2], n);
   int const pair0 = L.lb * nlp;
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
-    int const l = e / n, j = e - l * n;
+    int const l = fdiv(e, fn), j = e - l * n;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
     MT const re = (r0 < rows) ? rho[r0 * n + j] : MT(0);
     MT const im = (r1 < rows) ? rho[r1 * n + j] : MT(0);
@@ -442,14 +458,20 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   }
   __syncthreads();
   C *res = lds_fft(b0, b1, nlp, ls, plan, twl, false);
+  bool const kzm = d.kzmajor != 0;
   for (int e = threadIdx.x; e < nlp * Kz; e += blockDim.x) {
-    int const l = e / Kz, k = e - l * Kz;
+    // kz-major spectra: consecutive threads take consecutive rows
+    int const q0 = fdiv(e, kzm ? fnl : fkz);
+    int const l = kzm ? e - q0 * nlp : q0;
+    int const k = kzm ? q0 : e - q0 * Kz;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
     C const Z = res[l * ls + k];
-    C const Zc = cconj(res[l * ls + (n - k) % n]);
+    C const Zc = cconj(res[l * ls + (k == 0 ? 0 : n - k)]);
     C const sm = cadd(Z, Zc), df = csub(Z, Zc);
-    if (r0 < rows) khat[r0 * Kz + k] = C{MT(0.5) * sm.x, MT(0.5) * sm.y};
-    if (r1 < rows) khat[r1 * Kz + k] = C{MT(0.5) * df.y, MT(-0.5) * df.x};
+    int64_t const i0 = kzm ? (int64_t)k * rows + r0 : r0 * Kz + k;
+    int64_t const i1 = kzm ? i0 + 1 : i0 + Kz;
+    if (r0 < rows) khat[i0] = C{MT(0.5) * sm.x, MT(0.5) * sm.y};
+    if (r1 < rows) khat[i1] = C{MT(0.5) * df.y, MT(-0.5) * df.x};
   }
 }
 
@@ -468,6 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SysDesc const *__restrict__ sy
   int const ls = n + 1;
   int const Kz = d.g.Kz;
   int const ncb = (Kz + ncol - 1) / ncol;
+  FastDiv const fnc = make_fastdiv(ncol), fn = make_fastdiv(n);
   int const cb = L.lb % ncb;
   int64_t const x = L.lb / ncb;
   int const My = d.g.M[1];
@@ -479,13 +502,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SysDesc const *__restrict__ sy
   int const c0 = cb * ncol;
   int const nc = min(ncol, Kz - c0);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) b0[c * ls + j] = khat[(x * My + j) * Kz + c0 + c];
   }
   __syncthreads();
   C *res = lds_fft(b0, b1, nc, ls, plan, twl, false);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) khat[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
   }
 }
@@ -510,6 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   int const Kz = d.g.Kz;
   int const Mx = d.g.M[0], My = d.g.M[1], Mz = d.g.M[2];
   int const ncb = (Kz + ncol - 1) / ncol;
+  FastDiv const fnc = make_fastdiv(ncol), fn = make_fastdiv(n);
   int const cb = L.lb % ncb;
   int const y = L.lb / ncb;
   C const *khat = static_cast<C const *>(d.khat);
@@ -523,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   int const c0 = cb * ncol;
   int const nc = min(ncol, Kz - c0);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) X[c * ls + j] = khat[((int64_t)j * My + y) * Kz + c0 + c];
   }
   __syncthreads();
@@ -533,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   double const wvx = d.wv[0], wvy = d.wv[1], wvz = d.wv[2];
   double const ky = dop_shift(y, My) * wvy;
   for (int e = threadIdx.x; e < n * nc; e += blockDim.x) {
-    int const c = e / n, j = e - c * n;
+    int const c = fdiv(e, fn), j = e - c * n;
     int const kz = c0 + c;
     C const rh = res[c * ls + j];
     MT const gv = G[((int64_t)j * My + y) * Kz + kz];
@@ -567,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   if (!want_fields) return;
   C *ires = lds_fft(oth, res, 2 * ncol, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) {
       int64_t const gidx = ((int64_t)j * My + y) * Kz + c0 + c;
       Aout[gidx] = ires[c * ls + j];
@@ -594,6 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   int const Kz = d.g.Kz;
   int const My = d.g.M[1];
   int const ncb = (Kz + ncol - 1) / ncol;
+  FastDiv const fnc = make_fastdiv(ncol), fn = make_fastdiv(n);
   int const cb = L.lb % ncb;
   int64_t const x = L.lb / ncb;
   C *A = static_cast<C *>(d.A);
@@ -608,7 +633,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   int const c0 = cb * ncol;
   int const nc = min(ncol, Kz - c0);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) {
       C const v = A[(x * My + j) * Kz + c0 + c];
       S[c * ls + j] = v;
@@ -618,12 +643,12 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   __syncthreads();
   C *res = lds_fft(W0, W1, nc, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) A[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n * nc; e += blockDim.x) {
-    int const c = e / n, j = e - c * n;
+    int const c = fdiv(e, fn), j = e - c * n;
     MT const kyf = static_cast<MT>(dop_shift(j, My)) * static_cast<MT>(wvy);
     C const v = S[c * ls + j];
     W0[c * ls + j] = C{-kyf * v.y, kyf * v.x};
@@ -631,19 +656,129 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   __syncthreads();
   res = lds_fft(W0, W1, nc, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) Fy[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
   }
   __syncthreads();
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) W0[c * ls + j] = B[(x * My + j) * Kz + c0 + c];
   }
   __syncthreads();
   res = lds_fft(W0, W1, nc, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
-    int const j = e / ncol, c = e - j * ncol;
+    int const j = fdiv(e, fnc), c = e - j * ncol;
     if (c < nc) B[(x * My + j) * Kz + c0 + c] = res[c * ls + j];
+  }
+}
+
+/** Fused y/x pass for meshes whose x-y plane fits in LDS three times
+ *  (kz-major spectra), one block per kz column: forward c2c along y and x,
+ *  convolution with the influence function with energy and virial partial
+ *  sums (p3m.cpp:393-449, 551-567), then for each field component
+ *  (i k_x phi, i k_y phi, phi) the inverse transforms along x and y.
+ *  Outputs: B <- x field, khat <- y field, A <- z field before i k_z (which
+ *  the c2r z pass applies), all still transformed along z. */
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restrict__ sys,
+                                                        int const *__restrict__ sel,
+                                                        int const *__restrict__ wt, int nsel,
+                                                        cx_t<MT> const *__restrict__ tw,
+                                                        int want_fields, double *partials) {
+  using C = cx_t<MT>;
+  extern __shared__ double smem_d[];
+  __shared__ double red[8 * 16];
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int const kz = L.lb;
+  int const Mx = d.g.M[0], My = d.g.M[1], Mz = d.g.M[2];
+  int const lsy = My + 1, lsx = Mx + 1; // line strides of [x][y] and [y][x] planes
+  int const plane = max(Mx * lsy, My * lsx);
+  int const pl = Mx * My;
+  FastDiv const fMy = make_fastdiv(My), fMx = make_fastdiv(Mx);
+  C *twx = reinterpret_cast<C *>(smem_d);
+  C *twy = twx + Mx;
+  C *P0 = twy + My;
+  C *P1 = P0 + plane;
+  C *P2 = P1 + plane;
+  load_twiddles(twx, tw + d.tw_off[0], Mx);
+  load_twiddles(twy, tw + d.tw_off[1], My);
+  C *khat = static_cast<C *>(d.khat);
+  C const *src = khat + (int64_t)kz * pl;
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    P0[x * lsy + y] = src[e];
+  }
+  __syncthreads();
+  C *r = lds_fft(P0, P1, Mx, lsy, d.plan[1], twy, false); // along y, layout [x][y]
+  C *o = (r == P0) ? P1 : P0;
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    o[y * lsx + x] = r[x * lsy + y];
+  }
+  __syncthreads();
+  C *q = lds_fft(o, r, My, lsx, d.plan[0], twx, false); // along x, layout [y][x]
+  C *w = (q == o) ? r : o;
+
+  double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+  double const wvx = d.wv[0], wvy = d.wv[1], wvz = d.wv[2];
+  double const wgt = (kz == 0 || 2 * kz == Mz) ? 1. : 2.;
+  double const kzv = dop_shift(kz, Mz) * wvz;
+  MT const *G = static_cast<MT const *>(d.G) + (int64_t)kz * pl;
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    C const rh = q[y * lsx + x];
+    MT const gv = G[e];
+    double const ek = static_cast<double>(gv) *
+                      (static_cast<double>(rh.x) * rh.x + static_cast<double>(rh.y) * rh.y);
+    double const kx = dop_shift(x, Mx) * wvx;
+    double const ky = dop_shift(y, My) * wvy;
+    double const norm_sq = kx * kx + ky * ky + kzv * kzv;
+    acc[0] += wgt * ek;
+    if (norm_sq != 0.) {
+      double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
+      acc[1] += pref * kx * kx;
+      acc[2] += pref * kx * ky;
+      acc[3] += pref * kx * kzv;
+      acc[4] += pref * ky * ky;
+      acc[5] += pref * ky * kzv;
+      acc[6] += pref * kzv * kzv;
+      acc[7] += wgt * ek;
+    }
+    q[y * lsx + x] = C{gv * rh.x, gv * rh.y};
+  }
+  block_sum<8>(acc, red); // ends with a barrier: q is complete
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 8; ++k) partials[blockIdx.x * 8 + k] = acc[k];
+  if (!want_fields) return;
+  C *const outs[3] = {static_cast<C *>(d.B), khat, static_cast<C *>(d.A)};
+  for (int comp = 0; comp < 3; ++comp) {
+    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+      int const y = fdiv(e, fMx), x = e - y * Mx;
+      C const v = q[y * lsx + x];
+      if (comp == 2) {
+        P2[y * lsx + x] = v;
+      } else {
+        MT const kf = (comp == 0) ? static_cast<MT>(dop_shift(x, Mx)) * static_cast<MT>(wvx)
+                                  : static_cast<MT>(dop_shift(y, My)) * static_cast<MT>(wvy);
+        P2[y * lsx + x] = C{-kf * v.y, kf * v.x};
+      }
+    }
+    __syncthreads();
+    C *a = lds_fft(P2, w, My, lsx, d.plan[0], twx, true); // inverse along x
+    C *b = (a == P2) ? w : P2;
+    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+      int const x = fdiv(e, fMy), y = e - x * My;
+      b[x * lsy + y] = a[y * lsx + x];
+    }
+    __syncthreads();
+    C *c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true); // inverse along y
+    C *dst = outs[comp] + (int64_t)kz * pl;
+    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+      int const x = fdiv(e, fMy), y = e - x * My;
+      dst[e] = c[x * lsy + y];
+    }
+    __syncthreads();
   }
 }
 
@@ -665,6 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
   int const ls = n + 1;
   int const rows = d.g.M[0] * d.g.M[1];
   int const Kz = d.g.Kz;
+  FastDiv const fn = make_fastdiv(n), fnl = make_fastdiv(nlp);
   C const *S = static_cast<C const *>(comp == 0 ? d.B : comp == 1 ? d.khat : d.A);
   MT *E = static_cast<MT *>(comp == 0 ? d.rho : comp == 1 ? d.Ey : d.Ez);
   bool const mul_ikz = (comp == 2);
@@ -674,13 +810,18 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
   C *b1 = b0 + nlp * ls;
   load_twiddles(twl, tw + d.tw_off[2], n);
   int const pair0 = L.lb * nlp;
+  bool const kzm = d.kzmajor != 0;
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
-    int const l = e / n, k = e - l * n;
+    int const q0 = fdiv(e, kzm ? fnl : fn);
+    int const l = kzm ? e - q0 * nlp : q0;
+    int const k = kzm ? q0 : e - q0 * n;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
     bool const upper = (k >= Kz);
     int const ks = upper ? n - k : k;
-    C x1 = (r0 < rows) ? S[r0 * Kz + ks] : C{MT(0), MT(0)};
-    C x2 = (r1 < rows) ? S[r1 * Kz + ks] : C{MT(0), MT(0)};
+    int64_t const i0 = kzm ? (int64_t)ks * rows + r0 : r0 * Kz + ks;
+    int64_t const i1 = kzm ? i0 + 1 : i0 + Kz;
+    C x1 = (r0 < rows) ? S[i0] : C{MT(0), MT(0)};
+    C x2 = (r1 < rows) ? S[i1] : C{MT(0), MT(0)};
     if (mul_ikz) {
       MT const kz = static_cast<MT>(dop_shift(ks, n)) * static_cast<MT>(wvz);
       x1 = C{-kz * x1.y, kz * x1.x};
@@ -699,7 +840,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
   __syncthreads();
   C *res = lds_fft(b0, b1, nlp, ls, plan, twl, true);
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
-    int const l = e / n, j = e - l * n;
+    int const l = fdiv(e, fn), j = e - l * n;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
     C const v = res[l * ls + j];
     if (r0 < rows) E[r0 * n + j] = v.x;
@@ -950,6 +1091,16 @@ int tile_points(int cao) {
   return t * t * t;
 }
 
+/** LDS of the fused y/x pass: both twiddle tables and three x-y planes. */
+size_t fused_lds_bytes(int mx, int my, size_t csz) {
+  size_t const plane = static_cast<size_t>(std::max(mx * (my + 1), my * (mx + 1)));
+  return (static_cast<size_t>(mx + my) + 3 * plane) * csz;
+}
+bool uses_fused_pass(int mx, int my, MeshPrecision prec, LaunchShape const &shape) {
+  size_t const csz = (prec == MeshPrecision::FP64) ? 16 : 8;
+  return fused_lds_bytes(mx, my, csz) <= static_cast<size_t>(shape.fused_lds);
+}
+
 /** lines per block such that twiddles + lines_per_unit * units lines fit */
 int pick_lines(int n, size_t csz, int lines_per_unit, int want, size_t budget) {
   size_t const line = static_cast<size_t>(n + 1) * csz;
@@ -983,8 +1134,9 @@ struct Plan {
   int nsel = 0;
   int *dbuf = nullptr; // device copy of all tables below
   int *d_sel = nullptr, *wt_bin = nullptr, *wt_fill = nullptr, *wt_z = nullptr, *wt_y = nullptr,
-      *wt_x = nullptr, *wt_yi = nullptr;
-  int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_y = 0, blocks_x = 0, blocks_yi = 0;
+      *wt_x = nullptr, *wt_yi = nullptr, *wt_f = nullptr;
+  int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_y = 0, blocks_x = 0, blocks_yi = 0,
+      blocks_f = 0;
   struct Group {
     int cao;
     int nsel;
@@ -993,7 +1145,7 @@ struct Plan {
   };
   std::vector<Group> groups;
   int nlp_z = 1, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
-  size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_yi = 0;
+  size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_yi = 0, lds_f = 0;
   bool any_dipole = false;
   // graph
   hipGraphExec_t graph = nullptr;
@@ -1090,11 +1242,31 @@ int64_t BatchSolver::total_particles() const {
 size_t BatchSolver::system_bytes(int s) const { return m->sys.at(s).bytes; }
 LocalMesh const &BatchSolver::local_mesh(int s) const { return m->sys.at(s).lm; }
 
+namespace {
+void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st);
+} // namespace
+
 void BatchSolver::set_launch_shape(LaunchShape const &s) {
   m_shape = s;
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
+  m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
   hip_check(hipStreamSynchronize(m_stream), "sync");
+  // systems whose y/x pass changes switch spectrum layout: recompute G
+  bool changed = false;
+  for (auto &h : m->sys) {
+    int const kzm = uses_fused_pass(h.desc.g.M[0], h.desc.g.M[1], m_prec, m_shape) ? 1 : 0;
+    if (kzm != h.desc.kzmajor) {
+      h.desc.kzmajor = kzm;
+      compute_influence(h, m_prec, m_stream);
+      changed = true;
+    }
+  }
+  if (changed) {
+    hip_check(hipStreamSynchronize(m_stream), "sync influence");
+    m->descs_dirty = true;
+  }
   free_plan(m->plan);
 }
 
@@ -1122,20 +1294,37 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
   res.best = start;
   res.best_ms = res.default_ms;
   res.candidates = 1;
-  for (bool plane : {true, false})
-    for (int ncol : {2, 4, 8, 16})
-      for (int zl : {4, 8, 16}) {
-        LaunchShape const c{plane, ncol, zl};
-        if (c.gather_plane == start.gather_plane && c.ncol == start.ncol &&
-            c.zlines == start.zlines)
-          continue;
-        double const ms = time_shape(c);
-        ++res.candidates;
-        if (ms < res.best_ms) {
-          res.best_ms = ms;
-          res.best = c;
-        }
-      }
+  auto consider = [&](LaunchShape const &c) {
+    double const ms = time_shape(c);
+    ++res.candidates;
+    if (ms < res.best_ms) {
+      res.best_ms = ms;
+      res.best = c;
+    }
+  };
+  // coordinate descent: fused y/x pass, gather mapping, FFT columns, z lines
+  {
+    LaunchShape c = res.best;
+    c.fused_lds = (start.fused_lds > 0) ? 0 : LaunchShape{}.fused_lds;
+    consider(c);
+  }
+  {
+    LaunchShape c = res.best;
+    c.gather_plane = !c.gather_plane;
+    consider(c);
+  }
+  for (int v : {2, 4, 8, 16}) {
+    if (v == res.best.ncol) continue;
+    LaunchShape c = res.best;
+    c.ncol = v;
+    consider(c);
+  }
+  for (int v : {4, 8, 16}) {
+    if (v == res.best.zlines) continue;
+    LaunchShape c = res.best;
+    c.zlines = v;
+    consider(c);
+  }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   set_launch_shape(res.best);
@@ -1171,12 +1360,30 @@ template <typename MT> void set_kernel_attrs() {
   allow_lds(k_fft_y<MT>);
   allow_lds(k_fft_x_conv<MT>);
   allow_lds(k_fft_y_inv<MT>);
+  allow_lds(k_fft_yx_conv<MT>);
   for (int cao = 1; cao <= 7; ++cao)
     dispatch_cao(cao, [&](auto P) {
       allow_lds(k_spread<P.value, MT>);
       allow_lds(k_gather_plane<P.value, MT>);
       allow_lds(k_gather_layer<P.value, MT>);
     });
+}
+
+/** G on the half spectrum in the system's layout (FP64 arithmetic). */
+void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st) {
+  SysDesc const &d = h.desc;
+  DevGeo const &g = d.g;
+  int64_t const K = (int64_t)g.M[0] * g.M[1] * g.Kz;
+  int const gg = static_cast<int>(std::min<int64_t>((K + 255) / 256, 16384));
+  if (prec == MeshPrecision::FP64)
+    k_influence<double><<<gg, 256, 0, st>>>(
+        static_cast<double *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
+        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, d.cao, d.kzmajor);
+  else
+    k_influence<float><<<gg, 256, 0, st>>>(
+        static_cast<float *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
+        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, d.cao, d.kzmajor);
+  hip_check(hipGetLastError(), "k_influence");
 }
 
 void recompute_offsets(BatchSolver::Impl *m) {
@@ -1270,16 +1477,8 @@ int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, doub
   hip_check(hipMalloc(&d.B, K * 2 * msz), "hipMalloc B");
   hip_check(hipMalloc(&d.G, K * msz), "hipMalloc G");
   h.bytes = static_cast<size_t>(3 * R * msz + 3 * K * 2 * msz + K * msz);
-  int const gg = static_cast<int>(std::min<int64_t>((K + 255) / 256, 16384));
-  if (m_prec == MeshPrecision::FP64)
-    k_influence<double><<<gg, 256, 0, m_stream>>>(
-        static_cast<double *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
-        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, cao);
-  else
-    k_influence<float><<<gg, 256, 0, m_stream>>>(
-        static_cast<float *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
-        h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, cao);
-  hip_check(hipGetLastError(), "k_influence");
+  d.kzmajor = uses_fused_pass(g.M[0], g.M[1], m_prec, m_shape) ? 1 : 0;
+  compute_influence(h, m_prec, m_stream);
   hip_check(hipStreamSynchronize(m_stream), "sync influence");
   m->sys.push_back(h);
   recompute_offsets(m);
@@ -1368,13 +1567,18 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
   p.sel = sel;
   p.nsel = static_cast<int>(sel.size());
   size_t const csz = (prec == MeshPrecision::FP64) ? 16 : 8;
+  // split y/x passes: sizes over the systems that use them; z passes: all
   int max_mx = 1, max_my = 1, max_mz = 1;
   for (int s : sel) {
-    auto const &g = m->sys.at(s).desc.g;
-    max_mx = std::max(max_mx, g.M[0]);
-    max_my = std::max(max_my, g.M[1]);
-    max_mz = std::max(max_mz, g.M[2]);
-    if (m->sys[s].desc.dip_pref != 0.) p.any_dipole = true;
+    auto const &d = m->sys.at(s).desc;
+    max_mz = std::max(max_mz, d.g.M[2]);
+    if (d.kzmajor) {
+      p.lds_f = std::max(p.lds_f, fused_lds_bytes(d.g.M[0], d.g.M[1], csz));
+    } else {
+      max_mx = std::max(max_mx, d.g.M[0]);
+      max_my = std::max(max_my, d.g.M[1]);
+    }
+    if (d.dip_pref != 0.) p.any_dipole = true;
   }
   size_t const budget = 64 * 1024;
   p.nlp_z = pick_lines(max_mz, csz, 2, shape.zlines, budget);
@@ -1388,8 +1592,8 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
   p.lds_y = lds_lines(max_my, 2 * p.ncol_y);
   p.lds_x = lds_lines(max_mx, 4 * p.ncol_x);
   p.lds_yi = lds_lines(max_my, 3 * p.ncol_yi);
-  for (size_t b : {p.lds_z, p.lds_y, p.lds_x, p.lds_yi})
-    if (b > 96 * 1024) throw std::runtime_error("P3M: mesh too large for the LDS budget");
+  for (size_t b : {p.lds_z, p.lds_y, p.lds_x, p.lds_yi, p.lds_f})
+    if (b > 128 * 1024) throw std::runtime_error("P3M: mesh too large for the LDS budget");
 
   // tables: sel, then prefix arrays (each nsel+1), then per-cao groups
   std::vector<int> host;
@@ -1417,14 +1621,15 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
     return (pairs + nlp - 1) / nlp;
   });
   auto const fy = push_prefix([ncy](HostSys const &h) {
-    return ((h.desc.g.Kz + ncy - 1) / ncy) * h.desc.g.M[0];
+    return h.desc.kzmajor ? 0 : ((h.desc.g.Kz + ncy - 1) / ncy) * h.desc.g.M[0];
   });
   auto const fx = push_prefix([ncx](HostSys const &h) {
-    return ((h.desc.g.Kz + ncx - 1) / ncx) * h.desc.g.M[1];
+    return h.desc.kzmajor ? 0 : ((h.desc.g.Kz + ncx - 1) / ncx) * h.desc.g.M[1];
   });
   auto const fyi = push_prefix([ncyi](HostSys const &h) {
-    return ((h.desc.g.Kz + ncyi - 1) / ncyi) * h.desc.g.M[0];
+    return h.desc.kzmajor ? 0 : ((h.desc.g.Kz + ncyi - 1) / ncyi) * h.desc.g.M[0];
   });
+  auto const ff = push_prefix([](HostSys const &h) { return h.desc.kzmajor ? h.desc.g.Kz : 0; });
   struct GH {
     int cao, sel_start, nsel, wt_start, blocks;
   };
@@ -1462,6 +1667,8 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
   p.blocks_x = fx.second;
   p.wt_yi = p.dbuf + fyi.first;
   p.blocks_yi = fyi.second;
+  p.wt_f = p.dbuf + ff.first;
+  p.blocks_f = ff.second;
   for (auto const &g : gh)
     p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks});
   p.version = m->version;
@@ -1471,11 +1678,12 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
               "hipMalloc bin partials");
     m->bin_cap = p.blocks_bin;
   }
-  if (p.blocks_x > m->conv_cap) {
+  int const conv_blocks = p.blocks_x + p.blocks_f; // split partials, then fused
+  if (conv_blocks > m->conv_cap) {
     dfree(m->conv_part);
-    hip_check(hipMalloc(&m->conv_part, sizeof(double) * 8 * std::max(p.blocks_x, 1)),
+    hip_check(hipMalloc(&m->conv_part, sizeof(double) * 8 * std::max(conv_blocks, 1)),
               "hipMalloc conv partials");
-    m->conv_cap = p.blocks_x;
+    m->conv_cap = conv_blocks;
   }
 }
 
@@ -1494,7 +1702,7 @@ void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, uns
     k_bin<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->counts, m->p_brick,
                                             m->p_rank, p_nmp, m->bin_part, m->err_flag);
   }
-  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, m->bin_part, 7, 0, m->sums);
+  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, m->bin_part, 7, 0, m->sums, 0);
   k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, m->counts, m->offsets);
   if (p.blocks_bin > 0)
     k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->p_brick,
@@ -1511,15 +1719,25 @@ void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, uns
   if (ev) (void)hipEventRecord(ev[2], st);
   if (stop_after_spread) return;
   k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
-  k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);
+  if (p.blocks_y > 0)
+    k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);
   if (ev) (void)hipEventRecord(ev[3], st);
-  k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
-                                                        want_force ? 1 : 0, m->conv_part);
+  if (p.blocks_x > 0) {
+    k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
+                                                          want_force ? 1 : 0, m->conv_part);
+    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, m->conv_part, 8, 8, m->sums, 1);
+  }
+  if (p.blocks_f > 0) {
+    double *fpart = m->conv_part + 8 * static_cast<size_t>(p.blocks_x);
+    k_fft_yx_conv<MT><<<p.blocks_f, kBlock, p.lds_f, st>>>(sys, p.d_sel, p.wt_f, ns, tw,
+                                                           want_force ? 1 : 0, fpart);
+    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, m->sums, 1);
+  }
   if (ev) (void)hipEventRecord(ev[4], st);
-  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, m->conv_part, 8, 8, m->sums);
   if (want_force) {
-    k_fft_y_inv<MT><<<p.blocks_yi, kBlock, p.lds_yi, st>>>(sys, p.d_sel, p.wt_yi, ns, tw,
-                                                            p.ncol_yi);
+    if (p.blocks_yi > 0)
+      k_fft_y_inv<MT><<<p.blocks_yi, kBlock, p.lds_yi, st>>>(sys, p.d_sel, p.wt_yi, ns, tw,
+                                                              p.ncol_yi);
     dim3 const gz(p.blocks_z, 3);
     k_fft_z_c2r<MT><<<gz, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
   }

@@ -57,6 +57,8 @@ struct LaunchShape {
   bool gather_plane = true; // plane-mapped gather (else layer-mapped)
   int ncol = 8;             // kz columns per block in the y/x FFT passes
   int zlines = 16;          // pairs of real z lines per block in the z passes
+  int fused_lds = 96 * 1024; // fused y/x pass for meshes whose three x-y
+                             // planes fit in this many LDS bytes (0: never)
 };
 
 struct AutotuneResult {

@@ -615,20 +615,25 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double con
   });
 }
 
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines) {
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines,
+                               int fused_lds) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
     for (int v : {ncol, zlines})
       if (v < 1 || v > 64 || (v & (v - 1)) != 0)
         throw std::invalid_argument(
             "P3M batch: ncol and zlines must be powers of two in [1, 64]");
-    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol, zlines});
+    if (fused_lds < 0 || fused_lds > 128 * 1024)
+      throw std::invalid_argument("P3M batch: fused_lds must be in [0, 131072]");
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol, zlines, fused_lds});
   });
 }
 
 int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
                        double const *q, double *force, int reps, int *gather_plane,
-                       int *ncol, int *zlines, double *best_ms, double *default_ms) {
+                       int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                       double *default_ms) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
     auto const sel = selection(b, nsel, sys_ids);
@@ -641,6 +646,7 @@ int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const 
     if (gather_plane) *gather_plane = r.best.gather_plane ? 1 : 0;
     if (ncol) *ncol = r.best.ncol;
     if (zlines) *zlines = r.best.zlines;
+    if (fused_lds) *fused_lds = r.best.fused_lds;
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });

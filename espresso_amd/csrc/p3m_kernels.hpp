@@ -134,6 +134,21 @@ __device__ __forceinline__ int magic_div(int e, unsigned long long magic) {
   return static_cast<int>((static_cast<unsigned long long>(e) * magic) >> 32);
 }
 
+/** Division by a block-uniform divisor d >= 1 as a multiply-high with the
+ *  magic number m = floor((2^32 - 1) / d) + 1; exact for 0 <= e with
+ *  e * d < 2^32 (d = 1 is passed through).  Build it once per block or per
+ *  FFT stage from a uniform d. */
+struct FastDiv {
+  unsigned m;
+  int d;
+};
+__device__ __forceinline__ FastDiv make_fastdiv(int d) {
+  return FastDiv{d > 1 ? 0xFFFFFFFFu / static_cast<unsigned>(d) + 1u : 0u, d};
+}
+__device__ __forceinline__ int fdiv(int e, FastDiv f) {
+  return f.d == 1 ? e : static_cast<int>(__umulhi(static_cast<unsigned>(e), f.m));
+}
+
 /** Periodic mesh index of tile row/column v = o + a >= 0 (tiles may be
  *  wider than the mesh for tiny meshes, so wrap as often as needed). */
 __device__ __forceinline__ int wrap_once(int v, int M) {
@@ -333,20 +348,23 @@ __device__ __forceinline__ void dft_reg(C (&v)[R], C const *tw, int n, bool inv)
   }
 }
 
+/** One radix-R Stockham butterfly j of a stage with Ns = product of the
+ *  radices already applied; `fNs` divides by Ns and `tstride` = n / (Ns R). */
 template <typename C, int R>
 __device__ __forceinline__ void stockham_bfly(C const *src, C *dst, int j, int nb, int Ns, int n,
-                                              C const *tw, bool inv) {
+                                              FastDiv fNs, int tstride, C const *tw, bool inv) {
   C v[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
-  int const jm = j % Ns;
+  int const jq = fdiv(j, fNs);
+  int const jm = j - jq * Ns;
   if (Ns > 1) {
-    int const step = (n / (Ns * R)) * jm;
+    int const step = tstride * jm;
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_get(tw, r * step, inv));
   }
   dft_reg<C, R>(v, tw, n, inv);
-  int const idxD = (j / Ns) * Ns * R + jm;
+  int const idxD = jq * Ns * R + jm;
 #pragma unroll
   for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
 }
@@ -367,43 +385,49 @@ __device__ __forceinline__ C *lds_fft(C *buf0, C *buf1, int nlines, int ls, FftP
   int Ns = 1;
   for (int s = 0; s < nstages; ++s) {
     int const R = __builtin_amdgcn_readfirstlane(plan.radix[s]);
+    int const nb = __builtin_amdgcn_readfirstlane(n / R);
+    int const tstride = __builtin_amdgcn_readfirstlane(nb / Ns); // n / (Ns R)
+    FastDiv const fNs = make_fastdiv(Ns);
     if (R <= 8) {
-      int const nb = n / R;
+      FastDiv const fnb = make_fastdiv(nb);
       int const items = nlines * nb;
       for (int it = threadIdx.x; it < items; it += blockDim.x) {
-        int const l = it / nb;
+        int const l = fdiv(it, fnb);
         int const j = it - l * nb;
         C const *src = in + l * ls;
         C *dst = out + l * ls;
         switch (R) {
-        case 2: stockham_bfly<C, 2>(src, dst, j, nb, Ns, n, tw, inv); break;
-        case 3: stockham_bfly<C, 3>(src, dst, j, nb, Ns, n, tw, inv); break;
-        case 4: stockham_bfly<C, 4>(src, dst, j, nb, Ns, n, tw, inv); break;
-        case 5: stockham_bfly<C, 5>(src, dst, j, nb, Ns, n, tw, inv); break;
-        case 6: stockham_bfly<C, 6>(src, dst, j, nb, Ns, n, tw, inv); break;
-        case 7: stockham_bfly<C, 7>(src, dst, j, nb, Ns, n, tw, inv); break;
-        default: stockham_bfly<C, 8>(src, dst, j, nb, Ns, n, tw, inv); break;
+        case 2: stockham_bfly<C, 2>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        case 3: stockham_bfly<C, 3>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        case 4: stockham_bfly<C, 4>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        case 5: stockham_bfly<C, 5>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        case 6: stockham_bfly<C, 6>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        case 7: stockham_bfly<C, 7>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        default: stockham_bfly<C, 8>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
         }
       }
     } else {
       // generic radix: one output element per item
-      int const nb = n / R;
+      FastDiv const fn = make_fastdiv(n), fR = make_fastdiv(R);
       int const items = nlines * n;
       for (int it = threadIdx.x; it < items; it += blockDim.x) {
-        int const l = it / n;
+        int const l = fdiv(it, fn);
         int const e = it - l * n;
-        int const j = e / R;  // butterfly
+        int const j = fdiv(e, fR); // butterfly
         int const k = e - j * R;
-        int const jm = j % Ns;
+        int const jq = fdiv(j, fNs);
+        int const jm = j - jq * Ns;
         C const *src = in + l * ls;
         C *dst = out + l * ls;
-        int const base = (n / (Ns * R)) * (jm + k * Ns);
-        C s = src[j];
+        int const base = tstride * (jm + k * Ns); // < n
+        C sum = src[j];
+        int m = 0;
         for (int r = 1; r < R; ++r) {
-          int const m = static_cast<int>((static_cast<long long>(r) * base) % n);
-          s = cadd(s, cmul(src[j + r * nb], tw_get(tw, m, inv)));
+          m += base; // (r * base) mod n
+          if (m >= n) m -= n;
+          sum = cadd(sum, cmul(src[j + r * nb], tw_get(tw, m, inv)));
         }
-        dst[(j / Ns) * Ns * R + jm + k * Ns] = s;
+        dst[jq * Ns * R + jm + k * Ns] = sum;
       }
     }
     __syncthreads();

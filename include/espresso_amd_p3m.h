@@ -190,16 +190,21 @@ int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *po
 int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                           const double *q, double *force);
 /* Launch shape: gather mapping (1 = plane, 0 = layer), kz columns per block
- * of the y/x FFT passes and pairs of real lines per block of the z passes
- * (powers of two in [1, 64]). */
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines);
-/* Autotuner: time `reps` force solves of the selection for every candidate
- * launch shape and keep the fastest; returns the chosen shape and the mean
- * solve time before (default_ms) and after (best_ms).  `force` is scratch:
- * forces accumulate into it on every timed solve. */
+ * of the split y/x FFT passes and pairs of real lines per block of the z
+ * passes (powers of two in [1, 64]), and the LDS bytes up to which a system
+ * uses the fused y/x pass (its three x-y planes in LDS; 0 = never, at most
+ * 131072). */
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines,
+                               int fused_lds);
+/* Autotuner: time `reps` force solves of the selection for candidate launch
+ * shapes (coordinate descent over fused pass, gather mapping, FFT columns and
+ * z lines) and keep the fastest; returns the chosen shape and the mean solve
+ * time before (default_ms) and after (best_ms).  `force` is scratch: forces
+ * accumulate into it on every timed solve. */
 int p3m_batch_autotune(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                        const double *q, double *force, int reps, int *gather_plane,
-                       int *ncol, int *zlines, double *best_ms, double *default_ms);
+                       int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                       double *default_ms);
 int p3m_batch_synchronize(p3m_batch *b);
 int p3m_batch_get_stream(const p3m_batch *b, void **stream);
 int p3m_batch_profile_begin(p3m_batch *b, int max_solves);

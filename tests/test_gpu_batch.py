@@ -66,12 +66,17 @@ def _oracle(s):
                      s["prefactor"])
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("single,f_tol,e_tol", [(False, 1e-6, 1e-8),
                                                 (True, 2e-5, 2e-6)])
-def test_batch_matches_oracle_per_system(single, f_tol, e_tol):
+def test_batch_matches_oracle_per_system(single, f_tol, e_tol, fused):
+    # fused: every mesh here takes the fused y/x pass (x-y planes in LDS);
+    # otherwise the split y, x and inverse-y passes
     import torch
     systems = _systems()
     b = _batch(systems, single)
+    if not fused:
+        b.set_launch_shape(fused_lds=0)
     pos, q = _device_arrays(systems)
     force = torch.zeros_like(pos)
     energy, virial = b.solve(pos, q, force, energy=True, pressure=True)
@@ -210,18 +215,20 @@ def test_launch_shapes_and_autotune_keep_results():
     b.solve(pos, q, ref)
     ref_np = ref.cpu().numpy()
     f = torch.zeros_like(pos)
-    for plane in (True, False):
-        for ncol, zl in ((1, 1), (2, 4), (16, 16), (64, 64)):
-            b.set_launch_shape(plane, ncol, zl)
-            f.zero_()
-            torch.cuda.synchronize()
-            b.solve(pos, q, f)
-            assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (plane, ncol, zl)
+    for fused in (0, 96 * 1024):
+        for plane in (True, False):
+            for ncol, zl in ((1, 1), (2, 4), (16, 16), (64, 64)):
+                b.set_launch_shape(plane, ncol, zl, fused)
+                f.zero_()
+                torch.cuda.synchronize()
+                b.solve(pos, q, f)
+                assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (fused, plane, ncol, zl)
     with pytest.raises(ValueError, match="powers of two"):
         b.set_launch_shape(True, 3, 4)
     scratch = torch.zeros_like(pos)
     shape = b.autotune(pos, q, scratch, reps=1)
     assert shape["ncol"] in (2, 4, 8, 16) and shape["zlines"] in (4, 8, 16)
+    assert shape["fused_lds"] in (0, 96 * 1024)
     assert 0. < shape["best_ms"] <= shape["default_ms"]
     f.zero_()
     torch.cuda.synchronize()
