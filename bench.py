@@ -204,6 +204,103 @@ def bench_batched(steps, warmup, torch, dev):
     return res
 
 
+def make_c5_specs(n_sys=2048, seed0=5000):
+    """2048 resident systems: N log-uniform in [1000, 200000] (even), density
+    uniform in [0.2, 0.8]."""
+    out = []
+    for s in range(n_sys):
+        rng = np.random.default_rng(seed0 + s)
+        n = int(2 * round(math.exp(rng.uniform(math.log(1000), math.log(200000))) / 2))
+        rho = rng.uniform(0.2, 0.8)
+        out.append({"n": n, "L": (n / rho) ** (1. / 3.), "seed": seed0 + s})
+    return out
+
+
+def bench_serving(torch, dev, rates=(50, 100, 200, 500, 1000, 2000), seconds=1.5,
+                  n_sys=2048, lanes=4, budget_bytes=512 << 20, max_batch=64):
+    """Poisson request streams over resident systems: continuous batching,
+    admission by working memory, p50/p99 latency and solves/s per offered
+    load, and the saturation rate of a burst."""
+    from espresso_amd import P3MBatch, P3MServer
+    specs = make_c5_specs(n_sys)
+    t0 = time.perf_counter()
+    # tuning needs only N, sum q^2 and L
+    tuned = tune_systems([{"L": s["L"], "n": s["n"], "q": np.ones(1) * math.sqrt(s["n"])}
+                          for s in specs], 1e-4, (8, 128), (1, 7), workers=16)
+    t_tune = time.perf_counter() - t0
+    n_tot = sum(s["n"] for s in specs)
+    print(f"[bench] serving: {n_sys} systems, {n_tot} resident particles, tuned in "
+          f"{t_tune:.1f} s", flush=True)
+    pos = torch.empty((n_tot, 3), dtype=torch.float64, device=dev)
+    q = torch.empty((n_tot,), dtype=torch.float64, device=dev)
+    off = 0
+    for s in specs:
+        rng = np.random.default_rng(s["seed"])
+        n = s["n"]
+        pos[off:off + n] = torch.from_numpy(rng.uniform(0., s["L"], (n, 3))).to(dev)
+        q[off:off + n] = torch.from_numpy(np.where(np.arange(n) % 2 == 0, 1., -1.)).to(dev)
+        off += n
+    force = torch.zeros_like(pos)
+    b = P3MBatch(single_precision=True)
+    for s, t in zip(specs, tuned):
+        b.add_system(box_l=[s["L"]] * 3, n=s["n"], prefactor=1.0, mesh=t["mesh"],
+                     cao=t["cao"], alpha=t["alpha"], r_cut=t["r_cut"], accuracy=t["accuracy"])
+    resident = sum(b.system_bytes(i) for i in range(n_sys))
+    t_setup = time.perf_counter() - t0
+    print(f"[bench] serving: resident meshes {resident / 2**30:.2f} GiB, set up in "
+          f"{t_setup:.1f} s", flush=True)
+    srv = P3MServer(b, lanes=lanes, budget_bytes=budget_bytes, max_batch=max_batch,
+                    sigma=0.05)
+    stream_rng = np.random.default_rng(5999)
+    counts = np.array([s["n"] for s in specs])
+    # warm-up: every system once
+    srv.run(np.zeros(n_sys), np.arange(n_sys), np.arange(n_sys) + 10**9, pos, q, force)
+    points = []
+    seed_base = 0
+    for rate in rates:
+        n_req = max(50, int(rate * seconds))
+        arrival = np.cumsum(stream_rng.exponential(1. / rate, n_req))
+        ids = stream_rng.integers(0, n_sys, n_req)
+        seeds = seed_base + np.arange(n_req)
+        seed_base += n_req
+        r = srv.run(arrival, ids, seeds, pos, q, force)
+        lat = r["latency_s"] * 1e3
+        points.append({"offered_req_s": rate, "requests": n_req,
+                       "solves_per_s": round(n_req / (arrival[-1] + lat[-1] / 1e3), 1),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                       "mean_batch": round(n_req / r["batches"], 2),
+                       "peak_lanes": r["peak_lanes"],
+                       "peak_working_mib": round(r["peak_bytes"] / 2**20, 1),
+                       "mparticles_per_s": round(float(counts[ids].sum()) / (
+                           arrival[-1] + lat[-1] / 1e3) / 1e6, 1)})
+        print(f"[bench] serving {rate} req/s: p50 {points[-1]['p50_ms']} ms, p99 "
+              f"{points[-1]['p99_ms']} ms, batch {points[-1]['mean_batch']}", flush=True)
+    # saturation: a burst of requests all present at t = 0
+    n_b = 4 * n_sys
+    ids = stream_rng.integers(0, n_sys, n_b)
+    r = srv.run(np.zeros(n_b), ids, seed_base + np.arange(n_b), pos, q, force)
+    sat = {"requests": n_b, "wall_s": round(r["wall_s"], 4),
+           "solves_per_s": round(n_b / r["wall_s"], 1),
+           "mparticles_per_s": round(float(counts[ids].sum()) / r["wall_s"] / 1e6, 1),
+           "p50_ms": round(float(np.percentile(r["latency_s"], 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(r["latency_s"], 99)) * 1e3, 3),
+           "mean_batch": round(n_b / r["batches"], 2), "peak_lanes": r["peak_lanes"]}
+    print(f"[bench] serving saturation: {sat['solves_per_s']} solves/s, "
+          f"{sat['mparticles_per_s']} Mparticles/s", flush=True)
+    meshes = [t["mesh"][0] for t in tuned]
+    caos = [t["cao"] for t in tuned]
+    return {"workload": f"{n_sys} resident systems, N log-uniform [1000, 200000], density "
+                        "U[0.2, 0.8], +-1 alternating, tuned at 1e-4; Poisson arrivals, "
+                        "uniform system choice, Gaussian displacement sigma=0.05 per request",
+            "n_systems": n_sys, "resident_particles": n_tot,
+            "resident_mesh_gib": round(resident / 2**30, 3),
+            "mesh_range": [min(meshes), max(meshes)], "cao_range": [min(caos), max(caos)],
+            "lanes": lanes, "working_budget_mib": budget_bytes >> 20, "max_batch": max_batch,
+            "precision": "FP32 mesh, FP64 weights/reductions/forces",
+            "sweep": points, "saturation": sat, "setup_s": round(t_setup, 1)}
+
+
 def timed_loop(p3m, tp, tq, tf, steps, warmup, torch):
     for _ in range(warmup):
         p3m.add_long_range_forces(tp, tq, tf, sync=False)
@@ -230,6 +327,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 512-system batched workload")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the 2048-system serving workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference "
                          "dispatch for scaling HBM byte counters)")
@@ -327,6 +426,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev)
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(torch, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -376,6 +478,7 @@ def main():
         "host_pointer_ms": round(t_host * 1e3, 3),
         "fp32_mesh": fp32,
         "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
 

@@ -38,14 +38,17 @@ EXPORTED_SYMBOLS = (
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
-    "p3m_batch_particle_offset", "p3m_batch_system_bytes", "p3m_batch_solve",
+    "p3m_batch_particle_offset", "p3m_batch_system_bytes",
+    "p3m_batch_working_bytes", "p3m_batch_solve",
     "p3m_batch_graph_solve", "p3m_batch_set_launch_shape", "p3m_batch_autotune",
     "p3m_batch_synchronize", "p3m_batch_get_stream", "p3m_batch_profile_begin",
     "p3m_batch_profile_end", "p3m_batch_last_error", "p3m_tune_model",
+    "p3m_serve_create", "p3m_serve_destroy", "p3m_serve_run",
+    "p3m_serve_last_error",
 )
 
 _STRING_RESULT = ("p3m_last_error", "p3m_last_global_error",
-                  "p3m_batch_last_error")
+                  "p3m_batch_last_error", "p3m_serve_last_error")
 
 
 class p3m_params(C.Structure):
@@ -125,6 +128,7 @@ def load():
     L.p3m_batch_add_system.argtypes = [vp, C.POINTER(p3m_params), C.c_int64, ip]
     L.p3m_batch_particle_offset.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.p3m_batch_system_bytes.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    L.p3m_batch_working_bytes.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
     L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int,
@@ -140,6 +144,14 @@ def load():
     L.p3m_tune_model.argtypes = [C.POINTER(p3m_params), C.c_int, C.c_double,
                                  C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_double]
+    i64p = C.POINTER(C.c_int64)
+    L.p3m_serve_create.argtypes = [vp, C.c_int, C.c_int64, C.c_int, C.c_double,
+                                   C.POINTER(vp)]
+    L.p3m_serve_destroy.argtypes = [vp]
+    L.p3m_serve_run.argtypes = [vp, C.c_int, dp, ip, C.POINTER(C.c_uint64), vp,
+                                vp, vp, vp, i64p, dp, dp, ip, i64p, ip]
+    L.p3m_serve_last_error.argtypes = [vp]
+    L.p3m_serve_last_error.restype = C.c_char_p
     for name in EXPORTED_SYMBOLS:
         if name not in _STRING_RESULT:
             getattr(L, name).restype = C.c_int
@@ -156,12 +168,14 @@ def wait_for_torch(stream_handle):
             torch.cuda.current_stream())
 
 
-def raise_for(status, ctx=None, batch=None):
+def raise_for(status, ctx=None, batch=None, server=None):
     """Map a C status code to the Python exception the reference raises."""
     if status == P3M_OK:
         return
     L = load()
-    if batch:
+    if server:
+        msg = L.p3m_serve_last_error(server)
+    elif batch:
         msg = L.p3m_batch_last_error(batch)
     elif ctx:
         msg = L.p3m_last_error(ctx)

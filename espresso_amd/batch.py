@@ -107,6 +107,15 @@ class P3MBatch:
                                                        C.byref(out)))
         return out.value
 
+    def working_bytes_of(self, *systems):
+        """Device working memory of a solve of the given systems beyond
+        their resident data (the particle scratch)."""
+        nsel, ids = self._selection(systems)
+        out = C.c_int64(0)
+        self._check(_lib.load().p3m_batch_working_bytes(self._b, nsel, ids,
+                                                        C.byref(out)))
+        return out.value
+
     def _selection(self, systems):
         if systems is None:
             systems = range(self.num_systems)
@@ -237,3 +246,65 @@ def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
                                     float(pair_ns)))
     return {"mesh": [cp.mesh[d] for d in range(3)], "cao": cp.cao,
             "alpha": cp.alpha, "r_cut": cp.r_cut, "accuracy": cp.accuracy}
+
+
+class P3MServer:
+    """Continuous-batching request loop over the resident systems of a
+    ``P3MBatch``: requests are admitted into batches as lanes (device
+    streams) and working memory free up; the particle scratch of all batches
+    in flight comes from one device arena of ``budget_bytes``.  Each request
+    displaces its system's positions by ``sigma`` N(0, 1) (reproducible from
+    its seed), clears its forces and recomputes them."""
+
+    def __init__(self, batch, lanes=4, budget_bytes=256 << 20, max_batch=64,
+                 sigma=0.05):
+        L = _lib.load()
+        self._batch = batch  # keeps the batch alive
+        self._s = C.c_void_p()
+        _lib.raise_for(L.p3m_serve_create(batch._b, int(lanes), int(budget_bytes),
+                                          int(max_batch), float(sigma),
+                                          C.byref(self._s)), batch=batch._b)
+
+    def run(self, arrival_s, sys_ids, seeds, pos, q, force, capture=None):
+        """Serve the requests (sorted by arrival, seconds after the call) to
+        completion.  ``capture`` (device float64 tensor of shape
+        (sum of the requests' particle counts, 3)) receives each request's
+        forces in request order.  Returns a dict with per-request latencies
+        (s), wall time, number of batches, peak working memory and the most
+        batches in flight at once."""
+        b = self._batch
+        arrival = np.ascontiguousarray(arrival_s, dtype=np.float64)
+        ids = np.ascontiguousarray(sys_ids, dtype=np.int32)
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        n = len(arrival)
+        if len(ids) != n or len(sd) != n:
+            raise ValueError("arrival_s, sys_ids and seeds must have the same length")
+        p, qq, f = b._arrays(pos, q, force)
+        out_ptr, off = None, None
+        if capture is not None:
+            counts = np.array([b.particle_count(int(s)) for s in ids], dtype=np.int64)
+            off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(counts)[:-1]])
+                                       .astype(np.int64))
+            out_ptr = _device_ptr(capture, "capture", 8, 3 * int(counts.sum()))
+        lat = np.zeros(n)
+        wall = C.c_double()
+        nb, pl = C.c_int(), C.c_int()
+        peak = C.c_int64()
+        _lib.raise_for(_lib.load().p3m_serve_run(
+            self._s, n, arrival.ctypes.data_as(C.POINTER(C.c_double)),
+            ids.ctypes.data_as(C.POINTER(C.c_int)),
+            sd.ctypes.data_as(C.POINTER(C.c_uint64)), p, qq, f, out_ptr,
+            None if off is None else off.ctypes.data_as(C.POINTER(C.c_int64)),
+            lat.ctypes.data_as(C.POINTER(C.c_double)), C.byref(wall),
+            C.byref(nb), C.byref(peak), C.byref(pl)), server=self._s)
+        return {"latency_s": lat, "wall_s": wall.value, "batches": nb.value,
+                "peak_bytes": peak.value, "peak_lanes": pl.value}
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s:
+            try:
+                _lib.load().p3m_serve_destroy(s)
+            except Exception:  # pragma: no cover
+                pass
+            self._s = None

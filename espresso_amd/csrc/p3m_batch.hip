@@ -162,16 +162,17 @@ __global__ void k_zero_counts(SysDesc const *__restrict__ sys, int const *__rest
 __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
                                                 int const *__restrict__ sel,
                                                 int const *__restrict__ wt, int nsel,
-                                                ParticleArrays pa, int *counts, int *p_brick,
-                                                int *p_rank, int *p_nmp, double *partials,
-                                                int *err_flag) {
+                                                int const *__restrict__ soff, ParticleArrays pa,
+                                                int *counts, int *p_brick, int *p_rank,
+                                                int *p_nmp, double *partials, int *err_flag) {
   __shared__ double red[8 * 16];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
   double v[7] = {0., 0., 0., 0., 0., 0., 0.};
   if (li < d.n) {
-    int64_t const i = d.part_off + li;
+    int64_t const i = d.part_off + li;     // particle arrays
+    int64_t const si = soff[L.s] + li;     // lane scratch
     double const qi = pa.q[i];
     int b = -1;
     if (qi != 0.) {
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
       double dist[3];
       if (!particle_coordinates_rt(d.g, d.cao, x, nmp, gi, dist)) atomicOr(err_flag, 1);
       b = ((gi[0] / kBrick) * d.g.nb[1] + gi[1] / kBrick) * d.g.nb[2] + gi[2] / kBrick;
-      p_rank[i] = atomicAdd(&counts[d.brick_off + b], 1);
+      p_rank[si] = atomicAdd(&counts[d.brick_off + b], 1);
       if (p_nmp) {
         p_nmp[3 * li] = nmp[0];
         p_nmp[3 * li + 1] = nmp[1];
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
     } else if (p_nmp) {
       p_nmp[3 * li] = p_nmp[3 * li + 1] = p_nmp[3 * li + 2] = -1;
     }
-    p_brick[i] = b;
+    p_brick[si] = b;
   }
   block_sum<7>(v, red);
   if (threadIdx.x == 0)
@@ -265,6 +266,7 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
 __global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ sys,
                                                     int const *__restrict__ sel,
                                                     int const *__restrict__ wt, int nsel,
+                                                    int const *__restrict__ soff,
                                                     ParticleArrays pa, int const *p_brick,
                                                     int const *p_rank, int const *offsets,
                                                     double4 *spos, int *sidx) {
@@ -273,9 +275,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ 
   int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
   if (li >= d.n) return;
   int64_t const i = d.part_off + li;
-  int const b = p_brick[i];
+  int64_t const si = soff[L.s] + li;
+  int const b = p_brick[si];
   if (b < 0) return;
-  int64_t const slot = d.part_off + offsets[d.off_off + b] + p_rank[i];
+  int64_t const slot = soff[L.s] + offsets[d.off_off + b] + p_rank[si];
   spos[slot] = double4{pa.pos[3 * i], pa.pos[3 * i + 1], pa.pos[3 * i + 2], pa.q[i]};
   sidx[slot] = static_cast<int>(li);
 }
@@ -362,6 +365,7 @@ template <int P, typename MT>
 __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
                                                 int const *__restrict__ sel,
                                                 int const *__restrict__ wt, int nsel,
+                                                int const *__restrict__ soff,
                                                 int const *__restrict__ offsets,
                                                 double4 const *__restrict__ spos) {
   constexpr int T = kBrick + P - 1;
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   if (beg == end) return;
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
-  double4 const *sp = spos + d.part_off;
+  double4 const *sp = spos + soff[L.s];
   LaneSeg<P> const seg;
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
@@ -893,7 +897,8 @@ __device__ __forceinline__ void apply_force(double *force, int64_t idx, double q
 template <int P, typename MT>
 __global__ __launch_bounds__(kPG) void k_gather_plane(
     SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
-    int nsel, int const *__restrict__ offsets, double4 const *__restrict__ spos,
+    int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
+    double4 const *__restrict__ spos,
     int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
@@ -918,8 +923,8 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   double dm[3] = {0., 0., 0.};
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
-  double4 const *sp = spos + d.part_off;
-  int const *si = sidx + d.part_off;
+  double4 const *sp = spos + soff[L.s];
+  int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   LaneSeg<P> const seg;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
@@ -971,7 +976,8 @@ template <int S> __device__ __forceinline__ double small_segment_total(double v)
 template <int P, typename MT>
 __global__ __launch_bounds__(kPG) void k_gather_layer(
     SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
-    int nsel, int const *__restrict__ offsets, double4 const *__restrict__ spos,
+    int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
+    double4 const *__restrict__ spos,
     int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
@@ -998,8 +1004,8 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   double dm[3] = {0., 0., 0.};
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
-  double4 const *sp = spos + d.part_off;
-  int const *si = sidx + d.part_off;
+  double4 const *sp = spos + soff[L.s];
+  int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   int const lane = threadIdx.x & 63;
   int const wave = threadIdx.x >> 6;
@@ -1046,6 +1052,54 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
         apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// request preparation for served solves: positions of the selected systems
+// get a Gaussian displacement (x += sigma N(0,1), folded into the box) and
+// their forces are cleared.  Normal draws are Box-Muller over a counter-based
+// splitmix64 hash of (request seed, particle, component), so a request is
+// reproducible whatever else runs with it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double unit_open(unsigned long long h) {
+  return (static_cast<double>(h >> 11) + 0.5) * (1.0 / 9007199254740992.0); // (0, 1)
+}
+
+__global__ __launch_bounds__(kBlock) void k_prepare(SysDesc const *__restrict__ sys,
+                                                    int const *__restrict__ sel,
+                                                    int const *__restrict__ wt, int nsel,
+                                                    double *pos, double *force, double sigma,
+                                                    unsigned long long const *seeds) {
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
+  if (li >= d.n) return;
+  int64_t const i = d.part_off + li;
+  if (sigma > 0.) {
+    unsigned long long const seed = seeds[L.slot];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      unsigned long long const key = seed ^ (static_cast<unsigned long long>(3 * li + c) *
+                                             0xD1B54A32D192ED03ull);
+      double const u1 = unit_open(splitmix64(key));
+      double const u2 = unit_open(splitmix64(key ^ 0x8CB92BA72F3D8DD7ull));
+      double const z = sqrt(-2. * log(u1)) * cospi(2. * u2);
+      double const L0 = d.g.box_l[c];
+      double x = pos[3 * i + c] + sigma * z;
+      x -= L0 * floor(x / L0);
+      if (x >= L0) x -= L0;
+      pos[3 * i + c] = x;
+    }
+  }
+  force[3 * i] = 0.;
+  force[3 * i + 1] = 0.;
+  force[3 * i + 2] = 0.;
 }
 
 template <typename F> void dispatch_cao(int cao, F &&f) {
@@ -1132,11 +1186,13 @@ struct HostSys {
 struct Plan {
   std::vector<int> sel;
   int nsel = 0;
-  int *dbuf = nullptr; // device copy of all tables below
+  int *dbuf = nullptr; // device copy of all tables below (owned by the lane)
   int *d_sel = nullptr, *wt_bin = nullptr, *wt_fill = nullptr, *wt_z = nullptr, *wt_y = nullptr,
       *wt_x = nullptr, *wt_yi = nullptr, *wt_f = nullptr;
   int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_y = 0, blocks_x = 0, blocks_yi = 0,
       blocks_f = 0;
+  int *soff = nullptr; // per system id: first particle of the system in the lane scratch
+  int64_t npart = 0;   // particles of the selection
   struct Group {
     int cao;
     int nsel;
@@ -1154,6 +1210,37 @@ struct Plan {
   int64_t version = -1;
 };
 
+/** Working state of one stream: selection plan, particle scratch sized to
+ *  the selection, partial sums and events.  Systems (meshes, G,
+ *  descriptors) are shared by all lanes; a system is never in two lanes'
+ *  in-flight solves at once (the caller's contract). */
+struct Lane {
+  hipStream_t stream = nullptr;
+  int *counts = nullptr, *offsets = nullptr;
+  int counts_cap = 0, offsets_cap = 0;
+  // particle scratch of the current solve: carved from the lane's own block
+  // or from memory the caller passes in
+  int *p_brick = nullptr, *p_rank = nullptr, *sidx = nullptr;
+  double4 *spos = nullptr;
+  void *own = nullptr;
+  size_t own_bytes = 0;
+  double *bin_part = nullptr, *conv_part = nullptr;
+  int bin_cap = 0, conv_cap = 0;
+  double *sums = nullptr;
+  int sums_cap = 0;
+  int *err_flag = nullptr;
+  double *h_sums = nullptr;
+  unsigned long long *seeds = nullptr, *h_seeds = nullptr;
+  int seeds_cap = 0;
+  int *tables = nullptr, *h_tables = nullptr; // plan tables (device, pinned staging)
+  size_t tables_cap = 0;
+  hipEvent_t ev[8] = {};
+  std::vector<hipEvent_t> ring;
+  int ring_used = 0;
+  bool profiling = false;
+  Plan plan;
+};
+
 struct BatchSolver::Impl {
   std::vector<HostSys> sys;
   SysDesc *d_sys = nullptr;
@@ -1164,50 +1251,67 @@ struct BatchSolver::Impl {
   std::map<int, int> tw_offset; // size -> offset (complex elements)
   void *d_tw = nullptr;
   size_t tw_len = 0;
-  // brick counters / offsets / particle scratch
-  int *counts = nullptr, *offsets = nullptr;
-  int counts_cap = 0, offsets_cap = 0;
-  int64_t cap_n = 0;
-  int *p_brick = nullptr, *p_rank = nullptr, *sidx = nullptr;
-  double4 *spos = nullptr;
-  double *bin_part = nullptr, *conv_part = nullptr;
-  int bin_cap = 0, conv_cap = 0;
-  double *sums = nullptr;
-  int sums_cap = 0;
-  int *err_flag = nullptr;
-  double *h_sums = nullptr;
-  int h_sums_cap = 0;
-  hipEvent_t ev[8] = {};
-  std::vector<hipEvent_t> ring;
-  int ring_used = 0;
-  bool profiling = false;
-  Plan plan;
+  std::vector<std::unique_ptr<Lane>> lanes;
   bool kernels_ready = false;
 };
-
-BatchSolver::BatchSolver(int device, MeshPrecision precision)
-    : m(new Impl), m_device(device), m_prec(precision) {
-  hip_check(hipSetDevice(device), "hipSetDevice");
-  hip_check(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking), "hipStreamCreate");
-  for (auto &e : m->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
-  hip_check(hipMalloc(&m->err_flag, sizeof(int)), "hipMalloc err");
-  m_shape.gather_plane = (precision == MeshPrecision::FP64);
-  m_shape.ncol = 8;
-}
 
 namespace {
 void free_plan(Plan &p) {
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
-  p.graph = nullptr;
-  dfree(p.dbuf);
   p = Plan{};
+}
+
+void create_lane(Lane &ln) {
+  hip_check(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto &e : ln.ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipMalloc(&ln.err_flag, sizeof(int)), "hipMalloc err");
+}
+
+void destroy_lane(Lane &ln) {
+  if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+  free_plan(ln.plan);
+  dfree(ln.counts);
+  dfree(ln.offsets);
+  if (ln.own) (void)hipFree(ln.own);
+  ln.own = nullptr;
+  dfree(ln.bin_part);
+  dfree(ln.conv_part);
+  dfree(ln.sums);
+  dfree(ln.err_flag);
+  dfree(ln.seeds);
+  dfree(ln.tables);
+  if (ln.h_tables) (void)hipHostFree(ln.h_tables);
+  if (ln.h_sums) (void)hipHostFree(ln.h_sums);
+  if (ln.h_seeds) (void)hipHostFree(ln.h_seeds);
+  for (auto &e : ln.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : ln.ring) (void)hipEventDestroy(e);
+  if (ln.stream) (void)hipStreamDestroy(ln.stream);
+  ln.stream = nullptr;
+}
+
+/** Before shared state changes: wait for every lane and drop their plans. */
+void quiesce(BatchSolver::Impl *m) {
+  for (auto &ln : m->lanes) {
+    hip_check(hipStreamSynchronize(ln->stream), "sync lane");
+    free_plan(ln->plan);
+  }
 }
 } // namespace
 
+BatchSolver::BatchSolver(int device, MeshPrecision precision)
+    : m(new Impl), m_device(device), m_prec(precision) {
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  m->lanes.push_back(std::make_unique<Lane>());
+  create_lane(*m->lanes[0]);
+  m_stream = m->lanes[0]->stream;
+  m_shape.gather_plane = (precision == MeshPrecision::FP64);
+  m_shape.ncol = 8;
+}
+
 BatchSolver::~BatchSolver() {
   (void)hipSetDevice(m_device);
-  if (m_stream) (void)hipStreamSynchronize(m_stream);
-  free_plan(m->plan);
+  for (auto &ln : m->lanes) destroy_lane(*ln);
   for (auto &h : m->sys) {
     void *ptrs[] = {h.desc.rho, h.desc.Ey, h.desc.Ez, h.desc.khat, h.desc.A, h.desc.B, h.desc.G};
     for (void *p : ptrs)
@@ -1215,21 +1319,6 @@ BatchSolver::~BatchSolver() {
   }
   dfree(m->d_sys);
   dfree(m->d_tw);
-  dfree(m->counts);
-  dfree(m->offsets);
-  dfree(m->p_brick);
-  dfree(m->p_rank);
-  dfree(m->sidx);
-  dfree(m->spos);
-  dfree(m->bin_part);
-  dfree(m->conv_part);
-  dfree(m->sums);
-  dfree(m->err_flag);
-  if (m->h_sums) (void)hipHostFree(m->h_sums);
-  for (auto &e : m->ev)
-    if (e) (void)hipEventDestroy(e);
-  for (auto &e : m->ring) (void)hipEventDestroy(e);
-  if (m_stream) (void)hipStreamDestroy(m_stream);
   delete m;
 }
 
@@ -1242,6 +1331,43 @@ int64_t BatchSolver::total_particles() const {
 size_t BatchSolver::system_bytes(int s) const { return m->sys.at(s).bytes; }
 LocalMesh const &BatchSolver::local_mesh(int s) const { return m->sys.at(s).lm; }
 
+int BatchSolver::add_lane() {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  auto ln = std::make_unique<Lane>();
+  create_lane(*ln);
+  m->lanes.push_back(std::move(ln));
+  return static_cast<int>(m->lanes.size()) - 1;
+}
+int BatchSolver::num_lanes() const { return static_cast<int>(m->lanes.size()); }
+hipStream_t BatchSolver::lane_stream(int lane) const { return m->lanes.at(lane)->stream; }
+
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+/** Particle scratch of a solve over n particles: sorted (x, y, z, q) as
+ *  double4, brick id, rank in brick and original index. */
+size_t scratch_bytes(int64_t n) {
+  size_t const u = static_cast<size_t>(std::max<int64_t>(n, 1));
+  return align256(u * sizeof(double4)) + 3 * align256(u * sizeof(int));
+}
+void carve(Lane &ln, void *base, int64_t n) {
+  size_t const u = static_cast<size_t>(std::max<int64_t>(n, 1));
+  char *p = static_cast<char *>(base);
+  ln.spos = reinterpret_cast<double4 *>(p);
+  p += align256(u * sizeof(double4));
+  ln.p_brick = reinterpret_cast<int *>(p);
+  p += align256(u * sizeof(int));
+  ln.p_rank = reinterpret_cast<int *>(p);
+  p += align256(u * sizeof(int));
+  ln.sidx = reinterpret_cast<int *>(p);
+}
+} // namespace
+
+size_t BatchSolver::working_bytes(std::vector<int> const &sel) const {
+  int64_t n = 0;
+  for (int s : sel) n += m->sys.at(s).n;
+  return scratch_bytes(n);
+}
+
 namespace {
 void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st);
 } // namespace
@@ -1252,7 +1378,7 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
   m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
   hip_check(hipSetDevice(m_device), "hipSetDevice");
-  hip_check(hipStreamSynchronize(m_stream), "sync");
+  quiesce(m);
   // systems whose y/x pass changes switch spectrum layout: recompute G
   bool changed = false;
   for (auto &h : m->sys) {
@@ -1267,7 +1393,6 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
     hip_check(hipStreamSynchronize(m_stream), "sync influence");
     m->descs_dirty = true;
   }
-  free_plan(m->plan);
 }
 
 AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays const &pa,
@@ -1409,6 +1534,7 @@ void recompute_offsets(BatchSolver::Impl *m) {
 int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, double skin,
                             double prefactor, int64_t n_particles) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
+  quiesce(m);
   if (!m->kernels_ready) {
     if (m_prec == MeshPrecision::FP64)
       set_kernel_attrs<double>();
@@ -1485,16 +1611,15 @@ int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, doub
   // twiddle offsets of all systems (the table may have been reallocated)
   for (auto &hs : m->sys)
     for (int k = 0; k < 3; ++k) hs.desc.tw_off[k] = m->tw_offset.at(hs.desc.g.M[k]);
-  free_plan(m->plan);
   return static_cast<int>(m->sys.size()) - 1;
 }
 
 void BatchSolver::set_particle_count(int s, int64_t n) {
   if (m->sys.at(s).n == n) return;
-  hip_check(hipStreamSynchronize(m_stream), "sync");
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  quiesce(m);
   m->sys.at(s).n = n;
   recompute_offsets(m);
-  free_plan(m->plan);
 }
 
 namespace {
@@ -1514,53 +1639,57 @@ void upload_descs(BatchSolver::Impl *m, hipStream_t st) {
   m->descs_dirty = false;
 }
 
-void ensure_scratch(BatchSolver::Impl *m, hipStream_t st) {
-  int64_t const ntot = m->sys.empty() ? 0 : m->sys.back().part_off + m->sys.back().n;
+/** Particle scratch for `npart` particles of a selection, brick counters of
+ *  all systems, per-system sums; grows only, after draining the lane. */
+void ensure_scratch(BatchSolver::Impl *m, Lane &ln, int64_t npart, void *external,
+                    size_t external_bytes) {
   int nb_need = 0;
   for (auto const &h : m->sys) nb_need += h.nbricks;
-  bool const grow = ntot > m->cap_n || nb_need > m->counts_cap ||
-                    nb_need + static_cast<int>(m->sys.size()) > m->offsets_cap ||
-                    static_cast<int>(m->sys.size()) * 16 > m->sums_cap;
-  if (grow) hip_check(hipStreamSynchronize(st), "sync before realloc"); // queued work may use them
-  if (ntot > m->cap_n) {
-    dfree(m->p_brick);
-    dfree(m->p_rank);
-    dfree(m->sidx);
-    dfree(m->spos);
-    int64_t const cap = std::max<int64_t>(ntot + ntot / 4, 1024);
-    hip_check(hipMalloc(&m->p_brick, sizeof(int) * cap), "hipMalloc p_brick");
-    hip_check(hipMalloc(&m->p_rank, sizeof(int) * cap), "hipMalloc p_rank");
-    hip_check(hipMalloc(&m->sidx, sizeof(int) * cap), "hipMalloc sidx");
-    hip_check(hipMalloc(&m->spos, sizeof(double4) * cap), "hipMalloc spos");
-    m->cap_n = cap;
+  int const nsys = static_cast<int>(m->sys.size());
+  size_t const need = scratch_bytes(npart);
+  bool const grow_own = !external && need > ln.own_bytes;
+  bool const grow = grow_own || nb_need > ln.counts_cap || nb_need + nsys > ln.offsets_cap ||
+                    nsys * 16 > ln.sums_cap;
+  if (grow) hip_check(hipStreamSynchronize(ln.stream), "sync before realloc"); // queued work
+  if (external) {
+    if (external_bytes < need) throw std::invalid_argument("P3M: scratch block too small");
+    carve(ln, external, npart);
+  } else {
+    if (grow_own) {
+      if (ln.own) (void)hipFree(ln.own);
+      ln.own = nullptr;
+      ln.own_bytes = 0;
+      size_t const cap = scratch_bytes(std::max<int64_t>(npart + npart / 4, 1024));
+      hip_check(hipMalloc(&ln.own, cap), "hipMalloc particle scratch");
+      ln.own_bytes = cap;
+    }
+    carve(ln, ln.own, npart);
   }
-  int nb = 0;
-  for (auto const &h : m->sys) nb += h.nbricks;
-  if (nb > m->counts_cap) {
-    dfree(m->counts);
-    hip_check(hipMalloc(&m->counts, sizeof(int) * std::max(nb, 1)), "hipMalloc counts");
-    m->counts_cap = nb;
+  if (nb_need > ln.counts_cap) {
+    dfree(ln.counts);
+    hip_check(hipMalloc(&ln.counts, sizeof(int) * std::max(nb_need, 1)), "hipMalloc counts");
+    ln.counts_cap = nb_need;
   }
-  int const no = nb + static_cast<int>(m->sys.size());
-  if (no > m->offsets_cap) {
-    dfree(m->offsets);
-    hip_check(hipMalloc(&m->offsets, sizeof(int) * std::max(no, 1)), "hipMalloc offsets");
-    m->offsets_cap = no;
+  if (nb_need + nsys > ln.offsets_cap) {
+    dfree(ln.offsets);
+    hip_check(hipMalloc(&ln.offsets, sizeof(int) * std::max(nb_need + nsys, 1)),
+              "hipMalloc offsets");
+    ln.offsets_cap = nb_need + nsys;
   }
-  int const ns = static_cast<int>(m->sys.size()) * 16;
-  if (ns > m->sums_cap) {
-    dfree(m->sums);
-    hip_check(hipMalloc(&m->sums, sizeof(double) * std::max(ns, 16)), "hipMalloc sums");
-    if (m->h_sums) (void)hipHostFree(m->h_sums);
-    hip_check(hipHostMalloc(&m->h_sums, sizeof(double) * (std::max(ns, 16) + 8)),
+  if (nsys * 16 > ln.sums_cap) {
+    dfree(ln.sums);
+    hip_check(hipMalloc(&ln.sums, sizeof(double) * std::max(nsys * 16, 16)), "hipMalloc sums");
+    if (ln.h_sums) (void)hipHostFree(ln.h_sums);
+    hip_check(hipHostMalloc(&ln.h_sums, sizeof(double) * (std::max(nsys * 16, 16) + 8)),
               "hipHostMalloc sums");
-    m->sums_cap = ns;
+    ln.sums_cap = nsys * 16;
   }
 }
 
-void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision prec,
-                LaunchShape const &shape, hipStream_t st) {
-  Plan &p = m->plan;
+void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, MeshPrecision prec,
+                LaunchShape const &shape) {
+  Plan &p = ln.plan;
+  hipStream_t const st = ln.stream;
   if (p.version == m->version && p.sel == sel && p.dbuf) return;
   hip_check(hipStreamSynchronize(st), "sync before plan rebuild"); // queued work reads the tables
   free_plan(p);
@@ -1651,8 +1780,29 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
     g.blocks = acc;
     gh.push_back(g);
   }
-  hip_check(hipMalloc(&p.dbuf, sizeof(int) * std::max<size_t>(host.size(), 1)), "hipMalloc plan");
-  hip_check(hipMemcpy(p.dbuf, host.data(), sizeof(int) * host.size(), hipMemcpyHostToDevice),
+  // per-system offsets of the selection's particles in the lane scratch
+  int const soff_start = static_cast<int>(host.size());
+  host.resize(host.size() + m->sys.size(), 0);
+  int64_t npart = 0;
+  for (int s : sel) {
+    host[soff_start + s] = static_cast<int>(npart);
+    npart += m->sys[s].n;
+  }
+  if (npart > INT32_MAX) throw std::runtime_error("P3M: too many particles in one selection");
+  // the lane is idle (synchronised above): its table buffers can be reused
+  if (host.size() > ln.tables_cap) {
+    dfree(ln.tables);
+    if (ln.h_tables) (void)hipHostFree(ln.h_tables);
+    ln.h_tables = nullptr;
+    size_t const cap = host.size() + host.size() / 2 + 64;
+    hip_check(hipMalloc(&ln.tables, sizeof(int) * cap), "hipMalloc plan");
+    hip_check(hipHostMalloc(&ln.h_tables, sizeof(int) * cap), "hipHostMalloc plan");
+    ln.tables_cap = cap;
+  }
+  std::memcpy(ln.h_tables, host.data(), sizeof(int) * host.size());
+  p.dbuf = ln.tables;
+  hip_check(hipMemcpyAsync(p.dbuf, ln.h_tables, sizeof(int) * host.size(), hipMemcpyHostToDevice,
+                           st),
             "copy plan");
   p.d_sel = p.dbuf;
   p.wt_bin = p.dbuf + bin.first;
@@ -1669,51 +1819,56 @@ void build_plan(BatchSolver::Impl *m, std::vector<int> const &sel, MeshPrecision
   p.blocks_yi = fyi.second;
   p.wt_f = p.dbuf + ff.first;
   p.blocks_f = ff.second;
+  p.soff = p.dbuf + soff_start;
+  p.npart = npart;
   for (auto const &g : gh)
     p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks});
   p.version = m->version;
-  if (p.blocks_bin > m->bin_cap) {
-    dfree(m->bin_part); // the stream is idle (synchronised above)
-    hip_check(hipMalloc(&m->bin_part, sizeof(double) * 8 * std::max(p.blocks_bin, 1)),
+  if (p.blocks_bin > ln.bin_cap) {
+    dfree(ln.bin_part); // the stream is idle (synchronised above)
+    hip_check(hipMalloc(&ln.bin_part, sizeof(double) * 8 * std::max(p.blocks_bin, 1)),
               "hipMalloc bin partials");
-    m->bin_cap = p.blocks_bin;
+    ln.bin_cap = p.blocks_bin;
   }
   int const conv_blocks = p.blocks_x + p.blocks_f; // split partials, then fused
-  if (conv_blocks > m->conv_cap) {
-    dfree(m->conv_part);
-    hip_check(hipMalloc(&m->conv_part, sizeof(double) * 8 * std::max(conv_blocks, 1)),
+  if (conv_blocks > ln.conv_cap) {
+    dfree(ln.conv_part);
+    hip_check(hipMalloc(&ln.conv_part, sizeof(double) * 8 * std::max(conv_blocks, 1)),
               "hipMalloc conv partials");
-    m->conv_cap = conv_blocks;
+    ln.conv_cap = conv_blocks;
   }
 }
 
 template <typename MT>
-void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, unsigned flags,
+void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned flags,
              LaunchShape const &shape, hipEvent_t *ev, int *p_nmp, bool stop_after_spread) {
   using C = cx_t<MT>;
-  Plan &p = m->plan;
+  Plan &p = ln.plan;
+  hipStream_t const st = ln.stream;
   bool const want_force = (flags & 1u) && pa.force;
   int const ns = p.nsel;
   C const *tw = static_cast<C const *>(m->d_tw);
   SysDesc const *sys = m->d_sys;
   if (ev) (void)hipEventRecord(ev[0], st);
-  k_zero_counts<<<ns, kBlock, 0, st>>>(sys, p.d_sel, ns, m->counts);
+  k_zero_counts<<<ns, kBlock, 0, st>>>(sys, p.d_sel, ns, ln.counts);
   if (p.blocks_bin > 0) {
-    k_bin<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->counts, m->p_brick,
-                                            m->p_rank, p_nmp, m->bin_part, m->err_flag);
+    k_bin<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa, ln.counts,
+                                            ln.p_brick, ln.p_rank, p_nmp, ln.bin_part,
+                                            ln.err_flag);
   }
-  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, m->bin_part, 7, 0, m->sums, 0);
-  k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, m->counts, m->offsets);
+  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, ln.bin_part, 7, 0, ln.sums, 0);
+  k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, ln.counts, ln.offsets);
   if (p.blocks_bin > 0)
-    k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, pa, m->p_brick,
-                                                m->p_rank, m->offsets, m->spos, m->sidx);
+    k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa,
+                                                ln.p_brick, ln.p_rank, ln.offsets, ln.spos,
+                                                ln.sidx);
   k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
   if (ev) (void)hipEventRecord(ev[1], st);
   for (auto const &g : p.groups) {
     dispatch_cao(g.cao, [&](auto P) {
       size_t const lds = chunk_bytes(P.value) + tile_points(P.value) * sizeof(double);
-      k_spread<P.value, MT><<<g.blocks, kPG, lds, st>>>(sys, g.d_sel, g.wt, g.nsel, m->offsets,
-                                                          m->spos);
+      k_spread<P.value, MT><<<g.blocks, kPG, lds, st>>>(sys, g.d_sel, g.wt, g.nsel, p.soff,
+                                                          ln.offsets, ln.spos);
     });
   }
   if (ev) (void)hipEventRecord(ev[2], st);
@@ -1724,14 +1879,14 @@ void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, uns
   if (ev) (void)hipEventRecord(ev[3], st);
   if (p.blocks_x > 0) {
     k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
-                                                          want_force ? 1 : 0, m->conv_part);
-    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, m->conv_part, 8, 8, m->sums, 1);
+                                                          want_force ? 1 : 0, ln.conv_part);
+    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, ln.conv_part, 8, 8, ln.sums, 1);
   }
   if (p.blocks_f > 0) {
-    double *fpart = m->conv_part + 8 * static_cast<size_t>(p.blocks_x);
+    double *fpart = ln.conv_part + 8 * static_cast<size_t>(p.blocks_x);
     k_fft_yx_conv<MT><<<p.blocks_f, kBlock, p.lds_f, st>>>(sys, p.d_sel, p.wt_f, ns, tw,
                                                            want_force ? 1 : 0, fpart);
-    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, m->sums, 1);
+    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, ln.sums, 1);
   }
   if (ev) (void)hipEventRecord(ev[4], st);
   if (want_force) {
@@ -1748,23 +1903,25 @@ void enqueue(BatchSolver::Impl *m, hipStream_t st, ParticleArrays const &pa, uns
         size_t const lds = chunk_bytes(P.value) + 3 * tile_points(P.value) * sizeof(MT);
         if (shape.gather_plane)
           k_gather_plane<P.value, MT><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, m->offsets, m->spos, m->sidx, pa.force, m->sums);
+              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
+              ln.sums);
         else
           k_gather_layer<P.value, MT><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, m->offsets, m->spos, m->sidx, pa.force, m->sums);
+              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
+              ln.sums);
       });
     }
   }
   if (ev) (void)hipEventRecord(ev[6], st);
 }
 
-void finalize_results(BatchSolver::Impl *m, std::vector<int> const &sel,
+void finalize_results(BatchSolver::Impl *m, Lane const &ln, std::vector<int> const &sel,
                       std::vector<SolveResult> *out) {
   out->assign(sel.size(), SolveResult{});
   for (size_t i = 0; i < sel.size(); ++i) {
     int const s = sel[i];
     HostSys const &hs = m->sys[s];
-    double const *h = m->h_sums + 16 * s;
+    double const *h = ln.h_sums + 16 * s;
     SolveResult &r = (*out)[i];
     r.n_q = h[0];
     r.sum_q = h[1];
@@ -1792,6 +1949,12 @@ void finalize_results(BatchSolver::Impl *m, std::vector<int> const &sel,
 
 void BatchSolver::solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags,
                         bool sync, std::vector<SolveResult> *out) {
+  solve_on(0, sel, pa, flags, sync, out);
+}
+
+void BatchSolver::solve_on(int lane, std::vector<int> const &sel, ParticleArrays const &pa,
+                           unsigned flags, bool sync, std::vector<SolveResult> *out,
+                           void *scratch, size_t scratch_size) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   if (sel.empty()) {
     if (out) out->clear();
@@ -1799,29 +1962,30 @@ void BatchSolver::solve(std::vector<int> const &sel, ParticleArrays const &pa, u
   }
   for (int s : sel)
     if (s < 0 || s >= num_systems()) throw std::invalid_argument("P3M: invalid system id");
-  ensure_scratch(m, m_stream);
-  upload_descs(m, m_stream);
-  build_plan(m, sel, m_prec, m_shape, m_stream);
-  hipEvent_t *ev = m->ev;
-  if (m->profiling && static_cast<size_t>(m->ring_used + 1) * 8 <= m->ring.size())
-    ev = &m->ring[8 * static_cast<size_t>(m->ring_used++)];
-  k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+  Lane &ln = *m->lanes.at(lane);
+  upload_descs(m, ln.stream);
+  build_plan(m, ln, sel, m_prec, m_shape);
+  ensure_scratch(m, ln, ln.plan.npart, scratch, scratch_size);
+  hipEvent_t *ev = ln.ev;
+  if (ln.profiling && static_cast<size_t>(ln.ring_used + 1) * 8 <= ln.ring.size())
+    ev = &ln.ring[8 * static_cast<size_t>(ln.ring_used++)];
+  k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
   if (m_prec == MeshPrecision::FP64)
-    enqueue<double>(m, m_stream, pa, flags, m_shape, ev, nullptr, false);
+    enqueue<double>(m, ln, pa, flags, m_shape, ev, nullptr, false);
   else
-    enqueue<float>(m, m_stream, pa, flags, m_shape, ev, nullptr, false);
+    enqueue<float>(m, ln, pa, flags, m_shape, ev, nullptr, false);
   hip_check(hipGetLastError(), "P3M solve launch");
   if (!sync) return;
   int const ns = num_systems() * 16;
-  hip_check(hipMemcpyAsync(m->h_sums, m->sums, sizeof(double) * ns, hipMemcpyDeviceToHost,
-                           m_stream),
+  hip_check(hipMemcpyAsync(ln.h_sums, ln.sums, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                           ln.stream),
             "copy sums");
-  hip_check(hipMemcpyAsync(m->h_sums + ns, m->err_flag, sizeof(int), hipMemcpyDeviceToHost,
-                           m_stream),
+  hip_check(hipMemcpyAsync(ln.h_sums + ns, ln.err_flag, sizeof(int), hipMemcpyDeviceToHost,
+                           ln.stream),
             "copy err");
-  hip_check(hipStreamSynchronize(m_stream), "P3M solve");
+  hip_check(hipStreamSynchronize(ln.stream), "P3M solve");
   int err = 0;
-  std::memcpy(&err, m->h_sums + ns, sizeof(int));
+  std::memcpy(&err, ln.h_sums + ns, sizeof(int));
   if (err) throw std::runtime_error("P3M: non-finite particle position");
   float t = 0.f;
   (void)hipEventElapsedTime(&t, ev[0], ev[6]);
@@ -1830,17 +1994,49 @@ void BatchSolver::solve(std::vector<int> const &sel, ParticleArrays const &pa, u
     (void)hipEventElapsedTime(&t, ev[st - 1], ev[st]);
     last_ms[st] = t;
   }
-  if (out) finalize_results(m, sel, out);
+  if (out) finalize_results(m, ln, sel, out);
+}
+
+void BatchSolver::prepare_on(int lane, std::vector<int> const &sel, double *pos, double *force,
+                             double sigma, std::vector<unsigned long long> const &seeds) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  if (sel.empty()) return;
+  if (seeds.size() != sel.size()) throw std::invalid_argument("P3M: one seed per system");
+  Lane &ln = *m->lanes.at(lane);
+  upload_descs(m, ln.stream);
+  build_plan(m, ln, sel, m_prec, m_shape);
+  int const ns = static_cast<int>(sel.size());
+  if (ns > ln.seeds_cap) {
+    hip_check(hipStreamSynchronize(ln.stream), "sync before realloc");
+    dfree(ln.seeds);
+    if (ln.h_seeds) (void)hipHostFree(ln.h_seeds);
+    ln.h_seeds = nullptr;
+    hip_check(hipMalloc(&ln.seeds, sizeof(unsigned long long) * ns), "hipMalloc seeds");
+    hip_check(hipHostMalloc(&ln.h_seeds, sizeof(unsigned long long) * ns), "hipHostMalloc");
+    ln.seeds_cap = ns;
+  }
+  // the previous copy out of h_seeds has completed: the lane's earlier work
+  // was drained by the caller before it formed this batch
+  std::memcpy(ln.h_seeds, seeds.data(), sizeof(unsigned long long) * ns);
+  hip_check(hipMemcpyAsync(ln.seeds, ln.h_seeds, sizeof(unsigned long long) * ns,
+                           hipMemcpyHostToDevice, ln.stream),
+            "copy seeds");
+  Plan const &p = ln.plan;
+  if (p.blocks_bin > 0)
+    k_prepare<<<p.blocks_bin, kBlock, 0, ln.stream>>>(m->d_sys, p.d_sel, p.wt_bin, p.nsel, pos,
+                                                       force, sigma, ln.seeds);
+  hip_check(hipGetLastError(), "P3M prepare launch");
 }
 
 void BatchSolver::graph_solve(std::vector<int> const &sel, ParticleArrays const &pa,
                               unsigned flags) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   if (sel.empty()) return;
-  ensure_scratch(m, m_stream);
-  upload_descs(m, m_stream);
-  build_plan(m, sel, m_prec, m_shape, m_stream);
-  Plan &p = m->plan;
+  Lane &ln = *m->lanes[0];
+  upload_descs(m, ln.stream);
+  build_plan(m, ln, sel, m_prec, m_shape);
+  ensure_scratch(m, ln, ln.plan.npart, nullptr, 0);
+  Plan &p = ln.plan;
   bool const same = p.graph && p.graph_flags == flags && p.graph_pa.pos == pa.pos &&
                     p.graph_pa.q == pa.q && p.graph_pa.upos == pa.upos &&
                     p.graph_pa.force == pa.force;
@@ -1848,40 +2044,42 @@ void BatchSolver::graph_solve(std::vector<int> const &sel, ParticleArrays const 
     if (p.graph) (void)hipGraphExecDestroy(p.graph);
     p.graph = nullptr;
     hipGraph_t graph;
-    hip_check(hipStreamBeginCapture(m_stream, hipStreamCaptureModeRelaxed), "begin capture");
-    k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+    hip_check(hipStreamBeginCapture(ln.stream, hipStreamCaptureModeRelaxed), "begin capture");
+    k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
     if (m_prec == MeshPrecision::FP64)
-      enqueue<double>(m, m_stream, pa, flags, m_shape, nullptr, nullptr, false);
+      enqueue<double>(m, ln, pa, flags, m_shape, nullptr, nullptr, false);
     else
-      enqueue<float>(m, m_stream, pa, flags, m_shape, nullptr, nullptr, false);
-    hip_check(hipStreamEndCapture(m_stream, &graph), "end capture");
+      enqueue<float>(m, ln, pa, flags, m_shape, nullptr, nullptr, false);
+    hip_check(hipStreamEndCapture(ln.stream, &graph), "end capture");
     hip_check(hipGraphInstantiate(&p.graph, graph, nullptr, nullptr, 0), "graph instantiate");
     (void)hipGraphDestroy(graph);
     p.graph_pa = pa;
     p.graph_flags = flags;
   }
-  hip_check(hipGraphLaunch(p.graph, m_stream), "graph launch");
+  hip_check(hipGraphLaunch(p.graph, ln.stream), "graph launch");
 }
 
 void BatchSolver::profile_begin(int max_solves) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
-  hip_check(hipStreamSynchronize(m_stream), "sync");
+  Lane &ln = *m->lanes[0];
+  hip_check(hipStreamSynchronize(ln.stream), "sync");
   size_t const need = static_cast<size_t>(std::max(max_solves, 1)) * 8;
-  while (m->ring.size() < need) {
+  while (ln.ring.size() < need) {
     hipEvent_t e;
     hip_check(hipEventCreate(&e), "hipEventCreate");
-    m->ring.push_back(e);
+    ln.ring.push_back(e);
   }
-  m->ring_used = 0;
-  m->profiling = true;
+  ln.ring_used = 0;
+  ln.profiling = true;
 }
 
 int BatchSolver::profile_end(double *stage_ms) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
-  hip_check(hipStreamSynchronize(m_stream), "sync");
+  Lane &ln = *m->lanes[0];
+  hip_check(hipStreamSynchronize(ln.stream), "sync");
   for (int k = 0; k < 7; ++k) stage_ms[k] = 0.;
-  for (int i = 0; i < m->ring_used; ++i) {
-    hipEvent_t *ev = &m->ring[8 * static_cast<size_t>(i)];
+  for (int i = 0; i < ln.ring_used; ++i) {
+    hipEvent_t *ev = &ln.ring[8 * static_cast<size_t>(i)];
     float t = 0.f;
     (void)hipEventElapsedTime(&t, ev[0], ev[6]);
     stage_ms[0] += t;
@@ -1890,22 +2088,23 @@ int BatchSolver::profile_end(double *stage_ms) {
       stage_ms[s] += t;
     }
   }
-  m->profiling = false;
-  int const n = m->ring_used;
-  m->ring_used = 0;
+  ln.profiling = false;
+  int const n = ln.ring_used;
+  ln.ring_used = 0;
   return n;
 }
 
 void BatchSolver::debug_charge_mesh(int s, ParticleArrays const &pa, std::vector<double> &out) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
-  ensure_scratch(m, m_stream);
-  upload_descs(m, m_stream);
-  build_plan(m, {s}, m_prec, m_shape, m_stream);
-  k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+  Lane &ln = *m->lanes[0];
+  upload_descs(m, ln.stream);
+  build_plan(m, ln, {s}, m_prec, m_shape);
+  ensure_scratch(m, ln, ln.plan.npart, nullptr, 0);
+  k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
   if (m_prec == MeshPrecision::FP64)
-    enqueue<double>(m, m_stream, pa, 0, m_shape, m->ev, nullptr, true);
+    enqueue<double>(m, ln, pa, 0, m_shape, ln.ev, nullptr, true);
   else
-    enqueue<float>(m, m_stream, pa, 0, m_shape, m->ev, nullptr, true);
+    enqueue<float>(m, ln, pa, 0, m_shape, ln.ev, nullptr, true);
   hip_check(hipGetLastError(), "P3M spread launch");
   auto const &g = m->sys[s].desc.g;
   int64_t const R = (int64_t)g.M[0] * g.M[1] * g.M[2];
@@ -1927,18 +2126,19 @@ void BatchSolver::debug_charge_mesh(int s, ParticleArrays const &pa, std::vector
 
 void BatchSolver::debug_indices(int s, ParticleArrays const &pa, std::vector<int32_t> &out) {
   hip_check(hipSetDevice(m_device), "hipSetDevice");
-  ensure_scratch(m, m_stream);
-  upload_descs(m, m_stream);
-  build_plan(m, {s}, m_prec, m_shape, m_stream);
+  Lane &ln = *m->lanes[0];
+  upload_descs(m, ln.stream);
+  build_plan(m, ln, {s}, m_prec, m_shape);
+  ensure_scratch(m, ln, ln.plan.npart, nullptr, 0);
   int64_t const n = m->sys[s].n;
   int *d_nmp = nullptr;
   hip_check(hipMalloc(&d_nmp, sizeof(int) * 3 * std::max<int64_t>(n, 1)), "hipMalloc nmp");
   try {
-    k_fill_int<<<1, 64, 0, m_stream>>>(m->err_flag, 1);
+    k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
     if (m_prec == MeshPrecision::FP64)
-      enqueue<double>(m, m_stream, pa, 0, m_shape, m->ev, d_nmp, true);
+      enqueue<double>(m, ln, pa, 0, m_shape, ln.ev, d_nmp, true);
     else
-      enqueue<float>(m, m_stream, pa, 0, m_shape, m->ev, d_nmp, true);
+      enqueue<float>(m, ln, pa, 0, m_shape, ln.ev, d_nmp, true);
     hip_check(hipGetLastError(), "P3M bin launch");
     out.resize(3 * n);
     hip_check(hipMemcpyAsync(out.data(), d_nmp, sizeof(int) * 3 * n, hipMemcpyDeviceToHost,

@@ -95,6 +95,30 @@ public:
   void solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags, bool sync,
              std::vector<SolveResult> *out);
 
+  /** Lanes: independent streams with their own plans and particle scratch
+   *  over the shared resident systems, so that solves of disjoint selections
+   *  overlap on the device.  Lane 0 exists from the start and serves solve(),
+   *  graph_solve(), profiling and the debug calls. */
+  int add_lane();
+  int num_lanes() const;
+  hipStream_t lane_stream(int lane) const;
+  /** Enqueue a solve of `sel` on `lane` (no system of `sel` may be in another
+   *  lane's unfinished solve).  With `scratch`, the particle scratch of the
+   *  solve is taken from that device block of scratch_size >=
+   *  working_bytes(sel) bytes (it must stay untouched until the solve ends)
+   *  instead of the lane's own memory. */
+  void solve_on(int lane, std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags,
+                bool sync, std::vector<SolveResult> *out, void *scratch = nullptr,
+                size_t scratch_size = 0);
+  /** Enqueue on `lane`, for each system of `sel`: positions += sigma N(0,1)
+   *  from a counter-based generator keyed by seeds[i], folded into the box,
+   *  and forces zeroed.  The lane's previous work must be complete. */
+  void prepare_on(int lane, std::vector<int> const &sel, double *pos, double *force,
+                  double sigma, std::vector<unsigned long long> const &seeds);
+  /** Device working memory a solve of `sel` needs beyond the resident
+   *  systems (particle scratch). */
+  size_t working_bytes(std::vector<int> const &sel) const;
+
   /** Copy the charge mesh of system s after spreading ([Mx][My][Mz]). */
   void debug_charge_mesh(int s, ParticleArrays const &pa, std::vector<double> &out);
   /** CA-local first stencil point of every particle of system s. */

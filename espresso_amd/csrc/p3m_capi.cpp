@@ -8,6 +8,7 @@
 
 #include "p3m_host.hpp"
 #include "p3m_batch.hpp"
+#include "p3m_serve.hpp"
 #include "p3m_solver.hpp"
 #include "p3m_tuning.hpp"
 
@@ -42,6 +43,12 @@ struct p3m_context {
 struct p3m_batch {
   int device = 0;
   std::unique_ptr<BatchSolver> solver;
+  std::string last_error;
+};
+
+struct p3m_server {
+  p3m_batch *batch = nullptr;
+  std::unique_ptr<Server> server;
   std::string last_error;
 };
 
@@ -580,6 +587,14 @@ int p3m_batch_system_bytes(p3m_batch const *b, int sys_id, int64_t *bytes) {
   return P3M_OK;
 }
 
+int p3m_batch_working_bytes(p3m_batch const *b, int nsel, int const *sys_ids, int64_t *bytes) {
+  if (!b || !bytes) return P3M_ERR_INVALID;
+  return guarded(static_cast<std::string *>(nullptr), [&] {
+    auto const sel = selection(b, nsel, sys_ids);
+    *bytes = static_cast<int64_t>(b->solver->working_bytes(sel));
+  });
+}
+
 int p3m_batch_solve(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
                     double const *q, double *force, unsigned flags, double *energy,
                     double *virial) {
@@ -680,6 +695,58 @@ int p3m_batch_profile_end(p3m_batch *b, double *stage_ms, int *nsolves) {
 
 char const *p3m_batch_last_error(p3m_batch const *b) {
   return b ? b->last_error.c_str() : "null batch";
+}
+
+int p3m_serve_create(p3m_batch *b, int lanes, int64_t budget_bytes, int max_batch,
+                     double sigma, p3m_server **out) {
+  if (!b || !out) return P3M_ERR_INVALID;
+  *out = nullptr;
+  return guarded(&b->last_error, [&] {
+    if (budget_bytes <= 0) throw std::invalid_argument("P3M serve: budget must be > 0");
+    if (!(sigma >= 0.)) throw std::invalid_argument("P3M serve: sigma must be >= 0");
+    ServeConfig cfg;
+    cfg.lanes = lanes;
+    cfg.budget_bytes = static_cast<size_t>(budget_bytes);
+    cfg.max_batch = max_batch;
+    cfg.sigma = sigma;
+    auto s = std::make_unique<p3m_server>();
+    s->batch = b;
+    s->server = std::make_unique<Server>(*b->solver, cfg);
+    *out = s.release();
+  });
+}
+
+int p3m_serve_destroy(p3m_server *s) {
+  delete s;
+  return P3M_OK;
+}
+
+int p3m_serve_run(p3m_server *s, int n_req, double const *arrival_s, int const *sys_ids,
+                  uint64_t const *seeds, double *pos, double const *q, double *force,
+                  double *force_out, int64_t const *out_off, double *latency_s, double *wall_s,
+                  int *n_batches, int64_t *peak_bytes, int *peak_lanes) {
+  if (!s) return P3M_ERR_INVALID;
+  return guarded(&s->last_error, [&] {
+    if (n_req < 0 || (n_req > 0 && (!arrival_s || !sys_ids || !seeds)))
+      throw std::invalid_argument("P3M serve: request arrays required");
+    if (force_out && !out_off) throw std::invalid_argument("P3M serve: out_off required");
+    if (s->batch->solver->total_particles() > 0 && (!pos || !q || !force))
+      throw std::invalid_argument("P3M serve: particle arrays required");
+    hip_check(hipSetDevice(s->batch->device), "hipSetDevice");
+    std::vector<ServeRequest> reqs(static_cast<size_t>(n_req));
+    for (int r = 0; r < n_req; ++r) reqs[r] = ServeRequest{arrival_s[r], sys_ids[r], seeds[r]};
+    ServeStats const st = s->server->run(reqs, pos, q, force, force_out, out_off);
+    if (latency_s)
+      for (int r = 0; r < n_req; ++r) latency_s[r] = st.latency_s[r];
+    if (wall_s) *wall_s = st.wall_s;
+    if (n_batches) *n_batches = st.batches;
+    if (peak_bytes) *peak_bytes = static_cast<int64_t>(st.peak_bytes);
+    if (peak_lanes) *peak_lanes = st.peak_lanes;
+  });
+}
+
+char const *p3m_serve_last_error(p3m_server const *s) {
+  return s ? s->last_error.c_str() : "null server";
 }
 
 int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,

@@ -6,6 +6,10 @@
 #include <limits>
 #include <numbers>
 #include <sstream>
+#include <map>
+#include <mutex>
+#include <array>
+#include <vector>
 
 namespace espresso_amd {
 
@@ -151,6 +155,56 @@ double real_space_error(double pref, double r_cut_iL, int n_c_part, double sum_q
          std::sqrt(static_cast<double>(n_c_part) * r_cut_iL * box_l[0] * volume);
 }
 
+namespace {
+/** Weights of the k-space error sum grouped by k = |n|^2 (mesh index units).
+ *  With P3M_BRILLOUIN = 0 each term of p3m.cpp:187-243 is
+ *    d(n) = exp(-2 f k) / k * (1 - (U^2(n) / cs(n))^2),  f = (pi / alpha_L)^2,
+ *  and its inclusion test (d > 0, |d / alias1| > ROUND_ERROR_PREC) does not
+ *  depend on alpha.  So he_q(alpha) = sum_k w_k exp(-2 f k) with
+ *  w_k = sum over n with |n|^2 = k of (1 - (U^2 / cs)^2) / k: the weights are
+ *  computed once per (mesh, cao) and every trial of the tuner is O(k_max).
+ *  The summand is even in every component of n, so one octant is summed with
+ *  multiplicity 1 for n_d = 0 and n_d = M_d/2 and 2 otherwise. */
+std::vector<double> const &k_space_weights(Vec3i const &mesh, int cao) {
+  static std::mutex mutex;
+  static std::map<std::array<int, 4>, std::vector<double>> cache;
+  std::array<int, 4> const key{mesh[0], mesh[1], mesh[2], cao};
+  {
+    std::lock_guard<std::mutex> lock(mutex);
+    auto const it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  Vec3d const mesh_i{1. / mesh[0], 1. / mesh[1], 1. / mesh[2]};
+  Vec3i const stop{mesh[0] / 2, mesh[1] / 2, mesh[2] / 2};
+  std::array<std::vector<double>, 3> cs_tab, sinc_tab, mult;
+  for (int d = 0; d < 3; ++d) {
+    for (int n = 0; n <= stop[d]; ++n) {
+      cs_tab[d].push_back(analytic_cotangent_sum(cao, n, mesh_i[d]));
+      sinc_tab[d].push_back(sinc(n * mesh_i[d]));
+      mult[d].push_back((n == 0 or n == stop[d]) ? 1. : 2.);
+    }
+  }
+  int const kmax = stop[0] * stop[0] + stop[1] * stop[1] + stop[2] * stop[2];
+  std::vector<double> w(static_cast<size_t>(kmax) + 1, 0.);
+  for (int x = 0; x <= stop[0]; ++x)
+    for (int y = 0; y <= stop[1]; ++y) {
+      double const w_xy = mult[0][x] * mult[1][y];
+      double const cs_xy = cs_tab[0][x] * cs_tab[1][y];
+      double const f_xy = sinc_tab[0][x] * sinc_tab[1][y];
+      for (int z = 0; z <= stop[2]; ++z) {
+        if (x == 0 and y == 0 and z == 0) continue;
+        int const n2 = x * x + y * y + z * z;
+        double const ratio = std::pow(f_xy * sinc_tab[2][z], 2 * cao) / (cs_xy * cs_tab[2][z]);
+        double const rel = 1. - ratio * ratio; // d / alias1
+        if (rel > 0. and std::fabs(rel) > ROUND_ERROR_PREC)
+          w[n2] += w_xy * mult[2][z] * rel / n2;
+      }
+    }
+  std::lock_guard<std::mutex> lock(mutex);
+  return cache.emplace(key, std::move(w)).first->second;
+}
+} // namespace
+
 double k_space_error(double pref, Vec3i const &mesh, int cao, int n_c_part, double sum_q2,
                      double alpha_L, Vec3d const &box_l) {
   // p3m.cpp:187-243 with P3M_BRILLOUIN = 0: the aliasing sums reduce to the
@@ -191,6 +245,22 @@ double k_space_error(double pref, Vec3i const &mesh, int cao, int n_c_part, doub
          (box_l[1] * box_l[2]);
 }
 
+double k_space_error_fast(double pref, Vec3i const &mesh, int cao, int n_c_part,
+                          double sum_q2, double alpha_L, Vec3d const &box_l) {
+  // the terms of k_space_error summed by |n|^2 with the cached weights above
+  auto const &w = k_space_weights(mesh, cao);
+  double const alpha_L_i = 1. / alpha_L;
+  double const factor1 = (std::numbers::pi * alpha_L_i) * (std::numbers::pi * alpha_L_i);
+  double he_q = 0.;
+  for (size_t k = 1; k < w.size(); ++k) {
+    if (w[k] == 0.) continue;
+    double const ex = std::exp(-factor1 * static_cast<double>(k));
+    he_q += w[k] * (ex * ex);
+  }
+  return 2. * pref * sum_q2 * std::sqrt(he_q / static_cast<double>(n_c_part)) /
+         (box_l[1] * box_l[2]);
+}
+
 Accuracy calculate_accuracy(double pref, Vec3i const &mesh, int cao, double r_cut_iL,
                             int n_c_part, double sum_q2, double accuracy, Vec3d const &box_l) {
   double alpha_L;
@@ -200,7 +270,7 @@ Accuracy calculate_accuracy(double pref, Vec3i const &mesh, int cao, double r_cu
   else
     alpha_L = 0.1;
   rs_err = real_space_error(pref, r_cut_iL, n_c_part, sum_q2, alpha_L, box_l);
-  double const ks_err = k_space_error(pref, mesh, cao, n_c_part, sum_q2, alpha_L, box_l);
+  double const ks_err = k_space_error_fast(pref, mesh, cao, n_c_part, sum_q2, alpha_L, box_l);
   return {std::hypot(rs_err, ks_err), rs_err, ks_err, alpha_L};
 }
 

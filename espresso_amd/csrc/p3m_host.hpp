@@ -86,11 +86,18 @@ double analytic_cotangent_sum(int cao, int n, double mesh_i);
 /** Kolafa-Perram real-space error (p3m.cpp:145-160). */
 double real_space_error(double pref, double r_cut_iL, int n_c_part, double sum_q2,
                         double alpha_L, Vec3d const &box_l);
-/** Hockney-Eastwood k-space error (p3m.cpp:187-243). */
+/** Hockney-Eastwood k-space error (p3m.cpp:187-243), term by term. */
 double k_space_error(double pref, Vec3i const &mesh, int cao, int n_c_part, double sum_q2,
                      double alpha_L, Vec3d const &box_l);
+/** The same sum grouped by |n|^2 with per-(mesh, cao) weights cached across
+ *  calls: O(|n|^2_max) per call instead of O(M^3).  Agrees with
+ *  k_space_error to rounding, which reaches 1e-6 relative where the terms
+ *  cancel (small alpha_L, high cao); used by the tuner. */
+double k_space_error_fast(double pref, Vec3i const &mesh, int cao, int n_c_part,
+                          double sum_q2, double alpha_L, Vec3d const &box_l);
 
-/** Result of CoulombTuningAlgorithm::calculate_accuracy (p3m.cpp:636-674). */
+/** Result of CoulombTuningAlgorithm::calculate_accuracy (p3m.cpp:636-674),
+ *  with the k-space part from k_space_error_fast. */
 struct Accuracy {
   double total, rs, ks, alpha_L;
 };

@@ -180,6 +180,9 @@ int p3m_batch_destroy(p3m_batch *b);
 int p3m_batch_add_system(p3m_batch *b, const p3m_params *p, int64_t n, int *sys_id);
 int p3m_batch_particle_offset(const p3m_batch *b, int sys_id, int64_t *offset);
 int p3m_batch_system_bytes(const p3m_batch *b, int sys_id, int64_t *bytes);
+/* Device working memory (particle scratch) of a solve of the selection,
+ * beyond the resident systems. */
+int p3m_batch_working_bytes(const p3m_batch *b, int nsel, const int *sys_ids, int64_t *bytes);
 /* Solve the selected systems: flags P3M_FORCES / P3M_ENERGY / P3M_PRESSURE,
  * P3M_ASYNC (enqueue only).  energy[nsel] and virial[9*nsel] may be NULL. */
 int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
@@ -210,6 +213,32 @@ int p3m_batch_get_stream(const p3m_batch *b, void **stream);
 int p3m_batch_profile_begin(p3m_batch *b, int max_solves);
 int p3m_batch_profile_end(p3m_batch *b, double *stage_ms, int *nsolves);
 const char *p3m_batch_last_error(const p3m_batch *b);
+
+/* ------------------------------------------------------------------------
+ * Serving: continuous batching over the resident systems of a batch.  No
+ * counterpart in the reference.  Requests (arrival time, system, seed) are
+ * admitted into batches as lanes (streams) and working memory free up; the
+ * particle scratch of every batch in flight comes from one device arena of
+ * budget_bytes.  Per request the system's positions get a Gaussian
+ * displacement of width sigma (counter-based draws keyed by the seed, folded
+ * into the box), its forces are cleared and recomputed.
+ * ------------------------------------------------------------------------ */
+typedef struct p3m_server p3m_server;
+
+int p3m_serve_create(p3m_batch *b, int lanes, int64_t budget_bytes, int max_batch,
+                     double sigma, p3m_server **out);
+int p3m_serve_destroy(p3m_server *s);
+/* Serve n_req requests sorted by arrival_s (seconds after the call starts)
+ * to completion.  pos, q, force: device arrays of all systems of the batch.
+ * force_out (device, may be NULL): each request's forces are copied to
+ * force_out + 3 * out_off[r] when its batch completes.  Outputs: latency_s
+ * [n_req] (arrival to completion), wall_s, n_batches, peak_bytes (working
+ * memory in flight), peak_lanes (batches in flight at once). */
+int p3m_serve_run(p3m_server *s, int n_req, const double *arrival_s, const int *sys_ids,
+                  const uint64_t *seeds, double *pos, const double *q, double *force,
+                  double *force_out, const int64_t *out_off, double *latency_s, double *wall_s,
+                  int *n_batches, int64_t *peak_bytes, int *peak_lanes);
+const char *p3m_serve_last_error(const p3m_server *s);
 
 /* Parameter tuning without a device: the reference's tuning algorithm
  * (p3m.cpp:594-791, TuningAlgorithm.cpp:57-328) with a modelled cost in place

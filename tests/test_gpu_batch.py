@@ -225,6 +225,7 @@ def test_launch_shapes_and_autotune_keep_results():
                 assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (fused, plane, ncol, zl)
     with pytest.raises(ValueError, match="powers of two"):
         b.set_launch_shape(True, 3, 4)
+    b.set_launch_shape()  # autotuning starts from the default shape
     scratch = torch.zeros_like(pos)
     shape = b.autotune(pos, q, scratch, reps=1)
     assert shape["ncol"] in (2, 4, 8, 16) and shape["zlines"] in (4, 8, 16)
