@@ -301,6 +301,61 @@ def bench_serving(torch, dev, rates=(50, 100, 200, 500, 1000, 2000), seconds=1.5
             "sweep": points, "saturation": sat, "setup_s": round(t_setup, 1)}
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def total_force_error(p3m, tp, tq, idx, f_exact, torch):
+    """Absolute rms error of the total P3M force (device k-space + device
+    real-space pair forces) on the fixture subset against exact Ewald."""
+    f = torch.zeros_like(tp)
+    p3m.add_long_range_forces(tp, tq, f)
+    f_r = p3m.pair_forces_on(tp, tq, idx)
+    d = (f[torch.as_tensor(idx, device=tp.device)] + f_r).cpu().numpy() - f_exact
+    return float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+
+
+def accuracy_table_c3(torch, dev):
+    """Precision sweep: N=250,000 +-1 in L=79.37, mesh {48,64,96} x cao
+    {3,5,7} x accuracy {1e-3,1e-4,1e-5}; r_cut/alpha from the solver's tuning
+    at fixed mesh and cao; absolute rms force error of the FP64 solver and the
+    FP32-mesh form against exact Ewald on a 2000-particle subset."""
+    from espresso_amd.electrostatics import P3M
+    L = 79.37
+    rng = np.random.default_rng(1003)
+    pos = rng.uniform(0., L, (250_000, 3))
+    q = np.where(np.arange(250_000) % 2 == 0, 1., -1.)
+    fx = np.load(os.path.join(GOLDEN, "ewald_c3_subset.npz"))
+    tp, tq = torch.tensor(pos, device=dev), torch.tensor(q, device=dev)
+    rows = []
+    for mesh in (48, 64, 96):
+        for cao in (3, 5, 7):
+            for acc in (1e-3, 1e-4, 1e-5):
+                row = {"mesh": mesh, "cao": cao, "accuracy": acc}
+                try:
+                    p64 = P3M(prefactor=1.0, accuracy=acc, mesh=mesh, cao=cao, tune=True,
+                              timings=1, verbose=False)
+                    p64.activate(pos, q, box_l=[L] * 3)
+                except RuntimeError as e:
+                    row["infeasible"] = str(e)
+                    rows.append(row)
+                    continue
+                prm = p64.get_params()
+                p32 = P3M(prefactor=1.0, accuracy=acc, mesh=mesh, cao=cao, tune=False,
+                          r_cut=prm["r_cut"], alpha=prm["alpha"], single_precision=True)
+                p32.activate(pos, q, box_l=[L] * 3)
+                row.update({"r_cut": round(prm["r_cut"], 4), "alpha": round(prm["alpha"], 5),
+                            "estimate": prm["accuracy"],
+                            "rms_err_fp64": total_force_error(p64, tp, tq, fx["idx"],
+                                                              fx["force"], torch),
+                            "rms_err_fp32": total_force_error(p32, tp, tq, fx["idx"],
+                                                              fx["force"], torch)})
+                print(f"[bench] C3 mesh {mesh} cao {cao} acc {acc:g}: est "
+                      f"{row['estimate']:.2e} fp64 {row['rms_err_fp64']:.2e} fp32 "
+                      f"{row['rms_err_fp32']:.2e}", flush=True)
+                rows.append(row)
+    return rows
+
+
 def timed_loop(p3m, tp, tq, tf, steps, warmup, torch):
     for _ in range(warmup):
         p3m.add_long_range_forces(tp, tq, tf, sync=False)
@@ -329,6 +384,9 @@ def main():
                     help="skip the 512-system batched workload")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the 2048-system serving workload")
+    ap.add_argument("--accuracy-table", action="store_true",
+                    help="also run the 27-cell precision sweep (N=250,000) "
+                         "against exact Ewald forces")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference "
                          "dispatch for scaling HBM byte counters)")
@@ -423,6 +481,23 @@ def main():
               f"Mparticles/s, rel-rms vs FP64 {fp32['force_rel_rms_vs_fp64']:.2e}",
               flush=True)
 
+    # absolute rms force error of the measured solvers against exact Ewald
+    # forces on the first 2000 charged particles (full workload only)
+    accuracy = None
+    if args.n == 1_000_000:
+        fx = np.load(os.path.join(GOLDEN, "ewald_c2_subset.npz"))
+        accuracy = {"reference": "exact FP64 Ewald (tests/golden/ewald_c2_subset.npz), "
+                                 "2000 charged particles, total force = device k-space + "
+                                 "device real-space pair forces",
+                    "estimate": prm["accuracy"],
+                    "rms_err_fp64": total_force_error(p3m, tp, tq, fx["idx"], fx["force"],
+                                                      torch)}
+        if fp32 is not None:
+            accuracy["rms_err_fp32"] = total_force_error(p32, tp, tq, fx["idx"], fx["force"],
+                                                         torch)
+        print(f"[bench] rms force error vs Ewald: {accuracy}", flush=True)
+    table = accuracy_table_c3(torch, dev) if args.accuracy_table else None
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev)
@@ -477,9 +552,12 @@ def main():
         "solve_gbs_algorithmic": round(solve_achieved, 1),
         "host_pointer_ms": round(t_host * 1e3, 3),
         "fp32_mesh": fp32,
+        "accuracy": accuracy,
         "batched": batched,
         "serving": serving,
     }
+    if table is not None:
+        line["accuracy_table_c3"] = table
     print(json.dumps(line), flush=True)
 
 

@@ -33,7 +33,8 @@ EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
     "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
     "p3m_debug_charge_mesh", "p3m_debug_indices", "p3m_k_space_error",
-    "p3m_real_space_error", "p3m_pair_force", "p3m_pair_energy",
+    "p3m_real_space_error", "p3m_pair_forces_targets", "p3m_pair_force",
+    "p3m_pair_energy",
     "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
@@ -111,6 +112,7 @@ def load():
                                     C.c_int, C.c_double, C.c_double, dp, dp]
     L.p3m_real_space_error.argtypes = [C.c_double, C.c_double, C.c_int,
                                        C.c_double, C.c_double, dp, dp]
+    L.p3m_pair_forces_targets.argtypes = [vp, C.c_int64, vp, vp, C.c_int, vp, vp]
     L.p3m_pair_force.argtypes = [vp, C.c_double, dp, C.c_double, dp]
     L.p3m_pair_energy.argtypes = [vp, C.c_double, C.c_double, dp]
     L.p3m_last_timings.argtypes = [vp, dp, C.c_int]

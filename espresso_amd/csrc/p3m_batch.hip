@@ -1102,6 +1102,54 @@ __global__ __launch_bounds__(kBlock) void k_prepare(SysDesc const *__restrict__ 
   force[3 * i + 2] = 0.;
 }
 
+// ---------------------------------------------------------------------------
+// real-space pair forces (CoulombP3M::pair_force, p3m.hpp:146-165, with the
+// Abramowitz-Stegun erfc of AS_erfc_part.hpp:29-40) on listed target
+// particles from all particles within r_cut, minimum image; one block per
+// target scans all sources.  Used to complete the total force of selected
+// particles (error measurements); O(targets x N).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pair_forces_targets(
+    int64_t n, double const *__restrict__ pos, double const *__restrict__ q,
+    int const *__restrict__ targets, double Lx, double Ly, double Lz, double prefactor,
+    double alpha, double r_cut, double *__restrict__ out) {
+  __shared__ double red[3 * 16];
+  int const t = targets[blockIdx.x];
+  double const qi = q[t];
+  double const xi = pos[3 * (int64_t)t], yi = pos[3 * (int64_t)t + 1], zi = pos[3 * (int64_t)t + 2];
+  double const rc2 = r_cut * r_cut;
+  double const two_a_sqrt_pi_i = 2. * alpha * (1. / sqrt(std::numbers::pi));
+  double v[3] = {0., 0., 0.};
+  if (qi != 0.) {
+    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+      double const qj = q[j];
+      if (j == t || qj == 0.) continue;
+      double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+      dx -= Lx * rint(dx / Lx);
+      dy -= Ly * rint(dy / Ly);
+      dz -= Lz * rint(dz / Lz);
+      double const r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 >= rc2 || r2 == 0.) continue;
+      double const dist = sqrt(r2);
+      double const adist = alpha * dist;
+      double const tt = 1. / (1. + 0.3275911 * adist);
+      double const erfc_part = tt * (0.254829592 + tt * (-0.284496736 + tt * (1.421413741 +
+                                     tt * (-1.453152027 + tt * 1.061405429))));
+      double const ex = exp(-adist * adist);
+      double const fac = ex * (erfc_part / dist + two_a_sqrt_pi_i) / r2 * prefactor * qi * qj;
+      v[0] += fac * dx;
+      v[1] += fac * dy;
+      v[2] += fac * dz;
+    }
+  }
+  block_sum<3>(v, red);
+  if (threadIdx.x == 0) {
+    out[3 * (int64_t)blockIdx.x] = v[0];
+    out[3 * (int64_t)blockIdx.x + 1] = v[1];
+    out[3 * (int64_t)blockIdx.x + 2] = v[2];
+  }
+}
+
 template <typename F> void dispatch_cao(int cao, F &&f) {
   switch (cao) {
   case 1: f(std::integral_constant<int, 1>{}); break;
@@ -2150,6 +2198,17 @@ void BatchSolver::debug_indices(int s, ParticleArrays const &pa, std::vector<int
     throw;
   }
   (void)hipFree(d_nmp);
+}
+
+void pair_forces_targets(int device, hipStream_t stream, int64_t n, double const *pos,
+                         double const *q, int ntargets, int const *targets, Vec3d const &box,
+                         double prefactor, double alpha, double r_cut, double *out) {
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  if (ntargets <= 0) return;
+  k_pair_forces_targets<<<ntargets, kBlock, 0, stream>>>(n, pos, q, targets, box[0], box[1],
+                                                         box[2], prefactor, alpha, r_cut, out);
+  hip_check(hipGetLastError(), "pair force launch");
+  hip_check(hipStreamSynchronize(stream), "pair forces");
 }
 
 } // namespace espresso_amd

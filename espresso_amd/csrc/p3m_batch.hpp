@@ -155,4 +155,10 @@ private:
 
 void hip_check(hipError_t err, char const *what);
 
+/** Real-space pair forces (A&S erfc) on `targets` from all n particles
+ *  within r_cut, minimum image; device pointers, out[3 * ntargets]. */
+void pair_forces_targets(int device, hipStream_t stream, int64_t n, double const *pos,
+                         double const *q, int ntargets, int const *targets, Vec3d const &box,
+                         double prefactor, double alpha, double r_cut, double *out);
+
 } // namespace espresso_amd

@@ -434,6 +434,23 @@ int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double
   });
 }
 
+int p3m_pair_forces_targets(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                            int ntargets, int32_t const *targets, double *out) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    require_device(ctx);
+    if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
+    if (n < 0 || ntargets < 0) throw std::invalid_argument("P3M: negative count");
+    auto const box = v3(ctx->user.box_l);
+    for (int d = 0; d < 3; ++d)
+      if (!(ctx->params.r_cut < 0.5 * box[d]))
+        throw std::domain_error("P3M: r_cut must be below half the box for minimum image");
+    pair_forces_targets(ctx->device, ctx->solver->stream(), n, pos, q, ntargets, targets, box,
+                        ctx->user.prefactor, ctx->params.alpha, ctx->params.r_cut, out);
+  });
+}
+
 int p3m_pair_force(p3m_context const *ctx, double q1q2, double const d[3], double dist,
                    double out[3]) {
   if (!ctx || !out) return P3M_ERR_INVALID;

@@ -318,6 +318,24 @@ class _P3MBase:
             self._ctx)
         return e.value, np.array(v[:]).reshape(3, 3)
 
+    def pair_forces_on(self, pos, q, targets):
+        """Real-space pair forces (A&S erfc, p3m.hpp:146-165) on the listed
+        target particles from all particles within r_cut, minimum image;
+        device tensors in, a (len(targets), 3) device tensor out."""
+        import torch
+        self._require_active()
+        a = _Arrays(pos, q)
+        if not a.device:
+            raise TypeError("pair_forces_on takes device tensors")
+        a.order_after_torch(self._ctx)
+        t = torch.as_tensor(targets, dtype=torch.int32).to(pos.device).contiguous()
+        out = torch.empty((t.numel(), 3), dtype=torch.float64, device=pos.device)
+        torch.cuda.synchronize()
+        _lib.raise_for(_lib.load().p3m_pair_forces_targets(
+            self._ctx, a.n, a.pos, a.q, t.numel(), t.data_ptr(), out.data_ptr()),
+            self._ctx)
+        return out
+
     def pair_force(self, q1q2, d, dist):
         out = (C.c_double * 3)()
         dd = (C.c_double * 3)(*[float(x) for x in d])

@@ -135,6 +135,15 @@ int p3m_k_space_error(double prefactor, const int mesh[3], int cao, int n_c_part
 int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double sum_q2,
                          double alpha_L, const double box_l[3], double *out);
 
+/* Real-space pair forces of CoulombP3M::pair_force (p3m.hpp:146-165, A&S
+ * erfc) on the ntargets particles listed in targets from all n particles
+ * within the context's r_cut (minimum image, r_cut < half the box): device
+ * pointers pos [n][3], q [n], targets [ntargets], out [ntargets][3]
+ * (overwritten).  A scan over all sources per target: completes the total
+ * force of selected particles, e.g. for error measurements. */
+int p3m_pair_forces_targets(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                            int ntargets, const int32_t *targets, double *out);
+
 /* Real-space pair terms CoulombP3M::pair_force / pair_energy (p3m.hpp:146-180). */
 int p3m_pair_force(const p3m_context *ctx, double q1q2, const double d[3], double dist,
                    double out[3]);

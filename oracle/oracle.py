@@ -98,6 +98,10 @@ def lib():
     L.p3mo_ewald_kspace.restype = C.c_double
     L.p3mo_ewald_kspace.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
                                     C.c_double, C.c_double, C.c_int, _dp]
+    L.p3mo_real_space_subset.restype = None
+    L.p3mo_real_space_subset.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
+                                         C.c_double, C.c_double, C.c_int,
+                                         C.c_int, _dp]
     _lib = L
     return L
 
@@ -290,6 +294,18 @@ def real_space(pos, q, box_l, prefactor, alpha, r_cut, virial=False):
     if virial:
         return e, f, vir.reshape(3, 3)
     return e, f
+
+
+def real_space_subset(pos, q, box_l, prefactor, alpha, r_cut, n_out,
+                      exact_erfc=False):
+    """Real-space forces on the first n_out particles from all others:
+    the P3M pair force (A&S erfc) or, with exact_erfc, the exact Ewald one."""
+    pos = _pos(pos)
+    f = np.zeros((int(n_out), 3))
+    lib().p3mo_real_space_subset(len(q), pos, np.ascontiguousarray(q, np.float64),
+                                 _v3(box_l), prefactor, alpha, r_cut, int(n_out),
+                                 1 if exact_erfc else 0, f)
+    return f
 
 
 def ewald_kspace(pos, q, box_l, prefactor, alpha, kmax, n_out=None):

@@ -1130,7 +1130,8 @@ double p3mo_ewald_kspace(int n, double const *pos, double const *q, double const
   cplx *ez = malloc(sizeof(cplx) * (size_t)n * w2);
   cplx *tabs[3] = {ex, ey, ez};
   size_t const ws[3] = {w0, w1, w2};
-  for (int d = 0; d < 3; ++d)
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(static)
     for (int p = 0; p < n; ++p) {
       cplx *t = tabs[d] + (size_t)p * ws[d];
       int const c = nmax[d];
@@ -1140,44 +1141,103 @@ double p3mo_ewald_kspace(int n, double const *pos, double const *q, double const
         t[m + c].im = sin(m * ang);
       }
     }
-  for (int i = 0; i < 3 * n_out; ++i) force[i] = 0.;
-  double energy = 0.;
+  }
+  /* half space: (a>0) or (a==0,b>0) or (a==0,b==0,c>0), |k| <= kmax */
   double const kmax2 = kmax * kmax;
+  size_t nk = 0;
+  int *kv = malloc(sizeof(int) * 3 * w0 * w1 * w2);
   for (int a = 0; a <= nmax[0]; ++a)
     for (int b = -nmax[1]; b <= nmax[1]; ++b)
       for (int c = -nmax[2]; c <= nmax[2]; ++c) {
-        /* half space: (a>0) or (a==0,b>0) or (a==0,b==0,c>0) */
         if (a == 0 && (b < 0 || (b == 0 && c <= 0))) continue;
         double const k[3] = {two_pi * a / box_l[0], two_pi * b / box_l[1], two_pi * c / box_l[2]};
-        double const k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-        if (k2 > kmax2) continue;
-        double const g = 4. * P3MO_PI * exp(-k2 / (4. * alpha * alpha)) / k2;
-        double sr = 0., si = 0.;
-        for (int p = 0; p < n; ++p) {
-          if (q[p] == 0.) continue;
-          cplx const A = ex[(size_t)p * w0 + (size_t)(a + nmax[0])];
-          cplx const B = ey[(size_t)p * w1 + (size_t)(b + nmax[1])];
-          cplx const C = ez[(size_t)p * w2 + (size_t)(c + nmax[2])];
-          double const abr = A.re * B.re - A.im * B.im, abi = A.re * B.im + A.im * B.re;
-          double const er = abr * C.re - abi * C.im, ei = abr * C.im + abi * C.re;
-          sr += q[p] * er;
-          si += q[p] * ei;
-        }
-        energy += 2. * g * (sr * sr + si * si); /* doubled half space */
-        for (int p = 0; p < n_out; ++p) {
-          cplx const A = ex[(size_t)p * w0 + (size_t)(a + nmax[0])];
-          cplx const B = ey[(size_t)p * w1 + (size_t)(b + nmax[1])];
-          cplx const C = ez[(size_t)p * w2 + (size_t)(c + nmax[2])];
-          double const abr = A.re * B.re - A.im * B.im, abi = A.re * B.im + A.im * B.re;
-          double const er = abr * C.re - abi * C.im, ei = abr * C.im + abi * C.re;
-          /* Im(S* e^{ikr}) = sr*ei - si*er ; doubled half space */
-          double const im = sr * ei - si * er;
-          double const fac = 2. * g * q[p] * im * prefactor / volume;
-          for (int d = 0; d < 3; ++d) force[3 * (size_t)p + d] += fac * k[d];
-        }
+        if (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] > kmax2) continue;
+        kv[3 * nk] = a;
+        kv[3 * nk + 1] = b;
+        kv[3 * nk + 2] = c;
+        ++nk;
       }
+  for (int i = 0; i < 3 * n_out; ++i) force[i] = 0.;
+  double energy = 0.;
+#pragma omp parallel reduction(+ : energy)
+  {
+    double *fl = calloc(3 * (size_t)(n_out > 0 ? n_out : 1), sizeof(double));
+#pragma omp for schedule(dynamic, 4)
+    for (size_t kk = 0; kk < nk; ++kk) {
+      int const a = kv[3 * kk], b = kv[3 * kk + 1], c = kv[3 * kk + 2];
+      double const k[3] = {two_pi * a / box_l[0], two_pi * b / box_l[1], two_pi * c / box_l[2]};
+      double const k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+      double const g = 4. * P3MO_PI * exp(-k2 / (4. * alpha * alpha)) / k2;
+      double sr = 0., si = 0.;
+      for (int p = 0; p < n; ++p) {
+        if (q[p] == 0.) continue;
+        cplx const A = ex[(size_t)p * w0 + (size_t)(a + nmax[0])];
+        cplx const B = ey[(size_t)p * w1 + (size_t)(b + nmax[1])];
+        cplx const C = ez[(size_t)p * w2 + (size_t)(c + nmax[2])];
+        double const abr = A.re * B.re - A.im * B.im, abi = A.re * B.im + A.im * B.re;
+        double const er = abr * C.re - abi * C.im, ei = abr * C.im + abi * C.re;
+        sr += q[p] * er;
+        si += q[p] * ei;
+      }
+      energy += 2. * g * (sr * sr + si * si); /* doubled half space */
+      for (int p = 0; p < n_out; ++p) {
+        cplx const A = ex[(size_t)p * w0 + (size_t)(a + nmax[0])];
+        cplx const B = ey[(size_t)p * w1 + (size_t)(b + nmax[1])];
+        cplx const C = ez[(size_t)p * w2 + (size_t)(c + nmax[2])];
+        double const abr = A.re * B.re - A.im * B.im, abi = A.re * B.im + A.im * B.re;
+        double const er = abr * C.re - abi * C.im, ei = abr * C.im + abi * C.re;
+        /* Im(S* e^{ikr}) = sr*ei - si*er ; doubled half space */
+        double const im = sr * ei - si * er;
+        double const fac = 2. * g * q[p] * im * prefactor / volume;
+        for (int d = 0; d < 3; ++d) fl[3 * (size_t)p + d] += fac * k[d];
+      }
+    }
+#pragma omp critical
+    for (int i = 0; i < 3 * n_out; ++i) force[i] += fl[i];
+    free(fl);
+  }
+  free(kv);
   free(ex);
   free(ey);
   free(ez);
   return prefactor * energy / (2. * volume);
+}
+
+/* Real-space forces on the first n_out particles from all others within
+ * r_cut (minimum image, r_cut < half the smallest box side): the P3M pair
+ * force with the Abramowitz-Stegun erfc (p3m.hpp:146-165) when exact_erfc is
+ * 0, or the exact Ewald real-space force with libm erfc when it is 1. */
+void p3mo_real_space_subset(int n, double const *pos, double const *q, double const box_l[3],
+                            double prefactor, double alpha, double r_cut, int n_out,
+                            int exact_erfc, double *force) {
+  double const two_a_sqrt_pi_i = 2. * alpha / sqrt(P3MO_PI);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < n_out; ++i) {
+    double f[3] = {0., 0., 0.};
+    if (q[i] != 0.) {
+      for (int j = 0; j < n; ++j) {
+        if (j == i || q[j] == 0.) continue;
+        double d[3], r2 = 0.;
+        for (int k = 0; k < 3; ++k) {
+          double x = pos[3 * (size_t)i + k] - pos[3 * (size_t)j + k];
+          x -= box_l[k] * round(x / box_l[k]);
+          d[k] = x;
+          r2 += x * x;
+        }
+        if (r2 >= r_cut * r_cut || r2 == 0.) continue;
+        double const dist = sqrt(r2);
+        double fp[3];
+        if (exact_erfc) {
+          double const ad = alpha * dist;
+          double const fac = prefactor * q[i] * q[j] *
+                             (erfc(ad) / dist + two_a_sqrt_pi_i * exp(-ad * ad)) / r2;
+          for (int k = 0; k < 3; ++k) fp[k] = fac * d[k];
+        } else {
+          pair_force(prefactor, alpha, r_cut, q[i] * q[j], d, dist, fp);
+        }
+        for (int k = 0; k < 3; ++k) f[k] += fp[k];
+      }
+    }
+    for (int k = 0; k < 3; ++k) force[3 * (size_t)i + k] = f[k];
+  }
 }

@@ -60,6 +60,13 @@ def config_c2(n=1_000_000):
     return pos, q, [L] * 3, params
 
 
+def config_c3():
+    """N = 250,000 +1/-1 alternating charges in L = 79.37 (seed 1003)."""
+    L = 79.37
+    pos, q = uniform_system(250_000, L, 1003)
+    return pos, q, [L] * 3
+
+
 def madelung_cube(n=6):
     """NaCl cube of testsuite/python/p3m_madelung.py:279-312."""
     box = [2. * n] * 3

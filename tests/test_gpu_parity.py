@@ -225,3 +225,22 @@ def test_c2_subset_parity_fp64():
     pos, q, box, params = S.config_c2(n=100_000)
     params = dict(params, mesh=64)
     _compare(pos, q, box, params, pressure=False)
+
+
+def test_pair_forces_on_targets_match_oracle():
+    # device real-space pair forces (A&S erfc) on a subset from all sources
+    import torch
+    from oracle import oracle as O
+    pos, q, box, params = S.config_c1()
+    q = q.copy()
+    q[5] = 0.  # uncharged target and source
+    p3m = _p3m(box, params, check_neutrality=False)
+    p3m.activate(pos, q, box_l=box)
+    idx = np.array([0, 5, 17, 123, 9999, 4321], dtype=np.int32)
+    out = p3m.pair_forces_on(torch.tensor(pos, device="cuda"),
+                             torch.tensor(q, device="cuda"), idx).cpu().numpy()
+    order = np.concatenate([idx, np.setdiff1d(np.arange(len(q)), idx)])
+    ref = O.real_space_subset(pos[order], q[order], box, params["prefactor"], params["alpha"],
+                              params["r_cut"], len(idx))
+    assert np.all(out[1] == 0.)
+    assert S.rel_rms(out, ref) <= 1e-12

@@ -302,3 +302,32 @@ def test_error_estimates_known_values():
     t = O.tune_r_cut(1.0, [128] * 3, 7, 900000, 900000., 1e-5, [L2] * 3)
     assert abs(t[0] * L2 - 8.558) < 0.13 and abs(t[1] / L2 - 0.38896) < 4e-3
     assert t[2] <= 1e-5
+
+
+def test_exact_ewald_reference_is_independent_of_splitting():
+    # the total force of the exact Ewald reference (real space with libm erfc
+    # + reciprocal sum) must not depend on alpha: this pins both parts, their
+    # signs and prefactors, as used by tests/golden/make_ewald_fixtures.py
+    pos, q = S.uniform_system(1200, 10., 99)
+    box = [10.] * 3
+    forces = []
+    for alpha in (1.2, 1.6):
+        r_e = 5.6 / alpha
+        f_r = O.real_space_subset(pos, q, box, 1.0, alpha, r_e, 40, exact_erfc=True)
+        _, f_k = O.ewald_kspace(pos, q, box, 1.0, alpha, 11.2 * alpha, n_out=40)
+        forces.append(f_r + f_k)
+    assert S.rel_rms(forces[0], forces[1]) <= 1e-11
+    # the A&S pair force of the P3M real-space part stays within 1e-6 of erfc
+    f_as = O.real_space_subset(pos, q, box, 1.0, 1.2, 3.0, 40)
+    f_ex = O.real_space_subset(pos, q, box, 1.0, 1.2, 3.0, 40, exact_erfc=True)
+    assert S.rel_rms(f_as, f_ex) <= 1e-6
+
+
+def test_ewald_fixtures_present():
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for name, n_sys in (("ewald_c2_subset.npz", 1_000_000), ("ewald_c3_subset.npz", 250_000)):
+        d = np.load(os.path.join(here, name))
+        assert d["force"].shape == (2000, 3) and np.all(np.isfinite(d["force"]))
+        assert d["idx"].shape == (2000,) and d["idx"].max() < n_sys
+        assert float(d["alpha_e"]) * float(d["r_e"]) == pytest.approx(5.6)
