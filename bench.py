@@ -32,10 +32,16 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "hbm_traffic_c2_n1e6_mesh128_cao7.json")
-# kernel behind each single-kernel stage of the FP64 solve at cao 7
-STAGE_KERNEL = {"spread": "k_spread<7, double>",
-                "convolve": "k_fft_x_conv<double>",
-                "gather": "k_gather_plane<7, double>"}
+GATHER_KERNEL = {"plane": "k_gather_plane<7, double, 64>",
+                 "plane16": "k_gather_plane<7, double, 16>",
+                 "layer": "k_gather_layer<7, double>"}
+
+
+def stage_kernels(shape):
+    """Kernel behind each single-kernel stage of the FP64 solve at cao 7 and
+    mesh 128 (split FFT passes) for the chosen launch shape."""
+    return {"spread": "k_spread<7, double>", "convolve": "k_fft_x_conv<double>",
+            "gather": GATHER_KERNEL[shape["gather"]]}
 
 
 def make_system(n=1_000_000, seed=1002):
@@ -427,6 +433,8 @@ def main():
     tq = torch.tensor(q, device=dev)
     tf = torch.zeros((n, 3), dtype=torch.float64, device=dev)
 
+    shape64 = p3m.autotune(tp, tq, reps=3)
+    print(f"[bench] FP64 launch shape: {shape64}", flush=True)
     t_step, stages = timed_loop(p3m, tp, tq, tf, args.steps, args.warmup, torch)
     value = n / t_step / 1e6
     print(f"[bench] FP64: {t_step * 1e3:.3f} ms/step, {value:.1f} Mparticles/s; "
@@ -447,8 +455,8 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and args.n == 1_000_000:
         with open(TRAFFIC_FILE) as fh:
-            traffic = json.load(fh)["bytes_per_dispatch"].get(STAGE_KERNEL[dom])
-    roofline = {"bound": "hbm", "kernel": STAGE_KERNEL[dom],
+            traffic = json.load(fh)["bytes_per_dispatch"].get(stage_kernels(shape64)[dom])
+    roofline = {"bound": "hbm", "kernel": stage_kernels(shape64)[dom],
                 "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -463,6 +471,8 @@ def main():
                   r_cut=prm["r_cut"], alpha=prm["alpha"], single_precision=True)
         p32.activate(pos, q, box_l=box)
         tf32 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+        shape32 = p32.autotune(tp, tq, reps=3)
+        print(f"[bench] FP32 launch shape: {shape32}", flush=True)
         t32, st32 = timed_loop(p32, tp, tq, tf32, args.steps, args.warmup, torch)
         f64 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
         f32 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
@@ -472,6 +482,7 @@ def main():
         ref = f64.cpu().numpy()
         rms_abs = float(np.sqrt(np.mean(np.sum(d * d, axis=1)[q != 0])))
         fp32 = {"value": round(n / t32 / 1e6, 2), "ms_per_step": round(t32 * 1e3, 4),
+                "launch_shape": shape32,
                 "stages_ms": {k: round(v, 4) for k, v in st32.items()},
                 "force_rel_rms_vs_fp64": float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
                 "force_rms_abs_vs_fp64": rms_abs,
@@ -549,6 +560,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages_ms": {k: round(v, 5) for k, v in stages.items()},
+        "launch_shape": shape64,
         "solve_gbs_algorithmic": round(solve_achieved, 1),
         "host_pointer_ms": round(t_host * 1e3, 3),
         "fp32_mesh": fp32,

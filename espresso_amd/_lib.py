@@ -35,7 +35,7 @@ EXPORTED_SYMBOLS = (
     "p3m_debug_charge_mesh", "p3m_debug_indices", "p3m_k_space_error",
     "p3m_real_space_error", "p3m_pair_forces_targets", "p3m_pair_force",
     "p3m_pair_energy",
-    "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end",
+    "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end", "p3m_autotune",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -119,6 +119,9 @@ def load():
     L.p3m_profile_begin.argtypes = [vp, C.c_int]
     L.p3m_profile_end.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.p3m_synchronize.argtypes = [vp]
+    _ipp = C.POINTER(C.c_int)
+    L.p3m_autotune.argtypes = [vp, C.c_int64, vp, vp, C.c_int, _ipp, _ipp, _ipp,
+                               _ipp, dp, dp]
     L.p3m_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p

@@ -165,14 +165,17 @@ class P3MBatch:
         self._check(_lib.load().p3m_batch_graph_solve(self._b, nsel, ids, p,
                                                       qq, f))
 
-    def set_launch_shape(self, gather_plane=True, ncol=8, zlines=16,
+    GATHER = {"layer": 0, "plane": 1, "plane16": 2}
+
+    def set_launch_shape(self, gather="plane", ncol=8, zlines=16,
                          fused_lds=96 * 1024):
-        """Gather mapping, FFT columns per block (split y/x passes), z-line
-        pairs per block, and the LDS bytes up to which a system's x-y planes
-        are transformed by the fused y/x pass (0: never)."""
+        """Gather mapping ("layer", "plane", "plane16" or 0/1/2), FFT
+        columns per block (split y/x passes), z-line pairs per block, and
+        the LDS bytes up to which a system's x-y planes are transformed by
+        the fused y/x pass (0: never)."""
+        g = self.GATHER[gather] if isinstance(gather, str) else int(gather)
         self._check(_lib.load().p3m_batch_set_launch_shape(
-            self._b, 1 if gather_plane else 0, int(ncol), int(zlines),
-            int(fused_lds)))
+            self._b, g, int(ncol), int(zlines), int(fused_lds)))
 
     def autotune(self, pos, q, scratch_force, systems=None, reps=3):
         """Pick the fastest launch shape (gather mapping, FFT columns, z
@@ -186,7 +189,8 @@ class P3MBatch:
         self._check(_lib.load().p3m_batch_autotune(
             self._b, nsel, ids, p, qq, f, int(reps), C.byref(gp), C.byref(nc),
             C.byref(zl), C.byref(fl), C.byref(best), C.byref(default)))
-        return {"gather_plane": bool(gp.value), "ncol": nc.value,
+        names = {v: k for k, v in self.GATHER.items()}
+        return {"gather": names.get(gp.value, gp.value), "ncol": nc.value,
                 "zlines": zl.value, "fused_lds": fl.value,
                 "best_ms": best.value, "default_ms": default.value}
 

@@ -892,9 +892,12 @@ __device__ __forceinline__ void apply_force(double *force, int64_t idx, double q
 }
 
 /** Plane mapping: one lane segment per particle over the (j, k) plane, loop
- *  over i; LDS reads of an instruction serve one particle (no inter-particle
- *  bank conflicts); 64-lane DPP + lane-read reduction.  Used for FP64. */
-template <int P, typename MT>
+ *  over i.  SEGMAX = 64: a segment of the next power of two >= P^2 lanes, so
+ *  the LDS reads of an instruction serve one particle (no inter-particle bank
+ *  conflicts), reduced with 64-lane DPP and lane reads.  SEGMAX = 16: at most
+ *  16 lanes per particle, each sweeping (j, k) pairs r0, r0 + 16, ...; four
+ *  particles per wave and a 4-step DPP reduction. */
+template <int P, typename MT, int SEGMAX>
 __global__ __launch_bounds__(kPG) void k_gather_plane(
     SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
     int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
@@ -926,33 +929,48 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   double4 const *sp = spos + soff[L.s];
   int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
-  LaneSeg<P> const seg;
+  constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
+  constexpr int ppw = 64 / S;
+  constexpr int NPAIR = P * P;
+  constexpr int NT = (NPAIR + S - 1) / S;
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const r0 = lane - sub * S;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
     __syncthreads();
     stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
     __syncthreads();
-    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
-    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
-      int const p = grp * Seg<P>::ppw + seg.sub;
-      bool const valid = seg.active && p < cnt;
+    int const ngroups = (cnt + ppw - 1) / ppw;
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+      int const p = grp * ppw + sub;
       double fx = 0., fy = 0., fz = 0.;
-      if (valid) {
+      if (p < cnt) {
         double const *w = s.w + p * 3 * P;
-        double const wyz = w[P + seg.j] * w[2 * P + seg.k];
-        int const base = s.lo[p] + seg.j * T + seg.k;
+        int const lo = s.lo[p];
 #pragma unroll
-        for (int i = 0; i < P; ++i) {
-          double const ww = w[i] * wyz;
-          fx += ww * static_cast<double>(tx[base + i * TYZ]);
-          fy += ww * static_cast<double>(ty[base + i * TYZ]);
-          fz += ww * static_cast<double>(tz[base + i * TYZ]);
+        for (int t = 0; t < NT; ++t) {
+          int const r = r0 + t * S;
+          if (r < NPAIR) {
+            int const j = r / P, k = r - (r / P) * P;
+            double const wyz = w[P + j] * w[2 * P + k];
+            int const base = lo + j * T + k;
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+              double const ww = w[i] * wyz;
+              fx += ww * static_cast<double>(tx[base + i * TYZ]);
+              fy += ww * static_cast<double>(ty[base + i * TYZ]);
+              fz += ww * static_cast<double>(tz[base + i * TYZ]);
+            }
+          }
         }
       }
-      fx = segment_total<Seg<P>::S>(fx);
-      fy = segment_total<Seg<P>::S>(fy);
-      fz = segment_total<Seg<P>::S>(fz);
-      if (seg.r == 0 && p < cnt)
+      fx = segment_total<S>(fx);
+      fy = segment_total<S>(fy);
+      fz = segment_total<S>(fz);
+      if (r0 == 0 && p < cnt)
         apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
     }
   }
@@ -1353,7 +1371,7 @@ BatchSolver::BatchSolver(int device, MeshPrecision precision)
   m->lanes.push_back(std::make_unique<Lane>());
   create_lane(*m->lanes[0]);
   m_stream = m->lanes[0]->stream;
-  m_shape.gather_plane = (precision == MeshPrecision::FP64);
+  m_shape.gather = (precision == MeshPrecision::FP64) ? kGatherPlane : kGatherLayer;
   m_shape.ncol = 8;
 }
 
@@ -1425,6 +1443,8 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
   m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
+  if (s.gather < kGatherLayer || s.gather > kGatherPlane16)
+    throw std::invalid_argument("P3M: unknown gather mapping");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   quiesce(m);
   // systems whose y/x pass changes switch spectrum layout: recompute G
@@ -1481,9 +1501,10 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     c.fused_lds = (start.fused_lds > 0) ? 0 : LaunchShape{}.fused_lds;
     consider(c);
   }
-  {
+  for (int v : {kGatherLayer, kGatherPlane, kGatherPlane16}) {
+    if (v == res.best.gather) continue;
     LaunchShape c = res.best;
-    c.gather_plane = !c.gather_plane;
+    c.gather = v;
     consider(c);
   }
   for (int v : {2, 4, 8, 16}) {
@@ -1537,7 +1558,8 @@ template <typename MT> void set_kernel_attrs() {
   for (int cao = 1; cao <= 7; ++cao)
     dispatch_cao(cao, [&](auto P) {
       allow_lds(k_spread<P.value, MT>);
-      allow_lds(k_gather_plane<P.value, MT>);
+      allow_lds(k_gather_plane<P.value, MT, 64>);
+      allow_lds(k_gather_plane<P.value, MT, 16>);
       allow_lds(k_gather_layer<P.value, MT>);
     });
 }
@@ -1949,8 +1971,12 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     for (auto const &g : p.groups) {
       dispatch_cao(g.cao, [&](auto P) {
         size_t const lds = chunk_bytes(P.value) + 3 * tile_points(P.value) * sizeof(MT);
-        if (shape.gather_plane)
-          k_gather_plane<P.value, MT><<<g.blocks, kPG, lds, st>>>(
+        if (shape.gather == kGatherPlane16)
+          k_gather_plane<P.value, MT, 16><<<g.blocks, kPG, lds, st>>>(
+              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
+              ln.sums);
+        else if (shape.gather == kGatherPlane)
+          k_gather_plane<P.value, MT, 64><<<g.blocks, kPG, lds, st>>>(
               sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
               ln.sums);
         else

@@ -53,8 +53,13 @@ struct ParticleArrays {
 };
 
 /** Launch shapes picked by the autotuner for a class of systems. */
+/** Gather mappings: lanes over the stencil layer i (several particles per
+ *  wave), over the (j, k) plane with one segment of up to 64 lanes per
+ *  particle, or over the plane with 16-lane segments. */
+constexpr int kGatherLayer = 0, kGatherPlane = 1, kGatherPlane16 = 2;
+
 struct LaunchShape {
-  bool gather_plane = true; // plane-mapped gather (else layer-mapped)
+  int gather = kGatherPlane; // gather mapping
   int ncol = 8;             // kz columns per block in the y/x FFT passes
   int zlines = 16;          // pairs of real z lines per block in the z passes
   int fused_lds = 96 * 1024; // fused y/x pass for meshes whose three x-y

@@ -487,6 +487,28 @@ int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves) {
   });
 }
 
+int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q, int reps,
+                 int *gather, int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                 double *default_ms) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    require_device(ctx);
+    if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->initialized) init_solver(ctx);
+    ensure_staging(ctx, n);
+    AutotuneResult const r = ctx->solver->autotune(n, pos, q, ctx->d_force, reps);
+    if (gather) *gather = r.best.gather;
+    if (ncol) *ncol = r.best.ncol;
+    if (zlines) *zlines = r.best.zlines;
+    if (fused_lds) *fused_lds = r.best.fused_lds;
+    if (best_ms) *best_ms = r.best_ms;
+    if (default_ms) *default_ms = r.default_ms;
+  });
+}
+
 int p3m_synchronize(p3m_context *ctx) {
   if (!ctx || !ctx->solver) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
@@ -647,7 +669,7 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double con
   });
 }
 
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines,
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
                                int fused_lds) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
@@ -657,13 +679,15 @@ int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zli
             "P3M batch: ncol and zlines must be powers of two in [1, 64]");
     if (fused_lds < 0 || fused_lds > 128 * 1024)
       throw std::invalid_argument("P3M batch: fused_lds must be in [0, 131072]");
+    if (gather < kGatherLayer || gather > kGatherPlane16)
+      throw std::invalid_argument("P3M batch: gather must be 0 (layer), 1 (plane) or 2 (plane16)");
     hip_check(hipSetDevice(b->device), "hipSetDevice");
-    b->solver->set_launch_shape(LaunchShape{gather_plane != 0, ncol, zlines, fused_lds});
+    b->solver->set_launch_shape(LaunchShape{gather, ncol, zlines, fused_lds});
   });
 }
 
 int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
-                       double const *q, double *force, int reps, int *gather_plane,
+                       double const *q, double *force, int reps, int *gather,
                        int *ncol, int *zlines, int *fused_lds, double *best_ms,
                        double *default_ms) {
   if (!b) return P3M_ERR_INVALID;
@@ -675,7 +699,7 @@ int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const 
     hip_check(hipSetDevice(b->device), "hipSetDevice");
     AutotuneResult const r =
         b->solver->autotune(sel, ParticleArrays{pos, q, nullptr, force}, reps);
-    if (gather_plane) *gather_plane = r.best.gather_plane ? 1 : 0;
+    if (gather) *gather = r.best.gather;
     if (ncol) *ncol = r.best.ncol;
     if (zlines) *zlines = r.best.zlines;
     if (fused_lds) *fused_lds = r.best.fused_lds;

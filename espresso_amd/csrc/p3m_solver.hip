@@ -31,6 +31,16 @@ void Solver::set_launch_shape(LaunchShape const &s) {
   if (m_batch) m_batch->set_launch_shape(s);
 }
 
+AutotuneResult Solver::autotune(int64_t n, double const *d_pos, double const *d_q,
+                                double *d_force, int reps) {
+  if (!m_batch) throw std::runtime_error("P3M solver used before configure()");
+  m_batch->set_particle_count(0, n);
+  AutotuneResult const r = m_batch->autotune({0}, ParticleArrays{d_pos, d_q, nullptr, d_force}, reps);
+  m_shape = r.best;
+  m_shape_set = true;
+  return r;
+}
+
 void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
                        double prefactor, MeshPrecision precision) {
   if (m_batch) hip_check(hipStreamSynchronize(m_batch->stream()), "sync");

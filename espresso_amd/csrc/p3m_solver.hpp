@@ -61,6 +61,11 @@ public:
    *  fft_fwd, convolve, fft_bwd, gather. */
   float last_ms[8] = {};
   void set_launch_shape(LaunchShape const &s);
+  /** Pick the fastest launch shape for force solves of these device arrays
+   *  (BatchSolver::autotune); d_force is scratch.  The shape is kept across
+   *  later configure() calls. */
+  AutotuneResult autotune(int64_t n, double const *d_pos, double const *d_q, double *d_force,
+                          int reps);
   LocalMesh const &local_mesh() const;
 
 private:

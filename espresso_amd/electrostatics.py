@@ -258,6 +258,26 @@ class _P3MBase:
             self._ctx)
         return force
 
+    def autotune(self, pos, q, reps=3):
+        """Pick the fastest launch shape of the device kernels (gather
+        mapping, FFT columns, z lines, fused y/x pass) for force solves of
+        these device tensors and keep it.  Returns the shape and the mean
+        solve times before and after."""
+        self._require_active()
+        a = _Arrays(pos, q)
+        if not a.device:
+            raise TypeError("autotune takes device tensors")
+        a.order_after_torch(self._ctx)
+        g, nc, zl, fl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        best, default = C.c_double(), C.c_double()
+        _lib.raise_for(_lib.load().p3m_autotune(
+            self._ctx, a.n, a.pos, a.q, int(reps), C.byref(g), C.byref(nc),
+            C.byref(zl), C.byref(fl), C.byref(best), C.byref(default)), self._ctx)
+        names = {0: "layer", 1: "plane", 2: "plane16"}
+        return {"gather": names.get(g.value, g.value), "ncol": nc.value,
+                "zlines": zl.value, "fused_lds": fl.value, "best_ms": best.value,
+                "default_ms": default.value}
+
     def synchronize(self):
         _lib.raise_for(_lib.load().p3m_synchronize(self._ctx), self._ctx)
 

@@ -162,6 +162,14 @@ int p3m_last_timings(const p3m_context *ctx, double *ms, int n);
 int p3m_profile_begin(p3m_context *ctx, int max_solves);
 int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves);
 
+/* Launch-shape autotuner of the context's device kernels (no counterpart in
+ * the reference; see p3m_batch_autotune for the candidates): times `reps`
+ * force solves of the device arrays pos [n][3], q [n] per candidate and keeps
+ * the fastest for later solves.  Outputs may be NULL. */
+int p3m_autotune(p3m_context *ctx, int64_t n, const double *pos, const double *q, int reps,
+                 int *gather, int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                 double *default_ms);
+
 /* Block until all work enqueued on the context stream has finished. */
 int p3m_synchronize(p3m_context *ctx);
 
@@ -201,12 +209,14 @@ int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *po
  * first use, re-captured when the selection or arrays change); enqueue only. */
 int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                           const double *q, double *force);
-/* Launch shape: gather mapping (1 = plane, 0 = layer), kz columns per block
+/* Launch shape: gather mapping (0 = lanes over stencil layers, 1 = lanes
+ * over the (j, k) plane in segments of up to 64, 2 = the plane in 16-lane
+ * segments), kz columns per block
  * of the split y/x FFT passes and pairs of real lines per block of the z
  * passes (powers of two in [1, 64]), and the LDS bytes up to which a system
  * uses the fused y/x pass (its three x-y planes in LDS; 0 = never, at most
  * 131072). */
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zlines,
+int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
                                int fused_lds);
 /* Autotuner: time `reps` force solves of the selection for candidate launch
  * shapes (coordinate descent over fused pass, gather mapping, FFT columns and
@@ -214,7 +224,7 @@ int p3m_batch_set_launch_shape(p3m_batch *b, int gather_plane, int ncol, int zli
  * time before (default_ms) and after (best_ms).  `force` is scratch: forces
  * accumulate into it on every timed solve. */
 int p3m_batch_autotune(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
-                       const double *q, double *force, int reps, int *gather_plane,
+                       const double *q, double *force, int reps, int *gather,
                        int *ncol, int *zlines, int *fused_lds, double *best_ms,
                        double *default_ms);
 int p3m_batch_synchronize(p3m_batch *b);

@@ -244,3 +244,21 @@ def test_pair_forces_on_targets_match_oracle():
                               params["r_cut"], len(idx))
     assert np.all(out[1] == 0.)
     assert S.rel_rms(out, ref) <= 1e-12
+
+
+def test_actor_autotune_keeps_forces():
+    import torch
+    pos, q, box, params = S.config_c1()
+    for single in (False, True):
+        p3m = _p3m(box, params, single_precision=single)
+        p3m.activate(pos, q, box_l=box)
+        tp, tq = torch.tensor(pos, device="cuda"), torch.tensor(q, device="cuda")
+        ref = torch.zeros_like(tp)
+        p3m.add_long_range_forces(tp, tq, ref)
+        shape = p3m.autotune(tp, tq, reps=1)
+        assert shape["gather"] in ("layer", "plane", "plane16")
+        assert 0. < shape["best_ms"] <= shape["default_ms"]
+        f = torch.zeros_like(tp)
+        p3m.add_long_range_forces(tp, tq, f)
+        tol = 1e-5 if single else 1e-12
+        assert S.rel_rms(f.cpu().numpy(), ref.cpu().numpy()) <= tol
