@@ -34,13 +34,16 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "hbm_traffic_c2_n1e6_mesh128_cao7.json")
 GATHER_KERNEL = {"plane": "k_gather_plane<7, double, 64>",
                  "plane16": "k_gather_plane<7, double, 16>",
+                 "plane8": "k_gather_plane<7, double, 8>",
                  "layer": "k_gather_layer<7, double>"}
+SPREAD_KERNEL = {"plane": "k_spread<7, double, 64>", "plane16": "k_spread<7, double, 16>",
+                 "plane8": "k_spread<7, double, 8>"}
 
 
 def stage_kernels(shape):
     """Kernel behind each single-kernel stage of the FP64 solve at cao 7 and
     mesh 128 (split FFT passes) for the chosen launch shape."""
-    return {"spread": "k_spread<7, double>", "convolve": "k_fft_x_conv<double>",
+    return {"spread": SPREAD_KERNEL[shape["spread"]], "convolve": "k_fft_x_conv<double>",
             "gather": GATHER_KERNEL[shape["gather"]]}
 
 

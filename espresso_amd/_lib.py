@@ -121,7 +121,7 @@ def load():
     L.p3m_synchronize.argtypes = [vp]
     _ipp = C.POINTER(C.c_int)
     L.p3m_autotune.argtypes = [vp, C.c_int64, vp, vp, C.c_int, _ipp, _ipp, _ipp,
-                               _ipp, dp, dp]
+                               _ipp, _ipp, dp, dp]
     L.p3m_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
@@ -137,9 +137,9 @@ def load():
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
     L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int,
-                                             C.c_int]
+                                             C.c_int, C.c_int]
     L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, ip,
-                                     ip, ip, ip, dp, dp]
+                                     ip, ip, ip, ip, dp, dp]
     L.p3m_batch_synchronize.argtypes = [vp]
     L.p3m_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_batch_profile_begin.argtypes = [vp, C.c_int]

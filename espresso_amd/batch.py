@@ -39,6 +39,15 @@ def _device_ptr(t, name, dtype_size, numel):
     return t.data_ptr()
 
 
+def shape_dict(gather, ncol, zlines, fused_lds, spread, best_ms, default_ms):
+    """Launch shape returned by the autotuners, with mapping names."""
+    gnames = {v: k for k, v in P3MBatch.GATHER.items()}
+    snames = {v: k for k, v in P3MBatch.SPREAD.items()}
+    return {"gather": gnames.get(gather, gather), "ncol": ncol, "zlines": zlines,
+            "fused_lds": fused_lds, "spread": snames.get(spread, spread),
+            "best_ms": best_ms, "default_ms": default_ms}
+
+
 class P3MBatch:
     """Batched k-space solver (FP32 mesh by default, FP64 with
     ``single_precision=False``)."""
@@ -165,17 +174,20 @@ class P3MBatch:
         self._check(_lib.load().p3m_batch_graph_solve(self._b, nsel, ids, p,
                                                       qq, f))
 
-    GATHER = {"layer": 0, "plane": 1, "plane16": 2}
+    GATHER = {"layer": 0, "plane": 1, "plane16": 2, "plane8": 3}
+    SPREAD = {"plane": 0, "plane16": 1, "plane8": 2}
 
-    def set_launch_shape(self, gather="plane", ncol=8, zlines=16,
-                         fused_lds=96 * 1024):
-        """Gather mapping ("layer", "plane", "plane16" or 0/1/2), FFT
-        columns per block (split y/x passes), z-line pairs per block, and
-        the LDS bytes up to which a system's x-y planes are transformed by
-        the fused y/x pass (0: never)."""
+    def set_launch_shape(self, gather="plane16", ncol=8, zlines=16,
+                         fused_lds=96 * 1024, spread="plane"):
+        """Gather mapping ("layer", "plane", "plane16", "plane8" or 0-3),
+        FFT columns per block (split y/x passes), z-line pairs per block, the
+        LDS bytes up to which a system's x-y planes are transformed by the
+        fused y/x pass (0: never), and the spread mapping ("plane",
+        "plane16", "plane8" or 0-2)."""
         g = self.GATHER[gather] if isinstance(gather, str) else int(gather)
+        sp = self.SPREAD[spread] if isinstance(spread, str) else int(spread)
         self._check(_lib.load().p3m_batch_set_launch_shape(
-            self._b, g, int(ncol), int(zlines), int(fused_lds)))
+            self._b, g, int(ncol), int(zlines), int(fused_lds), sp))
 
     def autotune(self, pos, q, scratch_force, systems=None, reps=3):
         """Pick the fastest launch shape (gather mapping, FFT columns, z
@@ -184,15 +196,13 @@ class P3MBatch:
         and the mean solve times before and after."""
         p, qq, f = self._arrays(pos, q, scratch_force)
         nsel, ids = self._selection(systems)
-        gp, nc, zl, fl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        gp, nc, zl, fl, sp = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         best, default = C.c_double(), C.c_double()
         self._check(_lib.load().p3m_batch_autotune(
             self._b, nsel, ids, p, qq, f, int(reps), C.byref(gp), C.byref(nc),
-            C.byref(zl), C.byref(fl), C.byref(best), C.byref(default)))
-        names = {v: k for k, v in self.GATHER.items()}
-        return {"gather": names.get(gp.value, gp.value), "ncol": nc.value,
-                "zlines": zl.value, "fused_lds": fl.value,
-                "best_ms": best.value, "default_ms": default.value}
+            C.byref(zl), C.byref(fl), C.byref(sp), C.byref(best), C.byref(default)))
+        return shape_dict(gp.value, nc.value, zl.value, fl.value, sp.value,
+                          best.value, default.value)
 
     def synchronize(self):
         self._check(_lib.load().p3m_batch_synchronize(self._b))

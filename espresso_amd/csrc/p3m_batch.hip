@@ -8,6 +8,7 @@
 #include <numbers>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 
 namespace espresso_amd {
 
@@ -361,7 +362,7 @@ template <int P> __device__ __forceinline__ void brick_origin(SysDesc const &d, 
   o2 = bz * kBrick;
 }
 
-template <int P, typename MT>
+template <int P, typename MT, int SEGMAX>
 __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
                                                 int const *__restrict__ sel,
                                                 int const *__restrict__ wt, int nsel,
@@ -386,22 +387,41 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
   double4 const *sp = spos + soff[L.s];
-  LaneSeg<P> const seg;
+  // lane segments of S lanes per particle over the (j, k) stencil plane,
+  // each lane sweeping pairs r0, r0 + S, ... and the stencil layers i
+  constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
+  constexpr int ppw = 64 / S;
+  constexpr int NPAIR = P * P;
+  constexpr int NT = (NPAIR + S - 1) / S;
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const r0 = lane - sub * S;
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
     __syncthreads();
     stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
     __syncthreads();
-    int const ngroups = (cnt + Seg<P>::ppw - 1) / Seg<P>::ppw;
-    for (int grp = seg.wave; grp < ngroups; grp += seg.nwaves) {
-      int const p = grp * Seg<P>::ppw + seg.sub;
-      if (seg.active && p < cnt) {
+    int const ngroups = (cnt + ppw - 1) / ppw;
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+      int const p = grp * ppw + sub;
+      if (p < cnt) {
         double const *w = s.w + p * 3 * P;
-        double const wyzq = (w[P + seg.j] * w[2 * P + seg.k]) * s.qq[p];
-        double *dst = tile + s.lo[p] + seg.j * T + seg.k;
+        double const qp = s.qq[p];
+        int const lo = s.lo[p];
 #pragma unroll
-        for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, w[i] * wyzq);
+        for (int t = 0; t < NT; ++t) {
+          int const r = r0 + t * S;
+          if (r < NPAIR) {
+            int const jj = r / P, kk = r - (r / P) * P;
+            double const wyzq = (w[P + jj] * w[2 * P + kk]) * qp;
+            double *dst = tile + lo + jj * T + kk;
+#pragma unroll
+            for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, w[i] * wyzq);
+          }
+        }
       }
     }
   }
@@ -1371,7 +1391,7 @@ BatchSolver::BatchSolver(int device, MeshPrecision precision)
   m->lanes.push_back(std::make_unique<Lane>());
   create_lane(*m->lanes[0]);
   m_stream = m->lanes[0]->stream;
-  m_shape.gather = (precision == MeshPrecision::FP64) ? kGatherPlane : kGatherLayer;
+  m_shape.gather = (precision == MeshPrecision::FP64) ? kGatherPlane16 : kGatherLayer;
   m_shape.ncol = 8;
 }
 
@@ -1443,8 +1463,10 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
   m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
-  if (s.gather < kGatherLayer || s.gather > kGatherPlane16)
+  if (s.gather < kGatherLayer || s.gather > kGatherPlane8)
     throw std::invalid_argument("P3M: unknown gather mapping");
+  if (s.spread < kSpreadSeg64 || s.spread > kSpreadSeg8)
+    throw std::invalid_argument("P3M: unknown spread mapping");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   quiesce(m);
   // systems whose y/x pass changes switch spectrum layout: recompute G
@@ -1501,10 +1523,16 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     c.fused_lds = (start.fused_lds > 0) ? 0 : LaunchShape{}.fused_lds;
     consider(c);
   }
-  for (int v : {kGatherLayer, kGatherPlane, kGatherPlane16}) {
+  for (int v : {kGatherLayer, kGatherPlane, kGatherPlane16, kGatherPlane8}) {
     if (v == res.best.gather) continue;
     LaunchShape c = res.best;
     c.gather = v;
+    consider(c);
+  }
+  for (int v : {kSpreadSeg64, kSpreadSeg16, kSpreadSeg8}) {
+    if (v == res.best.spread) continue;
+    LaunchShape c = res.best;
+    c.spread = v;
     consider(c);
   }
   for (int v : {2, 4, 8, 16}) {
@@ -1557,9 +1585,12 @@ template <typename MT> void set_kernel_attrs() {
   allow_lds(k_fft_yx_conv<MT>);
   for (int cao = 1; cao <= 7; ++cao)
     dispatch_cao(cao, [&](auto P) {
-      allow_lds(k_spread<P.value, MT>);
+      allow_lds(k_spread<P.value, MT, 64>);
+      allow_lds(k_spread<P.value, MT, 16>);
+      allow_lds(k_spread<P.value, MT, 8>);
       allow_lds(k_gather_plane<P.value, MT, 64>);
       allow_lds(k_gather_plane<P.value, MT, 16>);
+      allow_lds(k_gather_plane<P.value, MT, 8>);
       allow_lds(k_gather_layer<P.value, MT>);
     });
 }
@@ -1937,8 +1968,17 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   for (auto const &g : p.groups) {
     dispatch_cao(g.cao, [&](auto P) {
       size_t const lds = chunk_bytes(P.value) + tile_points(P.value) * sizeof(double);
-      k_spread<P.value, MT><<<g.blocks, kPG, lds, st>>>(sys, g.d_sel, g.wt, g.nsel, p.soff,
-                                                          ln.offsets, ln.spos);
+      auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+                                        static_cast<double4 const *>(ln.spos));
+      auto launch = [&](auto kern) {
+        std::apply([&](auto... a) { kern<<<g.blocks, kPG, lds, st>>>(a...); }, args);
+      };
+      if (shape.spread == kSpreadSeg8)
+        launch(k_spread<P.value, MT, 8>);
+      else if (shape.spread == kSpreadSeg16)
+        launch(k_spread<P.value, MT, 16>);
+      else
+        launch(k_spread<P.value, MT, 64>);
     });
   }
   if (ev) (void)hipEventRecord(ev[2], st);
@@ -1971,7 +2011,11 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     for (auto const &g : p.groups) {
       dispatch_cao(g.cao, [&](auto P) {
         size_t const lds = chunk_bytes(P.value) + 3 * tile_points(P.value) * sizeof(MT);
-        if (shape.gather == kGatherPlane16)
+        if (shape.gather == kGatherPlane8)
+          k_gather_plane<P.value, MT, 8><<<g.blocks, kPG, lds, st>>>(
+              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
+              ln.sums);
+        else if (shape.gather == kGatherPlane16)
           k_gather_plane<P.value, MT, 16><<<g.blocks, kPG, lds, st>>>(
               sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
               ln.sums);

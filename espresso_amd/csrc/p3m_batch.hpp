@@ -54,16 +54,20 @@ struct ParticleArrays {
 
 /** Launch shapes picked by the autotuner for a class of systems. */
 /** Gather mappings: lanes over the stencil layer i (several particles per
- *  wave), over the (j, k) plane with one segment of up to 64 lanes per
- *  particle, or over the plane with 16-lane segments. */
-constexpr int kGatherLayer = 0, kGatherPlane = 1, kGatherPlane16 = 2;
+ *  wave), or over the (j, k) plane in segments of up to 64, 16 or 8 lanes per
+ *  particle. */
+constexpr int kGatherLayer = 0, kGatherPlane = 1, kGatherPlane16 = 2, kGatherPlane8 = 3;
+/** Spread mappings: lanes over the (j, k) plane in segments of up to 64, 16
+ *  or 8 lanes per particle. */
+constexpr int kSpreadSeg64 = 0, kSpreadSeg16 = 1, kSpreadSeg8 = 2;
 
 struct LaunchShape {
-  int gather = kGatherPlane; // gather mapping
+  int gather = kGatherPlane16; // gather mapping
   int ncol = 8;             // kz columns per block in the y/x FFT passes
   int zlines = 16;          // pairs of real z lines per block in the z passes
   int fused_lds = 96 * 1024; // fused y/x pass for meshes whose three x-y
                              // planes fit in this many LDS bytes (0: never)
+  int spread = kSpreadSeg64; // spread mapping
 };
 
 struct AutotuneResult {

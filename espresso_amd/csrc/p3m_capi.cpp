@@ -488,8 +488,8 @@ int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves) {
 }
 
 int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q, int reps,
-                 int *gather, int *ncol, int *zlines, int *fused_lds, double *best_ms,
-                 double *default_ms) {
+                 int *gather, int *ncol, int *zlines, int *fused_lds, int *spread,
+                 double *best_ms, double *default_ms) {
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
@@ -504,6 +504,7 @@ int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q
     if (ncol) *ncol = r.best.ncol;
     if (zlines) *zlines = r.best.zlines;
     if (fused_lds) *fused_lds = r.best.fused_lds;
+    if (spread) *spread = r.best.spread;
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });
@@ -670,7 +671,7 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double con
 }
 
 int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
-                               int fused_lds) {
+                               int fused_lds, int spread) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
     for (int v : {ncol, zlines})
@@ -679,16 +680,18 @@ int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
             "P3M batch: ncol and zlines must be powers of two in [1, 64]");
     if (fused_lds < 0 || fused_lds > 128 * 1024)
       throw std::invalid_argument("P3M batch: fused_lds must be in [0, 131072]");
-    if (gather < kGatherLayer || gather > kGatherPlane16)
-      throw std::invalid_argument("P3M batch: gather must be 0 (layer), 1 (plane) or 2 (plane16)");
+    if (gather < kGatherLayer || gather > kGatherPlane8)
+      throw std::invalid_argument("P3M batch: gather must be 0 (layer), 1, 2 or 3 (plane)");
+    if (spread < kSpreadSeg64 || spread > kSpreadSeg8)
+      throw std::invalid_argument("P3M batch: spread must be 0, 1 or 2");
     hip_check(hipSetDevice(b->device), "hipSetDevice");
-    b->solver->set_launch_shape(LaunchShape{gather, ncol, zlines, fused_lds});
+    b->solver->set_launch_shape(LaunchShape{gather, ncol, zlines, fused_lds, spread});
   });
 }
 
 int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
                        double const *q, double *force, int reps, int *gather,
-                       int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                       int *ncol, int *zlines, int *fused_lds, int *spread, double *best_ms,
                        double *default_ms) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
@@ -703,6 +706,7 @@ int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const 
     if (ncol) *ncol = r.best.ncol;
     if (zlines) *zlines = r.best.zlines;
     if (fused_lds) *fused_lds = r.best.fused_lds;
+    if (spread) *spread = r.best.spread;
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });

@@ -170,18 +170,18 @@ __device__ __forceinline__ double readlane_f64(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-/** Sum over aligned segments of S lanes (S in {1, 4, 16, 32, 64}); the
+/** Sum over aligned segments of S lanes (S in {1, 4, 8, 16, 32, 64}); the
  *  result is valid in the first lane of each segment.  Must be executed by
  *  all 64 lanes (lanes outside the work contribute zeros). */
 template <int S> __device__ __forceinline__ double segment_total(double v) {
+  static_assert(S == 1 || S == 4 || S == 8 || S == 16 || S == 32 || S == 64,
+                "unsupported segment size");
   if constexpr (S >= 4) {
     v += dpp_f64<0xB1>(v); // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v); // quad_perm [2,3,0,1]
+    v += dpp_f64<0x4E>(v); // quad_perm [2,3,0,1]: every lane holds its quad sum
   }
-  if constexpr (S >= 16) {
-    v += dpp_f64<0x141>(v); // row_half_mirror
-    v += dpp_f64<0x140>(v); // row_mirror: every lane holds its row sum
-  }
+  if constexpr (S >= 8) v += dpp_f64<0x141>(v);  // row_half_mirror: 8-lane sums
+  if constexpr (S >= 16) v += dpp_f64<0x140>(v); // row_mirror: 16-lane row sums
   if constexpr (S == 32) {
     double const a = readlane_f64(v, 0) + readlane_f64(v, 16);
     double const b = readlane_f64(v, 32) + readlane_f64(v, 48);

@@ -268,15 +268,15 @@ class _P3MBase:
         if not a.device:
             raise TypeError("autotune takes device tensors")
         a.order_after_torch(self._ctx)
-        g, nc, zl, fl = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        from .batch import shape_dict
+        g, nc, zl, fl, sp = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         best, default = C.c_double(), C.c_double()
         _lib.raise_for(_lib.load().p3m_autotune(
             self._ctx, a.n, a.pos, a.q, int(reps), C.byref(g), C.byref(nc),
-            C.byref(zl), C.byref(fl), C.byref(best), C.byref(default)), self._ctx)
-        names = {0: "layer", 1: "plane", 2: "plane16"}
-        return {"gather": names.get(g.value, g.value), "ncol": nc.value,
-                "zlines": zl.value, "fused_lds": fl.value, "best_ms": best.value,
-                "default_ms": default.value}
+            C.byref(zl), C.byref(fl), C.byref(sp), C.byref(best),
+            C.byref(default)), self._ctx)
+        return shape_dict(g.value, nc.value, zl.value, fl.value, sp.value,
+                          best.value, default.value)
 
     def synchronize(self):
         _lib.raise_for(_lib.load().p3m_synchronize(self._ctx), self._ctx)

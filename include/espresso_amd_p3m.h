@@ -167,8 +167,8 @@ int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves);
  * force solves of the device arrays pos [n][3], q [n] per candidate and keeps
  * the fastest for later solves.  Outputs may be NULL. */
 int p3m_autotune(p3m_context *ctx, int64_t n, const double *pos, const double *q, int reps,
-                 int *gather, int *ncol, int *zlines, int *fused_lds, double *best_ms,
-                 double *default_ms);
+                 int *gather, int *ncol, int *zlines, int *fused_lds, int *spread,
+                 double *best_ms, double *default_ms);
 
 /* Block until all work enqueued on the context stream has finished. */
 int p3m_synchronize(p3m_context *ctx);
@@ -209,15 +209,16 @@ int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *po
  * first use, re-captured when the selection or arrays change); enqueue only. */
 int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                           const double *q, double *force);
-/* Launch shape: gather mapping (0 = lanes over stencil layers, 1 = lanes
- * over the (j, k) plane in segments of up to 64, 2 = the plane in 16-lane
- * segments), kz columns per block
+/* Launch shape: gather mapping (0 = lanes over stencil layers, 1/2/3 = lanes
+ * over the (j, k) plane in segments of up to 64/16/8 lanes), kz columns per
+ * block
  * of the split y/x FFT passes and pairs of real lines per block of the z
  * passes (powers of two in [1, 64]), and the LDS bytes up to which a system
  * uses the fused y/x pass (its three x-y planes in LDS; 0 = never, at most
- * 131072). */
+ * 131072), and the spread mapping (0/1/2 = segments of up to 64/16/8 lanes
+ * per particle over the (j, k) plane). */
 int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
-                               int fused_lds);
+                               int fused_lds, int spread);
 /* Autotuner: time `reps` force solves of the selection for candidate launch
  * shapes (coordinate descent over fused pass, gather mapping, FFT columns and
  * z lines) and keep the fastest; returns the chosen shape and the mean solve
@@ -225,7 +226,7 @@ int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
  * accumulate into it on every timed solve. */
 int p3m_batch_autotune(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                        const double *q, double *force, int reps, int *gather,
-                       int *ncol, int *zlines, int *fused_lds, double *best_ms,
+                       int *ncol, int *zlines, int *fused_lds, int *spread, double *best_ms,
                        double *default_ms);
 int p3m_batch_synchronize(p3m_batch *b);
 int p3m_batch_get_stream(const p3m_batch *b, void **stream);

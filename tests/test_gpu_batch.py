@@ -216,13 +216,15 @@ def test_launch_shapes_and_autotune_keep_results():
     ref_np = ref.cpu().numpy()
     f = torch.zeros_like(pos)
     for fused in (0, 96 * 1024):
-        for gather in ("layer", "plane", "plane16"):
+        for gather, spread in (("layer", "plane"), ("plane", "plane16"),
+                               ("plane16", "plane8"), ("plane8", "plane")):
             for ncol, zl in ((1, 1), (2, 4), (16, 16), (64, 64)):
-                b.set_launch_shape(gather, ncol, zl, fused)
+                b.set_launch_shape(gather, ncol, zl, fused, spread)
                 f.zero_()
                 torch.cuda.synchronize()
                 b.solve(pos, q, f)
-                assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (fused, gather, ncol, zl)
+                assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (fused, gather, spread,
+                                                                     ncol, zl)
     with pytest.raises(ValueError, match="powers of two"):
         b.set_launch_shape("plane", 3, 4)
     with pytest.raises(ValueError, match="gather must be"):
@@ -232,7 +234,8 @@ def test_launch_shapes_and_autotune_keep_results():
     shape = b.autotune(pos, q, scratch, reps=1)
     assert shape["ncol"] in (2, 4, 8, 16) and shape["zlines"] in (4, 8, 16)
     assert shape["fused_lds"] in (0, 96 * 1024)
-    assert shape["gather"] in ("layer", "plane", "plane16")
+    assert shape["gather"] in ("layer", "plane", "plane16", "plane8")
+    assert shape["spread"] in ("plane", "plane16", "plane8")
     assert 0. < shape["best_ms"] <= shape["default_ms"]
     f.zero_()
     torch.cuda.synchronize()

@@ -256,7 +256,7 @@ def test_actor_autotune_keeps_forces():
         ref = torch.zeros_like(tp)
         p3m.add_long_range_forces(tp, tq, ref)
         shape = p3m.autotune(tp, tq, reps=1)
-        assert shape["gather"] in ("layer", "plane", "plane16")
+        assert shape["gather"] in ("layer", "plane", "plane16", "plane8")
         assert 0. < shape["best_ms"] <= shape["default_ms"]
         f = torch.zeros_like(tp)
         p3m.add_long_range_forces(tp, tq, f)
