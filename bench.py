@@ -396,6 +396,9 @@ def main():
     ap.add_argument("--accuracy-table", action="store_true",
                     help="also run the 27-cell precision sweep (N=250,000) "
                          "against exact Ewald forces")
+    ap.add_argument("--only", choices=("batched", "serving"),
+                    help="run only this workload and print its record "
+                         "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 256 MiB device copy first (reference "
                          "dispatch for scaling HBM byte counters)")
@@ -409,6 +412,13 @@ def main():
 
     import torch
     from espresso_amd.electrostatics import P3M
+
+    if args.only:
+        dev = torch.device("cuda:0")
+        res = (bench_batched(args.steps, args.warmup, torch, dev) if args.only == "batched"
+               else bench_serving(torch, dev))
+        print(json.dumps({"workload": args.only, "record": res}), flush=True)
+        return
 
     pos, q, L = make_system(args.n)
     box = [L, L, L]
