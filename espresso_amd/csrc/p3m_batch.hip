@@ -26,16 +26,22 @@ FftPlan1D make_fft_plan(int n) {
     if (p.nstages >= FftPlan1D::max_stages) throw std::runtime_error("FFT plan too long");
     p.radix[p.nstages++] = r;
   };
-  // few stages: radix 8 first, a leftover factor 2 merged with a 3 into a
-  // radix-6 register DFT (each stage is one LDS round trip and barrier)
+  // prime factors above 8 first: the first stage has no inter-stage
+  // twiddles, which lets a generic radix compute its outputs in symmetric
+  // pairs.  Then few stages: radix 8, a leftover factor 2 merged with a 3
+  // into a radix-6 register DFT (each stage is one LDS round trip and barrier)
+  int small = 1;
+  for (int f : {2, 3, 5, 7})
+    while (m % f == 0) { small *= f; m /= f; }
+  for (int f = 11; m > 1 && f * f <= m; f += 2) while (m % f == 0) { push(f); m /= f; }
+  if (m > 1) push(m);
+  m = small;
   while (m % 8 == 0) { push(8); m /= 8; }
   if (m % 4 == 0) { push(4); m /= 4; }
   if (m % 2 == 0) {
     if (m % 3 == 0) { push(6); m /= 6; } else { push(2); m /= 2; }
   }
   for (int f : {3, 5, 7}) while (m % f == 0) { push(f); m /= f; }
-  for (int f = 11; m > 1 && f * f <= m; f += 2) while (m % f == 0) { push(f); m /= f; }
-  if (m > 1) push(m);
   return p;
 }
 

@@ -12,14 +12,25 @@ template <> struct CxOf<double> { using type = double2; };
 template <> struct CxOf<float> { using type = float2; };
 template <typename T> using cx_t = typename CxOf<T>::type;
 
-template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+template <typename C> __host__ __device__ __forceinline__ C cmul(C a, C b) {
   return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
-template <typename C> __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+template <typename C> __host__ __device__ __forceinline__ C cadd(C a, C b) {
+  return C{a.x + b.x, a.y + b.y};
+}
+template <typename C> __host__ __device__ __forceinline__ C csub(C a, C b) {
+  return C{a.x - b.x, a.y - b.y};
+}
+template <typename C> __host__ __device__ __forceinline__ C cconj(C a) { return C{a.x, -a.y}; }
+template <typename C, typename T> __host__ __device__ __forceinline__ C cscale(C a, T s) {
+  return C{s * a.x, s * a.y};
+}
+/** a + s b */
+template <typename C, typename T> __host__ __device__ __forceinline__ C caxpy(C a, T s, C b) {
+  return C{a.x + s * b.x, a.y + s * b.y};
+}
 /** multiply by -i (forward) or +i (inverse) */
-template <typename C> __device__ __forceinline__ C cmul_mi(C a, bool inv) {
+template <typename C> __host__ __device__ __forceinline__ C cmul_mi(C a, bool inv) {
   return inv ? C{-a.y, a.x} : C{a.y, -a.x};
 }
 
@@ -287,15 +298,93 @@ struct FftPlanDev {
   int radix[16];
 };
 
-template <typename C> __device__ __forceinline__ C tw_get(C const *tw, int m, bool inv) {
+template <typename C> __host__ __device__ __forceinline__ C tw_get(C const *tw, int m, bool inv) {
   C w = tw[m];
   if (inv) w.y = -w.y;
   return w;
 }
 
+/** 3-point DFT of (a, b, c) with constant twiddles: with t = b + c,
+ *  X0 = a + t, X1,2 = a - t/2 -+ i sin(pi/3) (b - c) (signs of i flipped for
+ *  the inverse). */
+template <typename C> __host__ __device__ __forceinline__ void dft3(C &a, C &b, C &c, bool inv) {
+  using T = decltype(a.x);
+  T const s3 = T(0.86602540378443864676372317075293618347); // sin(pi/3)
+  C const t = cadd(b, c);
+  C const m = caxpy(a, T(-0.5), t);
+  C const d = cscale(cmul_mi(csub(b, c), inv), s3);
+  a = cadd(a, t);
+  b = cadd(m, d);
+  c = csub(m, d);
+}
+
+/** In-register R-point DFT, X_k = sum_r v_r w^(rk), w = exp(-+2 pi i / R).
+ *  Radices 2-8 use constant twiddles (radix 3, 5, 7: the symmetric
+ *  a_m = v_m + v_(R-m), b_m = v_m - v_(R-m) split; radix 6: two radix-3 DFTs
+ *  of the even and odd points merged with w_6^k); other radices use the
+ *  length-n twiddle table tw (stride n / R). */
 template <typename C, int R>
-__device__ __forceinline__ void dft_reg(C (&v)[R], C const *tw, int n, bool inv) {
-  if constexpr (R == 2) {
+__host__ __device__ __forceinline__ void dft_reg(C (&v)[R], C const *tw, int n, bool inv) {
+  using T = decltype(v[0].x);
+  if constexpr (R == 1) {
+  } else if constexpr (R == 3) {
+    dft3(v[0], v[1], v[2], inv);
+  } else if constexpr (R == 5) {
+    T const c1 = T(0.30901699437494742410229341718281905886);  // cos(2 pi / 5)
+    T const c2 = T(-0.80901699437494742410229341718281905886); // cos(4 pi / 5)
+    T const s1 = T(0.95105651629515357211643933337938214340);  // sin(2 pi / 5)
+    T const s2 = T(0.58778525229247312916870595463907276860);  // sin(4 pi / 5)
+    C const a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    C const a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    C const p1 = caxpy(caxpy(v[0], c1, a1), c2, a2);
+    C const p2 = caxpy(caxpy(v[0], c2, a1), c1, a2);
+    C const r1 = cmul_mi(caxpy(cscale(b1, s1), s2, b2), inv);
+    C const r2 = cmul_mi(caxpy(cscale(b1, s2), -s1, b2), inv);
+    v[0] = cadd(v[0], cadd(a1, a2));
+    v[1] = cadd(p1, r1);
+    v[4] = csub(p1, r1);
+    v[2] = cadd(p2, r2);
+    v[3] = csub(p2, r2);
+  } else if constexpr (R == 6) {
+    T const s3 = T(0.86602540378443864676372317075293618347); // sin(pi/3)
+    C e0 = v[0], e1 = v[2], e2 = v[4];
+    C o0 = v[1], o1 = v[3], o2 = v[5];
+    dft3(e0, e1, e2, inv);
+    dft3(o0, o1, o2, inv);
+    // w_6 = 1/2 -+ i sin(pi/3), w_6^2 = -1/2 -+ i sin(pi/3)
+    C const t1 = caxpy(cscale(o1, T(0.5)), s3, cmul_mi(o1, inv));
+    C const t2 = caxpy(cscale(o2, T(-0.5)), s3, cmul_mi(o2, inv));
+    v[0] = cadd(e0, o0);
+    v[3] = csub(e0, o0);
+    v[1] = cadd(e1, t1);
+    v[4] = csub(e1, t1);
+    v[2] = cadd(e2, t2);
+    v[5] = csub(e2, t2);
+  } else if constexpr (R == 7) {
+    T const c1 = T(0.62348980185873353052500488400423981063);  // cos(2 pi / 7)
+    T const c2 = T(-0.22252093395631440428890256449679475947); // cos(4 pi / 7)
+    T const c3 = T(-0.90096886790241912623610231950744505117); // cos(6 pi / 7)
+    T const s1 = T(0.78183148246802980870844452667405775023);  // sin(2 pi / 7)
+    T const s2 = T(0.97492791218182360701813168299393121723);  // sin(4 pi / 7)
+    T const s3 = T(0.43388373911755812047576833284835875461);  // sin(6 pi / 7)
+    C const a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+    C const a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+    C const a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+    // X_k = v0 + sum_m cos(2 pi mk/7) a_m -+ i sum_m sin(2 pi mk/7) b_m, k = 1..3
+    C const p1 = caxpy(caxpy(caxpy(v[0], c1, a1), c2, a2), c3, a3);
+    C const p2 = caxpy(caxpy(caxpy(v[0], c2, a1), c3, a2), c1, a3);
+    C const p3 = caxpy(caxpy(caxpy(v[0], c3, a1), c1, a2), c2, a3);
+    C const r1 = cmul_mi(caxpy(caxpy(cscale(b1, s1), s2, b2), s3, b3), inv);
+    C const r2 = cmul_mi(caxpy(caxpy(cscale(b1, s2), -s3, b2), -s1, b3), inv);
+    C const r3 = cmul_mi(caxpy(caxpy(cscale(b1, s3), -s1, b2), s2, b3), inv);
+    v[0] = cadd(v[0], cadd(a1, cadd(a2, a3)));
+    v[1] = cadd(p1, r1);
+    v[6] = csub(p1, r1);
+    v[2] = cadd(p2, r2);
+    v[5] = csub(p2, r2);
+    v[3] = cadd(p3, r3);
+    v[4] = csub(p3, r3);
+  } else if constexpr (R == 2) {
     C const a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -307,7 +396,6 @@ __device__ __forceinline__ void dft_reg(C (&v)[R], C const *tw, int n, bool inv)
     v[1] = cadd(d02, d13);
     v[3] = csub(d02, d13);
   } else if constexpr (R == 8) {
-    using T = decltype(v[0].x);
     T const h = T(0.70710678118654752440084436210484903928);
     C a[4], b[4];
 #pragma unroll
@@ -404,6 +492,42 @@ __device__ __forceinline__ C *lds_fft(C *buf0, C *buf1, int nlines, int ls, FftP
         case 6: stockham_bfly<C, 6>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
         case 7: stockham_bfly<C, 7>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
         default: stockham_bfly<C, 8>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv); break;
+        }
+      }
+    } else if (Ns == 1) {
+      // generic radix as the first stage (no inter-stage twiddles): item kp
+      // of butterfly j gives X_kp and X_(R-kp) from a_m = v_m + v_(R-m),
+      // b_m = v_m - v_(R-m): X_(kp, R-kp) = v_0 + sum_m cos(2 pi m kp / R) a_m
+      // -+ i sum_m sin(2 pi m kp / R) b_m; kp = 0 gives X_0 = sum_r v_r
+      int const half = (R + 1) >> 1;
+      FastDiv const fh = make_fastdiv(half), fnb = make_fastdiv(nb);
+      int const items = nlines * nb * half;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const lj = fdiv(it, fh);
+        int const kp = it - lj * half;
+        int const l = fdiv(lj, fnb);
+        int const j = lj - l * nb;
+        C const *src = in + l * ls;
+        C *dst = out + l * ls;
+        C const v0 = src[j];
+        if (kp == 0) {
+          C s = v0;
+          for (int r = 1; r < R; ++r) s = cadd(s, src[j + r * nb]);
+          dst[j * R] = s;
+        } else {
+          C sa = v0, sb = C{0, 0};
+          int idx = 0;
+          for (int mm = 1; mm < half; ++mm) {
+            idx += kp; // (mm kp) mod R
+            if (idx >= R) idx -= R;
+            C const w = tw[idx * nb]; // exp(-2 pi i idx / R) = cos - i sin
+            C const u = src[j + mm * nb], z = src[j + (R - mm) * nb];
+            sa = caxpy(sa, w.x, cadd(u, z));
+            sb = caxpy(sb, -w.y, csub(u, z));
+          }
+          C const isb = cmul_mi(sb, inv);
+          dst[j * R + kp] = cadd(sa, isb);
+          dst[j * R + R - kp] = csub(sa, isb);
         }
       }
     } else {

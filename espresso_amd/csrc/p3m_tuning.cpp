@@ -1,5 +1,6 @@
 // P3M tuning; see p3m_tuning.hpp.
 #include "p3m_tuning.hpp"
+#include "p3m_batch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -288,14 +289,34 @@ private:
 };
 } // namespace
 
+double fft_stage_cost(int n) {
+  // relative cost per point of the device FFT along a line of length n: one
+  // unit per radix-2..8 stage (constant-twiddle register DFTs); a larger
+  // prime radix R as the first stage computes symmetric output pairs with
+  // (R - 1) / 2 table cosines, about R / 8 units, and one output at a time
+  // with R - 1 table-twiddle products elsewhere, about (R - 1) / 2.5 units
+  FftPlan1D const p = make_fft_plan(n);
+  double c = 0.;
+  for (int s = 0; s < p.nstages; ++s) {
+    int const r = p.radix[s];
+    c += r <= 8 ? 1. : s == 0 ? r / 8. : (r - 1) / 2.5;
+  }
+  return c;
+}
+
 double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l,
                      double pair_ns) {
   // device k-space cost from the measured stage rates of the solver: about
   // 1.3 ns per particle stencil point (spread + gather) and 0.12 ns per mesh
-  // point (FFTs and convolution), plus a fixed launch cost
+  // point (FFTs and convolution) for three-stage lines (n = 128), half of it
+  // scaling with the FFT stage cost of the mesh, plus a fixed launch cost
   double const R = static_cast<double>(mesh[0]) * mesh[1] * mesh[2];
   double const stencil = static_cast<double>(n_q) * cao * cao * cao;
-  return 1.3e-9 * stencil + 1.2e-7 * R + 0.01 + model_real_space_ms(n_q, r_cut, box_l, pair_ns);
+  double const stages =
+      (fft_stage_cost(mesh[0]) + fft_stage_cost(mesh[1]) + fft_stage_cost(mesh[2])) / 3.;
+  double const mesh_ns = 0.12 * (0.5 + 0.5 * stages / 3.);
+  return 1.3e-9 * stencil + 1e-6 * mesh_ns * R + 0.01 +
+         model_real_space_ms(n_q, r_cut, box_l, pair_ns);
 }
 
 double model_real_space_ms(int n_q, double r_cut, Vec3d const &box_l, double pair_ns) {

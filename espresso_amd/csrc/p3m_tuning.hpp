@@ -29,6 +29,9 @@ struct TuningInput {
   int cao_min = 1, cao_max = 7;
 };
 
+/** Relative cost per point of the device FFT along a line of length n. */
+double fft_stage_cost(int n);
+
 /** Modelled cost (ms) of one k-space solve on the device plus the real-space
  *  pair loop at `pair_ns` nanoseconds per pair, for tuning without timing
  *  runs. */

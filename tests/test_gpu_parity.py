@@ -110,6 +110,8 @@ def test_all_orders(cao):
     ([10, 14, 26], [9., 11., 13.]),       # 5, 7, 13; non-cubic box and mesh
     ([16, 20, 24], [10., 12., 14.]),
     ([64, 32, 48], [20., 10., 15.]),
+    ([46, 34, 38], [15., 11., 12.]),      # 2 x 23, 2 x 17, 2 x 19 (paired first stage)
+    ([40, 36, 44], [13., 12., 14.]),      # 8 x 5, 4 x 3 x 3, 11 x 4
 ])
 def test_meshes(mesh, box):
     pos, q = S.uniform_system(2000, 1., 31)
