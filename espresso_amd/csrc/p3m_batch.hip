@@ -318,25 +318,62 @@ struct ChunkSlots {
   int *lo;    // [kChunk] tile index of the first stencil point
 };
 
+/** Weights of a chunk of cnt particles, one thread per (particle, dimension):
+ *  thread 3 p + d writes the P weights of dimension d of particle p; thread
+ *  3 p also writes the charge and the tile index of the first stencil point
+ *  (it evaluates the integer coordinates of all three dimensions). */
 template <int P>
 __device__ __forceinline__ void stage_chunk(DevGeo const &g, double4 const *spos, int64_t c0,
                                             int cnt, int o0, int o1, int o2, int TY, int TZ,
                                             ChunkSlots s) {
   int const t = threadIdx.x;
-  if (t < cnt) {
-    double4 const p = spos[c0 + t];
-    double const x[3] = {p.x, p.y, p.z};
-    int nmp[3], gi[3];
-    double dist[3];
-    particle_coordinates<P>(g, x, nmp, gi, dist);
-    double *w = s.w + t * 3 * P;
-    bspline_weights<P>(dist[0], w);
-    bspline_weights<P>(dist[1], w + P);
-    bspline_weights<P>(dist[2], w + 2 * P);
-    s.qq[t] = p.w;
-    s.lo[t] = ((gi[0] - o0) * TY + (gi[1] - o1)) * TZ + (gi[2] - o2);
+  if (t < 3 * cnt) {
+    int const p = t / 3;
+    int const dd = t - 3 * p;
+    double4 const pp = spos[c0 + p];
+    double const x[3] = {pp.x, pp.y, pp.z};
+    int nmp, gi;
+    double dist;
+    coordinate_1d(g, P, x[dd], dd, nmp, gi, dist);
+    bspline_weights<P>(dist, s.w + p * 3 * P + dd * P);
+    if (dd == 0) {
+      int n1, g1, n2, g2;
+      double d1, d2;
+      coordinate_1d(g, P, x[1], 1, n1, g1, d1);
+      coordinate_1d(g, P, x[2], 2, n2, g2, d2);
+      s.qq[p] = pp.w;
+      s.lo[p] = ((gi - o0) * TY + (g1 - o1)) * TZ + (g2 - o2);
+    }
   }
 }
+
+/** Row mapping of a brick tile of T^3 points (T = 8 + P - 1 <= 16) over a
+ *  block of kPG threads: lane c = threadIdx.x % 16 covers z offset c (active
+ *  for c < T), thread row threadIdx.x / 16 + k kPG / 16 covers tile row
+ *  r = a T + b (tile index r T + c).  One periodic wrap per coordinate. */
+template <int P> struct TileRows {
+  static constexpr int T = kBrick + P - 1;
+  static constexpr int ROWS = T * T;
+  static constexpr int RPP = kPG / 16;
+  static constexpr int NPASS = (ROWS + RPP - 1) / RPP;
+  static_assert(T <= 16, "tile rows wider than 16 lanes");
+  int c, row0, zz;
+  bool lane_ok;
+  __device__ TileRows(int o2, int M2) {
+    c = threadIdx.x & 15;
+    row0 = threadIdx.x >> 4;
+    lane_ok = c < T;
+    zz = wrap_once(o2 + c, M2);
+  }
+  __device__ bool valid(int k) const { return lane_ok && row0 + k * RPP < ROWS; }
+  __device__ int row(int k) const { return row0 + k * RPP; }
+  __device__ int64_t mesh_index(int k, int o0, int o1, int M0, int M1, int M2) const {
+    int const r = row(k);
+    int const a = r / T;
+    int const b = r - a * T;
+    return ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + b, M1)) * M2 + zz;
+  }
+};
 
 /** Lane segments over the (j, k) stencil plane: S = next power of two >= P^2
  *  lanes per particle, ppw particles per wave64. */
@@ -407,7 +444,7 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
-    __syncthreads();
+    if (c0 != beg) __syncthreads(); // the tile zeroing and the slots are disjoint
     stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
     __syncthreads();
     int const ngroups = (cnt + ppw - 1) / ppw;
@@ -434,15 +471,12 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   __syncthreads();
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
   MT *rho = static_cast<MT *>(d.rho);
-  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
-    double const v = tile[e];
-    if (v != 0.) {
-      int const a = e / TYZ;
-      int const rr = e - a * TYZ;
-      int const bb = rr / T;
-      int const c = rr - bb * T;
-      int const x = wrap_once(o0 + a, M0), y = wrap_once(o1 + bb, M1), z = wrap_once(o2 + c, M2);
-      unsafeAtomicAdd(rho + ((int64_t)x * M1 + y) * M2 + z, static_cast<MT>(v));
+  TileRows<P> const tr(o2, M2);
+#pragma unroll
+  for (int k = 0; k < TileRows<P>::NPASS; ++k) {
+    if (tr.valid(k)) {
+      double const v = tile[tr.row(k) * T + tr.c];
+      if (v != 0.) unsafeAtomicAdd(rho + tr.mesh_index(k, o0, o1, M0, M1, M2), static_cast<MT>(v));
     }
   }
 }
@@ -884,27 +918,42 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
 // accumulated in FP64 per particle (plus the dipole term for non-metallic
 // epsilon, p3m.cpp:538-547).
 // ---------------------------------------------------------------------------
+/** Field tiles of a brick fetched into registers with the row mapping, the
+ *  first chunk of particles staged while those loads are in flight, then the
+ *  tiles stored to LDS; ends with a barrier. */
 template <int P, typename MT>
-__device__ __forceinline__ void load_field_tiles(SysDesc const &d, int o0, int o1, int o2, MT *tx,
-                                                 MT *ty, MT *tz) {
+__device__ __forceinline__ void fetch_tiles_and_stage(SysDesc const &d, double4 const *sp, int beg,
+                                                      int cnt, int o0, int o1, int o2, MT *tx,
+                                                      MT *ty, MT *tz, ChunkSlots s) {
   constexpr int T = kBrick + P - 1;
-  constexpr int TYZ = T * T;
-  constexpr int tile_n = T * TYZ;
+  constexpr int NP = TileRows<P>::NPASS;
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
   MT const *Ex = static_cast<MT const *>(d.rho);
   MT const *Ey = static_cast<MT const *>(d.Ey);
   MT const *Ez = static_cast<MT const *>(d.Ez);
-  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) {
-    int const a = e / TYZ;
-    int const rr = e - a * TYZ;
-    int const bb = rr / T;
-    int const c = rr - bb * T;
-    int64_t const gidx =
-        ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + bb, M1)) * M2 + wrap_once(o2 + c, M2);
-    tx[e] = Ex[gidx];
-    ty[e] = Ey[gidx];
-    tz[e] = Ez[gidx];
+  TileRows<P> const tr(o2, M2);
+  MT vx[NP], vy[NP], vz[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    vx[k] = vy[k] = vz[k] = MT(0);
+    if (tr.valid(k)) {
+      int64_t const g = tr.mesh_index(k, o0, o1, M0, M1, M2);
+      vx[k] = Ex[g];
+      vy[k] = Ey[g];
+      vz[k] = Ez[g];
+    }
   }
+  stage_chunk<P>(d.g, sp, beg, cnt, o0, o1, o2, T, T, s);
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    if (tr.valid(k)) {
+      int const e = tr.row(k) * T + tr.c;
+      tx[e] = vx[k];
+      ty[e] = vy[k];
+      tz[e] = vz[k];
+    }
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ void apply_force(double *force, int64_t idx, double qv,
@@ -948,11 +997,11 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   if (beg == end) return;
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
-  load_field_tiles<P, MT>(d, o0, o1, o2, tx, ty, tz);
   double dm[3] = {0., 0., 0.};
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
+  fetch_tiles_and_stage<P, MT>(d, sp, beg, min(kChunk, end - beg), o0, o1, o2, tx, ty, tz, s);
   int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
@@ -966,9 +1015,11 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   int const r0 = lane - sub * S;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
-    __syncthreads();
-    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
-    __syncthreads();
+    if (c0 != beg) { // the first chunk was staged with the tiles
+      __syncthreads();
+      stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
+      __syncthreads();
+    }
     int const ngroups = (cnt + ppw - 1) / ppw;
     for (int grp = wave; grp < ngroups; grp += nwaves) {
       int const p = grp * ppw + sub;
@@ -1044,11 +1095,11 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   if (beg == end) return;
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
-  load_field_tiles<P, MT>(d, o0, o1, o2, tx, ty, tz);
   double dm[3] = {0., 0., 0.};
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
+  fetch_tiles_and_stage<P, MT>(d, sp, beg, min(kChunk, end - beg), o0, o1, o2, tx, ty, tz, s);
   int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   int const lane = threadIdx.x & 63;
@@ -1059,9 +1110,11 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   bool const active = li < P;
   for (int c0 = beg; c0 < end; c0 += kChunk) {
     int const cnt = min(kChunk, end - c0);
-    __syncthreads();
-    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
-    __syncthreads();
+    if (c0 != beg) { // the first chunk was staged with the tiles
+      __syncthreads();
+      stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
+      __syncthreads();
+    }
     int const ngroups = (cnt + ppw - 1) / ppw;
     for (int grp = wave; grp < ngroups; grp += nwaves) {
       int const p = grp * ppw + sub;

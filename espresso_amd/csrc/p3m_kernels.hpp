@@ -45,6 +45,7 @@ template <typename C> __host__ __device__ __forceinline__ C cmul_mi(C a, bool in
 template <int P> struct BsplineTable {
   double num[P][P];
   double den[P];
+  double inv_den[P]; // 1 / den, correctly rounded at compile time
 };
 
 constexpr long long cx_binom(int n, int k) {
@@ -87,12 +88,13 @@ template <int P> constexpr BsplineTable<P> make_bspline_table() {
     for (int m = 0; m < P; ++m) g = cx_gcd(g, c[m]);
     for (int m = 0; m < P; ++m) t.num[i][m] = static_cast<double>(c[m] / g);
     t.den[i] = static_cast<double>(den / g);
+    t.inv_den[i] = 1.0 / t.den[i];
   }
   return t;
 }
 
 /** w[i] = B-spline weight of stencil point i at offset x (FP64 Horner,
- *  integer-coefficient numerator divided by its denominator). */
+ *  integer-coefficient numerator times the reciprocal of its denominator). */
 template <int P> __device__ __forceinline__ void bspline_weights(double x, double *w) {
   constexpr BsplineTable<P> T = make_bspline_table<P>();
 #pragma unroll
@@ -100,7 +102,7 @@ template <int P> __device__ __forceinline__ void bspline_weights(double x, doubl
     double acc = T.num[i][P - 1];
 #pragma unroll
     for (int m = P - 2; m >= 0; --m) acc = T.num[i][m] + x * acc;
-    w[i] = acc / T.den[i];
+    w[i] = acc * T.inv_den[i];
   }
 }
 
@@ -233,6 +235,31 @@ __device__ __forceinline__ bool particle_coordinates_rt(DevGeo const &g, int P, 
   }
   return ok;
 }
+/** One dimension d of particle_coordinates_rt (same operations, so the
+ *  result is identical). */
+__device__ __forceinline__ bool coordinate_1d(DevGeo const &g, int P, double x, int d, int &nmp,
+                                              int &gi, double &dist) {
+  bool ok = true;
+  double t = ca_coordinate(x, g.ld_pos[d], g.ai[d], g.pos_shift);
+  if (!(t >= 0. && t < static_cast<double>(g.dim[d] - P + 1))) {
+    double xf = x - g.box_l[d] * floor(x / g.box_l[d]);
+    if (xf >= g.box_l[d]) xf -= g.box_l[d];
+    t = ca_coordinate(xf, g.ld_pos[d], g.ai[d], g.pos_shift);
+    if (!(t >= 0. && t < static_cast<double>(g.dim[d] - P + 1))) {
+      ok = false;
+      t = 0.;
+    }
+  }
+  int const n = static_cast<int>(t);
+  nmp = n;
+  dist = ca_distance(t, n);
+  int gg = n - g.marg[d];
+  gg %= g.M[d];
+  if (gg < 0) gg += g.M[d];
+  gi = gg;
+  return ok;
+}
+
 template <int P>
 __device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double const x[3],
                                                      int nmp[3], int gi[3], double dist[3]) {
