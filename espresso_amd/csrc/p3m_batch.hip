@@ -86,14 +86,27 @@ __device__ __forceinline__ int find_slot(int const *__restrict__ wt, int nsel, i
   return lo;
 }
 
+/** Work-table index of this block.  Workgroups are dealt to the 8 XCDs
+ *  round-robin by blockIdx.x; the index is remapped so that each XCD takes a
+ *  contiguous range of the table (neighbouring bricks and lines, whose halo
+ *  tiles overlap, then share that XCD's L2).  Bijective on [0, gridDim.x). */
+__device__ __forceinline__ int xcd_block() {
+  int const n = gridDim.x, b = blockIdx.x;
+  int const x = b & 7, k = b >> 3;
+  int const q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + k;
+}
+
 struct Slot {
-  int slot, s, lb;
+  int slot, s, lb, vb; // table slot, system, block within the system, table index
 };
+template <bool XCD = true>
 __device__ __forceinline__ Slot locate(int const *wt, int const *sel, int nsel) {
   Slot r;
-  r.slot = find_slot(wt, nsel, blockIdx.x);
+  r.vb = XCD ? xcd_block() : static_cast<int>(blockIdx.x);
+  r.slot = find_slot(wt, nsel, r.vb);
   r.s = sel[r.slot];
-  r.lb = blockIdx.x - wt[r.slot];
+  r.lb = r.vb - wt[r.slot];
   return r;
 }
 
@@ -173,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
                                                 int *counts, int *p_brick, int *p_rank,
                                                 int *p_nmp, double *partials, int *err_flag) {
   __shared__ double red[8 * 16];
-  Slot const L = locate(wt, sel, nsel);
+  Slot const L = locate<false>(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int64_t const li = (int64_t)L.lb * blockDim.x + threadIdx.x;
   double v[7] = {0., 0., 0., 0., 0., 0., 0.};
@@ -208,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
   }
   block_sum<7>(v, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 7; ++k) partials[blockIdx.x * 8 + k] = v[k];
+    for (int k = 0; k < 7; ++k) partials[L.vb * 8 + k] = v[k];
 }
 
 /** per-system deterministic reduction of block partials: one block per
@@ -651,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
   }
   block_sum<8>(acc, red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 8; ++k) partials[blockIdx.x * 8 + k] = acc[k];
+    for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
   if (!want_fields) return;
   C *ires = lds_fft(oth, res, 2 * ncol, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
@@ -752,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   using C = cx_t<MT>;
   extern __shared__ double smem_d[];
   __shared__ double red[8 * 16];
-  Slot const L = locate(wt, sel, nsel);
+  Slot const L = locate<false>(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int const kz = L.lb;
   int const Mx = d.g.M[0], My = d.g.M[1], Mz = d.g.M[2];
@@ -813,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   }
   block_sum<8>(acc, red); // ends with a barrier: q is complete
   if (threadIdx.x == 0)
-    for (int k = 0; k < 8; ++k) partials[blockIdx.x * 8 + k] = acc[k];
+    for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
   if (!want_fields) return;
   C *const outs[3] = {static_cast<C *>(d.B), khat, static_cast<C *>(d.A)};
   for (int comp = 0; comp < 3; ++comp) {
