@@ -749,13 +749,17 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   }
 }
 
-/** Fused y/x pass for meshes whose x-y plane fits in LDS three times
- *  (kz-major spectra), one block per kz column: forward c2c along y and x,
- *  convolution with the influence function with energy and virial partial
- *  sums (p3m.cpp:393-449, 551-567), then for each field component
- *  (i k_x phi, i k_y phi, phi) the inverse transforms along x and y.
- *  Outputs: B <- x field, khat <- y field, A <- z field before i k_z (which
- *  the c2r z pass applies), all still transformed along z. */
+/** Fused y/x pass for meshes whose x-y plane fits in LDS twice (kz-major
+ *  spectra), one block per kz column: forward c2c along y and x, convolution
+ *  with the influence function with energy and virial partial sums
+ *  (p3m.cpp:393-449, 551-567), then the inverse transforms of the three field
+ *  components (i k_x phi, i k_y phi, phi) along x and y.  k_y and k_z are
+ *  constant along an x line, so the y and z components share one inverse x
+ *  transform a = IFFT_x(phi): five inverse line passes instead of six.  The
+ *  block's own khat column keeps phi, then a, between the passes (two LDS
+ *  planes instead of three).  Outputs: B <- x field, khat <- y field,
+ *  A <- z field before i k_z (which the c2r z pass applies), all still
+ *  transformed along z. */
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restrict__ sys,
                                                         int const *__restrict__ sel,
@@ -777,14 +781,12 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   C *twy = twx + Mx;
   C *P0 = twy + My;
   C *P1 = P0 + plane;
-  C *P2 = P1 + plane;
   load_twiddles(twx, tw + d.tw_off[0], Mx);
   load_twiddles(twy, tw + d.tw_off[1], My);
-  C *khat = static_cast<C *>(d.khat);
-  C const *src = khat + (int64_t)kz * pl;
+  C *const col = static_cast<C *>(d.khat) + (int64_t)kz * pl; // this block's column
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
     int const x = fdiv(e, fMy), y = e - x * My;
-    P0[x * lsy + y] = src[e];
+    P0[x * lsy + y] = col[e];
   }
   __syncthreads();
   C *r = lds_fft(P0, P1, Mx, lsy, d.plan[1], twy, false); // along y, layout [x][y]
@@ -828,34 +830,65 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   if (threadIdx.x == 0)
     for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
   if (!want_fields) return;
-  C *const outs[3] = {static_cast<C *>(d.B), khat, static_cast<C *>(d.A)};
-  for (int comp = 0; comp < 3; ++comp) {
-    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-      int const y = fdiv(e, fMx), x = e - y * Mx;
-      C const v = q[y * lsx + x];
-      if (comp == 2) {
-        P2[y * lsx + x] = v;
-      } else {
-        MT const kf = (comp == 0) ? static_cast<MT>(dop_shift(x, Mx)) * static_cast<MT>(wvx)
-                                  : static_cast<MT>(dop_shift(y, My)) * static_cast<MT>(wvy);
-        P2[y * lsx + x] = C{-kf * v.y, kf * v.x};
-      }
-    }
-    __syncthreads();
-    C *a = lds_fft(P2, w, My, lsx, d.plan[0], twx, true); // inverse along x
-    C *b = (a == P2) ? w : P2;
-    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-      int const x = fdiv(e, fMy), y = e - x * My;
-      b[x * lsy + y] = a[y * lsx + x];
-    }
-    __syncthreads();
-    C *c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true); // inverse along y
-    C *dst = outs[comp] + (int64_t)kz * pl;
-    for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-      int const x = fdiv(e, fMy), y = e - x * My;
-      dst[e] = c[x * lsy + y];
-    }
-    __syncthreads();
+  C *const Bp = static_cast<C *>(d.B) + (int64_t)kz * pl;
+  C *const Ap = static_cast<C *>(d.A) + (int64_t)kz * pl;
+  // phi = G rho^ to the column ([y][x] order) and i k_x phi into w
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const y = fdiv(e, fMx), x = e - y * Mx;
+    C const v = q[y * lsx + x];
+    col[e] = v;
+    MT const kf = static_cast<MT>(dop_shift(x, Mx)) * static_cast<MT>(wvx);
+    w[y * lsx + x] = C{-kf * v.y, kf * v.x};
+  }
+  __syncthreads();
+  // x component: inverse along x, then along y, into B
+  C *a = lds_fft(w, q, My, lsx, d.plan[0], twx, true);
+  C *b = (a == w) ? q : w;
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    b[x * lsy + y] = a[y * lsx + x];
+  }
+  __syncthreads();
+  C *c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true);
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    Bp[e] = c[x * lsy + y];
+  }
+  __syncthreads(); // c may be P0
+  // a = IFFT_x(phi), shared by the y and z components
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const y = fdiv(e, fMx), x = e - y * Mx;
+    P0[y * lsx + x] = col[e];
+  }
+  __syncthreads();
+  a = lds_fft(P0, P1, My, lsx, d.plan[0], twx, true);
+  b = (a == P0) ? P1 : P0;
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const y = fdiv(e, fMx), x = e - y * Mx;
+    C const v = a[y * lsx + x];
+    col[e] = v;
+    b[x * lsy + y] = v;
+  }
+  __syncthreads();
+  // z component (i k_z is applied by the c2r pass): inverse along y into A
+  c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true);
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    Ap[e] = c[x * lsy + y];
+  }
+  __syncthreads(); // c may be P0
+  // y component: i k_y a, inverse along y into this column (khat)
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const y = fdiv(e, fMx), x = e - y * Mx;
+    C const v = col[e];
+    MT const kf = static_cast<MT>(dop_shift(y, My)) * static_cast<MT>(wvy);
+    P0[x * lsy + y] = C{-kf * v.y, kf * v.x};
+  }
+  __syncthreads();
+  c = lds_fft(P0, P1, Mx, lsy, d.plan[1], twy, true);
+  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
+    int const x = fdiv(e, fMy), y = e - x * My;
+    col[e] = c[x * lsy + y];
   }
 }
 
@@ -1303,10 +1336,10 @@ int tile_points(int cao) {
   return t * t * t;
 }
 
-/** LDS of the fused y/x pass: both twiddle tables and three x-y planes. */
+/** LDS of the fused y/x pass: both twiddle tables and two x-y planes. */
 size_t fused_lds_bytes(int mx, int my, size_t csz) {
   size_t const plane = static_cast<size_t>(std::max(mx * (my + 1), my * (mx + 1)));
-  return (static_cast<size_t>(mx + my) + 3 * plane) * csz;
+  return (static_cast<size_t>(mx + my) + 2 * plane) * csz;
 }
 bool uses_fused_pass(int mx, int my, MeshPrecision prec, LaunchShape const &shape) {
   size_t const csz = (prec == MeshPrecision::FP64) ? 16 : 8;
