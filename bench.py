@@ -32,19 +32,18 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "hbm_traffic_c2_n1e6_mesh128_cao7.json")
-GATHER_KERNEL = {"plane": "k_gather_plane<7, double, 64>",
-                 "plane16": "k_gather_plane<7, double, 16>",
-                 "plane8": "k_gather_plane<7, double, 8>",
-                 "layer": "k_gather_layer<7, double>"}
-SPREAD_KERNEL = {"plane": "k_spread<7, double, 64>", "plane16": "k_spread<7, double, 16>",
-                 "plane8": "k_spread<7, double, 8>"}
+SEGMENT = {"plane": 64, "plane16": 16, "plane8": 8}
 
 
 def stage_kernels(shape):
     """Kernel behind each single-kernel stage of the FP64 solve at cao 7 and
     mesh 128 (split FFT passes) for the chosen launch shape."""
-    return {"spread": SPREAD_KERNEL[shape["spread"]], "convolve": "k_fft_x_conv<double>",
-            "gather": GATHER_KERNEL[shape["gather"]]}
+    t = shape["threads"]
+    g = shape["gather"]
+    gather = (f"k_gather_layer<7, double, {t}>" if g == "layer"
+              else f"k_gather_plane<7, double, {SEGMENT[g]}, {t}>")
+    return {"spread": f"k_spread<7, double, {SEGMENT[shape['spread']]}, {t}>",
+            "convolve": "k_fft_x_conv<double>", "gather": gather}
 
 
 def make_system(n=1_000_000, seed=1002):

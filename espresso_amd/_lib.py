@@ -41,7 +41,8 @@ EXPORTED_SYMBOLS = (
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
     "p3m_batch_particle_offset", "p3m_batch_system_bytes",
     "p3m_batch_working_bytes", "p3m_batch_solve",
-    "p3m_batch_graph_solve", "p3m_batch_set_launch_shape", "p3m_batch_autotune",
+    "p3m_batch_graph_solve", "p3m_batch_set_launch_shape", "p3m_batch_get_launch_shape",
+    "p3m_batch_autotune",
     "p3m_batch_synchronize", "p3m_batch_get_stream", "p3m_batch_profile_begin",
     "p3m_batch_profile_end", "p3m_batch_last_error", "p3m_tune_model",
     "p3m_serve_create", "p3m_serve_destroy", "p3m_serve_run",
@@ -70,6 +71,11 @@ class p3m_params(C.Structure):
         ("verbose", C.c_int),
         ("charge_neutrality_tolerance", C.c_double),
     ]
+
+
+class p3m_launch_shape(C.Structure):
+    _fields_ = [(name, C.c_int) for name in
+                ("gather", "ncol", "zlines", "fused_lds", "spread", "threads", "chunk")]
 
 
 _lib = None
@@ -119,9 +125,8 @@ def load():
     L.p3m_profile_begin.argtypes = [vp, C.c_int]
     L.p3m_profile_end.argtypes = [vp, dp, C.POINTER(C.c_int)]
     L.p3m_synchronize.argtypes = [vp]
-    _ipp = C.POINTER(C.c_int)
-    L.p3m_autotune.argtypes = [vp, C.c_int64, vp, vp, C.c_int, _ipp, _ipp, _ipp,
-                               _ipp, _ipp, dp, dp]
+    shp = C.POINTER(p3m_launch_shape)
+    L.p3m_autotune.argtypes = [vp, C.c_int64, vp, vp, C.c_int, shp, dp, dp]
     L.p3m_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
@@ -136,10 +141,9 @@ def load():
     L.p3m_batch_working_bytes.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
-    L.p3m_batch_set_launch_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int,
-                                             C.c_int, C.c_int]
-    L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, ip,
-                                     ip, ip, ip, ip, dp, dp]
+    L.p3m_batch_set_launch_shape.argtypes = [vp, shp]
+    L.p3m_batch_get_launch_shape.argtypes = [vp, shp]
+    L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, shp, dp, dp]
     L.p3m_batch_synchronize.argtypes = [vp]
     L.p3m_batch_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_batch_profile_begin.argtypes = [vp, C.c_int]

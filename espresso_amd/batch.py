@@ -39,13 +39,18 @@ def _device_ptr(t, name, dtype_size, numel):
     return t.data_ptr()
 
 
-def shape_dict(gather, ncol, zlines, fused_lds, spread, best_ms, default_ms):
-    """Launch shape returned by the autotuners, with mapping names."""
+def shape_dict(shape, best_ms=None, default_ms=None):
+    """Launch shape (a p3m_launch_shape) as a dict with mapping names, plus
+    the autotuner's mean solve times when given."""
     gnames = {v: k for k, v in P3MBatch.GATHER.items()}
     snames = {v: k for k, v in P3MBatch.SPREAD.items()}
-    return {"gather": gnames.get(gather, gather), "ncol": ncol, "zlines": zlines,
-            "fused_lds": fused_lds, "spread": snames.get(spread, spread),
-            "best_ms": best_ms, "default_ms": default_ms}
+    d = {"gather": gnames.get(shape.gather, shape.gather), "ncol": shape.ncol,
+         "zlines": shape.zlines, "fused_lds": shape.fused_lds,
+         "spread": snames.get(shape.spread, shape.spread), "threads": shape.threads,
+         "chunk": shape.chunk}
+    if best_ms is not None:
+        d.update(best_ms=best_ms, default_ms=default_ms)
+    return d
 
 
 class P3MBatch:
@@ -178,31 +183,39 @@ class P3MBatch:
     SPREAD = {"plane": 0, "plane16": 1, "plane8": 2}
 
     def set_launch_shape(self, gather="plane16", ncol=8, zlines=16,
-                         fused_lds=96 * 1024, spread="plane"):
+                         fused_lds=96 * 1024, spread="plane", threads=512, chunk=64):
         """Gather mapping ("layer", "plane", "plane16", "plane8" or 0-3),
         FFT columns per block (split y/x passes), z-line pairs per block, the
         LDS bytes up to which a system's x-y planes are transformed by the
-        fused y/x pass (0: never), and the spread mapping ("plane",
-        "plane16", "plane8" or 0-2)."""
+        fused y/x pass (0: never), the spread mapping ("plane", "plane16",
+        "plane8" or 0-2), and the block size (256 or 512) and particles per
+        staged chunk (1-64, at most threads / 3) of the spread and gather."""
         g = self.GATHER[gather] if isinstance(gather, str) else int(gather)
         sp = self.SPREAD[spread] if isinstance(spread, str) else int(spread)
-        self._check(_lib.load().p3m_batch_set_launch_shape(
-            self._b, g, int(ncol), int(zlines), int(fused_lds), sp))
+        shape = _lib.p3m_launch_shape(g, int(ncol), int(zlines), int(fused_lds), sp,
+                                      int(threads), int(chunk))
+        self._check(_lib.load().p3m_batch_set_launch_shape(self._b, C.byref(shape)))
+
+    def launch_shape(self):
+        """The launch shape of later solves (dict with mapping names)."""
+        shape = _lib.p3m_launch_shape()
+        self._check(_lib.load().p3m_batch_get_launch_shape(self._b, C.byref(shape)))
+        return shape_dict(shape)
 
     def autotune(self, pos, q, scratch_force, systems=None, reps=3):
-        """Pick the fastest launch shape (gather mapping, FFT columns, z
-        lines) for force solves of the selection by timing every candidate;
-        forces accumulate into ``scratch_force``.  Returns the chosen shape
-        and the mean solve times before and after."""
+        """Pick the fastest launch shape (fused pass, particle-mesh blocks,
+        gather and spread mappings, FFT columns, z lines) for force solves of
+        the selection by timing every candidate; forces accumulate into
+        ``scratch_force``.  Returns the chosen shape and the mean solve times
+        before and after."""
         p, qq, f = self._arrays(pos, q, scratch_force)
         nsel, ids = self._selection(systems)
-        gp, nc, zl, fl, sp = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        shape = _lib.p3m_launch_shape()
         best, default = C.c_double(), C.c_double()
         self._check(_lib.load().p3m_batch_autotune(
-            self._b, nsel, ids, p, qq, f, int(reps), C.byref(gp), C.byref(nc),
-            C.byref(zl), C.byref(fl), C.byref(sp), C.byref(best), C.byref(default)))
-        return shape_dict(gp.value, nc.value, zl.value, fl.value, sp.value,
-                          best.value, default.value)
+            self._b, nsel, ids, p, qq, f, int(reps), C.byref(shape), C.byref(best),
+            C.byref(default)))
+        return shape_dict(shape, best.value, default.value)
 
     def synchronize(self):
         self._check(_lib.load().p3m_batch_synchronize(self._b))

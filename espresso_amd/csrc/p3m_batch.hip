@@ -49,8 +49,7 @@ namespace {
 using namespace dev;
 
 constexpr int kBlock = 256;  // block size of the mesh and particle-list kernels
-constexpr int kPG = 512;     // block size of the particle-mesh kernels
-constexpr int kChunk = 64;   // particles staged per brick pass
+constexpr int kChunkMax = 64; // most particles staged per brick pass (3 x chunk <= threads)
 constexpr int kBrick = 8;    // first-stencil points per brick edge
 constexpr int kFill = 1024;  // mesh elements zeroed per block
 
@@ -325,10 +324,17 @@ __global__ __launch_bounds__(kBlock) void k_fill_rho(SysDesc const *__restrict__
 // this scattered pattern on gfx950, and the sum is rounded once) and the
 // tile is flushed into the periodic mesh with device-scope atomics.
 // ---------------------------------------------------------------------------
+/** Bytes of the chunk slots ([chunk][3P] weights, [chunk] charges, [chunk]
+ *  tile indices) rounded up to 16, so that the brick tile after them is
+ *  16-byte aligned for any chunk (LDS b64 accesses must be aligned). */
+__host__ __device__ constexpr int chunk_slot_bytes(int P, int chunk) {
+  return (chunk * (3 * P + 1) * 8 + chunk * 4 + 15) & ~15;
+}
+
 struct ChunkSlots {
-  double *w;  // [kChunk][3P]
-  double *qq; // [kChunk]
-  int *lo;    // [kChunk] tile index of the first stencil point
+  double *w;  // [chunk][3P]
+  double *qq; // [chunk]
+  int *lo;    // [chunk] tile index of the first stencil point
 };
 
 /** Weights of a chunk of cnt particles, one thread per (particle, dimension):
@@ -361,13 +367,13 @@ __device__ __forceinline__ void stage_chunk(DevGeo const &g, double4 const *spos
 }
 
 /** Row mapping of a brick tile of T^3 points (T = 8 + P - 1 <= 16) over a
- *  block of kPG threads: lane c = threadIdx.x % 16 covers z offset c (active
- *  for c < T), thread row threadIdx.x / 16 + k kPG / 16 covers tile row
+ *  block of NTH threads: lane c = threadIdx.x % 16 covers z offset c (active
+ *  for c < T), thread row threadIdx.x / 16 + k NTH / 16 covers tile row
  *  r = a T + b (tile index r T + c).  One periodic wrap per coordinate. */
-template <int P> struct TileRows {
+template <int P, int NTH> struct TileRows {
   static constexpr int T = kBrick + P - 1;
   static constexpr int ROWS = T * T;
-  static constexpr int RPP = kPG / 16;
+  static constexpr int RPP = NTH / 16;
   static constexpr int NPASS = (ROWS + RPP - 1) / RPP;
   static_assert(T <= 16, "tile rows wider than 16 lanes");
   int c, row0, zz;
@@ -418,22 +424,23 @@ template <int P> __device__ __forceinline__ void brick_origin(SysDesc const &d, 
   o2 = bz * kBrick;
 }
 
-template <int P, typename MT, int SEGMAX>
-__global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
-                                                int const *__restrict__ sel,
-                                                int const *__restrict__ wt, int nsel,
-                                                int const *__restrict__ soff,
-                                                int const *__restrict__ offsets,
-                                                double4 const *__restrict__ spos) {
+template <int P, typename MT, int SEGMAX, int NTH>
+__global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
+                                               int const *__restrict__ sel,
+                                               int const *__restrict__ wt, int nsel,
+                                               int const *__restrict__ soff,
+                                               int const *__restrict__ offsets,
+                                               double4 const *__restrict__ spos, int chunk) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
   constexpr int tile_n = T * TYZ;
   extern __shared__ double smem_d[];
   ChunkSlots s;
   s.w = smem_d;
-  s.qq = s.w + kChunk * 3 * P;
-  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
-  double *tile = reinterpret_cast<double *>(s.lo + kChunk);
+  s.qq = s.w + chunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + chunk);
+  double *tile = reinterpret_cast<double *>(reinterpret_cast<char *>(smem_d) +
+                                              chunk_slot_bytes(P, chunk));
 
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
@@ -455,8 +462,8 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   int const sub = lane / S;
   int const r0 = lane - sub * S;
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
-  for (int c0 = beg; c0 < end; c0 += kChunk) {
-    int const cnt = min(kChunk, end - c0);
+  for (int c0 = beg; c0 < end; c0 += chunk) {
+    int const cnt = min(chunk, end - c0);
     if (c0 != beg) __syncthreads(); // the tile zeroing and the slots are disjoint
     stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
     __syncthreads();
@@ -484,9 +491,9 @@ __global__ __launch_bounds__(kPG) void k_spread(SysDesc const *__restrict__ sys,
   __syncthreads();
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
   MT *rho = static_cast<MT *>(d.rho);
-  TileRows<P> const tr(o2, M2);
+  TileRows<P, NTH> const tr(o2, M2);
 #pragma unroll
-  for (int k = 0; k < TileRows<P>::NPASS; ++k) {
+  for (int k = 0; k < TileRows<P, NTH>::NPASS; ++k) {
     if (tr.valid(k)) {
       double const v = tile[tr.row(k) * T + tr.c];
       if (v != 0.) unsafeAtomicAdd(rho + tr.mesh_index(k, o0, o1, M0, M1, M2), static_cast<MT>(v));
@@ -967,17 +974,17 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
 /** Field tiles of a brick fetched into registers with the row mapping, the
  *  first chunk of particles staged while those loads are in flight, then the
  *  tiles stored to LDS; ends with a barrier. */
-template <int P, typename MT>
+template <int P, typename MT, int NTH>
 __device__ __forceinline__ void fetch_tiles_and_stage(SysDesc const &d, double4 const *sp, int beg,
                                                       int cnt, int o0, int o1, int o2, MT *tx,
                                                       MT *ty, MT *tz, ChunkSlots s) {
   constexpr int T = kBrick + P - 1;
-  constexpr int NP = TileRows<P>::NPASS;
+  constexpr int NP = TileRows<P, NTH>::NPASS;
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
   MT const *Ex = static_cast<MT const *>(d.rho);
   MT const *Ey = static_cast<MT const *>(d.Ey);
   MT const *Ez = static_cast<MT const *>(d.Ez);
-  TileRows<P> const tr(o2, M2);
+  TileRows<P, NTH> const tr(o2, M2);
   MT vx[NP], vy[NP], vz[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
@@ -1018,21 +1025,21 @@ __device__ __forceinline__ void apply_force(double *force, int64_t idx, double q
  *  conflicts), reduced with 64-lane DPP and lane reads.  SEGMAX = 16: at most
  *  16 lanes per particle, each sweeping (j, k) pairs r0, r0 + 16, ...; four
  *  particles per wave and a 4-step DPP reduction. */
-template <int P, typename MT, int SEGMAX>
-__global__ __launch_bounds__(kPG) void k_gather_plane(
+template <int P, typename MT, int SEGMAX, int NTH>
+__global__ __launch_bounds__(NTH) void k_gather_plane(
     SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
     int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
-    double4 const *__restrict__ spos,
-    int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
+    double4 const *__restrict__ spos, int const *__restrict__ sidx, double *__restrict__ force,
+    double const *__restrict__ sums, int chunk) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
   constexpr int tile_n = T * TYZ;
   extern __shared__ double smem_d[];
   ChunkSlots s;
   s.w = smem_d;
-  s.qq = s.w + kChunk * 3 * P;
-  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
-  MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
+  s.qq = s.w + chunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + chunk);
+  MT *tx = reinterpret_cast<MT *>(reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
   MT *ty = tx + tile_n;
   MT *tz = ty + tile_n;
 
@@ -1047,7 +1054,7 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
-  fetch_tiles_and_stage<P, MT>(d, sp, beg, min(kChunk, end - beg), o0, o1, o2, tx, ty, tz, s);
+  fetch_tiles_and_stage<P, MT, NTH>(d, sp, beg, min(chunk, end - beg), o0, o1, o2, tx, ty, tz, s);
   int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
@@ -1059,8 +1066,8 @@ __global__ __launch_bounds__(kPG) void k_gather_plane(
   int const nwaves = blockDim.x >> 6;
   int const sub = lane / S;
   int const r0 = lane - sub * S;
-  for (int c0 = beg; c0 < end; c0 += kChunk) {
-    int const cnt = min(kChunk, end - c0);
+  for (int c0 = beg; c0 < end; c0 += chunk) {
+    int const cnt = min(chunk, end - c0);
     if (c0 != beg) { // the first chunk was staged with the tiles
       __syncthreads();
       stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
@@ -1114,12 +1121,12 @@ template <int S> __device__ __forceinline__ double small_segment_total(double v)
   return v;
 }
 
-template <int P, typename MT>
-__global__ __launch_bounds__(kPG) void k_gather_layer(
+template <int P, typename MT, int NTH>
+__global__ __launch_bounds__(NTH) void k_gather_layer(
     SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
     int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
-    double4 const *__restrict__ spos,
-    int const *__restrict__ sidx, double *__restrict__ force, double const *__restrict__ sums) {
+    double4 const *__restrict__ spos, int const *__restrict__ sidx, double *__restrict__ force,
+    double const *__restrict__ sums, int chunk) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
   constexpr int tile_n = T * TYZ;
@@ -1128,9 +1135,9 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   extern __shared__ double smem_d[];
   ChunkSlots s;
   s.w = smem_d;
-  s.qq = s.w + kChunk * 3 * P;
-  s.lo = reinterpret_cast<int *>(s.qq + kChunk);
-  MT *tx = reinterpret_cast<MT *>(s.lo + kChunk);
+  s.qq = s.w + chunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + chunk);
+  MT *tx = reinterpret_cast<MT *>(reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
   MT *ty = tx + tile_n;
   MT *tz = ty + tile_n;
 
@@ -1145,7 +1152,7 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   if (d.dip_pref != 0.)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
-  fetch_tiles_and_stage<P, MT>(d, sp, beg, min(kChunk, end - beg), o0, o1, o2, tx, ty, tz, s);
+  fetch_tiles_and_stage<P, MT, NTH>(d, sp, beg, min(chunk, end - beg), o0, o1, o2, tx, ty, tz, s);
   int const *si = sidx + soff[L.s];
   double *fbase = force + 3 * d.part_off;
   int const lane = threadIdx.x & 63;
@@ -1154,8 +1161,8 @@ __global__ __launch_bounds__(kPG) void k_gather_layer(
   int const sub = lane / S;
   int const li = lane - sub * S;
   bool const active = li < P;
-  for (int c0 = beg; c0 < end; c0 += kChunk) {
-    int const cnt = min(kChunk, end - c0);
+  for (int c0 = beg; c0 < end; c0 += chunk) {
+    int const cnt = min(chunk, end - c0);
     if (c0 != beg) { // the first chunk was staged with the tiles
       __syncthreads();
       stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
@@ -1306,6 +1313,20 @@ template <typename F> void dispatch_cao(int cao, F &&f) {
   }
 }
 
+/** Block sizes of the particle-mesh kernels (LaunchShape::threads). */
+template <typename F> void with_threads(int threads, F &&f) {
+  if (threads == 256)
+    f(std::integral_constant<int, 256>{});
+  else if (threads == 512)
+    f(std::integral_constant<int, 512>{});
+  else
+    throw std::runtime_error("particle-mesh block size must be 256 or 512");
+}
+template <typename F> void for_threads(F &&f) {
+  f(std::integral_constant<int, 256>{});
+  f(std::integral_constant<int, 512>{});
+}
+
 __global__ void k_fill_int(int *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
@@ -1328,9 +1349,7 @@ template <typename Kern> void allow_lds(Kern k) {
             "hipFuncSetAttribute");
 }
 
-size_t chunk_bytes(int cao) {
-  return kChunk * (3 * cao + 1) * sizeof(double) + kChunk * sizeof(int);
-}
+size_t chunk_bytes(int cao, int chunk) { return chunk_slot_bytes(cao, chunk); }
 int tile_points(int cao) {
   int const t = kBrick + cao - 1;
   return t * t * t;
@@ -1622,12 +1641,26 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
       res.best = c;
     }
   };
-  // coordinate descent: fused y/x pass, gather mapping, FFT columns, z lines
+  // coordinate descent: fused y/x pass, particle-mesh blocks, gather and
+  // spread mappings, blocks again, FFT columns, z lines
   {
     LaunchShape c = res.best;
     c.fused_lds = (start.fused_lds > 0) ? 0 : LaunchShape{}.fused_lds;
     consider(c);
   }
+  // particle-mesh block size and chunk: large blocks for dense bricks, small
+  // ones (more bricks in flight per CU) for sparse bricks
+  auto try_blocks = [&] {
+    LaunchShape const base = res.best;
+    for (auto [t, c] : {std::pair{512, 64}, std::pair{256, 64}, std::pair{256, 32}}) {
+      if (t == base.threads && c == base.chunk) continue;
+      LaunchShape cand = base;
+      cand.threads = t;
+      cand.chunk = c;
+      consider(cand);
+    }
+  };
+  try_blocks();
   for (int v : {kGatherLayer, kGatherPlane, kGatherPlane16, kGatherPlane8}) {
     if (v == res.best.gather) continue;
     LaunchShape c = res.best;
@@ -1640,6 +1673,7 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     c.spread = v;
     consider(c);
   }
+  try_blocks();
   for (int v : {2, 4, 8, 16}) {
     if (v == res.best.ncol) continue;
     LaunchShape c = res.best;
@@ -1690,13 +1724,15 @@ template <typename MT> void set_kernel_attrs() {
   allow_lds(k_fft_yx_conv<MT>);
   for (int cao = 1; cao <= 7; ++cao)
     dispatch_cao(cao, [&](auto P) {
-      allow_lds(k_spread<P.value, MT, 64>);
-      allow_lds(k_spread<P.value, MT, 16>);
-      allow_lds(k_spread<P.value, MT, 8>);
-      allow_lds(k_gather_plane<P.value, MT, 64>);
-      allow_lds(k_gather_plane<P.value, MT, 16>);
-      allow_lds(k_gather_plane<P.value, MT, 8>);
-      allow_lds(k_gather_layer<P.value, MT>);
+      for_threads([&](auto NT) {
+        allow_lds(k_spread<P.value, MT, 64, NT.value>);
+        allow_lds(k_spread<P.value, MT, 16, NT.value>);
+        allow_lds(k_spread<P.value, MT, 8, NT.value>);
+        allow_lds(k_gather_plane<P.value, MT, 64, NT.value>);
+        allow_lds(k_gather_plane<P.value, MT, 16, NT.value>);
+        allow_lds(k_gather_plane<P.value, MT, 8, NT.value>);
+        allow_lds(k_gather_layer<P.value, MT, NT.value>);
+      });
     });
 }
 
@@ -2072,18 +2108,21 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (ev) (void)hipEventRecord(ev[1], st);
   for (auto const &g : p.groups) {
     dispatch_cao(g.cao, [&](auto P) {
-      size_t const lds = chunk_bytes(P.value) + tile_points(P.value) * sizeof(double);
-      auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
-                                        static_cast<double4 const *>(ln.spos));
-      auto launch = [&](auto kern) {
-        std::apply([&](auto... a) { kern<<<g.blocks, kPG, lds, st>>>(a...); }, args);
-      };
-      if (shape.spread == kSpreadSeg8)
-        launch(k_spread<P.value, MT, 8>);
-      else if (shape.spread == kSpreadSeg16)
-        launch(k_spread<P.value, MT, 16>);
-      else
-        launch(k_spread<P.value, MT, 64>);
+      with_threads(shape.threads, [&](auto NT) {
+        size_t const lds =
+            chunk_bytes(P.value, shape.chunk) + tile_points(P.value) * sizeof(double);
+        auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+                                          static_cast<double4 const *>(ln.spos), shape.chunk);
+        auto launch = [&](auto kern) {
+          std::apply([&](auto... a) { kern<<<g.blocks, NT.value, lds, st>>>(a...); }, args);
+        };
+        if (shape.spread == kSpreadSeg8)
+          launch(k_spread<P.value, MT, 8, NT.value>);
+        else if (shape.spread == kSpreadSeg16)
+          launch(k_spread<P.value, MT, 16, NT.value>);
+        else
+          launch(k_spread<P.value, MT, 64, NT.value>);
+      });
     });
   }
   if (ev) (void)hipEventRecord(ev[2], st);
@@ -2115,23 +2154,26 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (want_force) {
     for (auto const &g : p.groups) {
       dispatch_cao(g.cao, [&](auto P) {
-        size_t const lds = chunk_bytes(P.value) + 3 * tile_points(P.value) * sizeof(MT);
-        if (shape.gather == kGatherPlane8)
-          k_gather_plane<P.value, MT, 8><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
-              ln.sums);
-        else if (shape.gather == kGatherPlane16)
-          k_gather_plane<P.value, MT, 16><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
-              ln.sums);
-        else if (shape.gather == kGatherPlane)
-          k_gather_plane<P.value, MT, 64><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
-              ln.sums);
-        else
-          k_gather_layer<P.value, MT><<<g.blocks, kPG, lds, st>>>(
-              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets, ln.spos, ln.sidx, pa.force,
-              ln.sums);
+        with_threads(shape.threads, [&](auto NT) {
+          size_t const lds =
+              chunk_bytes(P.value, shape.chunk) + 3 * tile_points(P.value) * sizeof(MT);
+          auto const args =
+              std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+                              static_cast<double4 const *>(ln.spos),
+                              static_cast<int const *>(ln.sidx), pa.force,
+                              static_cast<double const *>(ln.sums), shape.chunk);
+          auto launch = [&](auto kern) {
+            std::apply([&](auto... a) { kern<<<g.blocks, NT.value, lds, st>>>(a...); }, args);
+          };
+          if (shape.gather == kGatherPlane8)
+            launch(k_gather_plane<P.value, MT, 8, NT.value>);
+          else if (shape.gather == kGatherPlane16)
+            launch(k_gather_plane<P.value, MT, 16, NT.value>);
+          else if (shape.gather == kGatherPlane)
+            launch(k_gather_plane<P.value, MT, 64, NT.value>);
+          else
+            launch(k_gather_layer<P.value, MT, NT.value>);
+        });
       });
     }
   }

@@ -65,9 +65,11 @@ struct LaunchShape {
   int gather = kGatherPlane16; // gather mapping
   int ncol = 8;             // kz columns per block in the y/x FFT passes
   int zlines = 16;          // pairs of real z lines per block in the z passes
-  int fused_lds = 96 * 1024; // fused y/x pass for meshes whose three x-y
+  int fused_lds = 96 * 1024; // fused y/x pass for meshes whose two x-y
                              // planes fit in this many LDS bytes (0: never)
   int spread = kSpreadSeg64; // spread mapping
+  int threads = 512;         // block size of the spread and gather kernels (256, 512)
+  int chunk = 64;            // particles staged per brick pass (1..64, <= threads / 3)
 };
 
 struct AutotuneResult {

@@ -53,6 +53,35 @@ struct p3m_server {
 };
 
 namespace {
+
+p3m_launch_shape to_abi(LaunchShape const &s) {
+  return p3m_launch_shape{s.gather, s.ncol, s.zlines, s.fused_lds, s.spread, s.threads, s.chunk};
+}
+
+LaunchShape from_abi(p3m_launch_shape const &a) {
+  for (int v : {a.ncol, a.zlines})
+    if (v < 1 || v > 64 || (v & (v - 1)) != 0)
+      throw std::invalid_argument("P3M batch: ncol and zlines must be powers of two in [1, 64]");
+  if (a.fused_lds < 0 || a.fused_lds > 128 * 1024)
+    throw std::invalid_argument("P3M batch: fused_lds must be in [0, 131072]");
+  if (a.gather < kGatherLayer || a.gather > kGatherPlane8)
+    throw std::invalid_argument("P3M batch: gather must be 0 (layer), 1, 2 or 3 (plane)");
+  if (a.spread < kSpreadSeg64 || a.spread > kSpreadSeg8)
+    throw std::invalid_argument("P3M batch: spread must be 0, 1 or 2");
+  if (a.threads != 256 && a.threads != 512)
+    throw std::invalid_argument("P3M batch: threads must be 256 or 512");
+  if (a.chunk < 1 || a.chunk > 64 || 3 * a.chunk > a.threads)
+    throw std::invalid_argument("P3M batch: chunk must be in [1, 64] and at most threads / 3");
+  LaunchShape s;
+  s.gather = a.gather;
+  s.ncol = a.ncol;
+  s.zlines = a.zlines;
+  s.fused_lds = a.fused_lds;
+  s.spread = a.spread;
+  s.threads = a.threads;
+  s.chunk = a.chunk;
+  return s;
+}
 std::mutex g_err_mutex;
 std::string g_last_error;
 
@@ -488,8 +517,7 @@ int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves) {
 }
 
 int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q, int reps,
-                 int *gather, int *ncol, int *zlines, int *fused_lds, int *spread,
-                 double *best_ms, double *default_ms) {
+                 p3m_launch_shape *best, double *best_ms, double *default_ms) {
   if (!ctx) return P3M_ERR_INVALID;
   return guarded(ctx, [&] {
     require_params(ctx);
@@ -500,11 +528,7 @@ int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q
     if (!ctx->initialized) init_solver(ctx);
     ensure_staging(ctx, n);
     AutotuneResult const r = ctx->solver->autotune(n, pos, q, ctx->d_force, reps);
-    if (gather) *gather = r.best.gather;
-    if (ncol) *ncol = r.best.ncol;
-    if (zlines) *zlines = r.best.zlines;
-    if (fused_lds) *fused_lds = r.best.fused_lds;
-    if (spread) *spread = r.best.spread;
+    if (best) *best = to_abi(r.best);
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });
@@ -670,29 +694,25 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, int const *sys_ids, double con
   });
 }
 
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
-                               int fused_lds, int spread) {
+int p3m_batch_set_launch_shape(p3m_batch *b, p3m_launch_shape const *shape) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
-    for (int v : {ncol, zlines})
-      if (v < 1 || v > 64 || (v & (v - 1)) != 0)
-        throw std::invalid_argument(
-            "P3M batch: ncol and zlines must be powers of two in [1, 64]");
-    if (fused_lds < 0 || fused_lds > 128 * 1024)
-      throw std::invalid_argument("P3M batch: fused_lds must be in [0, 131072]");
-    if (gather < kGatherLayer || gather > kGatherPlane8)
-      throw std::invalid_argument("P3M batch: gather must be 0 (layer), 1, 2 or 3 (plane)");
-    if (spread < kSpreadSeg64 || spread > kSpreadSeg8)
-      throw std::invalid_argument("P3M batch: spread must be 0, 1 or 2");
+    if (!shape) throw std::invalid_argument("P3M batch: shape required");
+    LaunchShape const s = from_abi(*shape);
     hip_check(hipSetDevice(b->device), "hipSetDevice");
-    b->solver->set_launch_shape(LaunchShape{gather, ncol, zlines, fused_lds, spread});
+    b->solver->set_launch_shape(s);
   });
 }
 
+int p3m_batch_get_launch_shape(p3m_batch const *b, p3m_launch_shape *shape) {
+  if (!b || !shape) return P3M_ERR_INVALID;
+  *shape = to_abi(b->solver->launch_shape());
+  return P3M_OK;
+}
+
 int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,
-                       double const *q, double *force, int reps, int *gather,
-                       int *ncol, int *zlines, int *fused_lds, int *spread, double *best_ms,
-                       double *default_ms) {
+                       double const *q, double *force, int reps, p3m_launch_shape *best,
+                       double *best_ms, double *default_ms) {
   if (!b) return P3M_ERR_INVALID;
   return guarded(&b->last_error, [&] {
     auto const sel = selection(b, nsel, sys_ids);
@@ -702,11 +722,7 @@ int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const 
     hip_check(hipSetDevice(b->device), "hipSetDevice");
     AutotuneResult const r =
         b->solver->autotune(sel, ParticleArrays{pos, q, nullptr, force}, reps);
-    if (gather) *gather = r.best.gather;
-    if (ncol) *ncol = r.best.ncol;
-    if (zlines) *zlines = r.best.zlines;
-    if (fused_lds) *fused_lds = r.best.fused_lds;
-    if (spread) *spread = r.best.spread;
+    if (best) *best = to_abi(r.best);
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });

@@ -259,8 +259,9 @@ class _P3MBase:
         return force
 
     def autotune(self, pos, q, reps=3):
-        """Pick the fastest launch shape of the device kernels (gather
-        mapping, FFT columns, z lines, fused y/x pass) for force solves of
+        """Pick the fastest launch shape of the device kernels (fused y/x
+        pass, particle-mesh blocks, gather and spread mappings, FFT columns,
+        z lines) for force solves of
         these device tensors and keep it.  Returns the shape and the mean
         solve times before and after."""
         self._require_active()
@@ -269,14 +270,12 @@ class _P3MBase:
             raise TypeError("autotune takes device tensors")
         a.order_after_torch(self._ctx)
         from .batch import shape_dict
-        g, nc, zl, fl, sp = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        shape = _lib.p3m_launch_shape()
         best, default = C.c_double(), C.c_double()
         _lib.raise_for(_lib.load().p3m_autotune(
-            self._ctx, a.n, a.pos, a.q, int(reps), C.byref(g), C.byref(nc),
-            C.byref(zl), C.byref(fl), C.byref(sp), C.byref(best),
+            self._ctx, a.n, a.pos, a.q, int(reps), C.byref(shape), C.byref(best),
             C.byref(default)), self._ctx)
-        return shape_dict(g.value, nc.value, zl.value, fl.value, sp.value,
-                          best.value, default.value)
+        return shape_dict(shape, best.value, default.value)
 
     def synchronize(self):
         _lib.raise_for(_lib.load().p3m_synchronize(self._ctx), self._ctx)

@@ -162,13 +162,28 @@ int p3m_last_timings(const p3m_context *ctx, double *ms, int n);
 int p3m_profile_begin(p3m_context *ctx, int max_solves);
 int p3m_profile_end(p3m_context *ctx, double *stage_ms, int *nsolves);
 
+/* Launch shape of the device kernels (no counterpart in the reference). */
+typedef struct p3m_launch_shape {
+  int gather;    /* gather mapping: 0 = lanes over stencil layers, 1/2/3 = lanes
+                    over the (j, k) plane in segments of up to 64/16/8 lanes */
+  int ncol;      /* kz columns per block of the split y/x FFT passes
+                    (power of two in [1, 64]) */
+  int zlines;    /* pairs of real lines per block of the z passes (power of two
+                    in [1, 64]) */
+  int fused_lds; /* LDS bytes up to which a system uses the fused y/x pass (its
+                    two x-y planes in LDS; 0 = never, at most 131072) */
+  int spread;    /* spread mapping: 0/1/2 = segments of up to 64/16/8 lanes per
+                    particle over the (j, k) plane */
+  int threads;   /* block size of the spread and gather kernels: 256 or 512 */
+  int chunk;     /* particles staged per brick pass: 1..64, at most threads/3 */
+} p3m_launch_shape;
+
 /* Launch-shape autotuner of the context's device kernels (no counterpart in
  * the reference; see p3m_batch_autotune for the candidates): times `reps`
  * force solves of the device arrays pos [n][3], q [n] per candidate and keeps
  * the fastest for later solves.  Outputs may be NULL. */
 int p3m_autotune(p3m_context *ctx, int64_t n, const double *pos, const double *q, int reps,
-                 int *gather, int *ncol, int *zlines, int *fused_lds, int *spread,
-                 double *best_ms, double *default_ms);
+                 p3m_launch_shape *best, double *best_ms, double *default_ms);
 
 /* Block until all work enqueued on the context stream has finished. */
 int p3m_synchronize(p3m_context *ctx);
@@ -209,25 +224,19 @@ int p3m_batch_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *po
  * first use, re-captured when the selection or arrays change); enqueue only. */
 int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
                           const double *q, double *force);
-/* Launch shape: gather mapping (0 = lanes over stencil layers, 1/2/3 = lanes
- * over the (j, k) plane in segments of up to 64/16/8 lanes), kz columns per
- * block
- * of the split y/x FFT passes and pairs of real lines per block of the z
- * passes (powers of two in [1, 64]), and the LDS bytes up to which a system
- * uses the fused y/x pass (its three x-y planes in LDS; 0 = never, at most
- * 131072), and the spread mapping (0/1/2 = segments of up to 64/16/8 lanes
- * per particle over the (j, k) plane). */
-int p3m_batch_set_launch_shape(p3m_batch *b, int gather, int ncol, int zlines,
-                               int fused_lds, int spread);
+/* Set (validated as documented in p3m_launch_shape) or read the launch shape
+ * of later solves. */
+int p3m_batch_set_launch_shape(p3m_batch *b, const p3m_launch_shape *shape);
+int p3m_batch_get_launch_shape(const p3m_batch *b, p3m_launch_shape *shape);
 /* Autotuner: time `reps` force solves of the selection for candidate launch
- * shapes (coordinate descent over fused pass, gather mapping, FFT columns and
- * z lines) and keep the fastest; returns the chosen shape and the mean solve
+ * shapes (coordinate descent over fused pass, particle-mesh block size and
+ * chunk, gather and spread mappings, FFT columns and z lines) and keep the
+ * fastest; returns the chosen shape (`best` may be NULL) and the mean solve
  * time before (default_ms) and after (best_ms).  `force` is scratch: forces
  * accumulate into it on every timed solve. */
 int p3m_batch_autotune(p3m_batch *b, int nsel, const int *sys_ids, const double *pos,
-                       const double *q, double *force, int reps, int *gather,
-                       int *ncol, int *zlines, int *fused_lds, int *spread, double *best_ms,
-                       double *default_ms);
+                       const double *q, double *force, int reps, p3m_launch_shape *best,
+                       double *best_ms, double *default_ms);
 int p3m_batch_synchronize(p3m_batch *b);
 int p3m_batch_get_stream(const p3m_batch *b, void **stream);
 int p3m_batch_profile_begin(p3m_batch *b, int max_solves);

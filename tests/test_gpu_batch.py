@@ -215,20 +215,30 @@ def test_launch_shapes_and_autotune_keep_results():
     b.solve(pos, q, ref)
     ref_np = ref.cpu().numpy()
     f = torch.zeros_like(pos)
+    # particle-mesh block size and chunk, including one-particle and odd chunks
+    blocks = [(512, 64), (256, 32), (256, 7), (512, 1)]
+    k = 0
     for fused in (0, 96 * 1024):
         for gather, spread in (("layer", "plane"), ("plane", "plane16"),
                                ("plane16", "plane8"), ("plane8", "plane")):
             for ncol, zl in ((1, 1), (2, 4), (16, 16), (64, 64)):
-                b.set_launch_shape(gather, ncol, zl, fused, spread)
+                threads, chunk = blocks[k % len(blocks)]
+                k += 1
+                b.set_launch_shape(gather, ncol, zl, fused, spread, threads, chunk)
+                assert b.launch_shape()["chunk"] == chunk
                 f.zero_()
                 torch.cuda.synchronize()
                 b.solve(pos, q, f)
                 assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5, (fused, gather, spread,
-                                                                     ncol, zl)
+                                                                     ncol, zl, threads, chunk)
     with pytest.raises(ValueError, match="powers of two"):
         b.set_launch_shape("plane", 3, 4)
     with pytest.raises(ValueError, match="gather must be"):
         b.set_launch_shape(5)
+    with pytest.raises(ValueError, match="threads must be"):
+        b.set_launch_shape(threads=128)
+    with pytest.raises(ValueError, match="chunk must be"):
+        b.set_launch_shape(threads=256, chunk=90)
     b.set_launch_shape()  # autotuning starts from the default shape
     scratch = torch.zeros_like(pos)
     shape = b.autotune(pos, q, scratch, reps=1)
@@ -236,6 +246,7 @@ def test_launch_shapes_and_autotune_keep_results():
     assert shape["fused_lds"] in (0, 96 * 1024)
     assert shape["gather"] in ("layer", "plane", "plane16", "plane8")
     assert shape["spread"] in ("plane", "plane16", "plane8")
+    assert (shape["threads"], shape["chunk"]) in ((512, 64), (256, 64), (256, 32))
     assert 0. < shape["best_ms"] <= shape["default_ms"]
     f.zero_()
     torch.cuda.synchronize()
