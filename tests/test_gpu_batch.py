@@ -66,17 +66,19 @@ def _oracle(s):
                      s["prefactor"])
 
 
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("shape", [
+    None,                                   # default: every mesh here takes the fused y/x pass
+    dict(fused_lds=0),                      # split y, x and inverse-y passes
+    dict(spread="plane8", gather="layer", threads=256, chunk=32),  # small blocks
+], ids=["fused", "split", "small_blocks"])
 @pytest.mark.parametrize("single,f_tol,e_tol", [(False, 1e-6, 1e-8),
                                                 (True, 2e-5, 2e-6)])
-def test_batch_matches_oracle_per_system(single, f_tol, e_tol, fused):
-    # fused: every mesh here takes the fused y/x pass (x-y planes in LDS);
-    # otherwise the split y, x and inverse-y passes
+def test_batch_matches_oracle_per_system(single, f_tol, e_tol, shape):
     import torch
     systems = _systems()
     b = _batch(systems, single)
-    if not fused:
-        b.set_launch_shape(fused_lds=0)
+    if shape is not None:
+        b.set_launch_shape(**shape)
     pos, q = _device_arrays(systems)
     force = torch.zeros_like(pos)
     energy, virial = b.solve(pos, q, force, energy=True, pressure=True)
