@@ -1583,15 +1583,19 @@ void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st);
 } // namespace
 
 void BatchSolver::set_launch_shape(LaunchShape const &s) {
-  m_shape = s;
-  m_shape.ncol = std::max(1, std::min(64, s.ncol));
-  m_shape.zlines = std::max(1, std::min(64, s.zlines));
-  m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
+  // validate before anything changes
   if (s.gather < kGatherLayer || s.gather > kGatherPlane8)
     throw std::invalid_argument("P3M: unknown gather mapping");
   if (s.spread < kSpreadSeg64 || s.spread > kSpreadSeg8)
     throw std::invalid_argument("P3M: unknown spread mapping");
+  if ((s.threads != 256 && s.threads != 512) || s.chunk < 1 || s.chunk > kChunkMax ||
+      3 * s.chunk > s.threads)
+    throw std::invalid_argument("P3M: invalid particle-mesh block shape");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
+  m_shape = s;
+  m_shape.ncol = std::max(1, std::min(64, s.ncol));
+  m_shape.zlines = std::max(1, std::min(64, s.zlines));
+  m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
   quiesce(m);
   // systems whose y/x pass changes switch spectrum layout: recompute G
   bool changed = false;
