@@ -50,6 +50,7 @@ using namespace dev;
 
 constexpr int kBlock = 256;  // block size of the mesh and particle-list kernels
 constexpr int kChunkMax = 64; // most particles staged per brick pass (3 x chunk <= threads)
+constexpr int kCountCopies = 8; // brick counter copies (one per XCD-sized share of the blocks)
 constexpr int kBrick = 8;    // first-stencil points per brick edge
 constexpr int kFill = 1024;  // mesh elements zeroed per block
 
@@ -175,7 +176,9 @@ __global__ void k_zero_counts(SysDesc const *__restrict__ sys, int const *__rest
                               int nsel, int *counts) {
   int const s = sel[blockIdx.x];
   SysDesc const &d = sys[s];
-  for (int b = threadIdx.x; b < d.nbricks; b += blockDim.x) counts[d.brick_off + b] = 0;
+  int const n = d.nbricks * kCountCopies;
+  int *c = counts + d.brick_off * kCountCopies;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) c[b] = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
@@ -200,7 +203,11 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
       double dist[3];
       if (!particle_coordinates_rt(d.g, d.cao, x, nmp, gi, dist)) atomicOr(err_flag, 1);
       b = ((gi[0] / kBrick) * d.g.nb[1] + gi[1] / kBrick) * d.g.nb[2] + gi[2] / kBrick;
-      p_rank[si] = atomicAdd(&counts[d.brick_off + b], 1);
+      // brick counters [brick][copy]: blocks take copy blockIdx.x % 8, which
+      // spreads the returning atomics of one brick over eight addresses
+      int const copy = static_cast<int>(blockIdx.x) & (kCountCopies - 1);
+      int const rank = atomicAdd(&counts[(d.brick_off + b) * kCountCopies + copy], 1);
+      p_rank[si] = rank * kCountCopies + copy;
       if (p_nmp) {
         p_nmp[3 * li] = nmp[0];
         p_nmp[3 * li + 1] = nmp[1];
@@ -243,20 +250,26 @@ __global__ void k_reduce_sys(int const *__restrict__ sel, int const *__restrict_
 
 /** exclusive scan of one system's brick counts (one block per system) */
 __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
-                                               int const *__restrict__ sel, int const *counts,
+                                               int const *__restrict__ sel, int *counts,
                                                int *offsets) {
   __shared__ int wsum[16];
   __shared__ int carry;
   SysDesc const &d = sys[sel[blockIdx.x]];
   int const nb = d.nbricks;
-  int const *count = counts + d.brick_off;
+  int *count = counts + d.brick_off * kCountCopies;
   int *offset = offsets + d.off_off;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   int const lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int base = 0; base < nb; base += blockDim.x) {
     int const i = base + threadIdx.x;
-    int const c = (i < nb) ? count[i] : 0;
+    int cc[kCountCopies];
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kCountCopies; ++k) {
+      cc[k] = (i < nb) ? count[i * kCountCopies + k] : 0;
+      c += cc[k];
+    }
     int s = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -274,7 +287,16 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
       }
     }
     __syncthreads();
-    if (i < nb) offset[i] = carry + wsum[wid] + s - c;
+    if (i < nb) {
+      int o = carry + wsum[wid] + s - c;
+      offset[i] = o;
+      // the counters become the start of each copy's run within the brick
+#pragma unroll
+      for (int k = 0; k < kCountCopies; ++k) {
+        count[i * kCountCopies + k] = o;
+        o += cc[k];
+      }
+    }
     __syncthreads();
     if (threadIdx.x == blockDim.x - 1) carry += wsum[wid] + s;
     __syncthreads();
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ 
                                                     int const *__restrict__ wt, int nsel,
                                                     int const *__restrict__ soff,
                                                     ParticleArrays pa, int const *p_brick,
-                                                    int const *p_rank, int const *offsets,
+                                                    int const *p_rank, int const *counts,
                                                     double4 *spos, int *sidx) {
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
@@ -297,7 +319,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ 
   int64_t const si = soff[L.s] + li;
   int const b = p_brick[si];
   if (b < 0) return;
-  int64_t const slot = soff[L.s] + offsets[d.off_off + b] + p_rank[si];
+  int const rc = p_rank[si];
+  int64_t const slot =
+      soff[L.s] + counts[(d.brick_off + b) * kCountCopies + (rc & (kCountCopies - 1))] +
+      (rc / kCountCopies);
   spos[slot] = double4{pa.pos[3 * i], pa.pos[3 * i + 1], pa.pos[3 * i + 2], pa.q[i]};
   sidx[slot] = static_cast<int>(li);
 }
@@ -1913,7 +1938,8 @@ void ensure_scratch(BatchSolver::Impl *m, Lane &ln, int64_t npart, void *externa
   }
   if (nb_need > ln.counts_cap) {
     dfree(ln.counts);
-    hip_check(hipMalloc(&ln.counts, sizeof(int) * std::max(nb_need, 1)), "hipMalloc counts");
+    hip_check(hipMalloc(&ln.counts, sizeof(int) * kCountCopies * std::max(nb_need, 1)),
+              "hipMalloc counts");
     ln.counts_cap = nb_need;
   }
   if (nb_need + nsys > ln.offsets_cap) {
@@ -2106,7 +2132,7 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, ln.counts, ln.offsets);
   if (p.blocks_bin > 0)
     k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa,
-                                                ln.p_brick, ln.p_rank, ln.offsets, ln.spos,
+                                                ln.p_brick, ln.p_rank, ln.counts, ln.spos,
                                                 ln.sidx);
   k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
   if (ev) (void)hipEventRecord(ev[1], st);
