@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "p3m_real_space_error", "p3m_pair_forces_targets", "p3m_pair_force",
     "p3m_pair_energy",
     "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end", "p3m_autotune",
+    "p3m_short_range", "p3m_short_range_cells", "p3m_set_short_range_subdivision",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -128,6 +129,9 @@ def load():
     shp = C.POINTER(p3m_launch_shape)
     L.p3m_autotune.argtypes = [vp, C.c_int64, vp, vp, C.c_int, shp, dp, dp]
     L.p3m_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.p3m_short_range.argtypes = [vp, C.c_int64, vp, vp, C.c_uint, vp, dp, dp]
+    L.p3m_short_range_cells.argtypes = [vp, C.POINTER(C.c_int)]
+    L.p3m_set_short_range_subdivision.argtypes = [vp, C.c_int]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []

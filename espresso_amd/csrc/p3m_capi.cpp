@@ -9,6 +9,7 @@
 #include "p3m_host.hpp"
 #include "p3m_batch.hpp"
 #include "p3m_serve.hpp"
+#include "p3m_short_range.hpp"
 #include "p3m_solver.hpp"
 #include "p3m_tuning.hpp"
 
@@ -29,6 +30,8 @@ using namespace espresso_amd;
 struct p3m_context {
   int device = 0;
   std::unique_ptr<Solver> solver;
+  std::unique_ptr<ShortRange> short_range; // created on first use
+  int short_range_subdivision = 0;
   p3m_params user{};
   P3MParams params{};
   bool have_params = false;
@@ -478,6 +481,66 @@ int p3m_pair_forces_targets(p3m_context *ctx, int64_t n, double const *pos, doub
     pair_forces_targets(ctx->device, ctx->solver->stream(), n, pos, q, ntargets, targets, box,
                         ctx->user.prefactor, ctx->params.alpha, ctx->params.r_cut, out);
   });
+}
+
+int p3m_short_range(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                    unsigned flags, double *force, double *energy, double *virial) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    require_device(ctx);
+    if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    if (!force && n > 0) throw std::invalid_argument("P3M: force array required");
+    bool const async = (flags & P3M_ASYNC) != 0;
+    bool const want_ev = (flags & (P3M_ENERGY | P3M_PRESSURE)) != 0;
+    if (async && !(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
+    if (async && want_ev)
+      throw std::invalid_argument("P3M: energy and pressure need a synchronous call");
+    auto const box = v3(ctx->user.box_l);
+    for (int d = 0; d < 3; ++d)
+      if (!(ctx->params.r_cut < 0.5 * box[d]))
+        throw std::domain_error("P3M: r_cut must be below half the box for minimum image");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->initialized) init_solver(ctx);
+    if (!ctx->short_range) ctx->short_range = std::make_unique<ShortRange>(ctx->device);
+    DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, force, flags);
+    ShortRangeParams sp;
+    for (int k = 0; k < 3; ++k) sp.box[k] = box[k];
+    sp.prefactor = ctx->user.prefactor;
+    sp.alpha = ctx->params.alpha;
+    sp.r_cut = ctx->params.r_cut;
+    sp.subdivision = ctx->short_range_subdivision;
+    ShortRangeResult r;
+    hipStream_t const st = ctx->solver->stream();
+    ctx->short_range->compute(st, sp, n, d.pos, d.q, d.force, want_ev ? &r : nullptr);
+    if (!async && !want_ev) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!(flags & P3M_DEVICE_POINTERS) && n > 0)
+      hip_check(hipMemcpy(force, d.force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+                "copy force");
+    if (energy) *energy = (flags & P3M_ENERGY) ? r.energy : 0.;
+    if (virial)
+      for (int k = 0; k < 9; ++k) virial[k] = (flags & P3M_PRESSURE) ? r.virial[k] : 0.;
+  });
+}
+
+int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (cells_per_cutoff < 0 || cells_per_cutoff > 2)
+      throw std::invalid_argument("P3M: short-range subdivision must be 0, 1 or 2");
+    ctx->short_range_subdivision = cells_per_cutoff;
+  });
+}
+
+int p3m_short_range_cells(p3m_context const *ctx, int cells[3]) {
+  if (!ctx || !cells) return P3M_ERR_INVALID;
+  if (ctx->short_range)
+    ctx->short_range->cells(cells);
+  else
+    cells[0] = cells[1] = cells[2] = 0;
+  return P3M_OK;
 }
 
 int p3m_pair_force(p3m_context const *ctx, double q1q2, double const d[3], double dist,

@@ -1,0 +1,384 @@
+// Device real-space pair forces of P3M Coulomb over cell lists; see
+// p3m_short_range.hpp.
+#include "p3m_short_range.hpp"
+#include "p3m_batch.hpp"
+#include "p3m_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numbers>
+#include <stdexcept>
+#include <vector>
+
+namespace espresso_amd {
+namespace {
+using namespace dev;
+
+constexpr int kSRMaxCells = 64; // cells per dimension at most
+
+/** Cell grid: nc cells per dimension of edge L / nc >= r_cut / nsub. */
+struct CellGrid {
+  int nc[3];
+  double L[3];
+  double inv_cell[3]; // nc / L
+};
+
+/** x folded into [0, L). */
+__device__ __forceinline__ double fold(double x, double L) {
+  double xf = x - L * floor(x / L);
+  if (xf >= L) xf -= L;
+  return xf < 0. ? 0. : xf;
+}
+
+__global__ void k_sr_fill(int *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+/** Cell of every charged particle (uncharged ones take no part in pair
+ *  terms, p3m.hpp:150) and its rank in the cell. */
+__global__ __launch_bounds__(256) void k_sr_bin(int64_t n, double const *__restrict__ pos,
+                                                double const *__restrict__ q, CellGrid g,
+                                                int *counts, int *p_cell, int *p_rank) {
+  int64_t const i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (q[i] == 0.) {
+    p_cell[i] = -1;
+    return;
+  }
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    c[d] = min(g.nc[d] - 1, static_cast<int>(fold(pos[3 * i + d], g.L[d]) * g.inv_cell[d]));
+  int const cell = (c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2];
+  p_cell[i] = cell;
+  p_rank[i] = atomicAdd(&counts[cell], 1);
+}
+
+/** Exclusive scan of the cell counts (one block); offsets[ncells] = total. */
+__global__ __launch_bounds__(1024) void k_sr_scan(int const *counts, int ncells, int *offsets) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  int const lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int base = 0; base < ncells; base += blockDim.x) {
+    int const i = base + threadIdx.x;
+    int const c = (i < ncells) ? counts[i] : 0;
+    int s = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int const t = __shfl_up(s, off, 64);
+      if (lane >= off) s += t;
+    }
+    if (lane == 63) wsum[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+        int const t = wsum[w];
+        wsum[w] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+    if (i < ncells) offsets[i] = carry + wsum[wid] + s - c;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry += wsum[wid] + s;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) offsets[ncells] = carry;
+}
+
+/** Charged particles in cell order: folded position and charge, and the
+ *  original index. */
+__global__ __launch_bounds__(256) void k_sr_scatter(int64_t n, double const *__restrict__ pos,
+                                                    double const *__restrict__ q, CellGrid g,
+                                                    int const *p_cell, int const *p_rank,
+                                                    int const *offsets, double4 *spos,
+                                                    int *sidx) {
+  int64_t const i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int const cell = p_cell[i];
+  if (cell < 0) return;
+  int const slot = offsets[cell] + p_rank[i];
+  spos[slot] = double4{fold(pos[3 * i], g.L[0]), fold(pos[3 * i + 1], g.L[1]),
+                       fold(pos[3 * i + 2], g.L[2]), q[i]};
+  sidx[slot] = static_cast<int>(i);
+}
+
+/** Pair term of CoulombP3M::pair_force / pair_energy (p3m.hpp:146-180) with
+ *  the A&S erfc of AS_erfc_part.hpp:29-40 (the published 7.1.26 constants):
+ *  d = r_i - r_j, qq = prefactor q_i q_j.  Adds the force on i and, with EV,
+ *  the pair energy and F (x) d (upper triangle: xx xy xz yy yz zz). */
+template <bool EV>
+__device__ __forceinline__ void pair_term(double dx, double dy, double dz, double qq, double rc2,
+                                          double alpha, double two_a_sqrt_pi_i, double &fx,
+                                          double &fy, double &fz, double (&ev)[7]) {
+  double const r2 = dx * dx + dy * dy + dz * dz;
+  if (r2 < rc2 && r2 > 0.) {
+    double const dist = sqrt(r2);
+    double const adist = alpha * dist;
+    double const t = 1. / (1. + 0.3275911 * adist);
+    double const erfc_part =
+        t * (0.254829592 +
+             t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+    double const ex = exp(-adist * adist);
+    double const erfc_ri = erfc_part / dist;
+    double const fac = ex * (erfc_ri + two_a_sqrt_pi_i) / r2 * qq;
+    fx += fac * dx;
+    fy += fac * dy;
+    fz += fac * dz;
+    if constexpr (EV) {
+      ev[0] += qq * erfc_ri * ex;
+      ev[1] += fac * dx * dx;
+      ev[2] += fac * dx * dy;
+      ev[3] += fac * dx * dz;
+      ev[4] += fac * dy * dy;
+      ev[5] += fac * dy * dz;
+      ev[6] += fac * dz * dz;
+    }
+  }
+}
+
+/** One block of NT threads per cell: the cell's particles (NT at a time)
+ *  against the (2 NSUB + 1)^3 neighbour cells, each staged through LDS with
+ *  its periodic shift (nc >= 2 NSUB + 1 per dimension: every neighbour is a
+ *  distinct cell).  Each thread sums its particle's force over all partners
+ *  (both orders of a pair are evaluated), so no atomics are needed. */
+template <int NSUB, int NT, bool EV>
+__global__ __launch_bounds__(NT) void k_sr_cells(CellGrid g, int const *__restrict__ offsets,
+                                                 double4 const *__restrict__ spos,
+                                                 int const *__restrict__ sidx, double prefactor,
+                                                 double alpha, double rc2,
+                                                 double *__restrict__ force, double *partials) {
+  __shared__ double4 tile[NT];
+  __shared__ double red[8 * 16];
+  constexpr int W = 2 * NSUB + 1;
+  int const cell = blockIdx.x;
+  int const cz = cell % g.nc[2], cy = (cell / g.nc[2]) % g.nc[1], cx = cell / (g.nc[2] * g.nc[1]);
+  int const beg = offsets[cell], end = offsets[cell + 1];
+  double const two_a = 2. * alpha * std::numbers::inv_sqrtpi;
+  double ev[7] = {0., 0., 0., 0., 0., 0., 0.};
+  for (int hb = beg; hb < end; hb += NT) {
+    int const i = hb + static_cast<int>(threadIdx.x);
+    bool const has = i < end;
+    double4 const pi = has ? spos[i] : double4{0., 0., 0., 0.};
+    double const qpi = prefactor * pi.w;
+    double fx = 0., fy = 0., fz = 0.;
+    for (int o = 0; o < W * W * W; ++o) {
+      int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
+      double sx = 0., sy = 0., sz = 0.;
+      if (nx < 0) { nx += g.nc[0]; sx = -g.L[0]; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = g.L[0]; }
+      if (ny < 0) { ny += g.nc[1]; sy = -g.L[1]; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = g.L[1]; }
+      if (nz < 0) { nz += g.nc[2]; sz = -g.L[2]; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = g.L[2]; }
+      int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
+      int const jb = offsets[nb], je = offsets[nb + 1];
+      for (int tb = jb; tb < je; tb += NT) {
+        __syncthreads(); // the previous tile is consumed
+        int const j = tb + static_cast<int>(threadIdx.x);
+        if (j < je) {
+          double4 const pj = spos[j];
+          tile[threadIdx.x] = double4{pj.x + sx, pj.y + sy, pj.z + sz, pj.w};
+        }
+        __syncthreads();
+        int const cnt = min(NT, je - tb);
+        if (has)
+          for (int k = 0; k < cnt; ++k) {
+            double4 const pj = tile[k];
+            pair_term<EV>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, qpi * pj.w, rc2, alpha, two_a,
+                          fx, fy, fz, ev);
+          }
+      }
+    }
+    if (has) {
+      double *f = force + 3 * static_cast<int64_t>(sidx[i]);
+      f[0] += fx;
+      f[1] += fy;
+      f[2] += fz;
+    }
+  }
+  if constexpr (EV) {
+    block_sum<7>(ev, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 7; ++k) partials[blockIdx.x * 8 + k] = ev[k];
+  }
+}
+
+/** All charged pairs with the minimum-image displacement (boxes narrower
+ *  than three cells of edge r_cut); one block per NT sorted particles. */
+template <int NT, bool EV>
+__global__ __launch_bounds__(NT) void k_sr_allpairs(int nq, CellGrid g,
+                                                    double4 const *__restrict__ spos,
+                                                    int const *__restrict__ sidx,
+                                                    double prefactor, double alpha, double rc2,
+                                                    double *__restrict__ force,
+                                                    double *partials) {
+  __shared__ double4 tile[NT];
+  __shared__ double red[8 * 16];
+  int const i = blockIdx.x * NT + static_cast<int>(threadIdx.x);
+  bool const has = i < nq;
+  double4 const pi = has ? spos[i] : double4{0., 0., 0., 0.};
+  double const qpi = prefactor * pi.w;
+  double const two_a = 2. * alpha * std::numbers::inv_sqrtpi;
+  double fx = 0., fy = 0., fz = 0.;
+  double ev[7] = {0., 0., 0., 0., 0., 0., 0.};
+  for (int tb = 0; tb < nq; tb += NT) {
+    __syncthreads();
+    int const j = tb + static_cast<int>(threadIdx.x);
+    if (j < nq) tile[threadIdx.x] = spos[j];
+    __syncthreads();
+    int const cnt = min(NT, nq - tb);
+    if (has)
+      for (int k = 0; k < cnt; ++k) {
+        double4 const pj = tile[k];
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        dx -= g.L[0] * rint(dx / g.L[0]);
+        dy -= g.L[1] * rint(dy / g.L[1]);
+        dz -= g.L[2] * rint(dz / g.L[2]);
+        pair_term<EV>(dx, dy, dz, qpi * pj.w, rc2, alpha, two_a, fx, fy, fz, ev);
+      }
+  }
+  if (has) {
+    double *f = force + 3 * static_cast<int64_t>(sidx[i]);
+    f[0] += fx;
+    f[1] += fy;
+    f[2] += fz;
+  }
+  if constexpr (EV) {
+    block_sum<7>(ev, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 7; ++k) partials[blockIdx.x * 8 + k] = ev[k];
+  }
+}
+
+template <typename T> void sr_grow(T *&p, size_t &cap, size_t need, char const *what) {
+  if (need <= cap) return;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  size_t const n = need + need / 4 + 64;
+  hip_check(hipMalloc(&p, sizeof(T) * n), what);
+  cap = n;
+}
+} // namespace
+
+struct ShortRange::Impl {
+  int device = 0;
+  int nc[3] = {0, 0, 0};
+  int *counts = nullptr, *offsets = nullptr, *p_cell = nullptr, *p_rank = nullptr,
+      *sidx = nullptr;
+  double4 *spos = nullptr;
+  double *partials = nullptr;
+  size_t cap_cells = 0, cap_off = 0, cap_pc = 0, cap_pr = 0, cap_si = 0, cap_sp = 0, cap_part = 0;
+};
+
+ShortRange::ShortRange(int device) : m(new Impl) { m->device = device; }
+
+ShortRange::~ShortRange() {
+  (void)hipSetDevice(m->device);
+  for (void *p : {static_cast<void *>(m->counts), static_cast<void *>(m->offsets),
+                  static_cast<void *>(m->p_cell), static_cast<void *>(m->p_rank),
+                  static_cast<void *>(m->sidx), static_cast<void *>(m->spos),
+                  static_cast<void *>(m->partials)})
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+void ShortRange::cells(int out[3]) const {
+  for (int d = 0; d < 3; ++d) out[d] = m->nc[d];
+}
+
+void ShortRange::compute(hipStream_t st, ShortRangeParams const &p, int64_t n, double const *pos,
+                         double const *q, double *force, ShortRangeResult *out) {
+  if (out) *out = ShortRangeResult{};
+  if (!(p.r_cut > 0.)) throw std::invalid_argument("P3M: short range needs r_cut > 0");
+  for (int d = 0; d < 3; ++d)
+    if (!(p.box[d] > 0.)) throw std::invalid_argument("P3M: short range needs a box");
+  if (n > INT32_MAX) throw std::invalid_argument("P3M: too many particles for the short range");
+  if (n == 0) return;
+  hip_check(hipSetDevice(m->device), "hipSetDevice");
+  // cells of edge >= r_cut / nsub: two per r_cut when the box is wide enough
+  // for 5 distinct neighbour cells, one when it allows 3, else all pairs
+  if (p.subdivision < 0 || p.subdivision > 2)
+    throw std::invalid_argument("P3M: short-range subdivision must be 0, 1 or 2");
+  int nsub = (p.subdivision == 1) ? 1 : 2;
+  int nc[3];
+  auto fit = [&](int s) {
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = std::min(kSRMaxCells, static_cast<int>(std::floor(p.box[d] * s / p.r_cut)));
+      if (nc[d] < 2 * s + 1) ok = false;
+    }
+    return ok;
+  };
+  if (!fit(nsub)) {
+    nsub = 1;
+    if (!fit(1)) {
+      nsub = 0;
+      nc[0] = nc[1] = nc[2] = 1;
+    }
+  }
+  CellGrid g;
+  for (int d = 0; d < 3; ++d) {
+    m->nc[d] = g.nc[d] = nc[d];
+    g.L[d] = p.box[d];
+    g.inv_cell[d] = nc[d] / p.box[d];
+  }
+  int const ncells = nc[0] * nc[1] * nc[2];
+  sr_grow(m->counts, m->cap_cells, static_cast<size_t>(ncells), "hipMalloc cell counts");
+  sr_grow(m->offsets, m->cap_off, static_cast<size_t>(ncells) + 1, "hipMalloc cell offsets");
+  sr_grow(m->p_cell, m->cap_pc, static_cast<size_t>(n), "hipMalloc cell ids");
+  sr_grow(m->p_rank, m->cap_pr, static_cast<size_t>(n), "hipMalloc cell ranks");
+  sr_grow(m->sidx, m->cap_si, static_cast<size_t>(n), "hipMalloc cell order");
+  sr_grow(m->spos, m->cap_sp, static_cast<size_t>(n), "hipMalloc cell positions");
+  int const nb = static_cast<int>((n + 255) / 256);
+  k_sr_fill<<<std::min((ncells + 255) / 256, 1024), 256, 0, st>>>(m->counts, ncells);
+  k_sr_bin<<<nb, 256, 0, st>>>(n, pos, q, g, m->counts, m->p_cell, m->p_rank);
+  k_sr_scan<<<1, 1024, 0, st>>>(m->counts, ncells, m->offsets);
+  k_sr_scatter<<<nb, 256, 0, st>>>(n, pos, q, g, m->p_cell, m->p_rank, m->offsets, m->spos,
+                                   m->sidx);
+  double const rc2 = p.r_cut * p.r_cut;
+  int blocks = ncells;
+  int nq = 0;
+  if (nsub == 0) {
+    // all-pairs: the number of charged particles decides the grid
+    hip_check(hipMemcpyAsync(&nq, m->offsets + ncells, sizeof(int), hipMemcpyDeviceToHost, st),
+              "copy charged count");
+    hip_check(hipStreamSynchronize(st), "sync short range");
+    blocks = std::max(1, (nq + 255) / 256);
+  }
+  if (out) sr_grow(m->partials, m->cap_part, static_cast<size_t>(blocks) * 8, "hipMalloc partials");
+  auto launch = [&](auto EVc) {
+    constexpr bool EV = decltype(EVc)::value;
+    if (nsub == 2)
+      k_sr_cells<2, 64, EV><<<blocks, 64, 0, st>>>(g, m->offsets, m->spos, m->sidx, p.prefactor,
+                                                   p.alpha, rc2, force, m->partials);
+    else if (nsub == 1)
+      k_sr_cells<1, 256, EV><<<blocks, 256, 0, st>>>(g, m->offsets, m->spos, m->sidx,
+                                                     p.prefactor, p.alpha, rc2, force,
+                                                     m->partials);
+    else
+      k_sr_allpairs<256, EV><<<blocks, 256, 0, st>>>(nq, g, m->spos, m->sidx, p.prefactor,
+                                                     p.alpha, rc2, force, m->partials);
+  };
+  if (out)
+    launch(std::true_type{});
+  else
+    launch(std::false_type{});
+  hip_check(hipGetLastError(), "short-range kernels");
+  if (!out) return;
+  std::vector<double> part(static_cast<size_t>(blocks) * 8);
+  hip_check(hipMemcpyAsync(part.data(), m->partials, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, st),
+            "copy short-range partials");
+  hip_check(hipStreamSynchronize(st), "sync short range");
+  double s[7] = {0., 0., 0., 0., 0., 0., 0.};
+  for (int b = 0; b < blocks; ++b)
+    for (int k = 0; k < 7; ++k) s[k] += part[static_cast<size_t>(b) * 8 + k];
+  // every pair was evaluated from both sides
+  out->energy = 0.5 * s[0];
+  double const v[3][3] = {{s[1], s[2], s[3]}, {s[2], s[4], s[5]}, {s[3], s[5], s[6]}};
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) out->virial[3 * a + b] = 0.5 * v[a][b];
+}
+
+} // namespace espresso_amd

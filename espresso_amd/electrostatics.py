@@ -337,6 +337,50 @@ class _P3MBase:
             self._ctx)
         return e.value, np.array(v[:]).reshape(3, 3)
 
+    def add_short_range_forces(self, pos, q, force, energy=False, pressure=False,
+                               sync=True):
+        """force += real-space pair forces of all charged pairs closer than
+        r_cut (CoulombP3M::pair_force, p3m.hpp:146-165, over cell lists as the
+        short-range loop of forces.cpp:178-208 applies it); in place.  Returns
+        the pair energy with ``energy=True`` and the virial sum F (x) d with
+        ``pressure=True`` (a (energy, virial) pair with both).  With device
+        tensors and ``sync=False`` the work is only enqueued."""
+        self._require_active()
+        a = _Arrays(pos, q, force)
+        a.order_after_torch(self._ctx)
+        flags = a.flags
+        if energy:
+            flags |= _lib.P3M_ENERGY
+        if pressure:
+            flags |= _lib.P3M_PRESSURE
+        if not sync:
+            if not a.device:
+                raise ValueError("sync=False needs device tensors")
+            flags |= _lib.P3M_ASYNC
+        e = C.c_double(0.)
+        v = (C.c_double * 9)()
+        _lib.raise_for(_lib.load().p3m_short_range(
+            self._ctx, a.n, a.pos, a.q, flags, a.force, C.byref(e), v), self._ctx)
+        vir = np.array(v[:]).reshape(3, 3)
+        if energy and pressure:
+            return e.value, vir
+        if energy:
+            return e.value
+        if pressure:
+            return vir
+        return force
+
+    def set_short_range_subdivision(self, cells_per_cutoff):
+        """Cells per r_cut of the short-range cell lists (0: automatic)."""
+        _lib.raise_for(_lib.load().p3m_set_short_range_subdivision(
+            self._ctx, int(cells_per_cutoff)), self._ctx)
+
+    def short_range_cells(self):
+        """Cells per dimension of the last short-range call."""
+        c = (C.c_int * 3)()
+        _lib.raise_for(_lib.load().p3m_short_range_cells(self._ctx, c), self._ctx)
+        return list(c)
+
     def pair_forces_on(self, pos, q, targets):
         """Real-space pair forces (A&S erfc, p3m.hpp:146-165) on the listed
         target particles from all particles within r_cut, minimum image;

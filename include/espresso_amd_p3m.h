@@ -135,6 +135,26 @@ int p3m_k_space_error(double prefactor, const int mesh[3], int cao, int n_c_part
 int p3m_real_space_error(double prefactor, double r_cut_iL, int n_c_part, double sum_q2,
                          double alpha_L, const double box_l[3], double *out);
 
+/* Real-space (short-range) part on the device for all n particles: every
+ * pair of charged particles closer than the context's r_cut (minimum image,
+ * r_cut below half the box) adds the force of CoulombP3M::pair_force
+ * (p3m.hpp:146-165, A&S erfc of AS_erfc_part.hpp:29-40) to force, as the
+ * reference's short-range loop over its cell system applies it
+ * (forces.cpp:178-208, force on p1 += F(d), d = p1 - p2).  Cell lists of
+ * edge r_cut/2 or r_cut (all pairs for boxes under three cells wide).  flags:
+ * P3M_DEVICE_POINTERS, P3M_ASYNC (device pointers, no energy/pressure);
+ * P3M_ENERGY sets *energy to the sum of CoulombP3M::pair_energy
+ * (p3m.hpp:167-180), P3M_PRESSURE sets virial[9] to sum_pairs F (x) d. */
+int p3m_short_range(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                    unsigned flags, double *force, double *energy, double *virial);
+/* Cells per r_cut of the short-range cell lists: 0 = automatic (2 where the
+ * box is at least 2.5 r_cut wide), 1 or 2 (falls back to fewer cells, then
+ * all pairs, in narrow boxes). */
+int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff);
+/* Cells per dimension used by the last p3m_short_range call (0 0 0 before
+ * the first; 1 1 1 for all pairs). */
+int p3m_short_range_cells(const p3m_context *ctx, int cells[3]);
+
 /* Real-space pair forces of CoulombP3M::pair_force (p3m.hpp:146-165, A&S
  * erfc) on the ntargets particles listed in targets from all n particles
  * within the context's r_cut (minimum image, r_cut < half the box): device

@@ -1,0 +1,119 @@
+"""Device real-space pair forces (cell lists) against the FP64 oracle's
+brute-force pair loop (oracle.real_space: all pairs, minimum image, the
+reference's pair_force/pair_energy with the A&S erfc, p3m.hpp:146-180).
+
+Forces rel-rms <= 1e-12, energy and virial <= 1e-12 relative: both sides
+evaluate the same FP64 pair terms and differ only in summation order.  The
+cases cover cells of edge r_cut/2 and r_cut, the all-pairs path of narrow
+boxes, non-cubic boxes, uncharged particles and positions outside the box.
+The total device force (k-space + short range) is checked against exact
+Ewald on the C3 fixture subset.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from tests import p3m_systems as S
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _actor(box, r_cut, alpha, prefactor=1.3, mesh=16, cao=5):
+    from espresso_amd.electrostatics import P3M
+    p3m = P3M(prefactor=prefactor, accuracy=1e-3, mesh=mesh, cao=cao, r_cut=r_cut,
+              alpha=alpha, tune=False)
+    pos = np.random.default_rng(0).uniform(0., 1., (20, 3)) * np.asarray(box)
+    p3m.activate(pos, np.where(np.arange(20) % 2 == 0, 1., -1.), box_l=list(box))
+    return p3m
+
+
+def _system(n, box, seed, zero_frac=0.1):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0., 1., (n, 3)) * np.asarray(box)
+    q = np.where(np.arange(n) % 2 == 0, 1., -1.) * rng.uniform(0.5, 1.5, n)
+    q[rng.random(n) < zero_frac] = 0.
+    # a few positions outside the primary box (folded by the solver)
+    pos[:5, 0] -= box[0] * 1.3
+    pos[5:10, 1] += box[1] * 2.2
+    return pos, q
+
+
+@pytest.mark.parametrize("box,r_cut,subdiv,cells", [
+    ((12., 12., 12.), 2.0, 2, (12, 12, 12)),
+    ((12., 12., 12.), 2.0, 1, (6, 6, 6)),
+    ((10., 12., 14.), 2.4, 0, (8, 10, 11)),
+    ((4.5, 4.5, 4.5), 2.0, 0, (1, 1, 1)),       # narrower than 3 cells: all pairs
+    ((7.0, 4.6, 9.0), 2.2, 0, (1, 1, 1)),       # one narrow dimension: all pairs
+])
+def test_short_range_matches_oracle(box, r_cut, subdiv, cells):
+    import torch
+    from oracle import oracle as O
+    n = 3000 if box[0] > 5 else 300
+    pos, q = _system(n, box, 11)
+    alpha = 1.1
+    p3m = _actor(box, r_cut, alpha)
+    p3m.set_short_range_subdivision(subdiv)
+    tp = torch.tensor(pos, device="cuda")
+    tq = torch.tensor(q, device="cuda")
+    f0 = np.random.default_rng(5).normal(size=(n, 3))
+    tf = torch.tensor(f0, device="cuda")
+    e, vir = p3m.add_short_range_forces(tp, tq, tf, energy=True, pressure=True)
+    assert tuple(p3m.short_range_cells()) == cells
+    e_ref, f_ref, v_ref = O.real_space(pos, q, box, 1.3, alpha, r_cut, virial=True)
+    f = tf.cpu().numpy() - f0
+    assert S.rel_rms(f, f_ref) <= 1e-12
+    assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+    assert np.max(np.abs(vir - v_ref)) <= 1e-12 * np.max(np.abs(v_ref))
+    # forces only, asynchronous enqueue on device tensors
+    tf2 = torch.zeros_like(tp)
+    p3m.add_short_range_forces(tp, tq, tf2, sync=False)
+    p3m.synchronize()
+    assert S.rel_rms(tf2.cpu().numpy(), f_ref) <= 1e-12
+
+
+def test_short_range_host_arrays_and_errors():
+    from oracle import oracle as O
+    box = (9., 9., 9.)
+    pos, q = _system(800, box, 21)
+    p3m = _actor(box, 3.0, 0.9)
+    f = np.ones((800, 3))
+    e = p3m.add_short_range_forces(pos, q, f, energy=True)
+    e_ref, f_ref = O.real_space(pos, q, box, 1.3, 0.9, 3.0)
+    assert S.rel_rms(f - 1., f_ref) <= 1e-12
+    assert abs(e - e_ref) <= 1e-12 * abs(e_ref)
+    with pytest.raises(ValueError, match="sync=False needs device"):
+        p3m.add_short_range_forces(pos, q, f, sync=False)
+    with pytest.raises(ValueError, match="subdivision"):
+        p3m.set_short_range_subdivision(3)
+    wide = _actor((5., 5., 5.), 2.6, 1.0)
+    with pytest.raises(ValueError, match="half the box"):
+        wide.add_short_range_forces(pos, q, f)
+
+
+def test_c3_total_device_force_against_ewald():
+    """k-space + short range, both on the device, for the whole 250k system;
+    rms error on the exact-Ewald fixture subset within 1.5x the target."""
+    import torch
+    from espresso_amd.electrostatics import P3M
+    from oracle import oracle as O
+    pos, q, box = S.config_c3()
+    L = box[0]
+    fx = np.load(os.path.join(GOLDEN, "ewald_c3_subset.npz"))
+    acc = 1e-4
+    t = O.tune_r_cut(1.0, [64] * 3, 5, len(q), float(np.sum(q * q)), acc, [L] * 3)
+    r_cut, alpha = t[0] * L, t[1] / L
+    p3m = P3M(prefactor=1.0, accuracy=acc, mesh=64, cao=5, r_cut=r_cut, alpha=alpha,
+              tune=False)
+    p3m.activate(pos[:1000], np.where(np.arange(1000) % 2 == 0, 1., -1.), box_l=[L] * 3)
+    tp = torch.tensor(pos, device="cuda")
+    tq = torch.tensor(q, device="cuda")
+    f = torch.zeros_like(tp)
+    p3m.add_long_range_forces(tp, tq, f)
+    p3m.add_short_range_forces(tp, tq, f)
+    d = f.cpu().numpy()[fx["idx"]] - fx["force"]
+    err = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+    print(f"C3 mesh 64 cao 5: device total-force rms error {err:.3e} (estimate {t[2]:.3e})")
+    assert err <= 1.5 * acc
