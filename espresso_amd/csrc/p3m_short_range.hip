@@ -106,36 +106,97 @@ __global__ __launch_bounds__(256) void k_sr_scatter(int64_t n, double const *__r
   sidx[slot] = static_cast<int>(i);
 }
 
+/** 1/x and 1/sqrt(x) in FP64: the hardware estimate refined by two
+ *  Newton-Raphson steps (each doubles the correct bits). */
+__device__ __forceinline__ double nr_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double nr_rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x, y * y, 1.);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x, y * y, 1.);
+  return fma(0.5 * y, e, y);
+}
+
 /** Pair term of CoulombP3M::pair_force / pair_energy (p3m.hpp:146-180) with
- *  the A&S erfc of AS_erfc_part.hpp:29-40 (the published 7.1.26 constants):
- *  d = r_i - r_j, qq = prefactor q_i q_j.  Adds the force on i and, with EV,
- *  the pair energy and F (x) d (upper triangle: xx xy xz yy yz zz). */
+ *  the A&S erfc of AS_erfc_part.hpp:29-40 (the published 7.1.26 constants)
+ *  for a pair already known to lie within the cutoff: d = r_i - r_j,
+ *  r2 = |d|^2 > 0, qq = prefactor q_i q_j.  Adds the force on i and, with
+ *  EV, the pair energy and F (x) d (upper triangle: xx xy xz yy yz zz). */
 template <bool EV>
-__device__ __forceinline__ void pair_term(double dx, double dy, double dz, double qq, double rc2,
+__device__ __forceinline__ void pair_term(double dx, double dy, double dz, double r2, double qq,
                                           double alpha, double two_a_sqrt_pi_i, double &fx,
                                           double &fy, double &fz, double (&ev)[7]) {
-  double const r2 = dx * dx + dy * dy + dz * dz;
-  if (r2 < rc2 && r2 > 0.) {
-    double const dist = sqrt(r2);
-    double const adist = alpha * dist;
-    double const t = 1. / (1. + 0.3275911 * adist);
-    double const erfc_part =
-        t * (0.254829592 +
-             t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
-    double const ex = exp(-adist * adist);
-    double const erfc_ri = erfc_part / dist;
-    double const fac = ex * (erfc_ri + two_a_sqrt_pi_i) / r2 * qq;
-    fx += fac * dx;
-    fy += fac * dy;
-    fz += fac * dz;
-    if constexpr (EV) {
-      ev[0] += qq * erfc_ri * ex;
-      ev[1] += fac * dx * dx;
-      ev[2] += fac * dx * dy;
-      ev[3] += fac * dx * dz;
-      ev[4] += fac * dy * dy;
-      ev[5] += fac * dy * dz;
-      ev[6] += fac * dz * dz;
+  double const rinv = nr_rsq(r2);
+  double const adist = alpha * (r2 * rinv);
+  double const t = nr_rcp(fma(0.3275911, adist, 1.));
+  double const erfc_part =
+      t * (0.254829592 +
+           t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+  double const ex = exp(-adist * adist);
+  double const erfc_ri = erfc_part * rinv;
+  double const fac = ex * (erfc_ri + two_a_sqrt_pi_i) * (rinv * rinv) * qq;
+  fx += fac * dx;
+  fy += fac * dy;
+  fz += fac * dz;
+  if constexpr (EV) {
+    ev[0] += qq * erfc_ri * ex;
+    ev[1] += fac * dx * dx;
+    ev[2] += fac * dx * dy;
+    ev[3] += fac * dx * dz;
+    ev[4] += fac * dy * dy;
+    ev[5] += fac * dy * dz;
+    ev[6] += fac * dz * dz;
+  }
+}
+
+/** Displacement r_i - r_j, with the minimum image when MI. */
+template <bool MI>
+__device__ __forceinline__ void displacement(double4 const &pi, double4 const &pj,
+                                             CellGrid const &g, double &dx, double &dy,
+                                             double &dz) {
+  dx = pi.x - pj.x;
+  dy = pi.y - pj.y;
+  dz = pi.z - pj.z;
+  if constexpr (MI) {
+    dx -= g.L[0] * rint(dx / g.L[0]);
+    dy -= g.L[1] * rint(dy / g.L[1]);
+    dz -= g.L[2] * rint(dz / g.L[2]);
+  }
+}
+
+/** The staged candidates tile[0, cnt) against this thread's particle, in
+ *  groups of 64: a cheap pass marks the partners within the cutoff in a
+ *  64-bit mask, then each lane evaluates only its own marked pairs (the
+ *  expensive pair term runs with the lanes that have work, instead of on
+ *  every candidate that any lane of the wave needs). */
+template <bool MI, bool EV>
+__device__ __forceinline__ void tile_pairs(double4 const *tile, int cnt, double4 const &pi,
+                                           double qpi, CellGrid const &g, double rc2,
+                                           double alpha, double two_a, double &fx, double &fy,
+                                           double &fz, double (&ev)[7]) {
+  for (int kb = 0; kb < cnt; kb += 64) {
+    int const kn = min(64, cnt - kb);
+    unsigned long long mask = 0;
+    for (int k = 0; k < kn; ++k) {
+      double dx, dy, dz;
+      displacement<MI>(pi, tile[kb + k], g, dx, dy, dz);
+      double const r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < rc2 && r2 > 0.) mask |= 1ull << k;
+    }
+    while (mask) {
+      int const k = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      double4 const pj = tile[kb + k];
+      double dx, dy, dz;
+      displacement<MI>(pi, pj, g, dx, dy, dz);
+      double const r2 = dx * dx + dy * dy + dz * dz;
+      pair_term<EV>(dx, dy, dz, r2, qpi * pj.w, alpha, two_a, fx, fy, fz, ev);
     }
   }
 }
@@ -181,13 +242,9 @@ __global__ __launch_bounds__(NT) void k_sr_cells(CellGrid g, int const *__restri
           tile[threadIdx.x] = double4{pj.x + sx, pj.y + sy, pj.z + sz, pj.w};
         }
         __syncthreads();
-        int const cnt = min(NT, je - tb);
         if (has)
-          for (int k = 0; k < cnt; ++k) {
-            double4 const pj = tile[k];
-            pair_term<EV>(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z, qpi * pj.w, rc2, alpha, two_a,
-                          fx, fy, fz, ev);
-          }
+          tile_pairs<false, EV>(tile, min(NT, je - tb), pi, qpi, g, rc2, alpha, two_a, fx, fy,
+                                fz, ev);
       }
     }
     if (has) {
@@ -227,16 +284,9 @@ __global__ __launch_bounds__(NT) void k_sr_allpairs(int nq, CellGrid g,
     int const j = tb + static_cast<int>(threadIdx.x);
     if (j < nq) tile[threadIdx.x] = spos[j];
     __syncthreads();
-    int const cnt = min(NT, nq - tb);
     if (has)
-      for (int k = 0; k < cnt; ++k) {
-        double4 const pj = tile[k];
-        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        dx -= g.L[0] * rint(dx / g.L[0]);
-        dy -= g.L[1] * rint(dy / g.L[1]);
-        dz -= g.L[2] * rint(dz / g.L[2]);
-        pair_term<EV>(dx, dy, dz, qpi * pj.w, rc2, alpha, two_a, fx, fy, fz, ev);
-      }
+      tile_pairs<true, EV>(tile, min(NT, nq - tb), pi, qpi, g, rc2, alpha, two_a, fx, fy, fz,
+                           ev);
   }
   if (has) {
     double *f = force + 3 * static_cast<int64_t>(sidx[i]);
