@@ -392,6 +392,8 @@ def main():
                     help="skip the 512-system batched workload")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the 2048-system serving workload")
+    ap.add_argument("--no-short-range", action="store_true",
+                    help="skip the device real-space (short-range) timing")
     ap.add_argument("--accuracy-table", action="store_true",
                     help="also run the 27-cell precision sweep (N=250,000) "
                          "against exact Ewald forces")
@@ -521,6 +523,28 @@ def main():
         print(f"[bench] rms force error vs Ewald: {accuracy}", flush=True)
     table = accuracy_table_c3(torch, dev) if args.accuracy_table else None
 
+    # device real-space pair forces of the same system (cell lists), and the
+    # total force (k-space + real space) per step
+    short_range = None
+    if not args.no_short_range:
+        fs = torch.zeros_like(tp)
+        for _ in range(max(1, args.warmup)):
+            p3m.add_short_range_forces(tp, tq, fs, sync=False)
+        p3m.synchronize()
+        reps = max(3, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            p3m.add_short_range_forces(tp, tq, fs, sync=False)
+        p3m.synchronize()
+        t_sr = (time.perf_counter() - t0) / reps
+        short_range = {
+            "ms": round(t_sr * 1e3, 3),
+            "cells": p3m.short_range_cells(),
+            "r_cut": round(prm["r_cut"], 5),
+            "total_force_ms_fp64": round((t_sr + t_step) * 1e3, 3),
+            "total_force_mparticles_per_s_fp64": round(n / (t_sr + t_step) / 1e6, 2)}
+        print(f"[bench] short range: {short_range}", flush=True)
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev)
@@ -577,6 +601,7 @@ def main():
         "host_pointer_ms": round(t_host * 1e3, 3),
         "fp32_mesh": fp32,
         "accuracy": accuracy,
+        "short_range": short_range,
         "batched": batched,
         "serving": serving,
     }

@@ -235,6 +235,90 @@ def bspline_coeffs(order, i):
     return num, den.value
 
 
+def _bspline_eval(order, x):
+    """(order, len(x)) B-spline weights of the stencil points at offsets x in
+    [-1/2, 1/2] from the exact integer tables."""
+    x = np.asarray(x, np.float64)
+    w = np.empty((order, x.size))
+    for i in range(order):
+        num, den = bspline_coeffs(order, i)
+        acc = np.full(x.size, float(num[order - 1]))
+        for m in range(order - 2, -1, -1):
+            acc = float(num[m]) + x * acc
+        w[i] = acc / den
+    return w
+
+
+def gpu_kspace_forces(pos, q, box_l, mesh, cao, alpha, prefactor):
+    """k-space forces of the reference's GPU P3M (P3MGPU;
+    p3m_gpu_cuda.cu:165-723), FP64 arithmetic with the index rule of its
+    FP32 kernels (:315-360, 424-470):
+
+    - positions folded into the box and cast to float
+      (GpuParticleData.cpp:90); m = x hi - pos_shift with hi = float(M) /
+      float(L), pos_shift = float((cao - 1) / 2) (integer division), one
+      rounding (the CUDA compiler contracts it to an FMA); nmp =
+      floor(m + 1/2), dist = m - nmp, both in float
+    - stencil points nmp + k (k < cao), periodic wrap
+    - unnormalised r2c FFT; G_hat = 2/pi exp(-pi^2 n_L^2 / alpha^2) /
+      (n_L^2 U^2) with n_L = n / L, U^2 = prod_d sinc(n_d / M_d)^(2 cao)
+      (P3M_BRILLOUIN = 0), zero at n = 0 and where every index is a
+      multiple of M/2 (:213-248)
+    - i k = 2 pi i n / L, n = j - M for j > M/2, so the Nyquist index keeps
+      +M/2, and n_z = j on the half spectrum (:265-295)
+    - unnormalised c2r (imaginary parts of the kz = 0 and Nyquist bins are
+      dropped); F = -prefactor / (2 V) q sum_stencil w E (:424-723)."""
+    pos = np.asarray(pos, np.float64).reshape(-1, 3)
+    q = np.asarray(q, np.float64)
+    M = np.asarray(mesh, np.int64).reshape(3)
+    L = np.asarray(box_l, np.float64).reshape(3)
+    Lf = L.astype(np.float32)
+    hi = (M.astype(np.float32) / Lf).astype(np.float32)
+    ps = np.float32((cao - 1) // 2)
+    xf = pos - L * np.floor(pos / L)
+    xf = np.where(xf >= L, xf - L, xf).astype(np.float32)
+    m = (xf.astype(np.float64) * hi.astype(np.float64) - float(ps)).astype(np.float32)
+    nmp = np.floor((m + np.float32(0.5)).astype(np.float32)).astype(np.int64)
+    dist = (m - nmp.astype(np.float32)).astype(np.float32).astype(np.float64)
+    w = [_bspline_eval(cao, dist[:, d]) for d in range(3)]          # (cao, n)
+    idx = [np.mod(nmp[:, d][None, :] + np.arange(cao)[:, None], M[d]) for d in range(3)]
+    rho = np.zeros(tuple(M))
+    for i in range(cao):
+        for j in range(cao):
+            for k in range(cao):
+                np.add.at(rho, (idx[0][i], idx[1][j], idx[2][k]),
+                          q * w[0][i] * w[1][j] * w[2][k])
+    rhat = np.fft.rfftn(rho)
+    K2 = M[2] // 2 + 1
+    n = []
+    for d, size in enumerate((M[0], M[1], K2)):
+        j = np.arange(size)
+        n.append(np.where(j > M[d] // 2, j - M[d], j).astype(np.float64))
+    nx, ny, nz = np.meshgrid(n[0], n[1], n[2], indexing="ij")
+    nm2 = (nx / L[0]) ** 2 + (ny / L[1]) ** 2 + (nz / L[2]) ** 2
+    S = (np.sinc(nx / M[0]) * np.sinc(ny / M[1]) * np.sinc(nz / M[2])) ** (2 * cao)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        G = 2. / np.pi * np.exp(-(np.pi / alpha) ** 2 * nm2) / (nm2 * S)
+    jx, jy, jz = np.meshgrid(np.arange(M[0]), np.arange(M[1]), np.arange(K2), indexing="ij")
+    zero = ((jx == 0) & (jy == 0) & (jz == 0)) | (
+        (jx % (M[0] // 2) == 0) & (jy % (M[1] // 2) == 0) & (jz % (M[2] // 2) == 0))
+    G[zero] = 0.
+    phi = rhat * G
+    N = float(np.prod(M))
+    E = [np.fft.irfftn(2j * np.pi * nd / L[d] * phi, s=tuple(M)) * N
+         for d, nd in enumerate((nx, ny, nz))]
+    pref = prefactor / (2. * float(np.prod(L.astype(np.float32).astype(np.float64))))
+    f = np.zeros((len(q), 3))
+    for i in range(cao):
+        for j in range(cao):
+            for k in range(cao):
+                ww = w[0][i] * w[1][j] * w[2][k]
+                sel = (idx[0][i], idx[1][j], idx[2][k])
+                for d in range(3):
+                    f[:, d] += ww * E[d][sel]
+    return -pref * q[:, None] * f
+
+
 def meshift(n, zero_out_midpoint=False):
     out = np.zeros(n, np.int32)
     lib().p3mo_meshift(n, int(bool(zero_out_midpoint)), out)

@@ -331,3 +331,42 @@ def test_ewald_fixtures_present():
         assert d["force"].shape == (2000, 3) and np.all(np.isfinite(d["force"]))
         assert d["idx"].shape == (2000,) and d["idx"].max() < n_sys
         assert float(d["alpha_e"]) * float(d["r_e"]) == pytest.approx(5.6)
+
+
+@pytest.mark.parametrize("cao", [5, 7])
+def test_gpu_convention_equals_cpu_convention_with_zero_mesh_offset(cao):
+    """For odd cao the reference GPU index rule (nmp = floor(x M/L -
+    (cao-1)/2 + 1/2)) is the CPU rule with mesh_off = 0, so the P3MGPU
+    oracle must agree with the CPU oracle at mesh_off 0 up to its float
+    positions (Nyquist terms are negligible at this alpha)."""
+    L, mesh, alpha = 10., 24, 0.8
+    pos, q = S.uniform_system(1500, L, 71)
+    f_gpu = O.gpu_kspace_forces(pos, q, [L] * 3, [mesh] * 3, cao, alpha, 1.3)
+    ora = O.OracleP3M([L] * 3, [mesh] * 3, cao, alpha, 3.0, 1.3, mesh_off=(0., 0., 0.))
+    f_cpu = ora.forces(pos, q)
+    agree = S.rel_rms(f_gpu, f_cpu)
+    assert agree <= 2e-6
+    # the default CPU mesh (points at cell centres) is a different, equally
+    # accurate discretisation: the two differ several times more
+    f_mid = O.OracleP3M([L] * 3, [mesh] * 3, cao, alpha, 3.0, 1.3).forces(pos, q)
+    assert S.rel_rms(f_gpu, f_mid) > 5 * agree
+
+
+def test_gpu_convention_even_cao_against_ewald():
+    """Even cao (index rule differs from the CPU one): the P3MGPU oracle's
+    total force on the C3 fixture subset is within 1.5x the tuned error
+    estimate of exact Ewald."""
+    import os
+    pos, q, box = S.config_c3()
+    L = box[0]
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                              "ewald_c3_subset.npz"))
+    acc = 1e-3
+    t = O.tune_r_cut(1.0, [48] * 3, 4, len(q), float(np.sum(q * q)), acc, [L] * 3)
+    r_cut, alpha = t[0] * L, t[1] / L
+    f_k = O.gpu_kspace_forces(pos, q, box, [48] * 3, 4, alpha, 1.0)[fx["idx"]]
+    order = np.concatenate([fx["idx"], np.setdiff1d(np.arange(len(q)), fx["idx"])])
+    f_r = O.real_space_subset(pos[order], q[order], box, 1.0, alpha, r_cut, len(fx["idx"]))
+    d = f_k + f_r - fx["force"]
+    err = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+    assert err <= 1.5 * acc, (err, t[2])
