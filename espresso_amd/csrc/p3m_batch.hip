@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
     }
     if (want_fields) {
       C const phi = C{gv * rh.x, gv * rh.y};
-      MT const kxf = static_cast<MT>(dop_shift(j, Mx)) * static_cast<MT>(wvx);
+      MT const kxf = static_cast<MT>(ik_shift(j, Mx, d.g.conv)) * static_cast<MT>(wvx);
       oth[c * ls + j] = phi;                                   // A
       oth[(ncol + c) * ls + j] = C{-kxf * phi.y, kxf * phi.x}; // B = i kx phi
     }
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
   __syncthreads();
   for (int e = threadIdx.x; e < n * nc; e += blockDim.x) {
     int const c = fdiv(e, fn), j = e - c * n;
-    MT const kyf = static_cast<MT>(dop_shift(j, My)) * static_cast<MT>(wvy);
+    MT const kyf = static_cast<MT>(ik_shift(j, My, d.g.conv)) * static_cast<MT>(wvy);
     C const v = S[c * ls + j];
     W0[c * ls + j] = C{-kyf * v.y, kyf * v.x};
   }
@@ -869,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
     int const y = fdiv(e, fMx), x = e - y * Mx;
     C const v = q[y * lsx + x];
     col[e] = v;
-    MT const kf = static_cast<MT>(dop_shift(x, Mx)) * static_cast<MT>(wvx);
+    MT const kf = static_cast<MT>(ik_shift(x, Mx, d.g.conv)) * static_cast<MT>(wvx);
     w[y * lsx + x] = C{-kf * v.y, kf * v.x};
   }
   __syncthreads();
@@ -913,7 +913,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
     int const y = fdiv(e, fMx), x = e - y * Mx;
     C const v = col[e];
-    MT const kf = static_cast<MT>(dop_shift(y, My)) * static_cast<MT>(wvy);
+    MT const kf = static_cast<MT>(ik_shift(y, My, d.g.conv)) * static_cast<MT>(wvy);
     P0[x * lsy + y] = C{-kf * v.y, kf * v.x};
   }
   __syncthreads();
@@ -965,7 +965,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
     C x1 = (r0 < rows) ? S[i0] : C{MT(0), MT(0)};
     C x2 = (r1 < rows) ? S[i1] : C{MT(0), MT(0)};
     if (mul_ikz) {
-      MT const kz = static_cast<MT>(dop_shift(ks, n)) * static_cast<MT>(wvz);
+      MT const kz = static_cast<MT>(ik_shift(ks, n, d.g.conv)) * static_cast<MT>(wvz);
       x1 = C{-kz * x1.y, kz * x1.x};
       x2 = C{-kz * x2.y, kz * x2.x};
     }
@@ -1076,7 +1076,9 @@ __global__ __launch_bounds__(NTH) void k_gather_plane(
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
   double dm[3] = {0., 0., 0.};
-  if (d.dip_pref != 0.)
+  // the GPU index rule follows the reference's GPU force kernel, which has
+  // no dipole term (p3m.cpp:867-888)
+  if (d.dip_pref != 0. && !d.g.conv)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
   fetch_tiles_and_stage<P, MT, NTH>(d, sp, beg, min(chunk, end - beg), o0, o1, o2, tx, ty, tz, s);
@@ -1174,7 +1176,9 @@ __global__ __launch_bounds__(NTH) void k_gather_layer(
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
   double dm[3] = {0., 0., 0.};
-  if (d.dip_pref != 0.)
+  // the GPU index rule follows the reference's GPU force kernel, which has
+  // no dipole term (p3m.cpp:867-888)
+  if (d.dip_pref != 0. && !d.g.conv)
     for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
   double4 const *sp = spos + soff[L.s];
   fetch_tiles_and_stage<P, MT, NTH>(d, sp, beg, min(chunk, end - beg), o0, o1, o2, tx, ty, tz, s);
@@ -1843,6 +1847,10 @@ int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, doub
   g.tz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[2]) + 1;
   g.tyz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[1] * g.T[2]) + 1;
   g.pos_shift = std::floor((cao - 1) / 2.0) - (cao % 2) / 2.0;
+  g.conv = 0;
+  for (int k = 0; k < 3; ++k)
+    g.hi_f[k] = static_cast<float>(g.M[k]) / static_cast<float>(box_l[k]);
+  g.ps_f = static_cast<float>((cao - 1) / 2);
   d.cao = cao;
   h.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
   double const two_pi = 2. * std::numbers::pi;
@@ -1892,6 +1900,18 @@ void BatchSolver::set_particle_count(int s, int64_t n) {
   m->sys.at(s).n = n;
   recompute_offsets(m);
 }
+
+void BatchSolver::set_convention(int s, int conv) {
+  if (conv != 0 && conv != 1) throw std::invalid_argument("P3M: convention must be 0 or 1");
+  DevGeo &g = m->sys.at(s).desc.g;
+  if (g.conv == conv) return;
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  quiesce(m);
+  g.conv = conv;
+  m->descs_dirty = true;
+}
+
+int BatchSolver::convention(int s) const { return m->sys.at(s).desc.g.conv; }
 
 namespace {
 void upload_descs(BatchSolver::Impl *m, hipStream_t st) {

@@ -93,6 +93,12 @@ public:
   int num_systems() const;
   /** Change the particle count of system s (offsets of later systems move). */
   void set_particle_count(int s, int64_t n);
+  /** Index rule and ik of system s: 0 = the CPU kernels' (default), 1 = the
+   *  reference's GPU kernels' (FP32 index nmp = floor(x M / L - (cao - 1) / 2
+   *  + 1/2), Nyquist wave number kept at +M/2).  Drains the lanes when it
+   *  changes. */
+  void set_convention(int s, int conv);
+  int convention(int s) const;
   int64_t particle_offset(int s) const;
   int64_t particle_count(int s) const;
   int64_t total_particles() const;

@@ -32,6 +32,7 @@ struct p3m_context {
   std::unique_ptr<Solver> solver;
   std::unique_ptr<ShortRange> short_range; // created on first use
   int short_range_subdivision = 0;
+  int force_conv = P3M_CONV_CPU;
   p3m_params user{};
   P3MParams params{};
   bool have_params = false;
@@ -402,8 +403,21 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
       throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
     DevArgs const d = stage_inputs(ctx, n, pos, q, pos_unfolded, force, flags);
     SolveResult r;
-    ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
-                       &r, !async);
+    unsigned const reductions = flags & (P3M_ENERGY | P3M_PRESSURE);
+    if (ctx->force_conv == P3M_CONV_GPU && (flags & P3M_FORCES)) {
+      // forces with the GPU kernels' index rule; energy and virial, as in
+      // the reference's GPU actor, from the CPU kernels' mesh (p3m.cpp:867-892)
+      ctx->solver->set_convention(1);
+      ctx->solver->solve(n, d.pos, d.q, d.upos, d.force, flags & ~reductions, nullptr, !async);
+      if (reductions && !async) {
+        ctx->solver->set_convention(0);
+        ctx->solver->solve(n, d.pos, d.q, d.upos, nullptr, flags & ~P3M_FORCES, &r, true);
+      }
+    } else {
+      ctx->solver->set_convention(0);
+      ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
+                         &r, !async);
+    }
     if (async) {
       if (energy) *energy = 0.;
       if (virial)
@@ -430,6 +444,7 @@ int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, double const *pos, double
     if (!ctx->initialized) init_solver(ctx);
     DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);
     std::vector<double> out;
+    ctx->solver->set_convention(ctx->force_conv);
     ctx->solver->debug_charge_mesh(n, d.pos, d.q, out);
     std::memcpy(mesh_out, out.data(), sizeof(double) * out.size());
   });
@@ -445,6 +460,7 @@ int p3m_debug_indices(p3m_context *ctx, int64_t n, double const *pos, double con
     if (!ctx->initialized) init_solver(ctx);
     DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, 0);
     std::vector<int32_t> out;
+    ctx->solver->set_convention(ctx->force_conv);
     ctx->solver->debug_indices(n, d.pos, d.q, out);
     std::memcpy(nmp, out.data(), sizeof(int32_t) * out.size());
   });
@@ -523,6 +539,21 @@ int p3m_short_range(p3m_context *ctx, int64_t n, double const *pos, double const
     if (virial)
       for (int k = 0; k < 9; ++k) virial[k] = (flags & P3M_PRESSURE) ? r.virial[k] : 0.;
   });
+}
+
+int p3m_set_force_convention(p3m_context *ctx, int convention) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (convention != P3M_CONV_CPU && convention != P3M_CONV_GPU)
+      throw std::invalid_argument("P3M: force convention must be P3M_CONV_CPU or P3M_CONV_GPU");
+    ctx->force_conv = convention;
+  });
+}
+
+int p3m_get_force_convention(p3m_context const *ctx, int *convention) {
+  if (!ctx || !convention) return P3M_ERR_INVALID;
+  *convention = ctx->force_conv;
+  return P3M_OK;
 }
 
 int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff) {

@@ -139,6 +139,11 @@ struct DevGeo {
   double ai[3];
   double box_l[3];
   double pos_shift;
+  // index rule: 0 = the CPU kernels' CA mesh (above); 1 = the FP32 rule of
+  // the reference's GPU kernels, m = fma(x, hi_f, -ps_f), nmp = floor(m + 1/2)
+  int conv;
+  float hi_f[3]; // float(M) / float(L)
+  float ps_f;    // float((cao - 1) / 2), integer division
 };
 
 /** floor(e / d) for 0 <= e < 2^16 and 1 <= d < 2^16 from the magic number
@@ -205,6 +210,35 @@ template <int S> __device__ __forceinline__ double segment_total(double v) {
   return v;
 }
 
+/** One dimension of the index rule of the reference's GPU kernels
+ *  (p3m_gpu_cuda.cu:334-345): the position folded into [0, L) in FP64
+ *  (x - L floor(x / L), every operation rounded; the reference's particles
+ *  arrive folded, GpuParticleData.cpp:90) and cast to float, m = x hi -
+ *  pos_shift as one FMA, nmp = floor(m + 1/2), dist = m - nmp in float.
+ *  gi is the periodic index of the first stencil point nmp; the CA-local
+ *  index is reported as gi as well.  Returns false for non-finite x. */
+__device__ __forceinline__ bool gpu_coordinate_1d(DevGeo const &g, double x, int d, int &nmp,
+                                                  int &gi, double &dist) {
+#pragma clang fp contract(off)
+  bool const ok = isfinite(x);
+  double const L = g.box_l[d];
+  double xf = ok ? x : 0.;
+  double const k = floor(xf / L);
+  double const sh = L * k;
+  xf = xf - sh;
+  if (xf >= L) xf = xf - L;
+  float const xs = static_cast<float>(xf);
+  float const m = __fmaf_rn(xs, g.hi_f[d], -g.ps_f);
+  float const nf = floorf(m + 0.5f);
+  int const n = static_cast<int>(nf);
+  dist = static_cast<double>(m - nf);
+  int gg = n % g.M[d];
+  if (gg < 0) gg += g.M[d];
+  gi = gg;
+  nmp = gg;
+  return ok;
+}
+
 /** Mesh coordinates of one particle for stencil order P: CA-local first
  *  stencil point nmp, its periodic FFT-mesh index gi and the offsets dist.
  *  Positions whose stencil would leave the CA mesh (further outside the box
@@ -213,6 +247,11 @@ template <int S> __device__ __forceinline__ double segment_total(double v) {
 __device__ __forceinline__ bool particle_coordinates_rt(DevGeo const &g, int P, double const x[3],
                                                         int nmp[3], int gi[3], double dist[3]) {
   bool ok = true;
+  if (g.conv) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ok &= gpu_coordinate_1d(g, x[d], d, nmp[d], gi[d], dist[d]);
+    return ok;
+  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     double t = ca_coordinate(x[d], g.ld_pos[d], g.ai[d], g.pos_shift);
@@ -239,6 +278,7 @@ __device__ __forceinline__ bool particle_coordinates_rt(DevGeo const &g, int P, 
  *  result is identical). */
 __device__ __forceinline__ bool coordinate_1d(DevGeo const &g, int P, double x, int d, int &nmp,
                                               int &gi, double &dist) {
+  if (g.conv) return gpu_coordinate_1d(g, x, d, nmp, gi, dist);
   bool ok = true;
   double t = ca_coordinate(x, g.ld_pos[d], g.ai[d], g.pos_shift);
   if (!(t >= 0. && t < static_cast<double>(g.dim[d] - P + 1))) {
@@ -271,6 +311,14 @@ __device__ __forceinline__ bool particle_coordinates(DevGeo const &g, double con
 __device__ __forceinline__ int dop_shift(int j, int M) {
   if (2 * j == M) return 0;
   return (2 * j < M) ? j : j - M;
+}
+/** Wave-number index of the ik-differentiation of the field components:
+ *  the CPU kernels' d_op (Nyquist zero, dop_shift) or, for the GPU index
+ *  rule, n = j - M for j > M/2 with the Nyquist entry kept at +M/2
+ *  (apply_diff_op, p3m_gpu_cuda.cu:265-295). */
+__device__ __forceinline__ int ik_shift(int j, int M, int conv) {
+  if (conv) return (2 * j > M) ? j - M : j;
+  return dop_shift(j, M);
 }
 /** calc_meshift(mesh, false) entry (common.hpp:248-264). */
 __device__ __forceinline__ int plain_shift(int j, int M) { return (2 * j < M) ? j : j - M; }

@@ -49,7 +49,14 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
   auto b = std::make_unique<BatchSolver>(m_device, precision);
   if (m_shape_set) b->set_launch_shape(m_shape);
   b->add_system(params, box_l, skin, prefactor, 0);
+  b->set_convention(0, m_conv);
   m_batch = std::move(b);
+}
+
+void Solver::set_convention(int conv) {
+  if (conv != 0 && conv != 1) throw std::invalid_argument("P3M: convention must be 0 or 1");
+  m_conv = conv;
+  if (m_batch) m_batch->set_convention(0, conv);
 }
 
 void Solver::solve(int64_t n, double const *d_pos, double const *d_q, double const *d_upos,

@@ -61,6 +61,10 @@ public:
    *  fft_fwd, convolve, fft_bwd, gather. */
   float last_ms[8] = {};
   void set_launch_shape(LaunchShape const &s);
+  /** Index rule of the solves (BatchSolver::set_convention); kept across
+   *  later configure() calls. */
+  void set_convention(int conv);
+  int convention() const { return m_conv; }
   /** Pick the fastest launch shape for force solves of these device arrays
    *  (BatchSolver::autotune); d_force is scratch.  The shape is kept across
    *  later configure() calls. */
@@ -74,6 +78,7 @@ private:
   std::unique_ptr<BatchSolver> m_batch;
   LaunchShape m_shape;
   bool m_shape_set = false;
+  int m_conv = 0;
 };
 
 } // namespace espresso_amd

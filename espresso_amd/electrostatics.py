@@ -113,6 +113,7 @@ class _P3MBase:
 
     _name = "P3M"
     _gpu = False
+    _force_convention = _lib.P3M_CONV_CPU
 
     def required_keys(self):
         return {"prefactor", "accuracy"}
@@ -154,6 +155,8 @@ class _P3MBase:
         self._params = params
         self._ctx = C.c_void_p()
         _lib.raise_for(L.p3m_create(int(device), C.byref(self._ctx)))
+        _lib.raise_for(L.p3m_set_force_convention(self._ctx, self._force_convention),
+                       self._ctx)
         self._device = int(device)
         self._activated = False
         self._box_l = None
@@ -257,6 +260,14 @@ class _P3MBase:
             self._ctx, a.n, a.pos, a.q, a.upos, a.force, flags, None, None),
             self._ctx)
         return force
+
+    def force_convention(self):
+        """"cpu" (the CPU kernels' mesh) or "gpu" (the reference's GPU
+        kernels' index rule) for the forces."""
+        c = C.c_int(0)
+        _lib.raise_for(_lib.load().p3m_get_force_convention(self._ctx, C.byref(c)),
+                       self._ctx)
+        return "gpu" if c.value == _lib.P3M_CONV_GPU else "cpu"
 
     def autotune(self, pos, q, reps=3):
         """Pick the fastest launch shape of the device kernels (fused y/x
@@ -500,9 +511,14 @@ class P3M(_P3MBase):
 
 class P3MGPU(_P3MBase):
     """P3M electrostatics solver with GPU support (FP32 mesh, FP64
-    reductions and force accumulation)."""
+    reductions and force accumulation).  Forces follow the reference's GPU
+    kernels (FP32 mesh index without mesh offset, Nyquist wave number kept,
+    no dipole force term; p3m_gpu_cuda.cu:265-360, p3m.cpp:867-888); energy
+    and pressure the CPU kernels' mesh, as the reference's GPU actor
+    computes them (p3m.cpp:889-892)."""
 
     _gpu = True
+    _force_convention = _lib.P3M_CONV_GPU
 
     def default_params(self):
         return {"single_precision": True, **super().default_params()}

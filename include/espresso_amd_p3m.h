@@ -55,6 +55,10 @@ extern "C" {
 #define P3M_PREC_FP64 0 /* plain FP64 solver (drop-in for P3M, double) */
 #define P3M_PREC_FP32 1 /* FP32 mesh, FP64 weights/reductions/forces    */
 
+/* force conventions (p3m_set_force_convention) */
+#define P3M_CONV_CPU 0 /* CPU kernels: CA mesh with mesh_off, Nyquist ik 0  */
+#define P3M_CONV_GPU 1 /* GPU kernels: FP32 index, no mesh_off, +M/2 ik     */
+
 typedef struct p3m_context p3m_context;
 
 /* P3M parameters.  Field meaning and the -1 "to be tuned" sentinels follow
@@ -151,6 +155,21 @@ int p3m_short_range(p3m_context *ctx, int64_t n, const double *pos, const double
  * box is at least 2.5 r_cut wide), 1 or 2 (falls back to fewer cells, then
  * all pairs, in narrow boxes). */
 int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff);
+
+/* Force convention of p3m_solve.  P3M_CONV_GPU gives the forces of the
+ * reference's P3MGPU actor (p3m_gpu_add_farfield_force,
+ * p3m_gpu_cuda.cu:672-723, called from add_long_range_forces_gpu,
+ * p3m.cpp:867-888): first stencil point nmp = floor(m + 1/2) with
+ * m = fmaf(float(x), float(M) / float(L), -(float)((cao - 1) / 2)) on the
+ * folded position (p3m_gpu_cuda.cu:334-345), weights at m - nmp, ik with
+ * the Nyquist wave number kept at +M/2 (apply_diff_op, :265-295) and no
+ * dipole force term; mesh_off is ignored.  Energy and virial keep the CPU
+ * kernels' mesh, as the reference's GPU actor computes them on the CPU
+ * (p3m.cpp:889-892): a solve with P3M_FORCES and P3M_ENERGY or
+ * P3M_PRESSURE then runs twice.  p3m_debug_indices reports the periodic
+ * index of nmp under P3M_CONV_GPU.  Default P3M_CONV_CPU. */
+int p3m_set_force_convention(p3m_context *ctx, int convention);
+int p3m_get_force_convention(const p3m_context *ctx, int *convention);
 /* Cells per dimension used by the last p3m_short_range call (0 0 0 before
  * the first; 1 1 1 for all pairs). */
 int p3m_short_range_cells(const p3m_context *ctx, int cells[3]);

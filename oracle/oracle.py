@@ -249,6 +249,29 @@ def _bspline_eval(order, x):
     return w
 
 
+def gpu_mesh_coordinates(pos, box_l, mesh, cao):
+    """First stencil point nmp (not wrapped) and offset dist of each particle
+    under the index rule of the reference's GPU kernels
+    (p3m_gpu_cuda.cu:334-345): positions folded into [0, L) in FP64
+    (x - L floor(x / L)) and cast to float, m = x hi - pos_shift with
+    hi = float(M) / float(L) and pos_shift = float((cao - 1) / 2) (integer
+    division) rounded once (an FMA; the FP64 product of two floats is exact
+    and the difference needs at most 53 bits, so rounding it to float is
+    the FMA's single rounding), nmp = floor(m + 1/2) in float, dist = m - nmp.
+    Returns (nmp int64 [n, 3], dist float64 [n, 3])."""
+    pos = np.asarray(pos, np.float64).reshape(-1, 3)
+    M = np.asarray(mesh, np.int64).reshape(3)
+    L = np.asarray(box_l, np.float64).reshape(3)
+    hi = (M.astype(np.float32) / L.astype(np.float32)).astype(np.float32)
+    ps = np.float32((cao - 1) // 2)
+    xf = pos - L * np.floor(pos / L)
+    xf = np.where(xf >= L, xf - L, xf).astype(np.float32)
+    m = (xf.astype(np.float64) * hi.astype(np.float64) - float(ps)).astype(np.float32)
+    nmp = np.floor((m + np.float32(0.5)).astype(np.float32)).astype(np.int64)
+    dist = (m - nmp.astype(np.float32)).astype(np.float32).astype(np.float64)
+    return nmp, dist
+
+
 def gpu_kspace_forces(pos, q, box_l, mesh, cao, alpha, prefactor):
     """k-space forces of the reference's GPU P3M (P3MGPU;
     p3m_gpu_cuda.cu:165-723), FP64 arithmetic with the index rule of its
@@ -268,18 +291,10 @@ def gpu_kspace_forces(pos, q, box_l, mesh, cao, alpha, prefactor):
       +M/2, and n_z = j on the half spectrum (:265-295)
     - unnormalised c2r (imaginary parts of the kz = 0 and Nyquist bins are
       dropped); F = -prefactor / (2 V) q sum_stencil w E (:424-723)."""
-    pos = np.asarray(pos, np.float64).reshape(-1, 3)
     q = np.asarray(q, np.float64)
     M = np.asarray(mesh, np.int64).reshape(3)
     L = np.asarray(box_l, np.float64).reshape(3)
-    Lf = L.astype(np.float32)
-    hi = (M.astype(np.float32) / Lf).astype(np.float32)
-    ps = np.float32((cao - 1) // 2)
-    xf = pos - L * np.floor(pos / L)
-    xf = np.where(xf >= L, xf - L, xf).astype(np.float32)
-    m = (xf.astype(np.float64) * hi.astype(np.float64) - float(ps)).astype(np.float32)
-    nmp = np.floor((m + np.float32(0.5)).astype(np.float32)).astype(np.int64)
-    dist = (m - nmp.astype(np.float32)).astype(np.float32).astype(np.float64)
+    nmp, dist = gpu_mesh_coordinates(pos, box_l, mesh, cao)
     w = [_bspline_eval(cao, dist[:, d]) for d in range(3)]          # (cao, n)
     idx = [np.mod(nmp[:, d][None, :] + np.arange(cao)[:, None], M[d]) for d in range(3)]
     rho = np.zeros(tuple(M))

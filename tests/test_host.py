@@ -69,6 +69,18 @@ def test_gpu_actor_requires_single_precision():
     assert P3MGPU(device=-1, **VALID).get_params()["single_precision"] is True
 
 
+def test_force_convention_of_the_actors():
+    """P3MGPU forces follow the reference's GPU kernels, P3M's the CPU
+    kernels; other values are rejected with the library's message."""
+    from espresso_amd import _lib
+    assert P3MGPU(device=-1, **VALID).force_convention() == "gpu"
+    p = P3M(device=-1, **VALID)
+    assert p.force_convention() == "cpu"
+    with pytest.raises(ValueError, match="force convention must be"):
+        _lib.raise_for(_lib.load().p3m_set_force_convention(p._ctx, 2), p._ctx)
+    assert p.force_convention() == "cpu"
+
+
 def test_missing_required_keys():
     with pytest.raises(ValueError, match="prefactor"):
         P3M(device=-1, accuracy=1e-3)
