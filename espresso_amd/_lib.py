@@ -31,6 +31,9 @@ P3M_PREC_FP32 = 1
 P3M_CONV_CPU = 0
 P3M_CONV_GPU = 1
 
+P3M_TUNE_RS_MODEL = 0
+P3M_TUNE_RS_DEVICE = 1
+
 # every symbol declared in include/espresso_amd_p3m.h
 EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
@@ -40,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "p3m_pair_energy",
     "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end", "p3m_autotune",
     "p3m_short_range", "p3m_short_range_cells", "p3m_set_short_range_subdivision",
-    "p3m_set_force_convention", "p3m_get_force_convention",
+    "p3m_set_force_convention", "p3m_get_force_convention", "p3m_set_tuning_real_space",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -138,6 +141,7 @@ def load():
     L.p3m_set_short_range_subdivision.argtypes = [vp, C.c_int]
     L.p3m_set_force_convention.argtypes = [vp, C.c_int]
     L.p3m_get_force_convention.argtypes = [vp, C.POINTER(C.c_int)]
+    L.p3m_set_tuning_real_space.argtypes = [vp, C.c_int]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []

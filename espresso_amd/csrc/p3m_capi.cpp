@@ -33,6 +33,7 @@ struct p3m_context {
   std::unique_ptr<ShortRange> short_range; // created on first use
   int short_range_subdivision = 0;
   int force_conv = P3M_CONV_CPU;
+  int tune_real_space = P3M_TUNE_RS_MODEL;
   p3m_params user{};
   P3MParams params{};
   bool have_params = false;
@@ -310,6 +311,44 @@ int p3m_is_tuned(p3m_context const *ctx, int *out) {
   return P3M_OK;
 }
 
+namespace {
+/** Mean time of the device pair loop (p3m_short_range) for trial
+ *  parameters, in ms: one warm-up call, then `timings` calls between two
+ *  events on the solver stream.  Forces go to scratch. */
+double time_short_range(p3m_context *ctx, P3MParams const &trial, Vec3d const &box, int64_t n,
+                        double const *d_pos, double const *d_q, double *scratch_force,
+                        int timings) {
+  if (!ctx->short_range) ctx->short_range = std::make_unique<ShortRange>(ctx->device);
+  ShortRangeParams sp;
+  for (int k = 0; k < 3; ++k) sp.box[k] = box[k];
+  sp.prefactor = ctx->user.prefactor;
+  sp.alpha = trial.alpha;
+  sp.r_cut = trial.r_cut;
+  sp.subdivision = ctx->short_range_subdivision;
+  hipStream_t const st = ctx->solver->stream();
+  ctx->short_range->compute(st, sp, n, d_pos, d_q, scratch_force, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  float ms = 0.f;
+  try {
+    hip_check(hipEventRecord(e0, st), "hipEventRecord");
+    for (int t = 0; t < timings; ++t)
+      ctx->short_range->compute(st, sp, n, d_pos, d_q, scratch_force, nullptr);
+    hip_check(hipEventRecord(e1, st), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return static_cast<double>(ms) / timings;
+}
+} // namespace
+
 int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q,
                  unsigned flags) {
   if (!ctx) return P3M_ERR_INVALID;
@@ -376,7 +415,11 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
                              &r);
           total += ctx->solver->last_ms[0];
         }
-        return total / timings + model_real_space_ms(in.sum_qpart, trial.r_cut, box);
+        double rs_ms = model_real_space_ms(in.sum_qpart, trial.r_cut, box);
+        if (ctx->tune_real_space == P3M_TUNE_RS_DEVICE &&
+            trial.r_cut < 0.5 * std::min({box[0], box[1], box[2]}))
+          rs_ms = time_short_range(ctx, trial, box, n, d.pos, d.q, scratch_force, timings);
+        return total / timings + rs_ms;
       };
       tune_p3m(p, in, timer);
       ctx->is_tuned = true;
@@ -547,6 +590,16 @@ int p3m_set_force_convention(p3m_context *ctx, int convention) {
     if (convention != P3M_CONV_CPU && convention != P3M_CONV_GPU)
       throw std::invalid_argument("P3M: force convention must be P3M_CONV_CPU or P3M_CONV_GPU");
     ctx->force_conv = convention;
+  });
+}
+
+int p3m_set_tuning_real_space(p3m_context *ctx, int mode) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (mode != P3M_TUNE_RS_MODEL && mode != P3M_TUNE_RS_DEVICE)
+      throw std::invalid_argument(
+          "P3M: tuning real space must be P3M_TUNE_RS_MODEL or P3M_TUNE_RS_DEVICE");
+    ctx->tune_real_space = mode;
   });
 }
 

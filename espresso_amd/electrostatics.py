@@ -261,6 +261,17 @@ class _P3MBase:
             self._ctx)
         return force
 
+    def set_tuning_real_space(self, mode):
+        """Real-space cost of timed tuning: "model" (a host pair loop at
+        1 ns per pair, the default) or "device" (``add_short_range_forces``
+        timed for each trial on the activation particles).  Call before
+        ``activate``."""
+        modes = {"model": _lib.P3M_TUNE_RS_MODEL, "device": _lib.P3M_TUNE_RS_DEVICE}
+        if mode not in modes:
+            raise ValueError(f"tuning real space must be one of {sorted(modes)}")
+        _lib.raise_for(_lib.load().p3m_set_tuning_real_space(self._ctx, modes[mode]),
+                       self._ctx)
+
     def force_convention(self):
         """"cpu" (the CPU kernels' mesh) or "gpu" (the reference's GPU
         kernels' index rule) for the forces."""

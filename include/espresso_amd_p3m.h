@@ -55,6 +55,10 @@ extern "C" {
 #define P3M_PREC_FP64 0 /* plain FP64 solver (drop-in for P3M, double) */
 #define P3M_PREC_FP32 1 /* FP32 mesh, FP64 weights/reductions/forces    */
 
+/* real-space cost in timed tuning (p3m_set_tuning_real_space) */
+#define P3M_TUNE_RS_MODEL 0  /* host pair loop modelled at 1 ns per pair     */
+#define P3M_TUNE_RS_DEVICE 1 /* the device pair loop p3m_short_range, timed */
+
 /* force conventions (p3m_set_force_convention) */
 #define P3M_CONV_CPU 0 /* CPU kernels: CA mesh with mesh_off, Nyquist ik 0  */
 #define P3M_CONV_GPU 1 /* GPU kernels: FP32 index, no mesh_off, +M/2 ik     */
@@ -170,6 +174,14 @@ int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff);
  * index of nmp under P3M_CONV_GPU.  Default P3M_CONV_CPU. */
 int p3m_set_force_convention(p3m_context *ctx, int convention);
 int p3m_get_force_convention(const p3m_context *ctx, int *convention);
+
+/* Real-space cost that timed tuning (p3m_activate with tune set) adds to
+ * each timed k-space solve.  The reference times whole integration steps,
+ * real-space loop included (TuningAlgorithm.cpp:116-211 via benchmark_integration_step, tuning.cpp:73);
+ * P3M_TUNE_RS_DEVICE times p3m_short_range on the activation particles for
+ * each trial (r_cut, alpha), for callers whose pair loop runs there;
+ * P3M_TUNE_RS_MODEL (default) models a host loop at 1 ns per pair. */
+int p3m_set_tuning_real_space(p3m_context *ctx, int mode);
 /* Cells per dimension used by the last p3m_short_range call (0 0 0 before
  * the first; 1 1 1 for all pairs). */
 int p3m_short_range_cells(const p3m_context *ctx, int cells[3]);

@@ -81,6 +81,17 @@ def test_force_convention_of_the_actors():
     assert p.force_convention() == "cpu"
 
 
+def test_tuning_real_space_modes():
+    from espresso_amd import _lib
+    p = P3M(device=-1, **VALID)
+    p.set_tuning_real_space("device")
+    p.set_tuning_real_space("model")
+    with pytest.raises(ValueError, match="tuning real space must be one of"):
+        p.set_tuning_real_space("host")
+    with pytest.raises(ValueError, match="P3M_TUNE_RS_MODEL or P3M_TUNE_RS_DEVICE"):
+        _lib.raise_for(_lib.load().p3m_set_tuning_real_space(p._ctx, 7), p._ctx)
+
+
 def test_missing_required_keys():
     with pytest.raises(ValueError, match="prefactor"):
         P3M(device=-1, accuracy=1e-3)
