@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "p3m_last_timings", "p3m_profile_begin", "p3m_profile_end", "p3m_autotune",
     "p3m_short_range", "p3m_short_range_cells", "p3m_set_short_range_subdivision",
     "p3m_set_force_convention", "p3m_get_force_convention", "p3m_set_tuning_real_space",
+    "p3m_elc_params_default", "p3m_set_elc", "p3m_get_elc",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -84,6 +85,13 @@ class p3m_params(C.Structure):
 class p3m_launch_shape(C.Structure):
     _fields_ = [(name, C.c_int) for name in
                 ("gather", "ncol", "zlines", "fused_lds", "spread", "threads", "chunk")]
+
+
+class p3m_elc_params(C.Structure):
+    _fields_ = [("maxPWerror", C.c_double), ("gap_size", C.c_double),
+                ("far_cut", C.c_double), ("neutralize", C.c_int),
+                ("delta_mid_top", C.c_double), ("delta_mid_bot", C.c_double),
+                ("const_pot", C.c_int), ("pot_diff", C.c_double)]
 
 
 _lib = None
@@ -142,6 +150,10 @@ def load():
     L.p3m_set_force_convention.argtypes = [vp, C.c_int]
     L.p3m_get_force_convention.argtypes = [vp, C.POINTER(C.c_int)]
     L.p3m_set_tuning_real_space.argtypes = [vp, C.c_int]
+    ep = C.POINTER(p3m_elc_params)
+    L.p3m_elc_params_default.argtypes = [ep]
+    L.p3m_set_elc.argtypes = [vp, ep]
+    L.p3m_get_elc.argtypes = [vp, ep, C.POINTER(C.c_int)]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []

@@ -6,6 +6,7 @@
 // never cross the ABI; they become status codes plus a message.
 #include "espresso_amd_p3m.h"
 
+#include "p3m_elc.hpp"
 #include "p3m_host.hpp"
 #include "p3m_batch.hpp"
 #include "p3m_serve.hpp"
@@ -17,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -34,6 +36,12 @@ struct p3m_context {
   int short_range_subdivision = 0;
   int force_conv = P3M_CONV_CPU;
   int tune_real_space = P3M_TUNE_RS_MODEL;
+  // ELC on top of this solver (ElectrostaticLayerCorrection wrapping the
+  // P3M actor): parameters as given, far_cut as last used
+  bool elc_on = false;
+  p3m_elc_params elc{};
+  double elc_far_cut = -1.;
+  std::unique_ptr<ElcSolver> elc_solver;
   p3m_params user{};
   P3MParams params{};
   bool have_params = false;
@@ -204,6 +212,27 @@ MeshPrecision prec_of(p3m_params const &u) {
 
 /** init(): derive mesh constants, sanity checks, build device state
  *  (init_cpu_kernels / p3m_gpu_init, p3m.cpp:245-280). */
+/** ELC parameters for the current box: box_h = L_z - gap
+ *  (ElectrostaticLayerCorrection::recalc_box_h, elc.cpp:1020-1030) and
+ *  far_cut, tuned from maxPWerror unless given (recalc_far_cut,
+ *  elc.hpp:319-324). */
+ElcParams elc_geometry(p3m_context *ctx) {
+  ElcParams ep;
+  for (int d = 0; d < 3; ++d) ep.box[d] = ctx->user.box_l[d];
+  ep.prefactor = ctx->user.prefactor;
+  ep.gap_size = ctx->elc.gap_size;
+  ep.neutralize = ctx->elc.neutralize != 0;
+  if (ep.box[2] - ep.gap_size < 0.)
+    throw std::runtime_error("ELC gap size (" + std::to_string(ep.gap_size) +
+                             ") larger than box length in z-direction (" +
+                             std::to_string(ep.box[2]) + ")");
+  ep.far_cut = (ctx->elc.far_cut == -1.)
+                   ? elc_tune_far_cut(ep.box, ep.gap_size, ctx->elc.maxPWerror)
+                   : ctx->elc.far_cut;
+  ctx->elc_far_cut = ep.far_cut;
+  return ep;
+}
+
 void init_solver(p3m_context *ctx) {
   auto &p = ctx->params;
   auto const box = v3(ctx->user.box_l);
@@ -283,6 +312,10 @@ int p3m_set_params(p3m_context *ctx, p3m_params const *p) {
     params.tuning = false;
     ctx->params = params;
     ctx->user = *p;
+    if (ctx->elc_on) { // adapt_epsilon_elc (p3m.impl.hpp:112-114)
+      ctx->params.epsilon = P3M_EPSILON_METALLIC;
+      ctx->user.epsilon = P3M_EPSILON_METALLIC;
+    }
     ctx->have_params = true;
     ctx->initialized = false;
   });
@@ -367,6 +400,7 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
     }
     auto const charges = host_charges(ctx, n, q, flags);
     check_charge_neutrality(charges, ctx->user.charge_neutrality_tolerance);
+    if (ctx->elc_on) (void)elc_geometry(ctx); // gap and far_cut (elc.hpp:184-200)
     // tune (p3m.cpp:757-791)
     auto &p = ctx->params;
     if (p.alpha_L == 0. and p.alpha != 0.) p.alpha_L = p.alpha * box[0];
@@ -461,11 +495,28 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
       ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
                          &r, !async);
     }
+    double e_elc = 0.;
+    if (ctx->elc_on) {
+      // ElectrostaticLayerCorrection::add_long_range_forces / long_range_energy
+      // (elc.cpp:1189-1256): the P3M solve above plus the far-field terms
+      ElcParams const ep = elc_geometry(ctx);
+      if (!ctx->elc_solver) ctx->elc_solver = std::make_unique<ElcSolver>(ctx->device);
+      bool const want_e = (flags & P3M_ENERGY) && !async;
+      ctx->elc_solver->compute(ctx->solver->stream(), ep, n, d.pos, d.q,
+                               (flags & P3M_FORCES) ? d.force : nullptr, want_e ? &e_elc : nullptr,
+                               !async);
+    }
     if (async) {
       if (energy) *energy = 0.;
       if (virial)
         for (int k = 0; k < 9; ++k) virial[k] = 0.;
       return;
+    }
+    if (ctx->elc_on) {
+      r.energy += e_elc;
+      // the reference has no ELC pressure: it warns and returns zero
+      // (coulomb.cpp:125-131)
+      for (int k = 0; k < 9; ++k) r.virial[k] = 0.;
     }
     if ((flags & P3M_FORCES) && !(flags & P3M_DEVICE_POINTERS) && n > 0) {
       hip_check(hipMemcpy(force, d.force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
@@ -591,6 +642,73 @@ int p3m_set_force_convention(p3m_context *ctx, int convention) {
       throw std::invalid_argument("P3M: force convention must be P3M_CONV_CPU or P3M_CONV_GPU");
     ctx->force_conv = convention;
   });
+}
+
+int p3m_elc_params_default(p3m_elc_params *p) {
+  if (!p) return P3M_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->maxPWerror = -1.;
+  p->gap_size = -1.;
+  p->far_cut = -1.;
+  p->neutralize = 1;
+  return P3M_OK;
+}
+
+int p3m_set_elc(p3m_context *ctx, p3m_elc_params const *p) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    if (!p) {
+      ctx->elc_on = false;
+      return;
+    }
+    // elc_data constructor checks, in order (elc.cpp:1056-1102)
+    double const round_error_prec = 1e-14;
+    bool const far_calculated = (p->far_cut == -1.);
+    bool const dielectric = (p->delta_mid_top != 0. || p->delta_mid_bot != 0.);
+    bool const const_pot = p->const_pot != 0 && dielectric;
+    double const delta_range = 1. + std::sqrt(round_error_prec);
+    if (p->far_cut <= 0. && !far_calculated)
+      throw std::domain_error("Parameter 'far_cut' must be > 0");
+    if (p->maxPWerror <= 0.) throw std::domain_error("Parameter 'maxPWerror' must be > 0");
+    if (p->gap_size <= 0.) throw std::domain_error("Parameter 'gap_size' must be > 0");
+    if (p->pot_diff != 0. && !p->const_pot)
+      throw std::invalid_argument(
+          "Parameter 'const_pot' must be True when 'pot_diff' is non-zero");
+    if (p->delta_mid_top < -delta_range || p->delta_mid_top > delta_range)
+      throw std::domain_error("Parameter 'delta_mid_top' must be >= -1 and <= +1");
+    if (p->delta_mid_bot < -delta_range || p->delta_mid_bot > delta_range)
+      throw std::domain_error("Parameter 'delta_mid_bot' must be >= -1 and <= +1");
+    double const dt = std::clamp(p->delta_mid_top, -1., 1.);
+    double const db = std::clamp(p->delta_mid_bot, -1., 1.);
+    if (dielectric && !const_pot && std::fabs(1. - dt * db) < round_error_prec)
+      throw std::domain_error("ELC with two parallel metallic boundaries requires the "
+                              "const_pot option");
+    if (dielectric)
+      throw std::runtime_error("ELC: dielectric contrasts (delta_mid_top, delta_mid_bot) are "
+                               "not supported by this solver");
+    ctx->elc = *p;
+    ctx->elc.delta_mid_top = dt;
+    ctx->elc.delta_mid_bot = db;
+    ctx->elc.const_pot = const_pot ? 1 : 0;
+    ctx->elc.neutralize = (p->neutralize != 0 && !dielectric) ? 1 : 0;
+    ctx->elc.pot_diff = p->const_pot ? p->pot_diff : 0.;
+    ctx->elc_far_cut = far_calculated ? -1. : p->far_cut;
+    ctx->elc_on = true;
+    // adapt_epsilon_elc (p3m.impl.hpp:112-114): P3M under ELC is metallic
+    if (ctx->params.epsilon != P3M_EPSILON_METALLIC) {
+      ctx->params.epsilon = P3M_EPSILON_METALLIC;
+      ctx->user.epsilon = P3M_EPSILON_METALLIC;
+      ctx->initialized = false;
+    }
+  });
+}
+
+int p3m_get_elc(p3m_context const *ctx, p3m_elc_params *out, int *enabled) {
+  if (!ctx || !out || !enabled) return P3M_ERR_INVALID;
+  *enabled = ctx->elc_on ? 1 : 0;
+  *out = ctx->elc;
+  if (ctx->elc_on) out->far_cut = ctx->elc_far_cut;
+  return P3M_OK;
 }
 
 int p3m_set_tuning_real_space(p3m_context *ctx, int mode) {

@@ -533,3 +533,124 @@ class P3MGPU(_P3MBase):
 
     def default_params(self):
         return {"single_precision": True, **super().default_params()}
+
+
+class ELC:
+    """Electrostatic layer correction on top of a P3M or P3MGPU actor, for
+    systems with two periodic dimensions (``espressomd.electrostatics.ELC``,
+    electrostatics.py class ELC; core ElectrostaticLayerCorrection,
+    elc.cpp).  Charges stay in 0 <= z <= L_z - gap_size.  Dielectric
+    contrasts are not supported by this solver.
+
+    The forces, energy and virial calls of the wrapped actor include the
+    ELC terms once ELC is constructed on it."""
+
+    _KEYS = ("actor", "maxPWerror", "gap_size", "far_cut", "delta_mid_top",
+             "delta_mid_bot", "const_pot", "pot_diff", "neutralize",
+             "check_neutrality")
+
+    def required_keys(self):
+        return {"actor", "maxPWerror", "gap_size"}
+
+    def default_params(self):
+        return {"far_cut": -1.,
+                "delta_mid_top": 0.,
+                "delta_mid_bot": 0.,
+                "const_pot": False,
+                "pot_diff": 0.,
+                "neutralize": True,
+                "check_neutrality": True}
+
+    def __init__(self, **kwargs):
+        unknown = set(kwargs) - set(self._KEYS)
+        if unknown:
+            raise ValueError(
+                f"Only the following keys can be given as keyword arguments: "
+                f"{sorted(self._KEYS)}, got {sorted(kwargs)} (unknown {sorted(unknown)})")
+        missing = self.required_keys() - set(kwargs)
+        if missing:
+            raise ValueError(
+                f"At least the following keys have to be given as keyword "
+                f"arguments: {sorted(self.required_keys())}, got {sorted(kwargs)} "
+                f"(missing {sorted(missing)})")
+        params = self.default_params()
+        params.update(kwargs)
+        self.validate_params(params)
+        actor = params["actor"]
+        if not isinstance(actor, _P3MBase):
+            raise ValueError(f"Parameter 'actor' of type {type(actor).__name__} "
+                             f"isn't supported by ELC")
+        self._params = params
+        self.actor = actor
+        L = _lib.load()
+        ep = _lib.p3m_elc_params()
+        L.p3m_elc_params_default(C.byref(ep))
+        ep.maxPWerror = float(params["maxPWerror"])
+        ep.gap_size = float(params["gap_size"])
+        ep.far_cut = float(params["far_cut"])
+        ep.neutralize = 1 if params["neutralize"] else 0
+        ep.delta_mid_top = float(params["delta_mid_top"])
+        ep.delta_mid_bot = float(params["delta_mid_bot"])
+        ep.const_pot = 1 if params["const_pot"] else 0
+        ep.pot_diff = float(params["pot_diff"])
+        _lib.raise_for(L.p3m_set_elc(actor._ctx, C.byref(ep)), actor._ctx)
+        if not params["check_neutrality"]:
+            actor._params["check_neutrality"] = False
+            if actor._activated:
+                actor._build_cparams(box_l=actor._box_l, skin=actor._cp.skin)
+                _lib.raise_for(L.p3m_set_params(actor._ctx, C.byref(actor._cp)),
+                               actor._ctx)
+
+    @staticmethod
+    def validate_params(params):
+        """electrostatics.py ELC.validate_params (type checks with the
+        reference's messages)."""
+        def is_float(v):
+            return _is_float(v) or _is_int(v)
+        if not is_float(params["maxPWerror"]):
+            raise ValueError("maxPWerror has to be a float -- Got an "
+                             f"{type(params['maxPWerror']).__name__}")
+        if not is_float(params["gap_size"]):
+            raise ValueError("gap_size has to be a float -- Got an "
+                             f"{type(params['gap_size']).__name__}")
+        if not is_float(params["far_cut"]):
+            raise ValueError("far_cut has to be a float -- Got an "
+                             f"{type(params['far_cut']).__name__}")
+        if not isinstance(params["neutralize"], (bool, np.bool_)):
+            raise ValueError("neutralize has to be a bool -- Got an "
+                             f"{type(params['neutralize']).__name__}")
+
+    def get_params(self):
+        """ELC parameters as the script interface reports them, far_cut as
+        tuned by the last activation or solve."""
+        ep = _lib.p3m_elc_params()
+        on = C.c_int(0)
+        _lib.raise_for(_lib.load().p3m_get_elc(self.actor._ctx, C.byref(ep), C.byref(on)),
+                       self.actor._ctx)
+        return {"actor": self.actor, "maxPWerror": ep.maxPWerror, "gap_size": ep.gap_size,
+                "far_cut": ep.far_cut, "neutralize": bool(ep.neutralize),
+                "delta_mid_top": ep.delta_mid_top, "delta_mid_bot": ep.delta_mid_bot,
+                "const_pot": bool(ep.const_pot), "pot_diff": ep.pot_diff,
+                "check_neutrality": bool(self._params["check_neutrality"])}
+
+    def activate(self, pos, q, box_l, skin=0.4):
+        """ElectrostaticLayerCorrection::on_activation (elc.hpp:181-200):
+        gap and far_cut checks, then the P3M actor's activation."""
+        self.actor.activate(pos, q, box_l=box_l, skin=skin)
+        return self
+
+    def add_long_range_forces(self, pos, q, force, pos_unfolded=None, sync=True):
+        return self.actor.add_long_range_forces(pos, q, force, pos_unfolded, sync)
+
+    def long_range_energy(self, pos, q, pos_unfolded=None):
+        return self.actor.long_range_energy(pos, q, pos_unfolded)
+
+    def solve(self, pos, q, force=None, energy=True, pressure=False, pos_unfolded=None):
+        return self.actor.solve(pos, q, force, energy, pressure, pos_unfolded)
+
+    def add_short_range_forces(self, *args, **kwargs):
+        return self.actor.add_short_range_forces(*args, **kwargs)
+
+    def remove(self):
+        """Take ELC off the actor (its epsilon stays metallic)."""
+        _lib.raise_for(_lib.load().p3m_set_elc(self.actor._ctx, None), self.actor._ctx)

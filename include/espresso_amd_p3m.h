@@ -175,6 +175,42 @@ int p3m_set_short_range_subdivision(p3m_context *ctx, int cells_per_cutoff);
 int p3m_set_force_convention(p3m_context *ctx, int convention);
 int p3m_get_force_convention(const p3m_context *ctx, int *convention);
 
+/* ELC: electrostatic layer correction for slab systems (charges in
+ * 0 <= z <= L_z - gap_size, periodic in x and y).  Field meaning, defaults
+ * and checks follow elc_data (src/core/electrostatics/elc.hpp:62-118,
+ * elc.cpp:1056-1102) and the Python actor
+ * (src/python/espressomd/electrostatics.py, class ELC). */
+typedef struct {
+  double maxPWerror;    /* maximal pairwise error (required, > 0)          */
+  double gap_size;      /* empty layer below L_z (required, > 0)           */
+  double far_cut;       /* cutoff of the far-field sums, -1 = tune         */
+  int neutralize;       /* keep P3M's neutralising background (default 1)  */
+  double delta_mid_top; /* dielectric contrasts: 0 (only 0 is supported)   */
+  double delta_mid_bot;
+  int const_pot;        /* constant potential (with dielectric contrasts)  */
+  double pot_diff;      /* applied potential difference (needs const_pot) */
+} p3m_elc_params;
+
+/* Defaults of the Python actor: far_cut -1, neutralize 1, no contrasts;
+ * maxPWerror and gap_size -1 (must be set). */
+int p3m_elc_params_default(p3m_elc_params *p);
+
+/* Put ELC on top of the context's P3M solver, or remove it (p == NULL).
+ * Replaces the ElectrostaticLayerCorrection constructor with
+ * adapt_epsilon_elc (elc.cpp:1104-1108, 1010-1018; p3m.impl.hpp:112-114):
+ * the P3M epsilon becomes metallic.  From then on p3m_solve adds the ELC
+ * far-field forces and energy of ElectrostaticLayerCorrection::
+ * add_long_range_forces / long_range_energy (elc.cpp:1189-1256,
+ * 830-940); P3M_PRESSURE returns a zero virial, as the reference has no
+ * ELC pressure (coulomb.cpp:125-131).  A charged particle in the gap is
+ * reported by the solve (check_gap, elc.cpp:184-195).  Parameter errors
+ * carry the reference's messages; dielectric contrasts are rejected. */
+int p3m_set_elc(p3m_context *ctx, const p3m_elc_params *p);
+/* Current ELC parameters (far_cut as tuned by the last activation or
+ * solve) and whether ELC is on; read-only getters of the script interface
+ * (src/script_interface/electrostatics/ElectrostaticLayerCorrection.hpp:60-85). */
+int p3m_get_elc(const p3m_context *ctx, p3m_elc_params *out, int *enabled);
+
 /* Real-space cost that timed tuning (p3m_activate with tune set) adds to
  * each timed k-space solve.  The reference times whole integration steps,
  * real-space loop included (TuningAlgorithm.cpp:116-211 via benchmark_integration_step, tuning.cpp:73);

@@ -320,7 +320,7 @@ def gpu_kspace_forces(pos, q, box_l, mesh, cao, alpha, prefactor):
     G[zero] = 0.
     phi = rhat * G
     N = float(np.prod(M))
-    E = [np.fft.irfftn(2j * np.pi * nd / L[d] * phi, s=tuple(M)) * N
+    E = [np.fft.irfftn(2j * np.pi * nd / L[d] * phi, s=tuple(M), axes=(0, 1, 2)) * N
          for d, nd in enumerate((nx, ny, nz))]
     pref = prefactor / (2. * float(np.prod(L.astype(np.float32).astype(np.float64))))
     f = np.zeros((len(q), 3))

@@ -202,3 +202,41 @@ def test_grouped_k_space_estimate_of_the_tuner():
                                    float(n), acc, [L] * 3)
         assert abs(ref[0] - t["accuracy"]) <= 1e-6 * ref[0]
         assert ref[0] <= acc * (1. + 1e-6)
+
+
+def test_elc_parameters_and_messages():
+    """ELC construction checks with the reference's messages (elc_data,
+    elc.cpp:1056-1102; electrostatics.py ELC.validate_params) and the
+    metallic epsilon it imposes on the P3M actor (p3m.impl.hpp:112-114)."""
+    from espresso_amd.electrostatics import ELC
+    p = P3M(device=-1, epsilon=2.0, **VALID)
+    elc = ELC(actor=p, maxPWerror=1e-6, gap_size=3.)
+    g = elc.get_params()
+    assert g["far_cut"] == -1. and g["neutralize"] is True and g["gap_size"] == 3.
+    assert p.get_params()["epsilon"] == 0.
+    cases = [
+        (dict(maxPWerror=-1., gap_size=3.), ValueError, "'maxPWerror' must be > 0"),
+        (dict(maxPWerror=1e-6, gap_size=0.), ValueError, "'gap_size' must be > 0"),
+        (dict(maxPWerror=1e-6, gap_size=3., far_cut=-2.), ValueError, "'far_cut' must be > 0"),
+        (dict(maxPWerror=1e-6, gap_size=3., pot_diff=1.), ValueError,
+         "'const_pot' must be True when 'pot_diff' is non-zero"),
+        (dict(maxPWerror=1e-6, gap_size=3., delta_mid_top=2.), ValueError,
+         "'delta_mid_top' must be >= -1 and <= \\+1"),
+        (dict(maxPWerror=1e-6, gap_size=3., delta_mid_bot=-1.5), ValueError,
+         "'delta_mid_bot' must be >= -1 and <= \\+1"),
+        (dict(maxPWerror=1e-6, gap_size=3., delta_mid_top=1., delta_mid_bot=1.), ValueError,
+         "two parallel metallic boundaries requires the const_pot option"),
+        (dict(maxPWerror=1e-6, gap_size=3., delta_mid_bot=0.5), RuntimeError,
+         "dielectric contrasts"),
+        (dict(maxPWerror="x", gap_size=3.), ValueError, "maxPWerror has to be a float"),
+        (dict(maxPWerror=1e-6, gap_size=3., neutralize=1), ValueError,
+         "neutralize has to be a bool"),
+    ]
+    for kw, exc, msg in cases:
+        with pytest.raises(exc, match=msg):
+            ELC(actor=P3M(device=-1, **VALID), **kw)
+    with pytest.raises(ValueError, match="isn't supported by ELC"):
+        ELC(actor=object(), maxPWerror=1e-6, gap_size=3.)
+    with pytest.raises(ValueError, match="missing"):
+        ELC(actor=p, maxPWerror=1e-6)
+    elc.remove()
