@@ -380,6 +380,61 @@ def timed_loop(p3m, tp, tq, tf, steps, warmup, torch):
     return (t1 - t0) / steps, {k: v / max(nprof, 1) for k, v in stages.items()}
 
 
+def bench_elc(steps, warmup, torch, dev, n=100_000, L=60., gap=10., seed=7001):
+    """Slab workload: n charges (+-1 alternating) in an L x L x (L + gap) box
+    with the top gap empty; P3M (FP64, timed tuning at 1e-4) with and
+    without ELC (maxPWerror 1e-4).  The ELC share is the difference of the
+    two force solves."""
+    from espresso_amd.electrostatics import ELC, P3M
+    rng = np.random.default_rng(seed)
+    box = [L, L, L + gap]
+    pos = rng.uniform(0., 1., (n, 3)) * np.array([L, L, L])
+    q = np.where(np.arange(n) % 2 == 0, 1., -1.)
+    tp = torch.tensor(pos, device=dev)
+    tq = torch.tensor(q, device=dev)
+    tf = torch.zeros_like(tp)
+    plain = P3M(prefactor=1.0, accuracy=1e-4, timings=3, verbose=False)
+    plain.activate(tp, tq, box_l=box)
+    prm = plain.get_params()
+    fixed = dict(prefactor=1.0, accuracy=1e-4, mesh=prm["mesh"], cao=prm["cao"],
+                 r_cut=prm["r_cut"], alpha=prm["alpha"], tune=False)
+    elc = ELC(actor=P3M(**fixed), maxPWerror=1e-4, gap_size=gap)
+    elc.activate(tp, tq, box_l=box)
+    t_p3m, _ = timed_loop(plain, tp, tq, tf, steps, warmup, torch)
+    t_all, _ = timed_loop(elc.actor, tp, tq, tf, steps, warmup, torch)
+    from espresso_amd import _lib
+    import ctypes as C
+    ep = _lib.p3m_elc_params()
+    on = C.c_int(0)
+    _lib.load().p3m_get_elc(elc.actor._ctx, C.byref(ep), C.byref(on))
+    fc = ep.far_cut
+    nx, ny = int(np.ceil(fc * L) + 1), int(np.ceil(fc * L) + 1)
+    modes = 0
+    for p_ in range(1, nx + 1):
+        if (p_ - 1) / L >= fc:
+            break
+        modes += 1
+    modes *= 2
+    for p_ in range(1, nx + 1):
+        if (p_ - 1) / L >= fc:
+            break
+        for q_ in range(1, ny + 1):
+            if ((p_ - 1) / L) ** 2 + ((q_ - 1) / L) ** 2 >= fc * fc:
+                break
+            modes += 1
+    t_elc = max(t_all - t_p3m, 1e-9)
+    rec = {"workload": f"slab N={n}, box {L}x{L}x{L + gap} (gap {gap}), +-1 "
+                       f"alternating, P3M FP64 accuracy 1e-4 (timed tuning), "
+                       f"ELC maxPWerror 1e-4",
+           "mesh": prm["mesh"], "cao": prm["cao"], "r_cut": round(prm["r_cut"], 4),
+           "far_cut": round(fc, 5), "modes": modes,
+           "p3m_ms": round(t_p3m * 1e3, 4), "p3m_elc_ms": round(t_all * 1e3, 4),
+           "elc_ms": round(t_elc * 1e3, 4),
+           "elc_particle_modes_per_s": round(n * modes / t_elc, 1)}
+    print(f"[bench] elc: {rec}", flush=True)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +449,8 @@ def main():
                     help="skip the 2048-system serving workload")
     ap.add_argument("--no-short-range", action="store_true",
                     help="skip the device real-space (short-range) timing")
+    ap.add_argument("--no-elc", action="store_true",
+                    help="skip the ELC slab workload")
     ap.add_argument("--accuracy-table", action="store_true",
                     help="also run the 27-cell precision sweep (N=250,000) "
                          "against exact Ewald forces")
@@ -545,6 +602,10 @@ def main():
             "total_force_mparticles_per_s_fp64": round(n / (t_sr + t_step) / 1e6, 2)}
         print(f"[bench] short range: {short_range}", flush=True)
 
+    elc = None
+    if not args.no_elc:
+        elc = bench_elc(args.steps, args.warmup, torch, dev)
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev)
@@ -602,6 +663,7 @@ def main():
         "fp32_mesh": fp32,
         "accuracy": accuracy,
         "short_range": short_range,
+        "elc": elc,
         "batched": batched,
         "serving": serving,
     }
