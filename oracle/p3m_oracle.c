@@ -742,11 +742,13 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
   int const *M = s->mesh;
   size_t const M3 = (size_t)M[0] * M[1] * M[2];
   p3mo_count_charged(s, n, q);
-  if (s->sum_q2 > 0.) {
-    if (charge_assign(s, n, pos, q) != 0) return NAN;
-    halo_gather(s, s->rs_scalar);
-    forward_fft(s);
-  }
+  /* no charges: an empty mesh (the reference clears it in charge_assign /
+   * prepare_fft_mesh before the solve, p3m.impl.hpp:161-168), so forces and
+   * energy are zero */
+  if (!(s->sum_q2 > 0.)) return 0.;
+  if (charge_assign(s, n, pos, q) != 0) return NAN;
+  halo_gather(s, s->rs_scalar);
+  forward_fft(s);
   double const volume = s->box_l[0] * s->box_l[1] * s->box_l[2];
   int const has_dipole = (s->epsilon != 0.);
   double box_dipole[3] = {0., 0., 0.};

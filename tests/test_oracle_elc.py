@@ -5,7 +5,10 @@
   at the reference test's tolerances (1e-5 forces, 1e-4 energy);
 - the ELC forces are minus the gradient of the ELC energy (central
   differences), with and without neutralisation, for a charged system;
-- tune_far_cut returns the smallest grid value meeting maxPWerror.
+- tune_far_cut returns the smallest grid value meeting maxPWerror;
+- with a dielectric contrast (image charges in the mesh and pair terms
+  inside the space layer, the image series of the far field outside it),
+  the reference's analytic image-charge test (elc_vs_analytic.py).
 """
 import math
 import os
@@ -101,3 +104,35 @@ def test_gap_particles_are_rejected():
     pos[3, 2] = 6.5
     with pytest.raises(RuntimeError, match="gap"):
         E.elc_forces_energy(pos, q, [5., 5., 8.], 1., 2., 1.)
+
+
+@pytest.mark.parametrize("charge,zpos", [(-5., 0.1), (2.5, 3.3), (5., 17.), (-2.5, 120.)])
+def test_dielectric_contrast_image_charge_analytic(charge, zpos):
+    """Two opposite charges 1 apart above a dielectric boundary
+    (delta_mid_bot = 39/41) in a 200 x 200 x 275 box with a 75 gap: the
+    reference's analytic image-charge solution (elc_vs_analytic.py:35-76)
+    for the force on the lower charge and the total energy, at its
+    tolerances (atol 1e-4, rtol 1e-7)."""
+    from oracle import elc as E
+    from oracle import oracle as O
+    L, gap, dist = 200., 75., 1.
+    box = [L, L, L + gap]
+    db = 39. / 41.
+    pref = 2.0
+    pos = np.array([[L / 2, L / 2, zpos], [L / 2, L / 2, zpos + dist]])
+    q = np.array([charge, -charge])
+    mesh, cao = [58, 58, 70], 4
+    t = O.tune_r_cut(pref, mesh, cao, 2, float(np.sum(q * q)), 1e-7, box)
+    r_cut, alpha = t[0] * box[0], t[1] / box[0]
+    p3m = O.OracleP3M(box, mesh, cao, alpha, r_cut, pref)
+    sl = E.space_layer(gap, L, r_cut)
+    fc = E.tune_far_cut(box, gap, 1e-7, space_layer=sl, dielectric=True)
+    f, e = E.elc_p3m_total(pos, q, box, p3m, pref, alpha, r_cut, gap, fc, delta_bot=db,
+                           real_space=lambda p, qq: O.real_space(p, qq, box, pref, alpha, r_cut))
+    c2 = pref * charge * charge
+    f_ref = c2 * (1. / dist ** 2 + db * (1. / (2 * zpos) ** 2 - 1. / (2 * zpos + dist) ** 2))
+    e_ref = c2 * (-1. / dist + db * (1. / (4 * zpos) - 1. / (2 * zpos + dist) +
+                                     1. / (4 * (zpos + dist))))
+    print(f"q {charge} z {zpos}: force {f[0, 2]:.8f} vs {f_ref:.8f}, energy {e:.8f} vs {e_ref:.8f}")
+    np.testing.assert_allclose(f[0, 2], f_ref, atol=1e-4, rtol=1e-7)
+    assert abs(e - e_ref) <= 1e-4
