@@ -213,7 +213,9 @@ MeshPrecision prec_of(p3m_params const &u) {
 /** init(): derive mesh constants, sanity checks, build device state
  *  (init_cpu_kernels / p3m_gpu_init, p3m.cpp:245-280). */
 /** ELC parameters for the current box: box_h = L_z - gap
- *  (ElectrostaticLayerCorrection::recalc_box_h, elc.cpp:1020-1030) and
+ *  (ElectrostaticLayerCorrection::recalc_box_h, elc.cpp:1020-1030), the
+ *  space layer of dielectric contrasts (recalc_space_layer, :1032-1054;
+ *  gap / 3 before P3M is tuned, as the elc_data constructor sets it) and
  *  far_cut, tuned from maxPWerror unless given (recalc_far_cut,
  *  elc.hpp:319-324). */
 ElcParams elc_geometry(p3m_context *ctx) {
@@ -222,12 +224,23 @@ ElcParams elc_geometry(p3m_context *ctx) {
   ep.prefactor = ctx->user.prefactor;
   ep.gap_size = ctx->elc.gap_size;
   ep.neutralize = ctx->elc.neutralize != 0;
-  if (ep.box[2] - ep.gap_size < 0.)
+  ep.delta_top = ctx->elc.delta_mid_top;
+  ep.delta_bot = ctx->elc.delta_mid_bot;
+  ep.const_pot = ctx->elc.const_pot != 0;
+  ep.pot_diff = ctx->elc.pot_diff;
+  ep.alpha = ctx->params.alpha;
+  ep.r_cut = ctx->params.r_cut;
+  double const box_h = ep.box[2] - ep.gap_size;
+  if (box_h < 0.)
     throw std::runtime_error("ELC gap size (" + std::to_string(ep.gap_size) +
                              ") larger than box length in z-direction (" +
                              std::to_string(ep.box[2]) + ")");
+  if (ep.dielectric())
+    ep.space_layer = ctx->is_tuned ? elc_space_layer(ep.gap_size, box_h, ctx->params.r_cut)
+                                   : ep.gap_size / 3.;
   ep.far_cut = (ctx->elc.far_cut == -1.)
-                   ? elc_tune_far_cut(ep.box, ep.gap_size, ctx->elc.maxPWerror)
+                   ? elc_tune_far_cut(ep.box, ep.gap_size, ctx->elc.maxPWerror,
+                                      ep.dielectric(), ep.space_layer)
                    : ctx->elc.far_cut;
   ctx->elc_far_cut = ep.far_cut;
   return ep;
@@ -400,7 +413,23 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
     }
     auto const charges = host_charges(ctx, n, q, flags);
     check_charge_neutrality(charges, ctx->user.charge_neutrality_tolerance);
-    if (ctx->elc_on) (void)elc_geometry(ctx); // gap and far_cut (elc.hpp:184-200)
+    bool const elc_diel =
+        ctx->elc_on && (ctx->elc.delta_mid_top != 0. || ctx->elc.delta_mid_bot != 0.);
+    if (ctx->elc_on) {
+      (void)elc_geometry(ctx); // gap and far_cut (elc.hpp:181-200)
+      if (elc_diel) {
+        // sanity_checks_dielectric_contrasts (elc.cpp:990-1008)
+        double total = 0.;
+        for (double const qq : charges) total += qq;
+        if (std::fabs(total) >= std::sqrt(1e-14)) {
+          if (ctx->elc.const_pot)
+            throw std::runtime_error("ELC does not currently support non-neutral systems with a "
+                                     "dielectric contrast.");
+          throw std::runtime_error("ELC does not work for non-neutral systems and non-metallic "
+                                   "dielectric contrast.");
+        }
+      }
+    }
     // tune (p3m.cpp:757-791)
     auto &p = ctx->params;
     if (p.alpha_L == 0. and p.alpha != 0.) p.alpha_L = p.alpha * box[0];
@@ -426,6 +455,9 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
       double *scratch_force = ctx->d_force; // forces are computed but discarded
       TuningTimer timer = [&](Vec3i const &mesh, int cao, double r_cut_iL, double alpha_L,
                               int timings) -> double {
+        // ELC with dielectric contrasts vetoes r_cut >= gap_size
+        // (veto_r_cut, elc.hpp:243-248)
+        if (elc_diel && r_cut_iL * box[0] >= ctx->elc.gap_size) return -1.;
         P3MParams trial = p;
         trial.mesh = mesh;
         trial.cao = cao;
@@ -459,6 +491,7 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
       ctx->is_tuned = true;
     }
     init_solver(ctx);
+    if (ctx->elc_on) (void)elc_geometry(ctx); // space layer from the tuned r_cut
   });
 }
 
@@ -480,29 +513,68 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
       throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
     DevArgs const d = stage_inputs(ctx, n, pos, q, pos_unfolded, force, flags);
     SolveResult r;
-    unsigned const reductions = flags & (P3M_ENERGY | P3M_PRESSURE);
-    if (ctx->force_conv == P3M_CONV_GPU && (flags & P3M_FORCES)) {
-      // forces with the GPU kernels' index rule; energy and virial, as in
-      // the reference's GPU actor, from the CPU kernels' mesh (p3m.cpp:867-892)
-      ctx->solver->set_convention(1);
-      ctx->solver->solve(n, d.pos, d.q, d.upos, d.force, flags & ~reductions, nullptr, !async);
-      if (reductions && !async) {
+    // one P3M pass over (nn, pp, qq): forces with the context's convention
+    // (the GPU kernels' index rule makes a second pass for energy and
+    // virial from the CPU kernels' mesh, p3m.cpp:867-892)
+    auto p3m_pass = [&](int64_t nn, double const *pp, double const *qq, double const *up,
+                        double *ff, unsigned fl, SolveResult *res) {
+      unsigned const reductions = fl & (P3M_ENERGY | P3M_PRESSURE);
+      if (ctx->force_conv == P3M_CONV_GPU && (fl & P3M_FORCES)) {
+        ctx->solver->set_convention(1);
+        ctx->solver->solve(nn, pp, qq, up, ff, fl & ~reductions, nullptr, !async);
+        if (reductions && !async) {
+          ctx->solver->set_convention(0);
+          ctx->solver->solve(nn, pp, qq, up, nullptr, fl & ~P3M_FORCES, res, true);
+        }
+      } else {
         ctx->solver->set_convention(0);
-        ctx->solver->solve(n, d.pos, d.q, d.upos, nullptr, flags & ~P3M_FORCES, &r, true);
+        ctx->solver->solve(nn, pp, qq, up, (fl & P3M_FORCES) ? ff : nullptr, fl, res, !async);
       }
+    };
+    bool const diel =
+        ctx->elc_on && (ctx->elc.delta_mid_top != 0. || ctx->elc.delta_mid_bot != 0.);
+    ElcParams ep;
+    if (ctx->elc_on) {
+      ep = elc_geometry(ctx);
+      if (!ctx->elc_solver) ctx->elc_solver = std::make_unique<ElcSolver>(ctx->device);
+    }
+    hipStream_t const st = ctx->solver->stream();
+    if (!diel) {
+      p3m_pass(n, d.pos, d.q, d.upos, d.force, flags, &r);
     } else {
-      ctx->solver->set_convention(0);
-      ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
-                         &r, !async);
+      // dielectric contrasts (ElectrostaticLayerCorrection::
+      // add_long_range_forces / long_range_energy, elc.cpp:1189-1256): the
+      // mesh holds the real and the image charges; the forces are gathered
+      // at the real particles, plus each particle's pair term with its own
+      // image; the energy is 0.5 E(real) + 0.5 E_self + 0.5 E(real + images)
+      // - 0.5 E(images)
+      ElcSolver &es = *ctx->elc_solver;
+      double *pos3 = nullptr, *q_both = nullptr, *q_img = nullptr;
+      es.images(st, ep, n, d.pos, d.q, &pos3, &q_both, &q_img);
+      if (flags & P3M_FORCES) {
+        double *f3 = es.image_forces(st, n);
+        p3m_pass(3 * n, pos3, q_both, nullptr, f3, flags & ~(P3M_ENERGY | P3M_PRESSURE),
+                 nullptr);
+        es.add_real_rows(st, n, f3, d.force);
+        es.layer_self(st, ep, n, d.pos, d.q, d.force, nullptr);
+      }
+      if ((flags & P3M_ENERGY) && !async) {
+        unsigned const fe = (flags & ~(P3M_FORCES | P3M_PRESSURE | P3M_ASYNC)) | P3M_ENERGY;
+        SolveResult r_real, r_both, r_img;
+        p3m_pass(n, d.pos, d.q, d.upos, nullptr, fe, &r_real);
+        double e_self = 0.;
+        es.layer_self(st, ep, n, d.pos, d.q, nullptr, &e_self);
+        p3m_pass(3 * n, pos3, q_both, nullptr, nullptr, fe, &r_both);
+        p3m_pass(3 * n, pos3, q_img, nullptr, nullptr, fe, &r_img);
+        r.energy = 0.5 * r_real.energy + 0.5 * e_self + 0.5 * r_both.energy - 0.5 * r_img.energy;
+      }
     }
     double e_elc = 0.;
     if (ctx->elc_on) {
       // ElectrostaticLayerCorrection::add_long_range_forces / long_range_energy
       // (elc.cpp:1189-1256): the P3M solve above plus the far-field terms
-      ElcParams const ep = elc_geometry(ctx);
-      if (!ctx->elc_solver) ctx->elc_solver = std::make_unique<ElcSolver>(ctx->device);
       bool const want_e = (flags & P3M_ENERGY) && !async;
-      ctx->elc_solver->compute(ctx->solver->stream(), ep, n, d.pos, d.q,
+      ctx->elc_solver->compute(st, ep, n, d.pos, d.q,
                                (flags & P3M_FORCES) ? d.force : nullptr, want_e ? &e_elc : nullptr,
                                !async);
     }
@@ -622,6 +694,14 @@ int p3m_short_range(p3m_context *ctx, int64_t n, double const *pos, double const
     sp.alpha = ctx->params.alpha;
     sp.r_cut = ctx->params.r_cut;
     sp.subdivision = ctx->short_range_subdivision;
+    if (ctx->elc_on && (ctx->elc.delta_mid_top != 0. || ctx->elc.delta_mid_bot != 0.)) {
+      ElcParams const ep = elc_geometry(ctx);
+      sp.elc_images = true;
+      sp.elc_h = ep.box[2] - ep.gap_size;
+      sp.elc_sl = ep.space_layer;
+      sp.elc_dt = ep.delta_top;
+      sp.elc_db = ep.delta_bot;
+    }
     ShortRangeResult r;
     hipStream_t const st = ctx->solver->stream();
     ctx->short_range->compute(st, sp, n, d.pos, d.q, d.force, want_ev ? &r : nullptr);
@@ -683,9 +763,6 @@ int p3m_set_elc(p3m_context *ctx, p3m_elc_params const *p) {
     if (dielectric && !const_pot && std::fabs(1. - dt * db) < round_error_prec)
       throw std::domain_error("ELC with two parallel metallic boundaries requires the "
                               "const_pot option");
-    if (dielectric)
-      throw std::runtime_error("ELC: dielectric contrasts (delta_mid_top, delta_mid_bot) are "
-                               "not supported by this solver");
     ctx->elc = *p;
     ctx->elc.delta_mid_top = dt;
     ctx->elc.delta_mid_bot = db;

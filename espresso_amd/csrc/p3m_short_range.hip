@@ -155,6 +155,48 @@ __device__ __forceinline__ void pair_term(double dx, double dy, double dz, doubl
   }
 }
 
+/** ELC image terms of dielectric contrasts (off when on == 0). */
+struct SrImages {
+  int on;
+  double h, sl, dt, db;
+};
+
+/** Pair term of particle i with the image of itself in its layer against
+ *  partner j (add_pair_force_corrections / pair_energy_correction,
+ *  elc.hpp:250-296; dielectric_layers_contribution, :119-138): q_eff =
+ *  delta q_i q_j, d = r_j - r_i' with r_i' = (x_i, y_i, -z_i) below the
+ *  layer bound or (x_i, y_i, 2h - z_i) above it, minimum image along z.
+ *  dx, dy = r_i - r_j in the plane; zj is the partner's folded z.  Force on
+ *  i and, with EV, the pair energy (ev[0]). */
+template <bool EV>
+__device__ __forceinline__ void image_terms(SrImages const &im, double zi, double zj, double dx,
+                                            double dy, double Lz, double qq, double rc2,
+                                            double alpha, double two_a, double &fx, double &fy,
+                                            double &fz, double (&ev)[7]) {
+  auto one = [&](double delta, double dz) {
+    dz -= Lz * rint(dz / Lz);
+    double const r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < rc2) || !(r2 > 0.) || delta == 0.) return;
+    double ax = -dx, ay = -dy, az = dz; // d = r_j - r_i'
+    double const rinv = nr_rsq(r2);
+    double const adist = alpha * (r2 * rinv);
+    double const t = nr_rcp(fma(0.3275911, adist, 1.));
+    double const erfc_part =
+        t * (0.254829592 +
+             t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+    double const ex = exp(-adist * adist);
+    double const erfc_ri = erfc_part * rinv;
+    double const q = qq * delta;
+    double const fac = ex * (erfc_ri + two_a) * (rinv * rinv) * q;
+    fx += fac * ax;
+    fy += fac * ay;
+    fz += fac * az;
+    if constexpr (EV) ev[0] += q * erfc_ri * ex;
+  };
+  if (zi < im.sl) one(im.db, zj + zi);
+  if (zi > im.h - im.sl) one(im.dt, zj - (2. * im.h - zi));
+}
+
 /** Displacement r_i - r_j, with the minimum image when MI. */
 template <bool MI>
 __device__ __forceinline__ void displacement(double4 const &pi, double4 const &pj,
@@ -179,7 +221,8 @@ template <bool MI, bool EV>
 __device__ __forceinline__ void tile_pairs(double4 const *tile, int cnt, double4 const &pi,
                                            double qpi, CellGrid const &g, double rc2,
                                            double alpha, double two_a, double &fx, double &fy,
-                                           double &fz, double (&ev)[7]) {
+                                           double &fz, double (&ev)[7], SrImages const &im,
+                                           double sz) {
   for (int kb = 0; kb < cnt; kb += 64) {
     int const kn = min(64, cnt - kb);
     unsigned long long mask = 0;
@@ -197,6 +240,9 @@ __device__ __forceinline__ void tile_pairs(double4 const *tile, int cnt, double4
       displacement<MI>(pi, pj, g, dx, dy, dz);
       double const r2 = dx * dx + dy * dy + dz * dz;
       pair_term<EV>(dx, dy, dz, r2, qpi * pj.w, alpha, two_a, fx, fy, fz, ev);
+      if (im.on)
+        image_terms<EV>(im, pi.z, pj.z - sz, dx, dy, g.L[2], qpi * pj.w, rc2, alpha, two_a, fx,
+                        fy, fz, ev);
     }
   }
 }
@@ -211,7 +257,8 @@ __global__ __launch_bounds__(NT) void k_sr_cells(CellGrid g, int const *__restri
                                                  double4 const *__restrict__ spos,
                                                  int const *__restrict__ sidx, double prefactor,
                                                  double alpha, double rc2,
-                                                 double *__restrict__ force, double *partials) {
+                                                 double *__restrict__ force, double *partials,
+                                                 SrImages im) {
   __shared__ double4 tile[NT];
   __shared__ double red[8 * 16];
   constexpr int W = 2 * NSUB + 1;
@@ -244,7 +291,7 @@ __global__ __launch_bounds__(NT) void k_sr_cells(CellGrid g, int const *__restri
         __syncthreads();
         if (has)
           tile_pairs<false, EV>(tile, min(NT, je - tb), pi, qpi, g, rc2, alpha, two_a, fx, fy,
-                                fz, ev);
+                                fz, ev, im, sz);
       }
     }
     if (has) {
@@ -269,7 +316,7 @@ __global__ __launch_bounds__(NT) void k_sr_allpairs(int nq, CellGrid g,
                                                     int const *__restrict__ sidx,
                                                     double prefactor, double alpha, double rc2,
                                                     double *__restrict__ force,
-                                                    double *partials) {
+                                                    double *partials, SrImages im) {
   __shared__ double4 tile[NT];
   __shared__ double red[8 * 16];
   int const i = blockIdx.x * NT + static_cast<int>(threadIdx.x);
@@ -286,7 +333,7 @@ __global__ __launch_bounds__(NT) void k_sr_allpairs(int nq, CellGrid g,
     __syncthreads();
     if (has)
       tile_pairs<true, EV>(tile, min(NT, nq - tb), pi, qpi, g, rc2, alpha, two_a, fx, fy, fz,
-                           ev);
+                           ev, im, 0.);
   }
   if (has) {
     double *f = force + 3 * static_cast<int64_t>(sidx[i]);
@@ -397,18 +444,19 @@ void ShortRange::compute(hipStream_t st, ShortRangeParams const &p, int64_t n, d
     blocks = std::max(1, (nq + 255) / 256);
   }
   if (out) sr_grow(m->partials, m->cap_part, static_cast<size_t>(blocks) * 8, "hipMalloc partials");
+  SrImages im{p.elc_images ? 1 : 0, p.elc_h, p.elc_sl, p.elc_dt, p.elc_db};
   auto launch = [&](auto EVc) {
     constexpr bool EV = decltype(EVc)::value;
     if (nsub == 2)
       k_sr_cells<2, 64, EV><<<blocks, 64, 0, st>>>(g, m->offsets, m->spos, m->sidx, p.prefactor,
-                                                   p.alpha, rc2, force, m->partials);
+                                                   p.alpha, rc2, force, m->partials, im);
     else if (nsub == 1)
       k_sr_cells<1, 256, EV><<<blocks, 256, 0, st>>>(g, m->offsets, m->spos, m->sidx,
                                                      p.prefactor, p.alpha, rc2, force,
-                                                     m->partials);
+                                                     m->partials, im);
     else
       k_sr_allpairs<256, EV><<<blocks, 256, 0, st>>>(nq, g, m->spos, m->sidx, p.prefactor,
-                                                     p.alpha, rc2, force, m->partials);
+                                                     p.alpha, rc2, force, m->partials, im);
   };
   if (out)
     launch(std::true_type{});

@@ -17,6 +17,13 @@ struct ShortRangeParams {
   double alpha = 0.;
   double r_cut = 0.;
   int subdivision = 0; // cells per r_cut: 0 = automatic (2 when the box allows), 1 or 2
+  // ELC dielectric contrasts: each pair within r_cut also gets the terms of
+  // add_pair_force_corrections / pair_energy_correction (elc.hpp:250-296),
+  // particle i against the image of itself in the layer it sits in
+  bool elc_images = false;
+  double elc_h = 0.;  // L_z - gap
+  double elc_sl = 0.; // space layer
+  double elc_dt = 0., elc_db = 0.;
 };
 
 struct ShortRangeResult {

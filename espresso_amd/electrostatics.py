@@ -539,8 +539,10 @@ class ELC:
     """Electrostatic layer correction on top of a P3M or P3MGPU actor, for
     systems with two periodic dimensions (``espressomd.electrostatics.ELC``,
     electrostatics.py class ELC; core ElectrostaticLayerCorrection,
-    elc.cpp).  Charges stay in 0 <= z <= L_z - gap_size.  Dielectric
-    contrasts are not supported by this solver.
+    elc.cpp).  Charges stay in 0 <= z <= L_z - gap_size.  With dielectric
+    contrasts (delta_mid_top / delta_mid_bot) the mesh carries the image
+    charges of the boundary layers, and ``add_short_range_forces`` adds the
+    pair terms with the images.
 
     The forces, energy and virial calls of the wrapped actor include the
     ELC terms once ELC is constructed on it."""

@@ -163,7 +163,8 @@ def elc_forces_energy(pos, q, box_l, prefactor, gap_size, far_cut, neutralize=Tr
     if diel:
         pz = prefactor * 2. * math.pi / (Lx * Ly)
         delta = dt * db
-        fdb, fdt, fd = db / (1. - delta), dt / (1. - delta), delta / (1. - delta)
+        if not const_pot:  # (delta = 1 needs const_pot, elc.cpp:1094-1101)
+            fdb, fdt, fd = db / (1. - delta), dt / (1. - delta), delta / (1. - delta)
         if const_pot:
             c0 = np.sum((-db * q)[bot]) + np.sum((dt * q)[top])
         else:

@@ -125,3 +125,84 @@ def test_elc_gap_particle_is_reported():
     with pytest.raises(RuntimeError, match="larger than box length"):
         ELC(actor=_p3m(box, [16] * 3, 4, 1.2, 2.5, check_neutrality=False), maxPWerror=1e-4,
             gap_size=13.).activate(pos, q2, box_l=list(box))
+
+
+def test_dielectric_contrast_analytic_image_charges():
+    """The reference's analytic test (elc_vs_analytic.py): two opposite
+    charges 1 apart above a dielectric boundary (delta_mid_bot = 39/41) in a
+    200 x 200 x 275 box (gap 75), P3M mesh 58 x 58 x 70, cao 4, accuracy
+    1e-7, tuned once; total force on the lower charge and total energy
+    against the image-charge solution at atol 1e-4, rtol 1e-7."""
+    from espresso_amd.electrostatics import ELC, P3M
+    L, gap, dist = 200., 75., 1.
+    box = [L, L, L + gap]
+    db = 39. / 41.
+    pref = 2.0
+    p3m = P3M(prefactor=pref, accuracy=1e-7, mesh=[58, 58, 70], cao=4, verbose=False,
+              check_neutrality=False)
+    elc = ELC(actor=p3m, gap_size=gap, maxPWerror=1e-7, delta_mid_bot=db, delta_mid_top=0.)
+    pos = np.array([box[0] / 2, box[1] / 2, box[2] / 2]) * np.ones((2, 3))
+    pos[1, 2] += dist
+    elc.activate(pos, np.array([-5., 5.]), box_l=box)
+    zs = np.linspace(0.1, L - 0.1 - dist, 12)
+    worst_f = worst_e = 0.
+    for charge in np.arange(-5.0, 5.1, 2.5):
+        q = np.array([charge, -charge])
+        for z in zs:
+            pos = np.array([[L / 2, L / 2, z], [L / 2, L / 2, z + dist]])
+            f = np.zeros((2, 3))
+            e_lr, _ = elc.solve(pos, q, force=f, energy=True)
+            e_sr = elc.add_short_range_forces(pos, q, f, energy=True)
+            c2 = pref * charge * charge
+            f_ref = c2 * (1. / dist ** 2 + db * (1. / (2 * z) ** 2 - 1. / (2 * z + dist) ** 2))
+            e_ref = c2 * (-1. / dist + db * (1. / (4 * z) - 1. / (2 * z + dist) +
+                                             1. / (4 * (z + dist))))
+            np.testing.assert_allclose(f[0, 2], f_ref, atol=1e-4, rtol=1e-7)
+            assert abs(e_lr + e_sr - e_ref) <= 1e-4, (charge, z, e_lr + e_sr, e_ref)
+            worst_f = max(worst_f, abs(f[0, 2] - f_ref))
+            worst_e = max(worst_e, abs(e_lr + e_sr - e_ref))
+    print(f"r_cut {p3m.get_params()['r_cut']:.3f}: worst force error {worst_f:.2e}, "
+          f"energy {worst_e:.2e}")
+
+
+@pytest.mark.parametrize("dt,db,const_pot,pot_diff", [
+    (0.3, -0.5, False, 0.),
+    (-1., -1., True, 2.5),
+    (0., 0.8, False, 0.),
+])
+def test_dielectric_contrast_total_matches_oracle(dt, db, const_pot, pot_diff):
+    """P3M + ELC with contrasts, short range included, against the oracle's
+    composition (image charges in the mesh, layer pair and self terms, image
+    series): total forces rel-rms <= 1e-9, energy <= 1e-9 relative."""
+    from espresso_amd.electrostatics import ELC
+    from oracle import elc as E
+    from oracle import oracle as O
+    box = (9., 8., 14.)
+    gap = 4.
+    n = 300
+    pos, q = _slab(n, box, gap, 29)
+    q[::23] = 0.
+    q -= np.sum(q) / np.count_nonzero(q) * (q != 0)
+    # crowd the layers: some particles close to both boundaries
+    pos[:20, 2] = np.linspace(0.05, 1.0, 20)
+    pos[20:40, 2] = np.linspace(box[2] - gap - 1.0, box[2] - gap - 0.05, 20)
+    alpha, r_cut = 1.4, 2.5
+    p3m = _p3m(box, [24, 24, 36], 5, alpha, r_cut, prefactor=1.3)
+    elc = ELC(actor=p3m, maxPWerror=1e-6, gap_size=gap, delta_mid_top=dt, delta_mid_bot=db,
+              const_pot=const_pot, pot_diff=pot_diff)
+    elc.activate(pos, q, box_l=list(box))
+    f = np.zeros((n, 3))
+    e_lr, _ = elc.solve(pos, q, force=f, energy=True)
+    e_sr = elc.add_short_range_forces(pos, q, f, energy=True)
+    ep = elc.get_params()
+    ora = O.OracleP3M(list(box), [24, 24, 36], 5, alpha, r_cut, 1.3)
+    sl = E.space_layer(gap, box[2] - gap, r_cut)
+    f_ref, e_ref = E.elc_p3m_total(
+        pos, q, box, ora, 1.3, alpha, r_cut, gap, ep["far_cut"], delta_top=dt, delta_bot=db,
+        const_pot=const_pot, pot_diff=pot_diff, space_layer=sl,
+        real_space=lambda p, qq: O.real_space(p, qq, box, 1.3, alpha, r_cut))
+    err = S.rel_rms(f, f_ref)
+    print(f"dt {dt} db {db} const_pot {const_pot}: far_cut {ep['far_cut']:.3f}, "
+          f"forces rel-rms {err:.2e}, energy {e_lr + e_sr:.8f} vs {e_ref:.8f}")
+    assert err <= 1e-9
+    assert abs(e_lr + e_sr - e_ref) <= 1e-9 * abs(e_ref)

@@ -226,8 +226,6 @@ def test_elc_parameters_and_messages():
          "'delta_mid_bot' must be >= -1 and <= \\+1"),
         (dict(maxPWerror=1e-6, gap_size=3., delta_mid_top=1., delta_mid_bot=1.), ValueError,
          "two parallel metallic boundaries requires the const_pot option"),
-        (dict(maxPWerror=1e-6, gap_size=3., delta_mid_bot=0.5), RuntimeError,
-         "dielectric contrasts"),
         (dict(maxPWerror="x", gap_size=3.), ValueError, "maxPWerror has to be a float"),
         (dict(maxPWerror=1e-6, gap_size=3., neutralize=1), ValueError,
          "neutralize has to be a bool"),
@@ -240,3 +238,10 @@ def test_elc_parameters_and_messages():
     with pytest.raises(ValueError, match="missing"):
         ELC(actor=p, maxPWerror=1e-6)
     elc.remove()
+    # contrasts switch neutralisation off (elc_data, elc.cpp:1056-1070)
+    d = ELC(actor=P3M(device=-1, **VALID), maxPWerror=1e-6, gap_size=3., delta_mid_bot=0.5,
+            delta_mid_top=-2e-15).get_params()
+    assert d["delta_mid_bot"] == 0.5 and d["neutralize"] is False and d["const_pot"] is False
+    d = ELC(actor=P3M(device=-1, **VALID), maxPWerror=1e-6, gap_size=3., const_pot=True,
+            pot_diff=1.5).get_params()
+    assert d["const_pot"] is False and d["pot_diff"] == 1.5
