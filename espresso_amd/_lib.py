@@ -45,6 +45,10 @@ EXPORTED_SYMBOLS = (
     "p3m_short_range", "p3m_short_range_cells", "p3m_set_short_range_subdivision",
     "p3m_set_force_convention", "p3m_get_force_convention", "p3m_set_tuning_real_space",
     "p3m_elc_params_default", "p3m_set_elc", "p3m_get_elc",
+    "p3m_dipolar_params_default", "p3m_dipolar_create", "p3m_dipolar_destroy",
+    "p3m_dipolar_last_error", "p3m_dipolar_set_params", "p3m_dipolar_get_params",
+    "p3m_dipolar_activate", "p3m_dipolar_solve", "p3m_dipolar_short_range",
+    "p3m_dipolar_get_stream", "p3m_dipolar_synchronize",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -59,7 +63,8 @@ EXPORTED_SYMBOLS = (
 )
 
 _STRING_RESULT = ("p3m_last_error", "p3m_last_global_error",
-                  "p3m_batch_last_error", "p3m_serve_last_error")
+                  "p3m_batch_last_error", "p3m_serve_last_error",
+                  "p3m_dipolar_last_error")
 
 
 class p3m_params(C.Structure):
@@ -92,6 +97,15 @@ class p3m_elc_params(C.Structure):
                 ("far_cut", C.c_double), ("neutralize", C.c_int),
                 ("delta_mid_top", C.c_double), ("delta_mid_bot", C.c_double),
                 ("const_pot", C.c_int), ("pot_diff", C.c_double)]
+
+
+class p3m_dipolar_params(C.Structure):
+    _fields_ = [("mesh", C.c_int), ("cao", C.c_int), ("alpha", C.c_double),
+                ("r_cut", C.c_double), ("accuracy", C.c_double),
+                ("prefactor", C.c_double), ("epsilon", C.c_double),
+                ("mesh_off", C.c_double * 3), ("box_l", C.c_double),
+                ("skin", C.c_double), ("tune", C.c_int), ("timings", C.c_int),
+                ("verbose", C.c_int)]
 
 
 _lib = None
@@ -154,6 +168,19 @@ def load():
     L.p3m_elc_params_default.argtypes = [ep]
     L.p3m_set_elc.argtypes = [vp, ep]
     L.p3m_get_elc.argtypes = [vp, ep, C.POINTER(C.c_int)]
+    dpp = C.POINTER(p3m_dipolar_params)
+    L.p3m_dipolar_params_default.argtypes = [dpp]
+    L.p3m_dipolar_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.p3m_dipolar_destroy.argtypes = [vp]
+    L.p3m_dipolar_last_error.argtypes = [vp]
+    L.p3m_dipolar_last_error.restype = C.c_char_p
+    L.p3m_dipolar_set_params.argtypes = [vp, dpp]
+    L.p3m_dipolar_get_params.argtypes = [vp, dpp]
+    L.p3m_dipolar_activate.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]
+    L.p3m_dipolar_solve.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp]
+    L.p3m_dipolar_short_range.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp]
+    L.p3m_dipolar_get_stream.argtypes = [vp, C.POINTER(vp)]
+    L.p3m_dipolar_synchronize.argtypes = [vp]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []
@@ -203,12 +230,14 @@ def wait_for_torch(stream_handle):
             torch.cuda.current_stream())
 
 
-def raise_for(status, ctx=None, batch=None, server=None):
+def raise_for(status, ctx=None, batch=None, server=None, dipolar=None):
     """Map a C status code to the Python exception the reference raises."""
     if status == P3M_OK:
         return
     L = load()
-    if server:
+    if dipolar:
+        msg = L.p3m_dipolar_last_error(dipolar)
+    elif server:
         msg = L.p3m_serve_last_error(server)
     elif batch:
         msg = L.p3m_batch_last_error(batch)

@@ -17,7 +17,7 @@ LIBNAME = "libespresso_amd_p3m.so"
 LIB_PATH = os.path.join(LIBDIR, LIBNAME)
 
 SOURCES = ["p3m_host.cpp", "p3m_tuning.cpp", "p3m_capi.cpp", "p3m_solver.hip",
-           "p3m_batch.hip", "p3m_serve.cpp", "p3m_short_range.hip", "p3m_elc.hip"]
+           "p3m_batch.hip", "p3m_serve.cpp", "p3m_short_range.hip", "p3m_elc.hip", "p3m_dipolar.hip"]
 
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 VALU forms
 # (v_pk_*_f32); hipcc's host pass warns that it does not know the feature.

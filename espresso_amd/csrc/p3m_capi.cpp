@@ -6,6 +6,7 @@
 // never cross the ABI; they become status codes plus a message.
 #include "espresso_amd_p3m.h"
 
+#include "p3m_dipolar.hpp"
 #include "p3m_elc.hpp"
 #include "p3m_host.hpp"
 #include "p3m_batch.hpp"
@@ -20,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <numbers>
@@ -51,6 +53,17 @@ struct p3m_context {
   // staging for host pointers
   int64_t cap = 0;
   double *d_pos = nullptr, *d_q = nullptr, *d_upos = nullptr, *d_force = nullptr;
+};
+
+struct p3m_dipolar {
+  int device = 0;
+  std::unique_ptr<DipolarSolver> solver;
+  p3m_dipolar_params user{};
+  bool have_params = false;
+  bool is_tuned = false;
+  std::string last_error;
+  int64_t cap = 0;
+  double *d_pos = nullptr, *d_dip = nullptr, *d_f = nullptr, *d_t = nullptr;
 };
 
 struct p3m_batch {
@@ -1206,6 +1219,301 @@ int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min
     inout->r_cut = params.r_cut;
     inout->accuracy = params.accuracy;
     inout->tune = 0;
+  });
+}
+
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// dipolar P3M (DipolarP3MImpl, src/core/magnetostatics/dp3m.cpp)
+// ---------------------------------------------------------------------------
+namespace {
+void dp_free_staging(p3m_dipolar *c) {
+  for (double **p : {&c->d_pos, &c->d_dip, &c->d_f, &c->d_t}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  c->cap = 0;
+}
+
+struct DpArgs {
+  double const *pos, *dip;
+  double *f, *t;
+};
+
+DpArgs dp_stage(p3m_dipolar *c, int64_t n, double const *pos, double const *dip, double *f,
+                double *t, unsigned flags) {
+  if (n > 0 && (!pos || !dip)) throw std::invalid_argument("DipolarP3M: null particle arrays");
+  if (flags & P3M_DEVICE_POINTERS) return {pos, dip, f, t};
+  if (n > c->cap) {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    dp_free_staging(c);
+    for (double **p : {&c->d_pos, &c->d_dip, &c->d_f, &c->d_t})
+      hip_check(hipMalloc(p, sizeof(double) * 3 * static_cast<size_t>(n)), "hipMalloc staging");
+    c->cap = n;
+  }
+  size_t const bytes = sizeof(double) * 3 * static_cast<size_t>(n);
+  if (n > 0) {
+    hip_check(hipMemcpy(c->d_pos, pos, bytes, hipMemcpyHostToDevice), "copy pos");
+    hip_check(hipMemcpy(c->d_dip, dip, bytes, hipMemcpyHostToDevice), "copy dip");
+    if (f) hip_check(hipMemcpy(c->d_f, f, bytes, hipMemcpyHostToDevice), "copy force");
+    if (t) hip_check(hipMemcpy(c->d_t, t, bytes, hipMemcpyHostToDevice), "copy torque");
+  }
+  return {c->d_pos, c->d_dip, f ? c->d_f : nullptr, t ? c->d_t : nullptr};
+}
+
+void dp_unstage(p3m_dipolar *c, int64_t n, DpArgs const &d, double *f, double *t,
+                unsigned flags) {
+  if ((flags & P3M_DEVICE_POINTERS) || n <= 0) return;
+  size_t const bytes = sizeof(double) * 3 * static_cast<size_t>(n);
+  if (f) hip_check(hipMemcpy(f, d.f, bytes, hipMemcpyDeviceToHost), "copy force");
+  if (t) hip_check(hipMemcpy(t, d.t, bytes, hipMemcpyDeviceToHost), "copy torque");
+  (void)c;
+}
+
+DipolarParams dp_params(p3m_dipolar_params const &u) {
+  DipolarParams p;
+  p.mesh = u.mesh;
+  p.cao = u.cao;
+  p.alpha = u.alpha;
+  p.r_cut = u.r_cut;
+  p.accuracy = u.accuracy;
+  p.prefactor = u.prefactor;
+  p.epsilon = u.epsilon;
+  for (int d = 0; d < 3; ++d) p.mesh_off[d] = u.mesh_off[d];
+  p.box = u.box_l;
+  p.skin = u.skin;
+  return p;
+}
+} // namespace
+
+extern "C" {
+
+int p3m_dipolar_params_default(p3m_dipolar_params *p) {
+  if (!p) return P3M_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->mesh = -1;
+  p->cao = -1;
+  p->alpha = -1.;
+  p->r_cut = -1.;
+  p->accuracy = -1.;
+  p->mesh_off[0] = p->mesh_off[1] = p->mesh_off[2] = 0.5;
+  p->skin = 0.4;
+  p->tune = 1;
+  p->timings = 10;
+  return P3M_OK;
+}
+
+int p3m_dipolar_create(int device, p3m_dipolar **out) {
+  if (!out) return P3M_ERR_INVALID;
+  *out = nullptr;
+  return guarded(static_cast<std::string *>(nullptr), [&] {
+    auto c = std::make_unique<p3m_dipolar>();
+    c->device = device;
+    if (device >= 0) {
+      int count = 0;
+      hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+      if (device >= count) throw std::invalid_argument("P3M: invalid device id");
+      c->solver = std::make_unique<DipolarSolver>(device);
+    }
+    p3m_dipolar_params_default(&c->user);
+    *out = c.release();
+  });
+}
+
+int p3m_dipolar_destroy(p3m_dipolar *c) {
+  if (!c) return P3M_OK;
+  if (c->device >= 0) (void)hipSetDevice(c->device);
+  dp_free_staging(c);
+  delete c;
+  return P3M_OK;
+}
+
+char const *p3m_dipolar_last_error(p3m_dipolar const *c) {
+  return c ? c->last_error.c_str() : "";
+}
+
+int p3m_dipolar_set_params(p3m_dipolar *c, p3m_dipolar_params const *p) {
+  if (!c || !p) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    // DipolarP3MImpl constructor and P3MParameters checks (dp3m.impl.hpp:79-92,
+    // common.hpp:94-150)
+    P3MParams check(p->tune != 0, p->epsilon, p->r_cut, Vec3i{p->mesh, p->mesh, p->mesh},
+                    Vec3d{p->mesh_off[0], p->mesh_off[1], p->mesh_off[2]}, p->cao, p->alpha,
+                    p->accuracy);
+    (void)check;
+    if (p->timings <= 0) throw std::domain_error("Parameter 'timings' must be > 0");
+    if (p->prefactor <= 0.) throw std::domain_error("Parameter 'prefactor' must be > 0");
+    if (!(p->box_l > 0.)) throw std::domain_error("box_l must be > 0");
+    c->user = *p;
+    c->have_params = true;
+    c->is_tuned = (p->tune == 0);
+  });
+}
+
+int p3m_dipolar_get_params(p3m_dipolar const *c, p3m_dipolar_params *out) {
+  if (!c || !out) return P3M_ERR_INVALID;
+  *out = c->user;
+  out->tune = c->is_tuned ? 0 : 1;
+  return P3M_OK;
+}
+
+int p3m_dipolar_activate(p3m_dipolar *c, int64_t n, double const *pos, double const *dip,
+                         unsigned flags) {
+  if (!c) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    if (!c->have_params) throw std::runtime_error("DipolarP3M: parameters not set");
+    if (!c->solver) throw std::runtime_error("DipolarP3M: host-only context cannot solve");
+    if (n < 0) throw std::invalid_argument("DipolarP3M: negative particle count");
+    // count_magnetic_particles (dp3m.cpp:83-97)
+    std::vector<double> h(3 * static_cast<size_t>(n));
+    if (n > 0) {
+      if (flags & P3M_DEVICE_POINTERS)
+        hip_check(hipMemcpy(h.data(), dip, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
+                  "copy dip");
+      else
+        std::memcpy(h.data(), dip, sizeof(double) * h.size());
+    }
+    int n_dip = 0;
+    double mu2 = 0.;
+    for (int64_t i = 0; i < n; ++i) {
+      double const m2 = h[3 * i] * h[3 * i] + h[3 * i + 1] * h[3 * i + 1] +
+                        h[3 * i + 2] * h[3 * i + 2];
+      if (m2 != 0.) {
+        ++n_dip;
+        mu2 += m2;
+      }
+    }
+    auto &u = c->user;
+    double const L = u.box_l;
+    if (!c->is_tuned) {
+      if (n_dip == 0) throw std::runtime_error("DipolarP3M: no dipolar particles in the system");
+      // tuning (dp3m.cpp:533-682): for each (mesh, cao) the smallest r_cut
+      // meeting the accuracy (bisection to 1e-3 L, TuningAlgorithm.cpp:116-172)
+      // with alpha from the real-space error; the (mesh, cao) pair is chosen
+      // by a cost model in place of the reference's timings
+      int m_lo = u.mesh, m_hi = u.mesh;
+      if (u.mesh == -1) {
+        m_lo = static_cast<int>(std::round(std::pow(2., std::floor(std::log(std::cbrt(
+                                                               static_cast<double>(n_dip))) /
+                                                           std::log(2.)))));
+        m_lo = std::max(2, m_lo + (m_lo % 2));
+        m_hi = 128;
+      }
+      int c_lo = (u.cao == -1) ? 1 : u.cao, c_hi = (u.cao == -1) ? 7 : u.cao;
+      double const r_max_iL = (L / 2. - u.skin) / L;
+      double best = -1., best_r = 0., best_a = 0., best_acc = 0.;
+      int best_m = 0, best_c = 0;
+      double const density = n_dip / (L * L * L);
+      for (int M = m_lo; M <= m_hi; M += 2)
+        for (int cao = c_lo; cao <= c_hi; ++cao) {
+          double out[4];
+          double r_iL;
+          if (u.r_cut > 0.) {
+            r_iL = u.r_cut / L;
+            dp3m_calculate_accuracy(L, M, cao, r_iL, n_dip, mu2, u.accuracy, out);
+            if (out[0] > u.accuracy) continue;
+          } else {
+            dp3m_calculate_accuracy(L, M, cao, r_max_iL, n_dip, mu2, u.accuracy, out);
+            if (out[0] > u.accuracy) continue;
+            double lo = 0., hi = r_max_iL;
+            while (hi - lo >= 1e-3) {
+              double const mid = 0.5 * (lo + hi);
+              dp3m_calculate_accuracy(L, M, cao, mid, n_dip, mu2, u.accuracy, out);
+              if (out[0] > u.accuracy)
+                lo = mid;
+              else
+                hi = mid;
+            }
+            r_iL = hi;
+            dp3m_calculate_accuracy(L, M, cao, r_iL, n_dip, mu2, u.accuracy, out);
+          }
+          double const rc = r_iL * L;
+          double const cost = 2e-9 * n_dip * cao * cao * cao +
+                              5e-9 * static_cast<double>(M) * M * M +
+                              1e-9 * n_dip * density * 4.18879 * rc * rc * rc;
+          if (best < 0. || cost < best) {
+            best = cost;
+            best_m = M;
+            best_c = cao;
+            best_r = rc;
+            best_a = (u.alpha > 0.) ? u.alpha : out[3] / L;
+            best_acc = out[0];
+          }
+        }
+      if (best < 0.) throw std::runtime_error("DipolarP3M: failed to reach requested accuracy");
+      u.mesh = best_m;
+      u.cao = best_c;
+      u.r_cut = best_r;
+      u.alpha = best_a;
+      u.accuracy = best_acc;
+      c->is_tuned = true;
+    }
+    // sanity_checks_boxl (dp3m.cpp:822-846)
+    double const cao_cut = 0.5 * (L / u.mesh) * u.cao;
+    if (cao_cut >= 0.5 * L) {
+      std::ostringstream msg;
+      msg << "dipolar P3M_init: k-space cutoff " << cao_cut
+          << " is larger than half of box dimension " << L;
+      throw std::runtime_error(msg.str());
+    }
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    c->solver->configure(dp_params(u));
+  });
+}
+
+int p3m_dipolar_solve(p3m_dipolar *c, int64_t n, double const *pos, double const *dip,
+                      double *force, double *torque, unsigned flags, double *energy) {
+  if (!c) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    if (!c->solver || !c->solver->configured())
+      throw std::runtime_error("DipolarP3M: solver used before activation");
+    if (n < 0) throw std::invalid_argument("DipolarP3M: negative particle count");
+    bool const async = (flags & P3M_ASYNC) != 0;
+    if (async && !(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
+    bool const wf = (flags & P3M_FORCES) != 0;
+    DpArgs const d = dp_stage(c, n, pos, dip, wf ? force : nullptr, wf ? torque : nullptr, flags);
+    double e = 0.;
+    bool const we = (flags & P3M_ENERGY) && !async;
+    c->solver->solve(n, d.pos, d.dip, d.f, d.t, we ? &e : nullptr, !async);
+    dp_unstage(c, n, d, wf ? force : nullptr, wf ? torque : nullptr, flags);
+    if (energy) *energy = we ? e : 0.;
+  });
+}
+
+int p3m_dipolar_short_range(p3m_dipolar *c, int64_t n, double const *pos, double const *dip,
+                            double *force, double *torque, unsigned flags, double *energy) {
+  if (!c) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    if (!c->solver || !c->solver->configured())
+      throw std::runtime_error("DipolarP3M: solver used before activation");
+    if (n < 0) throw std::invalid_argument("DipolarP3M: negative particle count");
+    bool const async = (flags & P3M_ASYNC) != 0;
+    if (async && !(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
+    if (!(c->user.r_cut < 0.5 * c->user.box_l))
+      throw std::domain_error("P3M: r_cut must be below half the box for minimum image");
+    DpArgs const d = dp_stage(c, n, pos, dip, force, torque, flags);
+    double e = 0.;
+    bool const we = (flags & P3M_ENERGY) && !async;
+    c->solver->short_range(n, d.pos, d.dip, d.f, d.t, we ? &e : nullptr, !async);
+    dp_unstage(c, n, d, force, torque, flags);
+    if (energy) *energy = we ? e : 0.;
+  });
+}
+
+int p3m_dipolar_get_stream(p3m_dipolar const *c, void **stream) {
+  if (!c || !stream || !c->solver) return P3M_ERR_INVALID;
+  *stream = reinterpret_cast<void *>(c->solver->stream());
+  return P3M_OK;
+}
+
+int p3m_dipolar_synchronize(p3m_dipolar *c) {
+  if (!c || !c->solver) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    hip_check(hipStreamSynchronize(c->solver->stream()), "hipStreamSynchronize");
   });
 }
 

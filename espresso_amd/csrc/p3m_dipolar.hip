@@ -1,0 +1,803 @@
+// Dipolar P3M on the device; see p3m_dipolar.hpp.
+#include "p3m_dipolar.hpp"
+#include "p3m_batch.hpp"
+#include "p3m_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numbers>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace espresso_amd {
+namespace {
+using namespace dev;
+using C2 = double2;
+
+constexpr int kDpBlock = 256;
+constexpr int kDpLines = 8; // FFT lines per block
+
+__global__ void k_dp_clear(int *flag) { *flag = 0; }
+
+__global__ __launch_bounds__(kDpBlock) void k_dp_zero(C2 *a, int64_t n) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i < n) a[i] = C2{0., 0.};
+}
+
+/** Weights and periodic index of the first stencil point of one particle
+ *  (the Coulomb solver's bit-exact CA-mesh rule). */
+template <int P>
+__device__ __forceinline__ bool dp_weights(DevGeo const &g, double const x[3], int gi[3],
+                                           double w[3][P]) {
+  int nmp[3];
+  double dist[3];
+  bool const ok = particle_coordinates_rt(g, P, x, nmp, gi, dist);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) bspline_weights<P>(dist[d], w[d]);
+  return ok;
+}
+
+/** dipole_assign (dp3m.cpp:157-193): mesh_d += w mu_d for the three
+ *  components, one thread per particle, device atomics. */
+template <int P>
+__global__ __launch_bounds__(kDpBlock) void k_dp_spread(int64_t n, double const *__restrict__ pos,
+                                                        double const *__restrict__ dip,
+                                                        DevGeo g, C2 *A0, C2 *A1, C2 *A2,
+                                                        int *err) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i >= n) return;
+  double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+  if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) return;
+  double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  int gi[3];
+  double w[3][P];
+  if (!dp_weights<P>(g, x, gi, w)) {
+    atomicOr(err, 1);
+    return;
+  }
+  int const M = g.M[0];
+  for (int a = 0; a < P; ++a) {
+    int const ix = wrap_once(gi[0] + a, M);
+    for (int b = 0; b < P; ++b) {
+      int const iy = wrap_once(gi[1] + b, M);
+      double const wab = w[0][a] * w[1][b];
+      int64_t const row = ((int64_t)ix * M + iy) * M;
+      for (int c = 0; c < P; ++c) {
+        int64_t const idx = row + wrap_once(gi[2] + c, M);
+        double const wt = wab * w[2][c];
+        atomicAdd(&A0[idx].x, wt * mu[0]);
+        atomicAdd(&A1[idx].x, wt * mu[1]);
+        atomicAdd(&A2[idx].x, wt * mu[2]);
+      }
+    }
+  }
+}
+
+/** In-place complex FFT of the lines along `axis` of nmesh cubic meshes
+ *  (blockIdx.y: mesh), kDpLines lines per block, unnormalised, forward
+ *  sign -1 (inv: +1). */
+__global__ __launch_bounds__(kDpBlock) void k_dp_fft(C2 *const *meshes, int M, int axis,
+                                                     FftPlanDev plan, C2 const *__restrict__ tw,
+                                                     bool inv) {
+  extern __shared__ double smem_d[];
+  C2 *twl = reinterpret_cast<C2 *>(smem_d);
+  int const ls = M + 1;
+  C2 *X = twl + M;
+  C2 *Y = X + kDpLines * ls;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) twl[m] = tw[m];
+  C2 *data = meshes[blockIdx.y];
+  int64_t const M2 = (int64_t)M * M;
+  int const l0 = blockIdx.x * kDpLines;
+  int const nl = min(kDpLines, static_cast<int>(M2 - l0));
+  int64_t const stride = axis == 2 ? 1 : axis == 1 ? M : M2;
+  auto base = [&](int l) -> int64_t {
+    if (axis == 2) return (int64_t)l * M;
+    int const hi = l / M, lo = l - hi * M;
+    return axis == 1 ? (int64_t)hi * M2 + lo : (int64_t)hi * M + lo;
+  };
+  for (int e = threadIdx.x; e < kDpLines * M; e += blockDim.x) {
+    int const l = e / M, j = e - l * M;
+    X[l * ls + j] = (l < nl) ? data[base(l0 + l) + j * stride] : C2{0., 0.};
+  }
+  __syncthreads();
+  C2 *res = lds_fft(X, Y, kDpLines, ls, plan, twl, inv);
+  for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
+    int const l = e / M, j = e - l * M;
+    data[base(l0 + l) + j * stride] = res[l * ls + j];
+  }
+}
+
+/** k-space step (dp3m.cpp:270-311, 322-343, 369-389): z = sum_d D_d A_d
+ *  with D the integer wave numbers (Nyquist 0); energy partials of
+ *  G_e |z|^2; Kt = G_e z (torques), Kf = -i G_f z (forces). */
+__global__ __launch_bounds__(kDpBlock) void k_dp_kspace(C2 const *__restrict__ A0,
+                                                        C2 const *__restrict__ A1,
+                                                        C2 const *__restrict__ A2,
+                                                        double const *__restrict__ Ge,
+                                                        double const *__restrict__ Gf, int M,
+                                                        C2 *__restrict__ Kt, C2 *__restrict__ Kf,
+                                                        double *partials) {
+  __shared__ double red[16];
+  int64_t const M3 = (int64_t)M * M * M;
+  int64_t const k = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  double e = 0.;
+  if (k < M3) {
+    int const iz = static_cast<int>(k % M);
+    int const iy = static_cast<int>((k / M) % M);
+    int const ix = static_cast<int>(k / ((int64_t)M * M));
+    double const dx = dop_shift(ix, M), dy = dop_shift(iy, M), dz = dop_shift(iz, M);
+    C2 const a0 = A0[k], a1 = A1[k], a2 = A2[k];
+    double const re = dx * a0.x + dy * a1.x + dz * a2.x;
+    double const im = dx * a0.y + dy * a1.y + dz * a2.y;
+    double const ge = Ge[k], gf = Gf[k];
+    e = ge * (re * re + im * im);
+    Kt[k] = C2{ge * re, ge * im};
+    Kf[k] = C2{gf * im, -gf * re};
+  }
+  double v[1] = {e};
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+
+/** Torque field meshes T_d = D_d Kt (overwrite the component meshes) and
+ *  the six force meshes F_ab = D_a D_b Kf (xx xy xz yy yz zz). */
+__global__ __launch_bounds__(kDpBlock) void k_dp_back_meshes(C2 const *__restrict__ Kt,
+                                                             C2 const *__restrict__ Kf, int M,
+                                                             C2 *const *T, C2 *const *F,
+                                                             int with_forces) {
+  int64_t const M3 = (int64_t)M * M * M;
+  int64_t const k = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (k >= M3) return;
+  int const iz = static_cast<int>(k % M);
+  int const iy = static_cast<int>((k / M) % M);
+  int const ix = static_cast<int>(k / ((int64_t)M * M));
+  double const D[3] = {static_cast<double>(dop_shift(ix, M)),
+                       static_cast<double>(dop_shift(iy, M)),
+                       static_cast<double>(dop_shift(iz, M))};
+  C2 const kt = Kt[k];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) T[d][k] = C2{D[d] * kt.x, D[d] * kt.y};
+  if (with_forces) {
+    C2 const kf = Kf[k];
+    int c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = a; b < 3; ++b, ++c) {
+        double const s = D[a] * D[b];
+        F[c][k] = C2{s * kf.x, s * kf.y};
+      }
+  }
+}
+
+/** AssignTorques / AssignForces (dp3m.cpp:195-244): torque -= mu x (pt E),
+ *  force_a += pf sum_b mu_b H_ab with E and H interpolated from the real
+ *  parts of the back-transformed meshes. */
+template <int P>
+__global__ __launch_bounds__(kDpBlock) void k_dp_gather(int64_t n, double const *__restrict__ pos,
+                                                        double const *__restrict__ dip,
+                                                        DevGeo g, C2 *const *T, C2 *const *F,
+                                                        double pt, double pf,
+                                                        double *__restrict__ force,
+                                                        double *__restrict__ torque) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i >= n) return;
+  double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+  if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) return;
+  double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  int gi[3];
+  double w[3][P];
+  if (!dp_weights<P>(g, x, gi, w)) return;
+  int const M = g.M[0];
+  double E[3] = {0., 0., 0.}, H[6] = {0., 0., 0., 0., 0., 0.};
+  bool const wf = force != nullptr;
+  for (int a = 0; a < P; ++a) {
+    int const ix = wrap_once(gi[0] + a, M);
+    for (int b = 0; b < P; ++b) {
+      int const iy = wrap_once(gi[1] + b, M);
+      double const wab = w[0][a] * w[1][b];
+      int64_t const row = ((int64_t)ix * M + iy) * M;
+      for (int c = 0; c < P; ++c) {
+        int64_t const idx = row + wrap_once(gi[2] + c, M);
+        double const wt = wab * w[2][c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) E[d] += wt * T[d][idx].x;
+        if (wf)
+#pragma unroll
+          for (int d = 0; d < 6; ++d) H[d] += wt * F[d][idx].x;
+      }
+    }
+  }
+  if (torque) {
+    double const e0 = pt * E[0], e1 = pt * E[1], e2 = pt * E[2];
+    torque[3 * i] -= mu[1] * e2 - mu[2] * e1;
+    torque[3 * i + 1] -= mu[2] * e0 - mu[0] * e2;
+    torque[3 * i + 2] -= mu[0] * e1 - mu[1] * e0;
+  }
+  if (wf) {
+    // H: xx xy xz yy yz zz
+    force[3 * i] += pf * (mu[0] * H[0] + mu[1] * H[1] + mu[2] * H[2]);
+    force[3 * i + 1] += pf * (mu[0] * H[1] + mu[1] * H[3] + mu[2] * H[4]);
+    force[3 * i + 2] += pf * (mu[0] * H[2] + mu[1] * H[4] + mu[2] * H[5]);
+  }
+}
+
+/** Sums of |mu|^2 and of the dipole vector (block partials). */
+__global__ __launch_bounds__(kDpBlock) void k_dp_moments(int64_t n, double const *__restrict__ dip,
+                                                         double *partials) {
+  __shared__ double red[4 * 16];
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  double v[4] = {0., 0., 0., 0.};
+  if (i < n) {
+    double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
+    v[0] = m0 * m0 + m1 * m1 + m2 * m2;
+    v[1] = m0;
+    v[2] = m1;
+    v[3] = m2;
+  }
+  block_sum<4>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = v[k];
+}
+
+/** Surface term torques (calc_surface_term, dp3m.cpp:451-517):
+ *  torque -= pref mu x M_total. */
+__global__ __launch_bounds__(kDpBlock) void k_dp_surface(int64_t n, double const *__restrict__ dip,
+                                                         double mx, double my, double mz,
+                                                         double pref, double *torque) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i >= n || !torque) return;
+  double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
+  torque[3 * i] -= pref * (m1 * mz - m2 * my);
+  torque[3 * i + 1] -= pref * (m2 * mx - m0 * mz);
+  torque[3 * i + 2] -= pref * (m0 * my - m1 * mx);
+}
+
+/** Real-space dipolar pair terms (DipolarP3M::pair_force / pair_energy,
+ *  dp3m.hpp:160-235) of all pairs within r_cut with the minimum image; both
+ *  orders of a pair, so each thread accumulates its particle's force and
+ *  torque (the partner's torque of calc_opposing_force,
+ *  forces_inline.hpp:159-171, is the other order's torque). */
+template <bool EV>
+__global__ __launch_bounds__(kDpBlock) void k_dp_pairs(int64_t n, double const *__restrict__ pos,
+                                                       double const *__restrict__ dip, double L,
+                                                       double prefactor, double alpha,
+                                                       double r_cut, int exact_erfc,
+                                                       double *__restrict__ force,
+                                                       double *__restrict__ torque,
+                                                       double *partials) {
+  __shared__ double4 tp[kDpBlock], tm[kDpBlock];
+  __shared__ double red[16];
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  bool const has = i < n;
+  double4 pi{0., 0., 0., 0.}, mi{0., 0., 0., 0.};
+  auto fold = [&](double v) {
+    double f = v - L * floor(v / L);
+    return f >= L ? f - L : f;
+  };
+  if (has) {
+    pi = double4{fold(pos[3 * i]), fold(pos[3 * i + 1]), fold(pos[3 * i + 2]), 0.};
+    mi = double4{dip[3 * i], dip[3 * i + 1], dip[3 * i + 2], 0.};
+  }
+  bool const act = has && (mi.x != 0. || mi.y != 0. || mi.z != 0.);
+  double const rc2 = r_cut * r_cut;
+  double const coeff = 2. * alpha * std::numbers::inv_sqrtpi;
+  double const alpsq = alpha * alpha;
+  double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.}, e = 0.;
+  for (int64_t jb = 0; jb < n; jb += kDpBlock) {
+    __syncthreads();
+    int64_t const j = jb + threadIdx.x;
+    if (j < n) {
+      tp[threadIdx.x] = double4{fold(pos[3 * j]), fold(pos[3 * j + 1]), fold(pos[3 * j + 2]), 0.};
+      tm[threadIdx.x] = double4{dip[3 * j], dip[3 * j + 1], dip[3 * j + 2], 0.};
+    }
+    __syncthreads();
+    if (!act) continue;
+    int const cnt = static_cast<int>(min<int64_t>(kDpBlock, n - jb));
+    for (int k = 0; k < cnt; ++k) {
+      double4 const pj = tp[k], mj = tm[k];
+      if (mj.x == 0. && mj.y == 0. && mj.z == 0.) continue;
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      dx -= L * rint(dx / L);
+      dy -= L * rint(dy / L);
+      dz -= L * rint(dz / L);
+      double const r2 = dx * dx + dy * dy + dz * dz;
+      if (!(r2 < rc2) || !(r2 > 0.)) continue;
+      double const r = sqrt(r2);
+      double const ad = alpha * r;
+      double const ex = exp(-ad * ad);
+      double const r2i = 1. / r2;
+      double B;
+      if (exact_erfc) {
+        B = (erfc(ad) / r + coeff * ex) * r2i;
+      } else {
+        double const tt = 1. / (1. + 0.3275911 * ad);
+        double const ep =
+            tt * (0.254829592 +
+                  tt * (-0.284496736 + tt * (1.421413741 + tt * (-1.453152027 + tt * 1.061405429))));
+        B = (ep / r + coeff) * ex * r2i;
+      }
+      double const common = alpsq * coeff * ex;
+      double const Cr = r2i * (3. * B + 2. * common);
+      double const Dr = r2i * (5. * Cr + 4. * common * alpsq);
+      double const mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
+      double const mir = mi.x * dx + mi.y * dy + mi.z * dz;
+      double const mjr = mj.x * dx + mj.y * dy + mj.z * dz;
+      double const cd = mimj * Cr - mir * mjr * Dr;
+      f[0] += prefactor * (cd * dx + (mi.x * mjr + mj.x * mir) * Cr);
+      f[1] += prefactor * (cd * dy + (mi.y * mjr + mj.y * mir) * Cr);
+      f[2] += prefactor * (cd * dz + (mi.z * mjr + mj.z * mir) * Cr);
+      double const c1 = mjr * Cr;
+      t[0] += prefactor * (-(mi.y * mj.z - mi.z * mj.y) * B + (mi.y * dz - mi.z * dy) * c1);
+      t[1] += prefactor * (-(mi.z * mj.x - mi.x * mj.z) * B + (mi.z * dx - mi.x * dz) * c1);
+      t[2] += prefactor * (-(mi.x * mj.y - mi.y * mj.x) * B + (mi.x * dy - mi.y * dx) * c1);
+      if constexpr (EV) e += prefactor * (mimj * B - mir * mjr * Cr);
+    }
+  }
+  if (act) {
+    if (force) {
+      force[3 * i] += f[0];
+      force[3 * i + 1] += f[1];
+      force[3 * i + 2] += f[2];
+    }
+    if (torque) {
+      torque[3 * i] += t[0];
+      torque[3 * i + 1] += t[1];
+      torque[3 * i + 2] += t[2];
+    }
+  }
+  if constexpr (EV) {
+    double v[1] = {e};
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+  }
+}
+
+void dp_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+double sinc_pi(double x) {
+  if (std::fabs(x) < 1e-12) return 1.;
+  double const px = std::numbers::pi * x;
+  return std::sin(px) / px;
+}
+int dshift(int j, int M, bool zero_nyquist) {
+  if (zero_nyquist && 2 * j == M) return 0;
+  return (2 * j < M) ? j : j - M;
+}
+} // namespace
+
+// ---------------------------------------------------------------------------
+// error estimates (dp3m.cpp:684-820)
+// ---------------------------------------------------------------------------
+double dp3m_real_space_error(double box, double r_cut_iL, int n_c_part, double sum_mu2,
+                             double alpha_L) {
+  double const rc = r_cut_iL * box;
+  double const rc2 = rc * rc;
+  double const rc4 = rc2 * rc2;
+  double const a2 = alpha_L * alpha_L / (box * box);
+  double const c = sum_mu2 * std::exp(-a2 * rc2);
+  double const cc = 4. * a2 * a2 * rc2 * rc2 + 6. * a2 * rc2 + 3.;
+  double const dc = 8. * a2 * a2 * a2 * rc2 * rc2 * rc2 + 20. * a2 * a2 * rc4 + 30. * a2 * rc2 + 15.;
+  double const con =
+      1. / std::sqrt(box * box * box * a2 * a2 * rc * rc4 * rc4 * static_cast<double>(n_c_part));
+  return c * con *
+         std::sqrt((13. / 6.) * cc * cc + (2. / 15.) * dc * dc - (13. / 15.) * cc * dc);
+}
+
+double dp3m_k_space_error(double box, int mesh, int cao, int n_c_part, double sum_mu2,
+                          double alpha_L) {
+  double const mesh_i = 1. / mesh;
+  double const f1 = (std::numbers::pi / alpha_L) * (std::numbers::pi / alpha_L);
+  int const h = mesh / 2;
+  std::vector<double> cs(static_cast<size_t>(2 * h));
+  for (int n = -h; n < h; ++n) cs[static_cast<size_t>(n + h)] = analytic_cotangent_sum(cao, n, mesh_i);
+  double he_q = 0.;
+  for (int x = -h; x < h; ++x)
+    for (int y = -h; y < h; ++y)
+      for (int z = -h; z < h; ++z) {
+        if (x == 0 && y == 0 && z == 0) continue;
+        double const n2 = static_cast<double>(x * x + y * y + z * z);
+        double const c = cs[static_cast<size_t>(x + h)] * cs[static_cast<size_t>(y + h)] *
+                         cs[static_cast<size_t>(z + h)];
+        double const ex = std::exp(-f1 * n2);
+        double const U2 = std::pow(sinc_pi(x * mesh_i) * sinc_pi(y * mesh_i) * sinc_pi(z * mesh_i),
+                                   2 * cao);
+        double const alias1 = ex * ex * n2;
+        double const alias2 = U2 * ex * n2 * n2 * n2 / n2;
+        double const d = alias1 - (alias2 / c) * (alias2 / c) / (n2 * n2 * n2);
+        if (d > 0. && std::fabs(d / alias1) > 1e-14) he_q += d;
+      }
+  return 8. * std::numbers::pi * std::numbers::pi / 3. * sum_mu2 *
+         std::sqrt(he_q / n_c_part) / (box * box * box * box);
+}
+
+namespace {
+double dp3m_rtbisection(double box, double r_cut_iL, int n, double mu2, double x1, double x2,
+                        double xacc, double accuracy) {
+  double const cst = accuracy / std::numbers::sqrt2;
+  double const f1 = dp3m_real_space_error(box, r_cut_iL, n, mu2, x1) - cst;
+  double const f2 = dp3m_real_space_error(box, r_cut_iL, n, mu2, x2) - cst;
+  if (f1 * f2 >= 0.)
+    throw std::runtime_error("Root must be bracketed for bisection in dp3m_rtbisection");
+  double dx, rtb;
+  if (f1 < 0.) {
+    dx = x2 - x1;
+    rtb = x1;
+  } else {
+    dx = x1 - x2;
+    rtb = x2;
+  }
+  for (int j = 1; j <= 40; ++j) {
+    dx *= 0.5;
+    double const xmid = rtb + dx;
+    double const fmid = dp3m_real_space_error(box, r_cut_iL, n, mu2, xmid) - cst;
+    if (fmid <= 0.) rtb = xmid;
+    if (std::fabs(dx) < xacc || fmid == 0.) return rtb;
+  }
+  throw std::runtime_error("Too many bisections in dp3m_rtbisection");
+}
+} // namespace
+
+void dp3m_calculate_accuracy(double box, int mesh, int cao, double r_cut_iL, int n_c_part,
+                             double sum_mu2, double accuracy, double out[4]) {
+  double rs = dp3m_real_space_error(box, r_cut_iL, n_c_part, sum_mu2, 0.001);
+  double alpha_L;
+  if (std::numbers::sqrt2 * rs > accuracy)
+    alpha_L = dp3m_rtbisection(box, r_cut_iL, n_c_part, sum_mu2, 0.0001 * box, 5. * box, 0.0001,
+                               accuracy);
+  else
+    alpha_L = 0.1;
+  rs = dp3m_real_space_error(box, r_cut_iL, n_c_part, sum_mu2, alpha_L);
+  double const ks = dp3m_k_space_error(box, mesh, cao, n_c_part, sum_mu2, alpha_L);
+  out[0] = std::hypot(rs, ks);
+  out[1] = rs;
+  out[2] = ks;
+  out[3] = alpha_L;
+}
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+struct DipolarSolver::Impl {
+  int device = 0;
+  hipStream_t st = nullptr;
+  bool configured = false;
+  DipolarParams p;
+  DevGeo g{};
+  FftPlanDev plan{};
+  int M = 0;
+  C2 *tw = nullptr, *twi = nullptr;
+  C2 *mesh[11] = {};      // A0..A2 (T0..T2), Kt, Kf, F0..F5
+  C2 **d_ptrs = nullptr;  // device pointer table: A[3], F[6]
+  double *Ge = nullptr, *Gf = nullptr;
+  double *part = nullptr;
+  size_t cap_part = 0;
+  int *err = nullptr;
+  double self_sum = -1.; // sum_k G_e U2 |d_op|^2 (lazy, as the reference)
+  double *h_pin = nullptr;
+};
+
+DipolarSolver::DipolarSolver(int device) : m(std::make_unique<Impl>()) {
+  m->device = device;
+  hip_check(hipSetDevice(device), "hipSetDevice");
+  hip_check(hipStreamCreate(&m->st), "hipStreamCreate");
+  hip_check(hipMalloc(&m->err, sizeof(int)), "hipMalloc dp3m flag");
+  hip_check(hipMalloc(&m->d_ptrs, sizeof(C2 *) * 12), "hipMalloc dp3m table");
+}
+
+DipolarSolver::~DipolarSolver() {
+  (void)hipSetDevice(m->device);
+  if (m->st) (void)hipStreamSynchronize(m->st);
+  for (auto *x : m->mesh) dp_free(x);
+  dp_free(m->tw);
+  dp_free(m->twi);
+  dp_free(m->d_ptrs);
+  dp_free(m->Ge);
+  dp_free(m->Gf);
+  dp_free(m->part);
+  dp_free(m->err);
+  if (m->st) (void)hipStreamDestroy(m->st);
+}
+
+bool DipolarSolver::configured() const { return m->configured; }
+DipolarParams const &DipolarSolver::params() const { return m->p; }
+hipStream_t DipolarSolver::stream() const { return m->st; }
+
+void DipolarSolver::configure(DipolarParams const &p) {
+  if (p.mesh < 2 || p.mesh % 2) throw std::runtime_error("DipolarP3M: mesh must be even and >= 2");
+  if (p.cao < 1 || p.cao > 7) throw std::runtime_error("DipolarP3M: cao must be in [1, 7]");
+  if (!(p.alpha > 0.) || !(p.r_cut > 0.) || !(p.box > 0.))
+    throw std::runtime_error("DipolarP3M: alpha, r_cut and the box must be positive");
+  hip_check(hipSetDevice(m->device), "hipSetDevice");
+  hip_check(hipStreamSynchronize(m->st), "sync");
+  int const M = p.mesh;
+  Vec3d const box{p.box, p.box, p.box};
+  P3MParams prm(false, p.epsilon, p.r_cut, Vec3i{M, M, M},
+                Vec3d{p.mesh_off[0], p.mesh_off[1], p.mesh_off[2]}, p.cao, p.alpha, p.accuracy);
+  prm.recalc_a_ai_cao_cut(box);
+  LocalMesh lm;
+  lm.calc_local_ca_mesh(prm, box, p.skin);
+  DevGeo &g = m->g;
+  g = DevGeo{};
+  for (int k = 0; k < 3; ++k) {
+    g.M[k] = M;
+    g.dim[k] = lm.dim[k];
+    g.marg[k] = lm.margin[2 * k];
+    g.ld_pos[k] = lm.ld_pos[k];
+    g.ai[k] = prm.ai[k];
+    g.box_l[k] = p.box;
+  }
+  g.pos_shift = std::floor((p.cao - 1) / 2.0) - (p.cao % 2) / 2.0;
+  g.conv = 0;
+  FftPlan1D const fp = make_fft_plan(M);
+  m->plan.n = fp.n;
+  m->plan.nstages = fp.nstages;
+  for (int s = 0; s < 16; ++s) m->plan.radix[s] = fp.radix[s];
+  // buffers
+  if (M != m->M) {
+    for (auto *&x : m->mesh) {
+      dp_free(x);
+      x = nullptr;
+    }
+    dp_free(m->tw);
+    dp_free(m->Ge);
+    dp_free(m->Gf);
+    size_t const M3 = static_cast<size_t>(M) * M * M;
+    for (auto *&x : m->mesh) hip_check(hipMalloc(&x, sizeof(C2) * M3), "hipMalloc dp3m mesh");
+    hip_check(hipMalloc(&m->Ge, sizeof(double) * M3), "hipMalloc dp3m G");
+    hip_check(hipMalloc(&m->Gf, sizeof(double) * M3), "hipMalloc dp3m G");
+    hip_check(hipMalloc(&m->tw, sizeof(C2) * M), "hipMalloc dp3m twiddles");
+    std::vector<C2> tw(static_cast<size_t>(M));
+    for (int k = 0; k < M; ++k) {
+      long double const ang = -2.0L * std::numbers::pi_v<long double> * k / M;
+      tw[static_cast<size_t>(k)] =
+          C2{static_cast<double>(std::cos(ang)), static_cast<double>(std::sin(ang))};
+    }
+    hip_check(hipMemcpy(m->tw, tw.data(), sizeof(C2) * M, hipMemcpyHostToDevice), "copy tw");
+    C2 *tab[12] = {m->mesh[0], m->mesh[1], m->mesh[2], m->mesh[5], m->mesh[6],
+                   m->mesh[7], m->mesh[8], m->mesh[9], m->mesh[10], m->mesh[3],
+                   m->mesh[4], nullptr};
+    hip_check(hipMemcpy(m->d_ptrs, tab, sizeof(tab), hipMemcpyHostToDevice), "copy table");
+    m->M = M;
+  }
+  // influence functions (influence_function_dipolar.hpp:40-140), P3M_BRILLOUIN = 0
+  {
+    size_t const M3 = static_cast<size_t>(M) * M * M;
+    std::vector<double> ge(M3, 0.), gf(M3, 0.);
+    double const alpha_L = p.alpha * p.box;
+    double const f2 = (std::numbers::pi / alpha_L) * (std::numbers::pi / alpha_L);
+    double const pref = static_cast<double>(M) * M * M * 2. / (p.box * p.box);
+    int const half = M / 2;
+    std::vector<double> sn(static_cast<size_t>(M));
+    for (int j = 0; j < M; ++j)
+      sn[static_cast<size_t>(j)] = sinc_pi(static_cast<double>(dshift(j, M, false)) / M);
+    size_t idx = 0;
+    for (int x = 0; x < M; ++x)
+      for (int y = 0; y < M; ++y)
+        for (int z = 0; z < M; ++z, ++idx) {
+          if (x % half == 0 && y % half == 0 && z % half == 0) continue;
+          double const s[3] = {static_cast<double>(dshift(x, M, false)),
+                               static_cast<double>(dshift(y, M, false)),
+                               static_cast<double>(dshift(z, M, false))};
+          double const d[3] = {static_cast<double>(dshift(x, M, true)),
+                               static_cast<double>(dshift(y, M, true)),
+                               static_cast<double>(dshift(z, M, true))};
+          double const nm2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+          double const sz = std::pow(sn[static_cast<size_t>(x)] * sn[static_cast<size_t>(y)] *
+                                         sn[static_cast<size_t>(z)],
+                                     2 * p.cao);
+          double const dn = d[0] * s[0] + d[1] * s[1] + d[2] * s[2];
+          double const d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          double const et = f2 * nm2;
+          double num2 = 0., num3 = 0.;
+          if (et < 30.) {
+            double const f3 = sz * std::exp(-et) / nm2;
+            num2 = f3 * dn * dn;
+            num3 = f3 * dn * dn * dn;
+          }
+          double const den = sz * sz;
+          ge[idx] = pref * num2 / (d2 * d2 * den);
+          gf[idx] = pref * num3 / (d2 * d2 * d2 * den);
+          if (!std::isfinite(ge[idx])) ge[idx] = 0.;
+          if (!std::isfinite(gf[idx])) gf[idx] = 0.;
+        }
+    hip_check(hipMemcpy(m->Ge, ge.data(), sizeof(double) * M3, hipMemcpyHostToDevice), "copy G");
+    hip_check(hipMemcpy(m->Gf, gf.data(), sizeof(double) * M3, hipMemcpyHostToDevice), "copy G");
+    // self-energy sum of the energy correction (influence_function_dipolar.hpp:160-218)
+    double se = 0.;
+    idx = 0;
+    for (int x = 0; x < M; ++x)
+      for (int y = 0; y < M; ++y)
+        for (int z = 0; z < M; ++z, ++idx) {
+          if (x % half == 0 && y % half == 0 && z % half == 0) continue;
+          double const o[3] = {static_cast<double>(dshift(x, M, false)) / M,
+                               static_cast<double>(dshift(y, M, false)) / M,
+                               static_cast<double>(dshift(z, M, false)) / M};
+          double u2 = 0.;
+          for (int a = -1; a <= 1; ++a)
+            for (int b = -1; b <= 1; ++b)
+              for (int c = -1; c <= 1; ++c)
+                u2 += std::pow(sinc_pi(o[0] + a * M) * sinc_pi(o[1] + b * M) *
+                                   sinc_pi(o[2] + c * M),
+                               2 * p.cao);
+          double const d[3] = {static_cast<double>(dshift(x, M, true)),
+                               static_cast<double>(dshift(y, M, true)),
+                               static_cast<double>(dshift(z, M, true))};
+          se += ge[idx] * u2 * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        }
+    m->self_sum = se;
+  }
+  int const lds = static_cast<int>(sizeof(C2) * (M + 2 * kDpLines * (M + 1)));
+  if (lds > 64 * 1024)
+    hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k_dp_fft),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+              "hipFuncSetAttribute");
+  m->p = p;
+  m->configured = true;
+}
+
+namespace {
+template <int P> void launch_spread(hipStream_t st, int nb, int64_t n, double const *pos,
+                                    double const *dip, DevGeo const &g, C2 *const *mesh, int *err) {
+  k_dp_spread<P><<<nb, kDpBlock, 0, st>>>(n, pos, dip, g, mesh[0], mesh[1], mesh[2], err);
+}
+template <int P> void launch_gather(hipStream_t st, int nb, int64_t n, double const *pos,
+                                    double const *dip, DevGeo const &g, C2 *const *T,
+                                    C2 *const *F, double pt, double pf, double *force,
+                                    double *torque) {
+  k_dp_gather<P><<<nb, kDpBlock, 0, st>>>(n, pos, dip, g, T, F, pt, pf, force, torque);
+}
+} // namespace
+
+void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, double *force,
+                          double *torque, double *energy, bool sync) {
+  if (!m->configured) throw std::runtime_error("DipolarP3M solver used before configure()");
+  if (energy) *energy = 0.;
+  if (n <= 0) return;
+  hip_check(hipSetDevice(m->device), "hipSetDevice");
+  hipStream_t const st = m->st;
+  DipolarParams const &p = m->p;
+  int const M = m->M;
+  int64_t const M3 = static_cast<int64_t>(M) * M * M;
+  int const nb = static_cast<int>((n + kDpBlock - 1) / kDpBlock);
+  int const kb = static_cast<int>((M3 + kDpBlock - 1) / kDpBlock);
+  size_t const need = std::max(static_cast<size_t>(kb), static_cast<size_t>(nb) * 4);
+  if (need > m->cap_part) {
+    hip_check(hipStreamSynchronize(st), "sync");
+    dp_free(m->part);
+    hip_check(hipMalloc(&m->part, sizeof(double) * need), "hipMalloc dp3m partials");
+    m->cap_part = need;
+  }
+  // moments (sum |mu|^2, total dipole)
+  std::vector<double> mom(static_cast<size_t>(nb) * 4);
+  k_dp_moments<<<nb, kDpBlock, 0, st>>>(n, dip, m->part);
+  hip_check(hipMemcpyAsync(mom.data(), m->part, sizeof(double) * mom.size(),
+                           hipMemcpyDeviceToHost, st),
+            "copy dp3m moments");
+  hip_check(hipStreamSynchronize(st), "sync dp3m moments");
+  double mu2 = 0., Mt[3] = {0., 0., 0.};
+  for (int b = 0; b < nb; ++b) {
+    mu2 += mom[static_cast<size_t>(b) * 4];
+    for (int k = 0; k < 3; ++k) Mt[k] += mom[static_cast<size_t>(b) * 4 + 1 + k];
+  }
+  double const V = p.box * p.box * p.box;
+  double const wn = 2. * std::numbers::pi / p.box;
+  double const dipole_prefac = p.prefactor / static_cast<double>(M3);
+  if (mu2 > 0.) {
+    for (int c = 0; c < 3; ++c) k_dp_zero<<<kb, kDpBlock, 0, st>>>(m->mesh[c], M3);
+    k_dp_clear<<<1, 1, 0, st>>>(m->err);
+    switch (p.cao) {
+    case 1: launch_spread<1>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 2: launch_spread<2>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 3: launch_spread<3>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 4: launch_spread<4>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 5: launch_spread<5>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 6: launch_spread<6>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    default: launch_spread<7>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    }
+    size_t const lds = sizeof(C2) * (M + 2 * kDpLines * (M + 1));
+    int const lines = static_cast<int>((static_cast<int64_t>(M) * M + kDpLines - 1) / kDpLines);
+    auto fft = [&](C2 *const *table, int nmesh, bool inv) {
+      for (int axis = 2; axis >= 0; --axis)
+        k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, axis, m->plan, m->tw, inv);
+    };
+    // device table: [0..2] A (torque meshes after the k-space step),
+    // [3..8] F, [9] Kt, [10] Kf
+    fft(m->d_ptrs, 3, false);
+    k_dp_kspace<<<kb, kDpBlock, 0, st>>>(m->mesh[0], m->mesh[1], m->mesh[2], m->Ge, m->Gf, M,
+                                         m->mesh[3], m->mesh[4], m->part);
+    std::vector<double> ep(static_cast<size_t>(kb));
+    if (energy) {
+      hip_check(hipMemcpyAsync(ep.data(), m->part, sizeof(double) * kb, hipMemcpyDeviceToHost,
+                               st),
+                "copy dp3m energy");
+    }
+    bool const wf = force != nullptr, wt = torque != nullptr;
+    if (wf || wt) {
+      k_dp_back_meshes<<<kb, kDpBlock, 0, st>>>(m->mesh[3], m->mesh[4], M, m->d_ptrs,
+                                                m->d_ptrs + 3, wf ? 1 : 0);
+      fft(m->d_ptrs, 3, true);
+      if (wf) fft(m->d_ptrs + 3, 6, true);
+      double const pt = dipole_prefac * wn, pf = dipole_prefac * wn * wn;
+      switch (p.cao) {
+      case 1: launch_gather<1>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 2: launch_gather<2>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 3: launch_gather<3>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 4: launch_gather<4>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 5: launch_gather<5>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 6: launch_gather<6>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      default: launch_gather<7>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      }
+    }
+    hip_check(hipGetLastError(), "dipolar P3M kernels");
+    if (energy || sync) {
+      int err = 0;
+      hip_check(hipMemcpyAsync(&err, m->err, sizeof(int), hipMemcpyDeviceToHost, st),
+                "copy flag");
+      hip_check(hipStreamSynchronize(st), "sync dp3m");
+      if (err) throw std::runtime_error("DipolarP3M: particle outside the charge-assignment mesh");
+    }
+    if (energy) {
+      double ek = 0.;
+      for (double v : ep) ek += v;
+      double e = ek * dipole_prefac * std::numbers::pi / p.box;
+      e -= p.prefactor * mu2 * std::numbers::inv_sqrtpi * (2. / 3.) * p.alpha * p.alpha * p.alpha;
+      // calc_energy_correction (dp3m.cpp:892-901)
+      double const alpha_L = p.alpha * p.box;
+      double const ukp3m = std::numbers::pi * m->self_sum / (3. * p.box * static_cast<double>(M3)) * V;
+      double const eself = -2. * alpha_L * alpha_L * alpha_L * std::numbers::inv_sqrtpi / 3.;
+      double const corr = -mu2 * (ukp3m + eself + 2. * std::numbers::pi / 3.);
+      e += p.prefactor * corr / V;
+      *energy = e;
+    }
+  }
+  if (p.epsilon != 0.) {
+    // calc_surface_term (dp3m.cpp:451-517)
+    double const pref = p.prefactor * 4. * std::numbers::pi / V / (2. * p.epsilon + 1.);
+    if (torque) k_dp_surface<<<nb, kDpBlock, 0, st>>>(n, dip, Mt[0], Mt[1], Mt[2], pref, torque);
+    if (energy) *energy += 0.5 * pref * (Mt[0] * Mt[0] + Mt[1] * Mt[1] + Mt[2] * Mt[2]);
+  }
+  hip_check(hipGetLastError(), "dipolar P3M surface");
+  if (sync || energy) hip_check(hipStreamSynchronize(st), "sync dp3m");
+}
+
+void DipolarSolver::short_range(int64_t n, double const *pos, double const *dip, double *force,
+                                double *torque, double *energy, bool sync) {
+  if (!m->configured) throw std::runtime_error("DipolarP3M solver used before configure()");
+  if (energy) *energy = 0.;
+  if (n <= 0) return;
+  hip_check(hipSetDevice(m->device), "hipSetDevice");
+  hipStream_t const st = m->st;
+  DipolarParams const &p = m->p;
+  int const nb = static_cast<int>((n + kDpBlock - 1) / kDpBlock);
+  if (static_cast<size_t>(nb) > m->cap_part) {
+    hip_check(hipStreamSynchronize(st), "sync");
+    dp_free(m->part);
+    hip_check(hipMalloc(&m->part, sizeof(double) * nb), "hipMalloc dp3m partials");
+    m->cap_part = static_cast<size_t>(nb);
+  }
+  int const exact = p.accuracy > 5e-6 ? 0 : 1;
+  if (energy)
+    k_dp_pairs<true><<<nb, kDpBlock, 0, st>>>(n, pos, dip, p.box, p.prefactor, p.alpha, p.r_cut,
+                                              exact, force, torque, m->part);
+  else
+    k_dp_pairs<false><<<nb, kDpBlock, 0, st>>>(n, pos, dip, p.box, p.prefactor, p.alpha, p.r_cut,
+                                               exact, force, torque, nullptr);
+  hip_check(hipGetLastError(), "dipolar pair kernel");
+  if (energy) {
+    std::vector<double> e(static_cast<size_t>(nb));
+    hip_check(hipMemcpyAsync(e.data(), m->part, sizeof(double) * nb, hipMemcpyDeviceToHost, st),
+              "copy dp3m pair energy");
+    hip_check(hipStreamSynchronize(st), "sync dp3m pairs");
+    double s = 0.;
+    for (double v : e) s += v;
+    *energy = 0.5 * s;
+  } else if (sync) {
+    hip_check(hipStreamSynchronize(st), "sync dp3m pairs");
+  }
+}
+
+} // namespace espresso_amd

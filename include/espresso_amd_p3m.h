@@ -211,6 +211,63 @@ int p3m_set_elc(p3m_context *ctx, const p3m_elc_params *p);
  * (src/script_interface/electrostatics/ElectrostaticLayerCorrection.hpp:60-85). */
 int p3m_get_elc(const p3m_context *ctx, p3m_elc_params *out, int *enabled);
 
+/* ------------------------------------------------------------------------
+ * Dipolar P3M (magnetostatics), DipolarP3MImpl<double, CPU>
+ * (src/core/magnetostatics/dp3m.cpp, dp3m.hpp, dp3m.impl.hpp) for a cubic
+ * box and mesh.  Parameters and -1 sentinels follow the Python actor
+ * espressomd.magnetostatics.DipolarP3M (magnetostatics.py, class
+ * DipolarP3M): mesh, cao, alpha, r_cut tuned when -1; epsilon 0 = metallic.
+ * Arrays are [n][3] (positions, dipole moments, forces, torques), host or
+ * device pointers by P3M_DEVICE_POINTERS; forces and torques are added in
+ * place.
+ * ------------------------------------------------------------------------ */
+typedef struct p3m_dipolar p3m_dipolar;
+
+typedef struct {
+  int mesh;           /* cubic mesh, -1 = tune                              */
+  int cao;            /* 1..7, -1 = tune                                    */
+  double alpha;       /* -1 = tune                                          */
+  double r_cut;       /* -1 = tune                                          */
+  double accuracy;    /* target rms force error (required)                  */
+  double prefactor;   /* mu_0 / (4 pi)                                      */
+  double epsilon;     /* surrounding permeability, 0 = metallic             */
+  double mesh_off[3]; /* default 0.5                                        */
+  double box_l;       /* cubic box edge                                     */
+  double skin;        /* Verlet skin: fixes the charge-assignment mesh      */
+  int tune;           /* nonzero: tune what is -1                           */
+  int timings;        /* > 0                                                */
+  int verbose;
+} p3m_dipolar_params;
+
+int p3m_dipolar_params_default(p3m_dipolar_params *p);
+/* Context on HIP device `device` (-1: host only, parameter checks only);
+ * replaces the DipolarP3MImpl handle (dp3m.impl.hpp:79-92). */
+int p3m_dipolar_create(int device, p3m_dipolar **out);
+int p3m_dipolar_destroy(p3m_dipolar *ctx);
+const char *p3m_dipolar_last_error(const p3m_dipolar *ctx);
+/* Validate and store parameters (dp3m.impl.hpp:79-92, common.hpp:94-150). */
+int p3m_dipolar_set_params(p3m_dipolar *ctx, const p3m_dipolar_params *p);
+int p3m_dipolar_get_params(const p3m_dipolar *ctx, p3m_dipolar_params *out);
+/* count_magnetic_particles and tune (dp3m.cpp:83-97, 646-682; error
+ * estimates :684-820): for each (mesh, cao) the smallest r_cut meeting the
+ * accuracy, alpha from the real-space error; (mesh, cao) by a cost model.
+ * Then init (sanity_checks_boxl, :822-846; influence functions). */
+int p3m_dipolar_activate(p3m_dipolar *ctx, int64_t n, const double *pos, const double *dip,
+                         unsigned flags);
+/* k-space part: P3M_FORCES adds forces and torques
+ * (long_range_kernel with force_flag, dp3m.cpp:313-412, plus the surface
+ * term :451-517), P3M_ENERGY returns the k-space energy with self,
+ * correction and surface terms (:270-311, 892-901). */
+int p3m_dipolar_solve(p3m_dipolar *ctx, int64_t n, const double *pos, const double *dip,
+                      double *force, double *torque, unsigned flags, double *energy);
+/* Real-space pair terms of DipolarP3M::pair_force / pair_energy
+ * (dp3m.hpp:160-235) over all pairs within r_cut (minimum image, r_cut
+ * below half the box): forces and torques added, energy with P3M_ENERGY. */
+int p3m_dipolar_short_range(p3m_dipolar *ctx, int64_t n, const double *pos, const double *dip,
+                            double *force, double *torque, unsigned flags, double *energy);
+int p3m_dipolar_get_stream(const p3m_dipolar *ctx, void **stream);
+int p3m_dipolar_synchronize(p3m_dipolar *ctx);
+
 /* Real-space cost that timed tuning (p3m_activate with tune set) adds to
  * each timed k-space solve.  The reference times whole integration steps,
  * real-space loop included (TuningAlgorithm.cpp:116-211 via benchmark_integration_step, tuning.cpp:73);

@@ -245,3 +245,28 @@ def test_elc_parameters_and_messages():
     d = ELC(actor=P3M(device=-1, **VALID), maxPWerror=1e-6, gap_size=3., const_pot=True,
             pot_diff=1.5).get_params()
     assert d["const_pot"] is False and d["pot_diff"] == 1.5
+
+
+def test_dipolar_p3m_parameters_and_messages():
+    """DipolarP3M keyword checks (magnetostatics.py DipolarP3M,
+    dp3m.impl.hpp:79-92) on a host-only context."""
+    from espresso_amd.magnetostatics import DipolarP3M
+    d = DipolarP3M(device=-1, prefactor=1.1, accuracy=1e-6, mesh=32, epsilon="metallic")
+    g = d.get_params()
+    assert g["mesh"] == [32, 32, 32] and g["epsilon"] == 0. and g["mesh_off"] == [0.5] * 3
+    with pytest.raises(ValueError, match="missing"):
+        DipolarP3M(device=-1, prefactor=1.)
+    with pytest.raises(ValueError, match="requires a cubic mesh"):
+        DipolarP3M(device=-1, prefactor=1., accuracy=1e-4, mesh=[16, 16, 18])
+    with pytest.raises(ValueError, match="'epsilon' has to be a float or 'metallic'"):
+        DipolarP3M(device=-1, prefactor=1., accuracy=1e-4, epsilon="vacuum")
+    with pytest.raises(TypeError, match="'timings' has to be an integer"):
+        DipolarP3M(device=-1, prefactor=1., accuracy=1e-4, timings=2.5)
+    with pytest.raises(ValueError, match="'timings' must be > 0"):
+        DipolarP3M(device=-1, prefactor=1., accuracy=1e-4, timings=0)
+    with pytest.raises(ValueError, match="'prefactor' must be > 0"):
+        DipolarP3M(device=-1, prefactor=0., accuracy=1e-4)
+    with pytest.raises(RuntimeError, match="host-only"):
+        d.activate(np.zeros((2, 3)), np.ones((2, 3)), box_l=10.)
+    with pytest.raises(RuntimeError, match="cubic box"):
+        d.activate(np.zeros((2, 3)), np.ones((2, 3)), box_l=[10., 10., 11.])
