@@ -435,6 +435,57 @@ def bench_elc(steps, warmup, torch, dev, n=100_000, L=60., gap=10., seed=7001):
     return rec
 
 
+def bench_dp3m(steps, warmup, torch, dev, n=100_000, seed=7101):
+    """Dipolar P3M at the density of the reference's dipolar test
+    (0.09 per unit volume, dipolar_mdlc_p3m_scafacos_p2nfft.py): n random
+    unit dipoles in a cubic box, mesh 64, cao 5, accuracy 1e-4 (alpha and
+    r_cut from the dipolar error estimates).  Times the k-space solve
+    (forces and torques) and, at n = 10^4, the all-pairs real-space kernel."""
+    from espresso_amd.magnetostatics import DipolarP3M
+    rng = np.random.default_rng(seed)
+    L = (4. * np.pi * n / (3. * 0.09)) ** (1. / 3.)
+    pos = rng.uniform(0., L, (n, 3))
+    dip = rng.normal(size=(n, 3))
+    dip /= np.linalg.norm(dip, axis=1)[:, None]
+    tp, td = torch.tensor(pos, device=dev), torch.tensor(dip, device=dev)
+    tf, tt = torch.zeros_like(tp), torch.zeros_like(tp)
+    dp = DipolarP3M(prefactor=1.0, accuracy=1e-4, mesh=64, cao=5, verbose=False)
+    dp.activate(tp, td, box_l=L)
+    prm = dp.get_params()
+    for _ in range(max(1, warmup)):
+        dp.add_long_range_forces(tp, td, tf, tt, sync=False)
+    dp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dp.add_long_range_forces(tp, td, tf, tt, sync=False)
+    dp.synchronize()
+    t_k = (time.perf_counter() - t0) / steps
+    n_sr = 10_000
+    L_sr = (4. * np.pi * n_sr / (3. * 0.09)) ** (1. / 3.)
+    ps = torch.tensor(rng.uniform(0., L_sr, (n_sr, 3)), device=dev)
+    ds = torch.tensor(rng.normal(size=(n_sr, 3)), device=dev)
+    sr = DipolarP3M(prefactor=1.0, accuracy=1e-4, mesh=32, cao=5, verbose=False)
+    sr.activate(ps, ds, box_l=L_sr)
+    fs, ts = torch.zeros_like(ps), torch.zeros_like(ps)
+    sr.add_short_range_forces(ps, ds, fs, ts, sync=False)
+    sr.synchronize()
+    t0 = time.perf_counter()
+    reps = max(3, steps // 4)
+    for _ in range(reps):
+        sr.add_short_range_forces(ps, ds, fs, ts, sync=False)
+    sr.synchronize()
+    t_sr = (time.perf_counter() - t0) / reps
+    rec = {"workload": f"dipolar P3M: N={n} unit dipoles, density 0.09, box {L:.2f}, "
+                       f"mesh 64, cao 5, accuracy 1e-4",
+           "alpha": round(prm["alpha"], 5), "r_cut": round(prm["r_cut"], 4),
+           "kspace_ms": round(t_k * 1e3, 4),
+           "kspace_mparticles_per_s": round(n / t_k / 1e6, 2),
+           "real_space_allpairs_n": n_sr,
+           "real_space_allpairs_ms": round(t_sr * 1e3, 4)}
+    print(f"[bench] dp3m: {rec}", flush=True)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,10 +502,12 @@ def main():
                     help="skip the device real-space (short-range) timing")
     ap.add_argument("--no-elc", action="store_true",
                     help="skip the ELC slab workload")
+    ap.add_argument("--no-dp3m", action="store_true",
+                    help="skip the dipolar P3M workload")
     ap.add_argument("--accuracy-table", action="store_true",
                     help="also run the 27-cell precision sweep (N=250,000) "
                          "against exact Ewald forces")
-    ap.add_argument("--only", choices=("batched", "serving"),
+    ap.add_argument("--only", choices=("batched", "serving", "elc", "dp3m"),
                     help="run only this workload and print its record "
                          "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -473,8 +526,11 @@ def main():
 
     if args.only:
         dev = torch.device("cuda:0")
-        res = (bench_batched(args.steps, args.warmup, torch, dev) if args.only == "batched"
-               else bench_serving(torch, dev))
+        runs = {"batched": lambda: bench_batched(args.steps, args.warmup, torch, dev),
+                "serving": lambda: bench_serving(torch, dev),
+                "elc": lambda: bench_elc(args.steps, args.warmup, torch, dev),
+                "dp3m": lambda: bench_dp3m(args.steps, args.warmup, torch, dev)}
+        res = runs[args.only]()
         print(json.dumps({"workload": args.only, "record": res}), flush=True)
         return
 
@@ -606,6 +662,10 @@ def main():
     if not args.no_elc:
         elc = bench_elc(args.steps, args.warmup, torch, dev)
 
+    dp3m = None
+    if not args.no_dp3m:
+        dp3m = bench_dp3m(args.steps, args.warmup, torch, dev)
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev)
@@ -664,6 +724,7 @@ def main():
         "accuracy": accuracy,
         "short_range": short_range,
         "elc": elc,
+        "dp3m": dp3m,
         "batched": batched,
         "serving": serving,
     }
