@@ -39,37 +39,39 @@ __device__ __forceinline__ bool dp_weights(DevGeo const &g, double const x[3], i
 }
 
 /** dipole_assign (dp3m.cpp:157-193): mesh_d += w mu_d for the three
- *  components, one thread per particle, device atomics. */
+ *  components with device atomics.  Eight lanes per particle, lane a < P
+ *  takes the stencil points with z offset a. */
 template <int P>
 __global__ __launch_bounds__(kDpBlock) void k_dp_spread(int64_t n, double const *__restrict__ pos,
                                                         double const *__restrict__ dip,
                                                         DevGeo g, C2 *A0, C2 *A1, C2 *A2,
                                                         int *err) {
-  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  if (i >= n) return;
+  int64_t const t = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  int64_t const i = t >> 3;
+  int const a = static_cast<int>(t & 7);
+  if (i >= n || a >= P) return;
   double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
   if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) return;
   double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
   int gi[3];
   double w[3][P];
   if (!dp_weights<P>(g, x, gi, w)) {
-    atomicOr(err, 1);
+    if (a == 0) atomicOr(err, 1);
     return;
   }
+  // lane a takes z offset a: the lanes of a particle hit consecutive
+  // addresses of each z row
   int const M = g.M[0];
-  for (int a = 0; a < P; ++a) {
-    int const ix = wrap_once(gi[0] + a, M);
+  int const iz = wrap_once(gi[2] + a, M);
+  double const wz = w[2][a];
+  for (int r = 0; r < P; ++r) {
+    int const ix = wrap_once(gi[0] + r, M);
     for (int b = 0; b < P; ++b) {
-      int const iy = wrap_once(gi[1] + b, M);
-      double const wab = w[0][a] * w[1][b];
-      int64_t const row = ((int64_t)ix * M + iy) * M;
-      for (int c = 0; c < P; ++c) {
-        int64_t const idx = row + wrap_once(gi[2] + c, M);
-        double const wt = wab * w[2][c];
-        atomicAdd(&A0[idx].x, wt * mu[0]);
-        atomicAdd(&A1[idx].x, wt * mu[1]);
-        atomicAdd(&A2[idx].x, wt * mu[2]);
-      }
+      int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
+      double const wt = w[0][r] * w[1][b] * wz;
+      atomicAdd(&A0[idx].x, wt * mu[0]);
+      atomicAdd(&A1[idx].x, wt * mu[1]);
+      atomicAdd(&A2[idx].x, wt * mu[2]);
     }
   }
 }
@@ -173,7 +175,9 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_back_meshes(C2 const *__restric
 
 /** AssignTorques / AssignForces (dp3m.cpp:195-244): torque -= mu x (pt E),
  *  force_a += pf sum_b mu_b H_ab with E and H interpolated from the real
- *  parts of the back-transformed meshes. */
+ *  parts of the back-transformed meshes.  Eight lanes per particle (lane
+ *  a < P sums the stencil points with z offset a), combined by lane
+ *  shuffles. */
 template <int P>
 __global__ __launch_bounds__(kDpBlock) void k_dp_gather(int64_t n, double const *__restrict__ pos,
                                                         double const *__restrict__ dip,
@@ -181,42 +185,58 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_gather(int64_t n, double const 
                                                         double pt, double pf,
                                                         double *__restrict__ force,
                                                         double *__restrict__ torque) {
-  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  if (i >= n) return;
-  double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
-  if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) return;
-  double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-  int gi[3];
-  double w[3][P];
-  if (!dp_weights<P>(g, x, gi, w)) return;
-  int const M = g.M[0];
-  double E[3] = {0., 0., 0.}, H[6] = {0., 0., 0., 0., 0., 0.};
+  int64_t const t = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  int64_t const i = t >> 3;
+  int const a = static_cast<int>(t & 7);
   bool const wf = force != nullptr;
-  for (int a = 0; a < P; ++a) {
-    int const ix = wrap_once(gi[0] + a, M);
-    for (int b = 0; b < P; ++b) {
-      int const iy = wrap_once(gi[1] + b, M);
-      double const wab = w[0][a] * w[1][b];
-      int64_t const row = ((int64_t)ix * M + iy) * M;
-      for (int c = 0; c < P; ++c) {
-        int64_t const idx = row + wrap_once(gi[2] + c, M);
-        double const wt = wab * w[2][c];
+  double v[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.}; // E[3], H[6]
+  double mu[3] = {0., 0., 0.};
+  bool act = false;
+  if (i < n) {
+    mu[0] = dip[3 * i];
+    mu[1] = dip[3 * i + 1];
+    mu[2] = dip[3 * i + 2];
+    act = (mu[0] != 0. || mu[1] != 0. || mu[2] != 0.);
+  }
+  if (act && a < P) {
+    double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int gi[3];
+    double w[3][P];
+    if (dp_weights<P>(g, x, gi, w)) {
+      // lane a takes z offset a (consecutive addresses across the lanes)
+      int const M = g.M[0];
+      int const iz = wrap_once(gi[2] + a, M);
+      double const wz = w[2][a];
+      for (int r = 0; r < P; ++r) {
+        int const ix = wrap_once(gi[0] + r, M);
+        for (int b = 0; b < P; ++b) {
+          int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
+          double const wt = w[0][r] * w[1][b] * wz;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) E[d] += wt * T[d][idx].x;
-        if (wf)
+          for (int d = 0; d < 3; ++d) v[d] += wt * T[d][idx].x;
+          if (wf)
 #pragma unroll
-          for (int d = 0; d < 6; ++d) H[d] += wt * F[d][idx].x;
+            for (int d = 0; d < 6; ++d) v[3 + d] += wt * F[d][idx].x;
+        }
       }
     }
   }
+  // sum over the eight lanes of the particle (all lanes take part)
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    v[k] += __shfl_xor(v[k], 1, 64);
+    v[k] += __shfl_xor(v[k], 2, 64);
+    v[k] += __shfl_xor(v[k], 4, 64);
+  }
+  if (!act || a != 0) return;
   if (torque) {
-    double const e0 = pt * E[0], e1 = pt * E[1], e2 = pt * E[2];
+    double const e0 = pt * v[0], e1 = pt * v[1], e2 = pt * v[2];
     torque[3 * i] -= mu[1] * e2 - mu[2] * e1;
     torque[3 * i + 1] -= mu[2] * e0 - mu[0] * e2;
     torque[3 * i + 2] -= mu[0] * e1 - mu[1] * e0;
   }
   if (wf) {
-    // H: xx xy xz yy yz zz
+    double const *H = v + 3; // xx xy xz yy yz zz
     force[3 * i] += pf * (mu[0] * H[0] + mu[1] * H[1] + mu[2] * H[2]);
     force[3 * i + 1] += pf * (mu[0] * H[1] + mu[1] * H[3] + mu[2] * H[4]);
     force[3 * i + 2] += pf * (mu[0] * H[2] + mu[1] * H[4] + mu[2] * H[5]);
@@ -640,15 +660,20 @@ void DipolarSolver::configure(DipolarParams const &p) {
 }
 
 namespace {
-template <int P> void launch_spread(hipStream_t st, int nb, int64_t n, double const *pos,
+int lanes_grid(int64_t n) { return static_cast<int>((8 * n + kDpBlock - 1) / kDpBlock); }
+
+template <int P> void launch_spread(hipStream_t st, int, int64_t n, double const *pos,
                                     double const *dip, DevGeo const &g, C2 *const *mesh, int *err) {
-  k_dp_spread<P><<<nb, kDpBlock, 0, st>>>(n, pos, dip, g, mesh[0], mesh[1], mesh[2], err);
+  k_dp_spread<P><<<lanes_grid(n), kDpBlock, 0, st>>>(n, pos, dip, g, mesh[0], mesh[1], mesh[2],
+                                                     err);
 }
 template <int P> void launch_gather(hipStream_t st, int nb, int64_t n, double const *pos,
                                     double const *dip, DevGeo const &g, C2 *const *T,
                                     C2 *const *F, double pt, double pf, double *force,
                                     double *torque) {
-  k_dp_gather<P><<<nb, kDpBlock, 0, st>>>(n, pos, dip, g, T, F, pt, pf, force, torque);
+  (void)nb;
+  k_dp_gather<P><<<lanes_grid(n), kDpBlock, 0, st>>>(n, pos, dip, g, T, F, pt, pf, force,
+                                                     torque);
 }
 } // namespace
 
