@@ -2,6 +2,7 @@
 #include "p3m_dipolar.hpp"
 #include "p3m_batch.hpp"
 #include "p3m_kernels.hpp"
+#include "p3m_short_range.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -496,8 +497,8 @@ struct DipolarSolver::Impl {
   double *part = nullptr;
   size_t cap_part = 0;
   int *err = nullptr;
-  double self_sum = -1.; // sum_k G_e U2 |d_op|^2 (lazy, as the reference)
-  double *h_pin = nullptr;
+  double self_sum = -1.; // sum_k G_e U2 |d_op|^2
+  std::unique_ptr<ShortRange> sr; // cell lists of the real space
 };
 
 DipolarSolver::DipolarSolver(int device) : m(std::make_unique<Impl>()) {
@@ -805,6 +806,17 @@ void DipolarSolver::short_range(int64_t n, double const *pos, double const *dip,
     m->cap_part = static_cast<size_t>(nb);
   }
   int const exact = p.accuracy > 5e-6 ? 0 : 1;
+  // cell lists when the box holds three cells of edge r_cut, else all pairs
+  if (!m->sr) m->sr = std::make_unique<ShortRange>(m->device);
+  ShortRangeParams sp;
+  for (int d = 0; d < 3; ++d) sp.box[d] = p.box;
+  sp.prefactor = p.prefactor;
+  sp.alpha = p.alpha;
+  sp.r_cut = p.r_cut;
+  if (m->sr->compute_dipolar(st, sp, exact != 0, n, pos, dip, force, torque, energy)) {
+    if (sync && !energy) hip_check(hipStreamSynchronize(st), "sync dp3m pairs");
+    return;
+  }
   if (energy)
     k_dp_pairs<true><<<nb, kDpBlock, 0, st>>>(n, pos, dip, p.box, p.prefactor, p.alpha, p.r_cut,
                                               exact, force, torque, m->part);

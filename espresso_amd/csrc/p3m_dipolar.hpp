@@ -66,7 +66,8 @@ public:
   void solve(int64_t n, double const *pos, double const *dip, double *force, double *torque,
              double *energy, bool sync);
   /** Real-space pair forces, torques and (energy != null) energy of all
-   *  pairs within r_cut (minimum image), all pairs on the device. */
+   *  pairs within r_cut (minimum image): cell lists of edge r_cut/2 or
+   *  r_cut, all pairs in boxes narrower than three cells. */
   void short_range(int64_t n, double const *pos, double const *dip, double *force,
                    double *torque, double *energy, bool sync);
 

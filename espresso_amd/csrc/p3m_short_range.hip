@@ -348,6 +348,142 @@ __global__ __launch_bounds__(NT) void k_sr_allpairs(int nq, CellGrid g,
   }
 }
 
+/** |mu|^2 of every particle (the dipolar binning key: zero dipoles take no
+ *  part, dp3m.hpp:162). */
+__global__ __launch_bounds__(256) void k_sr_mu2(int64_t n, double const *__restrict__ dip,
+                                                double *__restrict__ mu2) {
+  int64_t const i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double const a = dip[3 * i], b = dip[3 * i + 1], c = dip[3 * i + 2];
+  mu2[i] = a * a + b * b + c * c;
+}
+
+/** Dipoles in cell order. */
+__global__ __launch_bounds__(256) void k_sr_sort_dip(int64_t n, int const *__restrict__ total,
+                                                     int const *__restrict__ sidx,
+                                                     double const *__restrict__ dip,
+                                                     double4 *__restrict__ sdip) {
+  int64_t const s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= n || s >= *total) return;
+  int64_t const i = sidx[s];
+  sdip[s] = double4{dip[3 * i], dip[3 * i + 1], dip[3 * i + 2], 0.};
+}
+
+/** Dipolar pair term (DipolarP3M::pair_force / pair_energy,
+ *  dp3m.hpp:160-235) of particle i with partner j at d = r_i - r_j, r2 =
+ *  |d|^2 within the cutoff: force and torque on i, pair energy with EV. */
+template <bool EV>
+__device__ __forceinline__ void dip_pair(double dx, double dy, double dz, double r2,
+                                         double4 const &mi, double4 const &mj, double prefactor,
+                                         double alpha, int exact, double (&f)[3],
+                                         double (&t)[3], double &e) {
+  double const r = sqrt(r2);
+  double const ad = alpha * r;
+  double const ex = exp(-ad * ad);
+  double const r2i = 1. / r2;
+  double const coeff = 2. * alpha * std::numbers::inv_sqrtpi;
+  double B;
+  if (exact) {
+    B = (erfc(ad) / r + coeff * ex) * r2i;
+  } else {
+    double const tt = 1. / (1. + 0.3275911 * ad);
+    double const ep =
+        tt * (0.254829592 +
+              tt * (-0.284496736 + tt * (1.421413741 + tt * (-1.453152027 + tt * 1.061405429))));
+    B = (ep / r + coeff) * ex * r2i;
+  }
+  double const alpsq = alpha * alpha;
+  double const common = alpsq * coeff * ex;
+  double const Cr = r2i * (3. * B + 2. * common);
+  double const Dr = r2i * (5. * Cr + 4. * common * alpsq);
+  double const mimj = mi.x * mj.x + mi.y * mj.y + mi.z * mj.z;
+  double const mir = mi.x * dx + mi.y * dy + mi.z * dz;
+  double const mjr = mj.x * dx + mj.y * dy + mj.z * dz;
+  double const cd = mimj * Cr - mir * mjr * Dr;
+  f[0] += prefactor * (cd * dx + (mi.x * mjr + mj.x * mir) * Cr);
+  f[1] += prefactor * (cd * dy + (mi.y * mjr + mj.y * mir) * Cr);
+  f[2] += prefactor * (cd * dz + (mi.z * mjr + mj.z * mir) * Cr);
+  double const c1 = mjr * Cr;
+  t[0] += prefactor * (-(mi.y * mj.z - mi.z * mj.y) * B + (mi.y * dz - mi.z * dy) * c1);
+  t[1] += prefactor * (-(mi.z * mj.x - mi.x * mj.z) * B + (mi.z * dx - mi.x * dz) * c1);
+  t[2] += prefactor * (-(mi.x * mj.y - mi.y * mj.x) * B + (mi.x * dy - mi.y * dx) * c1);
+  if constexpr (EV) e += prefactor * (mimj * B - mir * mjr * Cr);
+}
+
+/** Dipolar counterpart of k_sr_cells: the cell's particles against the
+ *  (2 NSUB + 1)^3 neighbour cells staged through LDS (positions with the
+ *  periodic shift, dipoles), both orders of a pair. */
+template <int NSUB, int NT, bool EV>
+__global__ __launch_bounds__(NT) void k_sr_cells_dip(CellGrid g, int const *__restrict__ offsets,
+                                                     double4 const *__restrict__ spos,
+                                                     double4 const *__restrict__ sdip,
+                                                     int const *__restrict__ sidx,
+                                                     double prefactor, double alpha, double rc2,
+                                                     int exact, double *__restrict__ force,
+                                                     double *__restrict__ torque,
+                                                     double *partials) {
+  __shared__ double4 tp[NT], tm[NT];
+  __shared__ double red[16];
+  constexpr int W = 2 * NSUB + 1;
+  int const cell = blockIdx.x;
+  int const cz = cell % g.nc[2], cy = (cell / g.nc[2]) % g.nc[1], cx = cell / (g.nc[2] * g.nc[1]);
+  int const beg = offsets[cell], end = offsets[cell + 1];
+  double e = 0.;
+  for (int hb = beg; hb < end; hb += NT) {
+    int const i = hb + static_cast<int>(threadIdx.x);
+    bool const has = i < end;
+    double4 const pi = has ? spos[i] : double4{0., 0., 0., 0.};
+    double4 const mi = has ? sdip[i] : double4{0., 0., 0., 0.};
+    double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+    for (int o = 0; o < W * W * W; ++o) {
+      int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
+      double sx = 0., sy = 0., sz = 0.;
+      if (nx < 0) { nx += g.nc[0]; sx = -g.L[0]; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = g.L[0]; }
+      if (ny < 0) { ny += g.nc[1]; sy = -g.L[1]; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = g.L[1]; }
+      if (nz < 0) { nz += g.nc[2]; sz = -g.L[2]; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = g.L[2]; }
+      int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
+      int const jb = offsets[nb], je = offsets[nb + 1];
+      for (int tb = jb; tb < je; tb += NT) {
+        __syncthreads();
+        int const j = tb + static_cast<int>(threadIdx.x);
+        if (j < je) {
+          double4 const pj = spos[j];
+          tp[threadIdx.x] = double4{pj.x + sx, pj.y + sy, pj.z + sz, 0.};
+          tm[threadIdx.x] = sdip[j];
+        }
+        __syncthreads();
+        if (has) {
+          int const cnt = min(NT, je - tb);
+          for (int k = 0; k < cnt; ++k) {
+            double const dx = pi.x - tp[k].x, dy = pi.y - tp[k].y, dz = pi.z - tp[k].z;
+            double const r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < rc2 && r2 > 0.)
+              dip_pair<EV>(dx, dy, dz, r2, mi, tm[k], prefactor, alpha, exact, f, t, e);
+          }
+        }
+      }
+    }
+    if (has) {
+      int64_t const o = 3 * static_cast<int64_t>(sidx[i]);
+      if (force) {
+        force[o] += f[0];
+        force[o + 1] += f[1];
+        force[o + 2] += f[2];
+      }
+      if (torque) {
+        torque[o] += t[0];
+        torque[o + 1] += t[1];
+        torque[o + 2] += t[2];
+      }
+    }
+  }
+  if constexpr (EV) {
+    double v[1] = {e};
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x * 8] = v[0];
+  }
+}
+
 template <typename T> void sr_grow(T *&p, size_t &cap, size_t need, char const *what) {
   if (need <= cap) return;
   if (p) (void)hipFree(p);
@@ -361,6 +497,9 @@ template <typename T> void sr_grow(T *&p, size_t &cap, size_t need, char const *
 struct ShortRange::Impl {
   int device = 0;
   int nc[3] = {0, 0, 0};
+  double *mu2 = nullptr;
+  double4 *sdip = nullptr;
+  size_t cap_mu2 = 0, cap_sdip = 0;
   int *counts = nullptr, *offsets = nullptr, *p_cell = nullptr, *p_rank = nullptr,
       *sidx = nullptr;
   double4 *spos = nullptr;
@@ -375,7 +514,8 @@ ShortRange::~ShortRange() {
   for (void *p : {static_cast<void *>(m->counts), static_cast<void *>(m->offsets),
                   static_cast<void *>(m->p_cell), static_cast<void *>(m->p_rank),
                   static_cast<void *>(m->sidx), static_cast<void *>(m->spos),
-                  static_cast<void *>(m->partials)})
+                  static_cast<void *>(m->partials), static_cast<void *>(m->mu2),
+                  static_cast<void *>(m->sdip)})
     if (p) (void)hipFree(p);
   delete m;
 }
@@ -477,6 +617,82 @@ void ShortRange::compute(hipStream_t st, ShortRangeParams const &p, int64_t n, d
   double const v[3][3] = {{s[1], s[2], s[3]}, {s[2], s[4], s[5]}, {s[3], s[5], s[6]}};
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) out->virial[3 * a + b] = 0.5 * v[a][b];
+}
+
+bool ShortRange::compute_dipolar(hipStream_t st, ShortRangeParams const &p, bool exact_erfc,
+                                 int64_t n, double const *pos, double const *dip, double *force,
+                                 double *torque, double *energy) {
+  if (energy) *energy = 0.;
+  if (!(p.r_cut > 0.)) throw std::invalid_argument("P3M: short range needs r_cut > 0");
+  if (n > INT32_MAX) throw std::invalid_argument("P3M: too many particles for the short range");
+  int nsub = (p.subdivision == 1) ? 1 : 2;
+  int nc[3];
+  auto fit = [&](int s) {
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = std::min(kSRMaxCells, static_cast<int>(std::floor(p.box[d] * s / p.r_cut)));
+      if (nc[d] < 2 * s + 1) ok = false;
+    }
+    return ok;
+  };
+  if (!fit(nsub)) {
+    nsub = 1;
+    if (!fit(1)) return false;
+  }
+  if (n == 0) return true;
+  hip_check(hipSetDevice(m->device), "hipSetDevice");
+  CellGrid g;
+  for (int d = 0; d < 3; ++d) {
+    m->nc[d] = g.nc[d] = nc[d];
+    g.L[d] = p.box[d];
+    g.inv_cell[d] = nc[d] / p.box[d];
+  }
+  int const ncells = nc[0] * nc[1] * nc[2];
+  sr_grow(m->counts, m->cap_cells, static_cast<size_t>(ncells), "hipMalloc cell counts");
+  sr_grow(m->offsets, m->cap_off, static_cast<size_t>(ncells) + 1, "hipMalloc cell offsets");
+  sr_grow(m->p_cell, m->cap_pc, static_cast<size_t>(n), "hipMalloc cell ids");
+  sr_grow(m->p_rank, m->cap_pr, static_cast<size_t>(n), "hipMalloc cell ranks");
+  sr_grow(m->sidx, m->cap_si, static_cast<size_t>(n), "hipMalloc cell order");
+  sr_grow(m->spos, m->cap_sp, static_cast<size_t>(n), "hipMalloc cell positions");
+  sr_grow(m->mu2, m->cap_mu2, static_cast<size_t>(n), "hipMalloc dipole norms");
+  sr_grow(m->sdip, m->cap_sdip, static_cast<size_t>(n), "hipMalloc cell dipoles");
+  if (energy) sr_grow(m->partials, m->cap_part, static_cast<size_t>(ncells) * 8, "hipMalloc partials");
+  int const nb = static_cast<int>((n + 255) / 256);
+  k_sr_mu2<<<nb, 256, 0, st>>>(n, dip, m->mu2);
+  k_sr_fill<<<std::min((ncells + 255) / 256, 1024), 256, 0, st>>>(m->counts, ncells);
+  k_sr_bin<<<nb, 256, 0, st>>>(n, pos, m->mu2, g, m->counts, m->p_cell, m->p_rank);
+  k_sr_scan<<<1, 1024, 0, st>>>(m->counts, ncells, m->offsets);
+  k_sr_scatter<<<nb, 256, 0, st>>>(n, pos, m->mu2, g, m->p_cell, m->p_rank, m->offsets, m->spos,
+                                   m->sidx);
+  k_sr_sort_dip<<<nb, 256, 0, st>>>(n, m->offsets + ncells, m->sidx, dip, m->sdip);
+  double const rc2 = p.r_cut * p.r_cut;
+  int const ex = exact_erfc ? 1 : 0;
+  auto launch = [&](auto EVc) {
+    constexpr bool EV = decltype(EVc)::value;
+    if (nsub == 2)
+      k_sr_cells_dip<2, 64, EV><<<ncells, 64, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
+                                                       p.prefactor, p.alpha, rc2, ex, force,
+                                                       torque, m->partials);
+    else
+      k_sr_cells_dip<1, 256, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip,
+                                                         m->sidx, p.prefactor, p.alpha, rc2, ex,
+                                                         force, torque, m->partials);
+  };
+  if (energy)
+    launch(std::true_type{});
+  else
+    launch(std::false_type{});
+  hip_check(hipGetLastError(), "dipolar short-range kernels");
+  if (!energy) return true;
+  std::vector<double> part(static_cast<size_t>(ncells) * 8);
+  hip_check(hipMemcpyAsync(part.data(), m->partials, sizeof(double) * part.size(),
+                           hipMemcpyDeviceToHost, st),
+            "copy partials");
+  hip_check(hipStreamSynchronize(st), "sync dipolar short range");
+  double s = 0.;
+  for (int c = 0; c < ncells; ++c) s += part[static_cast<size_t>(c) * 8];
+  *energy = 0.5 * s;
+  return true;
 }
 
 } // namespace espresso_amd

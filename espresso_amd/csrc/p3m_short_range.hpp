@@ -54,6 +54,17 @@ public:
   /** Cells per dimension of the last call (1 1 1: all pairs). */
   void cells(int out[3]) const;
 
+  /** Dipolar real space (DipolarP3M::pair_force / pair_energy,
+   *  dp3m.hpp:160-235) over the same cell lists, for particles with a
+   *  non-zero dipole: adds forces and torques ([n][3] device arrays, either
+   *  may be null) and, with energy != null, returns the pair energy (the
+   *  call then synchronises).  exact_erfc selects libm erfc for B_r (the
+   *  reference's choice at accuracy <= 5e-6).  Returns false, doing
+   *  nothing, when the box is narrower than three cells of edge r_cut. */
+  bool compute_dipolar(hipStream_t st, ShortRangeParams const &p, bool exact_erfc, int64_t n,
+                       double const *pos, double const *dip, double *force, double *torque,
+                       double *energy);
+
 private:
   struct Impl;
   Impl *m;
