@@ -24,6 +24,7 @@ P3M_ENERGY = 2
 P3M_PRESSURE = 4
 P3M_DEVICE_POINTERS = 8
 P3M_ASYNC = 16
+P3M_DEVICE_RESULTS = 32
 
 P3M_PREC_FP64 = 0
 P3M_PREC_FP32 = 1

@@ -2230,6 +2230,30 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (ev) (void)hipEventRecord(ev[6], st);
 }
 
+/** finalize_results on the device for one system: the k-space energy
+ *  (p3m.cpp:551-589) and virial (:393-449) from the per-system sums, written
+ *  in stream order (no host synchronisation). */
+__global__ void k_finalize_device(double const *__restrict__ h, double volume, double alpha,
+                                  double epsilon, double prefactor, double *energy,
+                                  double *virial) {
+  if (threadIdx.x != 0) return;
+  if (energy) {
+    double e = h[8] / (2. * volume);
+    e -= h[2] * alpha * std::numbers::inv_sqrtpi;
+    e -= (h[1] * h[1]) * std::numbers::pi / (2. * volume * alpha * alpha);
+    if (epsilon != 0.) {
+      double const pref = 4. * std::numbers::pi / volume / (2. * epsilon + 1.);
+      e += pref * (h[3] * h[3] + h[4] * h[4] + h[5] * h[5]);
+    }
+    *energy = (h[0] > 0.) ? e * prefactor : 0.;
+  }
+  if (virial) {
+    double const v[9] = {h[9] + h[15], h[10], h[11], h[10], h[12] + h[15], h[13], h[11], h[13],
+                         h[14] + h[15]};
+    for (int k = 0; k < 9; ++k) virial[k] = v[k] * prefactor / (2. * volume);
+  }
+}
+
 void finalize_results(BatchSolver::Impl *m, Lane const &ln, std::vector<int> const &sel,
                       std::vector<SolveResult> *out) {
   out->assign(sel.size(), SolveResult{});
@@ -2310,6 +2334,18 @@ void BatchSolver::solve_on(int lane, std::vector<int> const &sel, ParticleArrays
     last_ms[st] = t;
   }
   if (out) finalize_results(m, ln, sel, out);
+}
+
+void BatchSolver::device_results(int lane, int s, double *d_energy, double *d_virial) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  Lane &ln = *m->lanes.at(lane);
+  HostSys const &hs = m->sys.at(s);
+  if (!ln.sums || 16 * (s + 1) > ln.sums_cap)
+    throw std::runtime_error("P3M: device results before a solve of this system");
+  double const volume = hs.box_l[0] * hs.box_l[1] * hs.box_l[2];
+  k_finalize_device<<<1, 64, 0, ln.stream>>>(ln.sums + 16 * s, volume, hs.params.alpha,
+                                             hs.params.epsilon, hs.prefactor, d_energy, d_virial);
+  hip_check(hipGetLastError(), "P3M device results");
 }
 
 void BatchSolver::prepare_on(int lane, std::vector<int> const &sel, double *pos, double *force,

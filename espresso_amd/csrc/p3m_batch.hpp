@@ -124,6 +124,10 @@ public:
    *  solve is taken from that device block of scratch_size >=
    *  working_bytes(sel) bytes (it must stay untouched until the solve ends)
    *  instead of the lane's own memory. */
+  /** Energy and virial of system s from the last solve on the lane,
+   *  computed on the device into d_energy (1 double) and d_virial (9),
+   *  either may be null, in stream order. */
+  void device_results(int lane, int s, double *d_energy, double *d_virial);
   void solve_on(int lane, std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags,
                 bool sync, std::vector<SolveResult> *out, void *scratch = nullptr,
                 size_t scratch_size = 0);

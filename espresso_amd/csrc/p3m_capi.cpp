@@ -524,6 +524,11 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
     bool const async = (flags & P3M_ASYNC) != 0;
     if (async && !(flags & P3M_DEVICE_POINTERS))
       throw std::invalid_argument("P3M: P3M_ASYNC requires P3M_DEVICE_POINTERS");
+    bool const dev_res = (flags & P3M_DEVICE_RESULTS) != 0;
+    if (dev_res && !(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: P3M_DEVICE_RESULTS requires P3M_DEVICE_POINTERS");
+    if (dev_res && ctx->elc_on)
+      throw std::invalid_argument("P3M: P3M_DEVICE_RESULTS is not available with ELC");
     DevArgs const d = stage_inputs(ctx, n, pos, q, pos_unfolded, force, flags);
     SolveResult r;
     // one P3M pass over (nn, pp, qq): forces with the context's convention
@@ -552,6 +557,29 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
       if (!ctx->elc_solver) ctx->elc_solver = std::make_unique<ElcSolver>(ctx->device);
     }
     hipStream_t const st = ctx->solver->stream();
+    if (dev_res) {
+      // energy and virial stay on the device (an MD loop's NpT virial,
+      // p3m.cpp:580-583, without a host round trip)
+      unsigned const red = flags & (P3M_ENERGY | P3M_PRESSURE);
+      if (ctx->force_conv == P3M_CONV_GPU && (flags & P3M_FORCES)) {
+        // forces with the GPU rule, energy and virial from the CPU mesh
+        ctx->solver->set_convention(1);
+        ctx->solver->solve(n, d.pos, d.q, d.upos, d.force, flags & ~red, nullptr, false);
+        if (red) {
+          ctx->solver->set_convention(0);
+          ctx->solver->solve(n, d.pos, d.q, d.upos, nullptr, flags & ~P3M_FORCES, nullptr,
+                             false);
+        }
+      } else {
+        ctx->solver->set_convention(0);
+        ctx->solver->solve(n, d.pos, d.q, d.upos, (flags & P3M_FORCES) ? d.force : nullptr, flags,
+                           nullptr, false);
+      }
+      ctx->solver->device_results((flags & P3M_ENERGY) ? energy : nullptr,
+                                  (flags & P3M_PRESSURE) ? virial : nullptr);
+      if (!async) hip_check(hipStreamSynchronize(ctx->solver->stream()), "P3M solve");
+      return;
+    }
     if (!diel) {
       p3m_pass(n, d.pos, d.q, d.upos, d.force, flags, &r);
     } else {

@@ -53,6 +53,11 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
   m_batch = std::move(b);
 }
 
+void Solver::device_results(double *d_energy, double *d_virial) {
+  if (!m_batch) throw std::runtime_error("P3M solver used before configure()");
+  m_batch->device_results(0, 0, d_energy, d_virial);
+}
+
 void Solver::set_convention(int conv) {
   if (conv != 0 && conv != 1) throw std::invalid_argument("P3M: convention must be 0 or 1");
   m_conv = conv;

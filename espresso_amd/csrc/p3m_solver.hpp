@@ -43,6 +43,10 @@ public:
   void solve(int64_t n, double const *d_pos, double const *d_q, double const *d_upos,
              double *d_force, unsigned flags, SolveResult *res, bool sync = true);
 
+  /** Energy and virial of the last solve written to device memory in
+   *  stream order (BatchSolver::device_results). */
+  void device_results(double *d_energy, double *d_virial);
+
   /** Charge mesh after spreading ([Mx][My][Mz], device-to-host copy). */
   void debug_charge_mesh(int64_t n, double const *d_pos, double const *d_q,
                          std::vector<double> &out);

@@ -359,6 +359,37 @@ class _P3MBase:
             self._ctx)
         return e.value, np.array(v[:]).reshape(3, 3)
 
+    def solve_to_device(self, pos, q, force=None, energy=None, virial=None, sync=False):
+        """One solve on device tensors whose energy (1-element float64
+        tensor) and virial (9 elements) are written on the device in stream
+        order, without a host round trip (P3M_DEVICE_RESULTS): an MD loop's
+        per-step energy or NpT virial (p3m.cpp:580-583).  Without sync the
+        call only enqueues; ``synchronize()`` waits."""
+        self._require_active()
+        a = _Arrays(pos, q, force)
+        if not a.device:
+            raise TypeError("solve_to_device takes device tensors")
+        for t, name in ((energy, "energy"), (virial, "virial")):
+            if t is not None and not (_is_device_tensor(t) and t.is_contiguous()
+                                      and str(t.dtype) == "torch.float64"):
+                raise TypeError(f"'{name}' must be a contiguous float64 device tensor")
+        a.order_after_torch(self._ctx)
+        flags = a.flags | _lib.P3M_DEVICE_RESULTS
+        if force is not None:
+            flags |= _lib.P3M_FORCES
+        if energy is not None:
+            flags |= _lib.P3M_ENERGY
+        if virial is not None:
+            flags |= _lib.P3M_PRESSURE
+        if not sync:
+            flags |= _lib.P3M_ASYNC
+        ep = C.c_void_p(energy.data_ptr()) if energy is not None else None
+        vp = C.c_void_p(virial.data_ptr()) if virial is not None else None
+        _lib.raise_for(_lib.load().p3m_solve(
+            self._ctx, a.n, a.pos, a.q, a.upos, a.force, flags,
+            C.cast(ep, C.POINTER(C.c_double)) if ep else None,
+            C.cast(vp, C.POINTER(C.c_double)) if vp else None), self._ctx)
+
     def add_short_range_forces(self, pos, q, force, energy=False, pressure=False,
                                sync=True):
         """force += real-space pair forces of all charged pairs closer than

@@ -50,6 +50,10 @@ extern "C" {
 #define P3M_DEVICE_POINTERS 8u
 #define P3M_ASYNC 16u /* enqueue only: no host sync, no energy/virial output;
                          requires P3M_DEVICE_POINTERS */
+#define P3M_DEVICE_RESULTS 32u /* energy / virial arguments are device arrays
+                         (1 and 9 doubles) written in stream order, also
+                         with P3M_ASYNC; requires P3M_DEVICE_POINTERS, not
+                         with ELC */
 
 /* mesh precision */
 #define P3M_PREC_FP64 0 /* plain FP64 solver (drop-in for P3M, double) */
