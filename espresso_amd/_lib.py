@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "p3m_dipolar_last_error", "p3m_dipolar_set_params", "p3m_dipolar_get_params",
     "p3m_dipolar_activate", "p3m_dipolar_solve", "p3m_dipolar_short_range",
     "p3m_dipolar_get_stream", "p3m_dipolar_synchronize",
+    "p3m_dipolar_set_short_range_subdivision",
     "p3m_synchronize", "p3m_get_stream", "p3m_last_error",
     "p3m_last_global_error",
     "p3m_batch_create", "p3m_batch_destroy", "p3m_batch_add_system",
@@ -182,6 +183,7 @@ def load():
     L.p3m_dipolar_short_range.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp]
     L.p3m_dipolar_get_stream.argtypes = [vp, C.POINTER(vp)]
     L.p3m_dipolar_synchronize.argtypes = [vp]
+    L.p3m_dipolar_set_short_range_subdivision.argtypes = [vp, C.c_int]
     L.p3m_last_error.argtypes = [vp]
     L.p3m_last_error.restype = C.c_char_p
     L.p3m_last_global_error.argtypes = []

@@ -1532,6 +1532,14 @@ int p3m_dipolar_short_range(p3m_dipolar *c, int64_t n, double const *pos, double
   });
 }
 
+int p3m_dipolar_set_short_range_subdivision(p3m_dipolar *c, int cells_per_cutoff) {
+  if (!c) return P3M_ERR_INVALID;
+  return guarded(&c->last_error, [&] {
+    if (!c->solver) throw std::runtime_error("DipolarP3M: host-only context cannot solve");
+    c->solver->set_short_range_subdivision(cells_per_cutoff);
+  });
+}
+
 int p3m_dipolar_get_stream(p3m_dipolar const *c, void **stream) {
   if (!c || !stream || !c->solver) return P3M_ERR_INVALID;
   *stream = reinterpret_cast<void *>(c->solver->stream());

@@ -72,6 +72,8 @@ public:
                    double *torque, double *energy, bool sync);
 
   hipStream_t stream() const;
+  /** Cells per r_cut of the real space: 0 = by occupancy, 1 or 2. */
+  void set_short_range_subdivision(int s);
 
 private:
   struct Impl;

@@ -234,6 +234,11 @@ class DipolarP3M:
             dipolar=self._ctx)
         return e.value if energy else None
 
+    def set_short_range_subdivision(self, cells_per_cutoff):
+        """Cells per r_cut of the real space: 0 (by occupancy), 1 or 2."""
+        _lib.raise_for(_lib.load().p3m_dipolar_set_short_range_subdivision(
+            self._ctx, int(cells_per_cutoff)), dipolar=self._ctx)
+
     def synchronize(self):
         _lib.raise_for(_lib.load().p3m_dipolar_synchronize(self._ctx), dipolar=self._ctx)
 

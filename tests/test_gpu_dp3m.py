@@ -50,6 +50,27 @@ def test_kspace_matches_oracle(M, cao, epsilon):
     assert abs(e - er) <= 1e-10 * abs(er)
 
 
+@pytest.mark.parametrize("subdiv", [0, 1, 2])
+def test_short_range_cell_lists_match_oracle(subdiv):
+    """Cell lists of edge r_cut (subdivision 1) and r_cut/2 (2), and the
+    occupancy choice (0), against the oracle's all-pairs sum."""
+    from espresso_amd.magnetostatics import DipolarP3M
+    from oracle import dp3m as D
+    L = 20.
+    pos, dip = _system(3000, L, 8)
+    dp = DipolarP3M(prefactor=1.3, accuracy=1e-4, mesh=16, cao=4, alpha=0.9, r_cut=3.5,
+                    tune=False)
+    dp.activate(pos, dip, box_l=L)
+    dp.set_short_range_subdivision(subdiv)
+    f, t = np.zeros((3000, 3)), np.zeros((3000, 3))
+    e = dp.add_short_range_forces(pos, dip, f, t, energy=True)
+    fr, tr, er = D.pair_terms(pos - L * np.floor(pos / L), dip, L, 1.3, 0.9, 3.5, 1e-4)
+    assert S.rel_rms(f, fr) <= 1e-12 and S.rel_rms(t, tr) <= 1e-12
+    assert abs(e - er) <= 1e-12 * abs(er)
+    with pytest.raises(ValueError, match="subdivision must be 0, 1 or 2"):
+        dp.set_short_range_subdivision(3)
+
+
 @pytest.mark.parametrize("accuracy", [1e-4, 1e-7])
 def test_short_range_matches_oracle(accuracy):
     """A&S erfc for accuracy > 5e-6, libm erfc below (dp3m.hpp:176-178)."""
