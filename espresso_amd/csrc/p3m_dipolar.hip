@@ -5,6 +5,7 @@
 #include "p3m_short_range.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <numbers>
 #include <stdexcept>
@@ -18,8 +19,17 @@ using C2 = double2;
 
 constexpr int kDpBlock = 256;
 constexpr int kDpLines = 8; // FFT lines per block
+constexpr int kDpGatherBlock = 512;
+// dynamic LDS of the gather: gfx950's 160 KiB per workgroup less a margin
+// for the kernel's static LDS
+constexpr int kDpGatherLds = 160 * 1024 - 256;
 
 __global__ void k_dp_clear(int *flag) { *flag = 0; }
+
+__global__ __launch_bounds__(kDpBlock) void k_dp_izero(int *a, int n) {
+  int const i = blockIdx.x * kDpBlock + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
 
 __global__ __launch_bounds__(kDpBlock) void k_dp_zero(C2 *a, int64_t n) {
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
@@ -39,41 +49,105 @@ __device__ __forceinline__ bool dp_weights(DevGeo const &g, double const x[3], i
   return ok;
 }
 
-/** dipole_assign (dp3m.cpp:157-193): mesh_d += w mu_d for the three
- *  components with device atomics.  Eight lanes per particle, lane a < P
- *  takes the stencil points with z offset a. */
+/** dipole_assign (dp3m.cpp:157-193) over particles sorted into buckets of
+ *  mesh cells: one
+ *  block per bucket accumulates its particles in an LDS tile of E^3 points
+ *  per component (LDS atomics) and adds the tile to the meshes with one
+ *  device atomic per point.  The tile origin is the lowest stencil base of
+ *  the bucket's particles (taken modulo M relative to its first particle);
+ *  a particle whose stencil leaves the tile uses device atomics directly,
+ *  so every placement is correct, and tile points wrap modulo M. */
 template <int P>
-__global__ __launch_bounds__(kDpBlock) void k_dp_spread(int64_t n, double const *__restrict__ pos,
-                                                        double const *__restrict__ dip,
-                                                        DevGeo g, C2 *A0, C2 *A1, C2 *A2,
-                                                        int *err) {
-  int64_t const t = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  int64_t const i = t >> 3;
-  int const a = static_cast<int>(t & 7);
-  if (i >= n || a >= P) return;
-  double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
-  if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) return;
-  double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-  int gi[3];
-  double w[3][P];
-  if (!dp_weights<P>(g, x, gi, w)) {
-    if (a == 0) atomicOr(err, 1);
-    return;
-  }
-  // lane a takes z offset a: the lanes of a particle hit consecutive
-  // addresses of each z row
+__global__ __launch_bounds__(kDpBlock) void k_dp_spread_tile(
+    int const *__restrict__ offsets, int const *__restrict__ counts,
+    double const *__restrict__ pos, double const *__restrict__ dip, DevGeo g, int E, C2 *A0,
+    C2 *A1, C2 *A2, int *err) {
+  extern __shared__ double tile[]; // [3][E][E][E]
+  __shared__ int org[3];
+  int const beg = offsets[blockIdx.x], cnt = counts[blockIdx.x];
+  if (cnt == 0) return;
   int const M = g.M[0];
-  int const iz = wrap_once(gi[2] + a, M);
-  double const wz = w[2][a];
-  for (int r = 0; r < P; ++r) {
-    int const ix = wrap_once(gi[0] + r, M);
-    for (int b = 0; b < P; ++b) {
-      int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
-      double const wt = w[0][r] * w[1][b] * wz;
-      atomicAdd(&A0[idx].x, wt * mu[0]);
-      atomicAdd(&A1[idx].x, wt * mu[1]);
-      atomicAdd(&A2[idx].x, wt * mu[2]);
+  int const E3 = E * E * E;
+  for (int e = threadIdx.x; e < 3 * E3; e += kDpBlock) tile[e] = 0.;
+  if (threadIdx.x < 3) org[threadIdx.x] = INT_MAX;
+  int gref[3];
+  {
+    double const x[3] = {pos[3 * beg], pos[3 * beg + 1], pos[3 * beg + 2]};
+    int nmp[3];
+    double dist[3];
+    (void)particle_coordinates_rt(g, P, x, nmp, gref, dist);
+  }
+  auto rel = [M](int gi, int gr) {
+    int d = gi - gr;
+    if (d < -(M / 2)) d += M;
+    if (d >= M - M / 2) d -= M;
+    return d;
+  };
+  __syncthreads();
+  for (int j = threadIdx.x; j < cnt; j += kDpBlock) {
+    int64_t const i = beg + j;
+    double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int nmp[3], gi[3];
+    double dist[3];
+    if (!particle_coordinates_rt(g, P, x, nmp, gi, dist)) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) atomicMin(&org[d], rel(gi[d], gref[d]));
+  }
+  __syncthreads();
+  int const o[3] = {org[0] == INT_MAX ? 0 : org[0], org[1] == INT_MAX ? 0 : org[1],
+                    org[2] == INT_MAX ? 0 : org[2]};
+  int const a = static_cast<int>(threadIdx.x & 7);
+  for (int j = threadIdx.x >> 3; j < cnt; j += kDpBlock / 8) {
+    if (a >= P) continue;
+    int64_t const i = beg + j;
+    double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+    if (mu[0] == 0. && mu[1] == 0. && mu[2] == 0.) continue;
+    double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    int gi[3];
+    double w[3][P];
+    if (!dp_weights<P>(g, x, gi, w)) {
+      if (a == 0) atomicOr(err, 1);
+      continue;
     }
+    int r[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r[d] = rel(gi[d], gref[d]) - o[d];
+    double const wz = w[2][a];
+    if (r[0] + P <= E && r[1] + P <= E && r[2] + P <= E) {
+      for (int rr = 0; rr < P; ++rr)
+        for (int b = 0; b < P; ++b) {
+          int const t = ((r[0] + rr) * E + r[1] + b) * E + r[2] + a;
+          double const wt = w[0][rr] * w[1][b] * wz;
+          atomicAdd(&tile[t], wt * mu[0]);
+          atomicAdd(&tile[E3 + t], wt * mu[1]);
+          atomicAdd(&tile[2 * E3 + t], wt * mu[2]);
+        }
+    } else {
+      int const iz = wrap_once(gi[2] + a, M);
+      for (int rr = 0; rr < P; ++rr) {
+        int const ix = wrap_once(gi[0] + rr, M);
+        for (int b = 0; b < P; ++b) {
+          int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
+          double const wt = w[0][rr] * w[1][b] * wz;
+          atomicAdd(&A0[idx].x, wt * mu[0]);
+          atomicAdd(&A1[idx].x, wt * mu[1]);
+          atomicAdd(&A2[idx].x, wt * mu[2]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  int const b0 = ((gref[0] + o[0]) % M + M) % M, b1 = ((gref[1] + o[1]) % M + M) % M,
+            b2 = ((gref[2] + o[2]) % M + M) % M;
+  for (int e = threadIdx.x; e < E3; e += kDpBlock) {
+    double const v0 = tile[e], v1 = tile[E3 + e], v2 = tile[2 * E3 + e];
+    if (v0 == 0. && v1 == 0. && v2 == 0.) continue;
+    int const tz = e % E, ty = (e / E) % E, tx = e / (E * E);
+    int64_t const idx =
+        ((int64_t)((b0 + tx) % M) * M + (b1 + ty) % M) * M + (b2 + tz) % M;
+    if (v0 != 0.) atomicAdd(&A0[idx].x, v0);
+    if (v1 != 0.) atomicAdd(&A1[idx].x, v1);
+    if (v2 != 0.) atomicAdd(&A2[idx].x, v2);
   }
 }
 
@@ -176,72 +250,208 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_back_meshes(C2 const *__restric
 
 /** AssignTorques / AssignForces (dp3m.cpp:195-244): torque -= mu x (pt E),
  *  force_a += pf sum_b mu_b H_ab with E and H interpolated from the real
- *  parts of the back-transformed meshes.  Eight lanes per particle (lane
- *  a < P sums the stencil points with z offset a), combined by lane
- *  shuffles. */
+ *  parts of the back-transformed meshes, over particles sorted into buckets
+ *  of mesh cells.  One block per bucket stages the bucket's E^3 tile of the
+ *  three field meshes (and the six force meshes when forces are wanted) in
+ *  LDS, tile origin as in the spread; eight lanes per particle (lane a < P
+ *  sums the stencil points with z offset a), combined by lane shuffles.  A
+ *  particle whose stencil leaves the tile reads the meshes directly.
+ *  Results go to the caller's row perm[i]. */
 template <int P>
-__global__ __launch_bounds__(kDpBlock) void k_dp_gather(int64_t n, double const *__restrict__ pos,
-                                                        double const *__restrict__ dip,
-                                                        DevGeo g, C2 *const *T, C2 *const *F,
-                                                        double pt, double pf,
-                                                        double *__restrict__ force,
-                                                        double *__restrict__ torque) {
-  int64_t const t = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  int64_t const i = t >> 3;
-  int const a = static_cast<int>(t & 7);
+__global__ __launch_bounds__(kDpGatherBlock) void k_dp_gather_tile(
+    int const *__restrict__ offsets, int const *__restrict__ counts,
+    double const *__restrict__ pos, double const *__restrict__ dip, DevGeo g, int E,
+    C2 *const *T, C2 *const *F, double pt, double pf, int const *__restrict__ perm,
+    double *__restrict__ force, double *__restrict__ torque) {
+  extern __shared__ double tile[]; // [nm][E][E][E]: E_x E_y E_z H_xx .. H_zz
+  __shared__ int org[3];
+  int const beg = offsets[blockIdx.x], cnt = counts[blockIdx.x];
+  if (cnt == 0) return;
   bool const wf = force != nullptr;
-  double v[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.}; // E[3], H[6]
-  double mu[3] = {0., 0., 0.};
-  bool act = false;
-  if (i < n) {
-    mu[0] = dip[3 * i];
-    mu[1] = dip[3 * i + 1];
-    mu[2] = dip[3 * i + 2];
-    act = (mu[0] != 0. || mu[1] != 0. || mu[2] != 0.);
+  int const nm = wf ? 9 : 3;
+  int const M = g.M[0];
+  int const E3 = E * E * E;
+  if (threadIdx.x < 3) org[threadIdx.x] = INT_MAX;
+  int gref[3];
+  {
+    double const x[3] = {pos[3 * beg], pos[3 * beg + 1], pos[3 * beg + 2]};
+    int nmp[3];
+    double dist[3];
+    (void)particle_coordinates_rt(g, P, x, nmp, gref, dist);
   }
-  if (act && a < P) {
+  auto rel = [M](int gi, int gr) {
+    int d = gi - gr;
+    if (d < -(M / 2)) d += M;
+    if (d >= M - M / 2) d -= M;
+    return d;
+  };
+  __syncthreads();
+  for (int j = threadIdx.x; j < cnt; j += kDpGatherBlock) {
+    int64_t const i = beg + j;
     double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-    int gi[3];
-    double w[3][P];
-    if (dp_weights<P>(g, x, gi, w)) {
-      // lane a takes z offset a (consecutive addresses across the lanes)
-      int const M = g.M[0];
-      int const iz = wrap_once(gi[2] + a, M);
-      double const wz = w[2][a];
-      for (int r = 0; r < P; ++r) {
-        int const ix = wrap_once(gi[0] + r, M);
-        for (int b = 0; b < P; ++b) {
-          int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
-          double const wt = w[0][r] * w[1][b] * wz;
+    int nmp[3], gi[3];
+    double dist[3];
+    if (!particle_coordinates_rt(g, P, x, nmp, gi, dist)) continue;
 #pragma unroll
-          for (int d = 0; d < 3; ++d) v[d] += wt * T[d][idx].x;
-          if (wf)
+    for (int d = 0; d < 3; ++d) atomicMin(&org[d], rel(gi[d], gref[d]));
+  }
+  __syncthreads();
+  int const o[3] = {org[0] == INT_MAX ? 0 : org[0], org[1] == INT_MAX ? 0 : org[1],
+                    org[2] == INT_MAX ? 0 : org[2]};
+  int const b0 = ((gref[0] + o[0]) % M + M) % M, b1 = ((gref[1] + o[1]) % M + M) % M,
+            b2 = ((gref[2] + o[2]) % M + M) % M;
+  for (int e = threadIdx.x; e < nm * E3; e += kDpGatherBlock) {
+    int const c = e / E3, q = e - c * E3;
+    int const tz = q % E, ty = (q / E) % E, tx = q / (E * E);
+    int64_t const idx =
+        ((int64_t)((b0 + tx) % M) * M + (b1 + ty) % M) * M + (b2 + tz) % M;
+    tile[e] = (c < 3 ? T[c] : F[c - 3])[idx].x;
+  }
+  __syncthreads();
+  int const a = static_cast<int>(threadIdx.x & 7);
+  // the eight lanes of a particle share j and run the loop together
+  for (int j = threadIdx.x >> 3; j < cnt; j += kDpGatherBlock / 8) {
+    int64_t const i = beg + j;
+    double v[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.}; // E[3], H[6]
+    double const mu[3] = {dip[3 * i], dip[3 * i + 1], dip[3 * i + 2]};
+    bool const act = (mu[0] != 0. || mu[1] != 0. || mu[2] != 0.);
+    if (act && a < P) {
+      double const x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+      int gi[3];
+      double w[3][P];
+      if (dp_weights<P>(g, x, gi, w)) {
+        int r[3];
 #pragma unroll
-            for (int d = 0; d < 6; ++d) v[3 + d] += wt * F[d][idx].x;
+        for (int d = 0; d < 3; ++d) r[d] = rel(gi[d], gref[d]) - o[d];
+        double const wz = w[2][a];
+        if (r[0] + P <= E && r[1] + P <= E && r[2] + P <= E) {
+          for (int rr = 0; rr < P; ++rr)
+            for (int b = 0; b < P; ++b) {
+              int const t = ((r[0] + rr) * E + r[1] + b) * E + r[2] + a;
+              double const wt = w[0][rr] * w[1][b] * wz;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) v[d] += wt * tile[d * E3 + t];
+              if (wf)
+#pragma unroll
+                for (int d = 0; d < 6; ++d) v[3 + d] += wt * tile[(3 + d) * E3 + t];
+            }
+        } else {
+          int const iz = wrap_once(gi[2] + a, M);
+          for (int rr = 0; rr < P; ++rr) {
+            int const ix = wrap_once(gi[0] + rr, M);
+            for (int b = 0; b < P; ++b) {
+              int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
+              double const wt = w[0][rr] * w[1][b] * wz;
+#pragma unroll
+              for (int d = 0; d < 3; ++d) v[d] += wt * T[d][idx].x;
+              if (wf)
+#pragma unroll
+                for (int d = 0; d < 6; ++d) v[3 + d] += wt * F[d][idx].x;
+            }
+          }
         }
       }
     }
-  }
-  // sum over the eight lanes of the particle (all lanes take part)
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    v[k] += __shfl_xor(v[k], 1, 64);
-    v[k] += __shfl_xor(v[k], 2, 64);
-    v[k] += __shfl_xor(v[k], 4, 64);
+    for (int k = 0; k < 9; ++k) {
+      v[k] += __shfl_xor(v[k], 1, 64);
+      v[k] += __shfl_xor(v[k], 2, 64);
+      v[k] += __shfl_xor(v[k], 4, 64);
+    }
+    if (!act || a != 0) continue;
+    int64_t const ro = 3 * static_cast<int64_t>(perm[i]); // caller's row
+    if (torque) {
+      double const e0 = pt * v[0], e1 = pt * v[1], e2 = pt * v[2];
+      torque[ro] -= mu[1] * e2 - mu[2] * e1;
+      torque[ro + 1] -= mu[2] * e0 - mu[0] * e2;
+      torque[ro + 2] -= mu[0] * e1 - mu[1] * e0;
+    }
+    if (wf) {
+      double const *H = v + 3; // xx xy xz yy yz zz
+      force[ro] += pf * (mu[0] * H[0] + mu[1] * H[1] + mu[2] * H[2]);
+      force[ro + 1] += pf * (mu[0] * H[1] + mu[1] * H[3] + mu[2] * H[4]);
+      force[ro + 2] += pf * (mu[0] * H[2] + mu[1] * H[4] + mu[2] * H[5]);
+    }
   }
-  if (!act || a != 0) return;
-  if (torque) {
-    double const e0 = pt * v[0], e1 = pt * v[1], e2 = pt * v[2];
-    torque[3 * i] -= mu[1] * e2 - mu[2] * e1;
-    torque[3 * i + 1] -= mu[2] * e0 - mu[0] * e2;
-    torque[3 * i + 2] -= mu[0] * e1 - mu[1] * e0;
+}
+
+/** Counting sort of the particles into nbk^3 buckets of mesh cells (the
+ *  spread and gather take one bucket per block): bucket key and rank per
+ *  particle. */
+__global__ __launch_bounds__(kDpBlock) void k_dp_key(int64_t n, double const *__restrict__ pos,
+                                                     double L, int nbk, int *__restrict__ counts,
+                                                     int *__restrict__ key,
+                                                     int *__restrict__ rank) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[3 * i + d];
+    x -= L * floor(x / L);
+    c[d] = isfinite(x) ? min(max(static_cast<int>(x / L * nbk), 0), nbk - 1) : 0;
   }
-  if (wf) {
-    double const *H = v + 3; // xx xy xz yy yz zz
-    force[3 * i] += pf * (mu[0] * H[0] + mu[1] * H[1] + mu[2] * H[2]);
-    force[3 * i + 1] += pf * (mu[0] * H[1] + mu[1] * H[3] + mu[2] * H[4]);
-    force[3 * i + 2] += pf * (mu[0] * H[2] + mu[1] * H[4] + mu[2] * H[5]);
+  int const k = (c[0] * nbk + c[1]) * nbk + c[2];
+  key[i] = k;
+  rank[i] = atomicAdd(&counts[k], 1);
+}
+
+/** Exclusive scan of the block counts (one block of 1024 threads). */
+__global__ __launch_bounds__(1024) void k_dp_scan(int const *__restrict__ counts, int nk,
+                                                  int *__restrict__ offsets) {
+  __shared__ int part[1024];
+  int const per = (nk + 1023) / 1024;
+  int const b = threadIdx.x * per;
+  int s = 0;
+  for (int k = b; k < min(nk, b + per); ++k) s += counts[k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < 1024; ++t) {
+      int const v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
   }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int k = b; k < min(nk, b + per); ++k) {
+    offsets[k] = acc;
+    acc += counts[k];
+  }
+}
+
+__global__ __launch_bounds__(kDpBlock) void k_dp_scatter(int64_t n, double const *__restrict__ pos,
+                                                         double const *__restrict__ dip,
+                                                         int const *__restrict__ key,
+                                                         int const *__restrict__ rank,
+                                                         int const *__restrict__ offsets,
+                                                         double *__restrict__ spos,
+                                                         double *__restrict__ sdip,
+                                                         int *__restrict__ perm) {
+  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
+  if (i >= n) return;
+  int64_t const s = offsets[key[i]] + rank[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    spos[3 * s + d] = pos[3 * i + d];
+    sdip[3 * s + d] = dip[3 * i + d];
+  }
+  perm[s] = static_cast<int>(i);
+}
+
+/** Deterministic sum of the moment partials into out[4]. */
+__global__ __launch_bounds__(kDpBlock) void k_dp_reduce4(double const *__restrict__ part, int nb,
+                                                         double *__restrict__ out) {
+  __shared__ double red[4 * 16];
+  double v[4] = {0., 0., 0., 0.};
+  for (int b = threadIdx.x; b < nb; b += kDpBlock)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] += part[b * 4 + k];
+  block_sum<4>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) out[k] = v[k];
 }
 
 /** Sums of |mu|^2 and of the dipole vector (block partials). */
@@ -265,10 +475,11 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_moments(int64_t n, double const
 /** Surface term torques (calc_surface_term, dp3m.cpp:451-517):
  *  torque -= pref mu x M_total. */
 __global__ __launch_bounds__(kDpBlock) void k_dp_surface(int64_t n, double const *__restrict__ dip,
-                                                         double mx, double my, double mz,
+                                                         double const *__restrict__ mom,
                                                          double pref, double *torque) {
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
   if (i >= n || !torque) return;
+  double const mx = mom[1], my = mom[2], mz = mom[3];
   double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
   torque[3 * i] -= pref * (m1 * mz - m2 * my);
   torque[3 * i + 1] -= pref * (m2 * mx - m0 * mz);
@@ -500,6 +711,12 @@ struct DipolarSolver::Impl {
   double self_sum = -1.; // sum_k G_e U2 |d_op|^2
   std::unique_ptr<ShortRange> sr; // cell lists of the real space
   int subdivision = 0;
+  double *mom = nullptr; // device: sum |mu|^2, total dipole
+  // particles sorted by blocks of mesh cells
+  double *spos = nullptr, *sdip = nullptr;
+  int *key = nullptr, *rank = nullptr, *perm = nullptr, *counts = nullptr, *offsets = nullptr;
+  int64_t cap_sort = 0;
+  int cap_keys = 0;
 };
 
 DipolarSolver::DipolarSolver(int device) : m(std::make_unique<Impl>()) {
@@ -508,6 +725,7 @@ DipolarSolver::DipolarSolver(int device) : m(std::make_unique<Impl>()) {
   hip_check(hipStreamCreate(&m->st), "hipStreamCreate");
   hip_check(hipMalloc(&m->err, sizeof(int)), "hipMalloc dp3m flag");
   hip_check(hipMalloc(&m->d_ptrs, sizeof(C2 *) * 12), "hipMalloc dp3m table");
+  hip_check(hipMalloc(&m->mom, sizeof(double) * 4), "hipMalloc dp3m moments");
 }
 
 DipolarSolver::~DipolarSolver() {
@@ -521,6 +739,11 @@ DipolarSolver::~DipolarSolver() {
   dp_free(m->Gf);
   dp_free(m->part);
   dp_free(m->err);
+  for (void *x : {static_cast<void *>(m->mom), static_cast<void *>(m->spos),
+                  static_cast<void *>(m->sdip), static_cast<void *>(m->key),
+                  static_cast<void *>(m->rank), static_cast<void *>(m->perm),
+                  static_cast<void *>(m->counts), static_cast<void *>(m->offsets)})
+    dp_free(x);
   if (m->st) (void)hipStreamDestroy(m->st);
 }
 
@@ -668,20 +891,30 @@ void DipolarSolver::configure(DipolarParams const &p) {
 }
 
 namespace {
-int lanes_grid(int64_t n) { return static_cast<int>((8 * n + kDpBlock - 1) / kDpBlock); }
 
-template <int P> void launch_spread(hipStream_t st, int, int64_t n, double const *pos,
-                                    double const *dip, DevGeo const &g, C2 *const *mesh, int *err) {
-  k_dp_spread<P><<<lanes_grid(n), kDpBlock, 0, st>>>(n, pos, dip, g, mesh[0], mesh[1], mesh[2],
-                                                     err);
+template <int P> void launch_spread(hipStream_t st, int nk, int E, int const *offsets,
+                                    int const *counts, double const *pos, double const *dip,
+                                    DevGeo const &g, C2 *const *mesh, int *err) {
+  size_t const lds = sizeof(double) * 3 * static_cast<size_t>(E) * E * E;
+  k_dp_spread_tile<P><<<nk, kDpBlock, lds, st>>>(offsets, counts, pos, dip, g, E, mesh[0],
+                                                 mesh[1], mesh[2], err);
 }
-template <int P> void launch_gather(hipStream_t st, int nb, int64_t n, double const *pos,
-                                    double const *dip, DevGeo const &g, C2 *const *T,
-                                    C2 *const *F, double pt, double pf, double *force,
-                                    double *torque) {
-  (void)nb;
-  k_dp_gather<P><<<lanes_grid(n), kDpBlock, 0, st>>>(n, pos, dip, g, T, F, pt, pf, force,
-                                                     torque);
+template <int P> void launch_gather(hipStream_t st, int nk, int E, int const *offsets,
+                                    int const *counts, double const *pos, double const *dip,
+                                    DevGeo const &g, C2 *const *T, C2 *const *F, double pt,
+                                    double pf, int const *perm, double *force, double *torque) {
+  static bool const raised = [] {
+    hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k_dp_gather_tile<P>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kDpGatherLds),
+              "hipFuncSetAttribute");
+    return true;
+  }();
+  (void)raised;
+  size_t const lds = sizeof(double) * (force ? 9 : 3) * static_cast<size_t>(E) * E * E;
+  if (lds > static_cast<size_t>(kDpGatherLds))
+    throw std::runtime_error("DipolarP3M: gather tile exceeds the LDS of a workgroup");
+  k_dp_gather_tile<P><<<nk, kDpGatherBlock, lds, st>>>(offsets, counts, pos, dip, g, E, T, F,
+                                                       pt, pf, perm, force, torque);
 }
 } // namespace
 
@@ -704,32 +937,62 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     hip_check(hipMalloc(&m->part, sizeof(double) * need), "hipMalloc dp3m partials");
     m->cap_part = need;
   }
-  // moments (sum |mu|^2, total dipole)
-  std::vector<double> mom(static_cast<size_t>(nb) * 4);
+  // moments (sum |mu|^2, total dipole) on the device, no host round trip
   k_dp_moments<<<nb, kDpBlock, 0, st>>>(n, dip, m->part);
-  hip_check(hipMemcpyAsync(mom.data(), m->part, sizeof(double) * mom.size(),
-                           hipMemcpyDeviceToHost, st),
-            "copy dp3m moments");
-  hip_check(hipStreamSynchronize(st), "sync dp3m moments");
-  double mu2 = 0., Mt[3] = {0., 0., 0.};
-  for (int b = 0; b < nb; ++b) {
-    mu2 += mom[static_cast<size_t>(b) * 4];
-    for (int k = 0; k < 3; ++k) Mt[k] += mom[static_cast<size_t>(b) * 4 + 1 + k];
+  k_dp_reduce4<<<1, kDpBlock, 0, st>>>(m->part, nb, m->mom);
+  // particles sorted into buckets of S^3 mesh cells for the spread and
+  // gather; the spread's LDS tile of E = ceil(M / nbk) + cao points per edge
+  // and component stays within 64 KiB
+  int nbk = 1, E = 0;
+  for (int S : {8, 4, 2, 1}) {
+    nbk = std::max(1, M / S);
+    E = (M + nbk - 1) / nbk + p.cao;
+    if (3 * E * E * E * static_cast<int>(sizeof(double)) <= 65536) break;
   }
+  int const nk = nbk * nbk * nbk;
+  if (n > m->cap_sort || nk > m->cap_keys) {
+    hip_check(hipStreamSynchronize(st), "sync");
+    if (n > m->cap_sort) {
+      for (void *x : {static_cast<void *>(m->spos), static_cast<void *>(m->sdip),
+                      static_cast<void *>(m->key), static_cast<void *>(m->rank),
+                      static_cast<void *>(m->perm)})
+        dp_free(x);
+      hip_check(hipMalloc(&m->spos, sizeof(double) * 3 * n), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->sdip, sizeof(double) * 3 * n), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->key, sizeof(int) * n), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->rank, sizeof(int) * n), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->perm, sizeof(int) * n), "hipMalloc dp3m sort");
+      m->cap_sort = n;
+    }
+    if (nk > m->cap_keys) {
+      dp_free(m->counts);
+      dp_free(m->offsets);
+      hip_check(hipMalloc(&m->counts, sizeof(int) * nk), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->offsets, sizeof(int) * nk), "hipMalloc dp3m sort");
+      m->cap_keys = nk;
+    }
+  }
+  k_dp_izero<<<(nk + kDpBlock - 1) / kDpBlock, kDpBlock, 0, st>>>(m->counts, nk);
+  k_dp_key<<<nb, kDpBlock, 0, st>>>(n, pos, p.box, nbk, m->counts, m->key, m->rank);
+  k_dp_scan<<<1, 1024, 0, st>>>(m->counts, nk, m->offsets);
+  k_dp_scatter<<<nb, kDpBlock, 0, st>>>(n, pos, dip, m->key, m->rank, m->offsets, m->spos,
+                                        m->sdip, m->perm);
+  double const *sp = m->spos, *sd = m->sdip;
+  int const *perm = m->perm;
   double const V = p.box * p.box * p.box;
   double const wn = 2. * std::numbers::pi / p.box;
   double const dipole_prefac = p.prefactor / static_cast<double>(M3);
-  if (mu2 > 0.) {
+  {
     for (int c = 0; c < 3; ++c) k_dp_zero<<<kb, kDpBlock, 0, st>>>(m->mesh[c], M3);
     k_dp_clear<<<1, 1, 0, st>>>(m->err);
     switch (p.cao) {
-    case 1: launch_spread<1>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    case 2: launch_spread<2>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    case 3: launch_spread<3>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    case 4: launch_spread<4>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    case 5: launch_spread<5>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    case 6: launch_spread<6>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
-    default: launch_spread<7>(st, nb, n, pos, dip, m->g, m->mesh, m->err); break;
+    case 1: launch_spread<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 2: launch_spread<2>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 3: launch_spread<3>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 4: launch_spread<4>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 5: launch_spread<5>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 6: launch_spread<6>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    default: launch_spread<7>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
     }
     size_t const lds = sizeof(C2) * (M + 2 * kDpLines * (M + 1));
     int const lines = static_cast<int>((static_cast<int64_t>(M) * M + kDpLines - 1) / kDpLines);
@@ -743,10 +1006,13 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     k_dp_kspace<<<kb, kDpBlock, 0, st>>>(m->mesh[0], m->mesh[1], m->mesh[2], m->Ge, m->Gf, M,
                                          m->mesh[3], m->mesh[4], m->part);
     std::vector<double> ep(static_cast<size_t>(kb));
+    double mom[4] = {0., 0., 0., 0.};
     if (energy) {
       hip_check(hipMemcpyAsync(ep.data(), m->part, sizeof(double) * kb, hipMemcpyDeviceToHost,
                                st),
                 "copy dp3m energy");
+      hip_check(hipMemcpyAsync(mom, m->mom, sizeof(mom), hipMemcpyDeviceToHost, st),
+                "copy dp3m moments");
     }
     bool const wf = force != nullptr, wt = torque != nullptr;
     if (wf || wt) {
@@ -756,13 +1022,13 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       if (wf) fft(m->d_ptrs + 3, 6, true);
       double const pt = dipole_prefac * wn, pf = dipole_prefac * wn * wn;
       switch (p.cao) {
-      case 1: launch_gather<1>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      case 2: launch_gather<2>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      case 3: launch_gather<3>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      case 4: launch_gather<4>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      case 5: launch_gather<5>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      case 6: launch_gather<6>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
-      default: launch_gather<7>(st, nb, n, pos, dip, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, force, torque); break;
+      case 1: launch_gather<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 2: launch_gather<2>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 3: launch_gather<3>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 4: launch_gather<4>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 5: launch_gather<5>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 6: launch_gather<6>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      default: launch_gather<7>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
       }
     }
     hip_check(hipGetLastError(), "dipolar P3M kernels");
@@ -774,6 +1040,7 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       if (err) throw std::runtime_error("DipolarP3M: particle outside the charge-assignment mesh");
     }
     if (energy) {
+      double const mu2 = mom[0];
       double ek = 0.;
       for (double v : ep) ek += v;
       double e = ek * dipole_prefac * std::numbers::pi / p.box;
@@ -784,14 +1051,18 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       double const eself = -2. * alpha_L * alpha_L * alpha_L * std::numbers::inv_sqrtpi / 3.;
       double const corr = -mu2 * (ukp3m + eself + 2. * std::numbers::pi / 3.);
       e += p.prefactor * corr / V;
-      *energy = e;
+      *energy = (mu2 > 0.) ? e : 0.;
+      if (p.epsilon != 0.) {
+        // calc_surface_term energy (dp3m.cpp:483-491)
+        double const pref = p.prefactor * 4. * std::numbers::pi / V / (2. * p.epsilon + 1.);
+        *energy += 0.5 * pref * (mom[1] * mom[1] + mom[2] * mom[2] + mom[3] * mom[3]);
+      }
     }
   }
-  if (p.epsilon != 0.) {
-    // calc_surface_term (dp3m.cpp:451-517)
+  if (p.epsilon != 0. && torque) {
+    // calc_surface_term torques (dp3m.cpp:493-514)
     double const pref = p.prefactor * 4. * std::numbers::pi / V / (2. * p.epsilon + 1.);
-    if (torque) k_dp_surface<<<nb, kDpBlock, 0, st>>>(n, dip, Mt[0], Mt[1], Mt[2], pref, torque);
-    if (energy) *energy += 0.5 * pref * (Mt[0] * Mt[0] + Mt[1] * Mt[1] + Mt[2] * Mt[2]);
+    k_dp_surface<<<nb, kDpBlock, 0, st>>>(n, dip, m->mom, pref, torque);
   }
   hip_check(hipGetLastError(), "dipolar P3M surface");
   if (sync || energy) hip_check(hipStreamSynchronize(st), "sync dp3m");
