@@ -116,12 +116,12 @@ def test_p3mgpu_actor_forces_energy_pressure():
     c.add_long_range_forces(pos, q, fc)
     assert S.rel_rms(fc, f) > 5 * err
     # energy and virial: the CPU kernels' mesh, identical to the P3M actor's
-    # (spreading flushes with atomics, so repeated FP32 solves agree to
-    # about the float rounding, not bit for bit)
+    # (spreading flushes with atomics in any order, so repeated FP32 solves
+    # agree to a few float ulps, not bit for bit: 1e-6 is about 8 ulps)
     e_g, e_c = g.long_range_energy(pos, q), c.long_range_energy(pos, q)
-    assert abs(e_g - e_c) <= 1e-7 * abs(e_c)
+    assert abs(e_g - e_c) <= 1e-6 * abs(e_c)
     v_c = c.long_range_pressure(pos, q)
-    assert np.max(np.abs(g.long_range_pressure(pos, q) - v_c)) <= 1e-7 * np.max(np.abs(v_c))
+    assert np.max(np.abs(g.long_range_pressure(pos, q) - v_c)) <= 1e-6 * np.max(np.abs(v_c))
     ora = O.OracleP3M(box, mesh, cao, PARAMS["alpha"], PARAMS["r_cut"], PARAMS["prefactor"])
     assert abs(e_g - ora.energy(pos, q)) <= 2e-6 * abs(e_g)
     # one call with forces, energy and pressure: GPU-rule forces, CPU-mesh
@@ -129,8 +129,8 @@ def test_p3mgpu_actor_forces_energy_pressure():
     f2 = np.zeros((len(q), 3))
     e2, v2 = g.solve(pos, q, force=f2, energy=True, pressure=True)
     assert S.rel_rms(f2, f) <= 1e-6
-    assert abs(e2 - e_g) <= 1e-7 * abs(e_g)
-    assert np.max(np.abs(v2 - v_c)) <= 1e-7 * np.max(np.abs(v_c))
+    assert abs(e2 - e_g) <= 1e-6 * abs(e_g)
+    assert np.max(np.abs(v2 - v_c)) <= 1e-6 * np.max(np.abs(v_c))
     # after the energy pass the force convention is back in force
     f3 = np.zeros((len(q), 3))
     g.add_long_range_forces(pos, q, f3)
