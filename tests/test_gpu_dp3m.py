@@ -28,7 +28,10 @@ def _system(n, L, seed, zero_frac=0.1):
     return pos, dip
 
 
-@pytest.mark.parametrize("M,cao,epsilon", [(16, 3, 0.), (24, 5, 0.), (32, 7, 2.), (20, 4, 0.)])
+# (8, 7): the per-bucket LDS tile (11 points per edge) wraps around the mesh;
+# (10, 5) and (12, 3): meshes that the buckets of 8 cells do not divide
+@pytest.mark.parametrize("M,cao,epsilon", [(16, 3, 0.), (24, 5, 0.), (32, 7, 2.), (20, 4, 0.),
+                                           (8, 7, 0.), (10, 5, 0.), (12, 3, 2.)])
 def test_kspace_matches_oracle(M, cao, epsilon):
     from espresso_amd.magnetostatics import DipolarP3M
     from oracle import dp3m as D
@@ -48,6 +51,30 @@ def test_kspace_matches_oracle(M, cao, epsilon):
           f"energy {e:.10f} vs {er:.10f}")
     assert ef <= 1e-10 and et <= 1e-10
     assert abs(e - er) <= 1e-10 * abs(er)
+
+
+def test_kspace_clustered_and_partial_outputs():
+    """Most dipoles in one corner (a few buckets holding far more particles
+    than a block has threads), a spread across the periodic boundary, and
+    torque-only / force-only calls equal to the combined call."""
+    from espresso_amd.magnetostatics import DipolarP3M
+    from oracle import dp3m as D
+    L, M, cao, n = 12., 24, 5, 2500
+    rng = np.random.default_rng(31)
+    pos = np.concatenate([rng.uniform(-1., 1.5, (2000, 3)), rng.uniform(0., L, (n - 2000, 3))])
+    dip = rng.normal(size=(n, 3))
+    dp = DipolarP3M(prefactor=0.7, accuracy=1e-4, mesh=M, cao=cao, alpha=0.8, r_cut=3.,
+                    tune=False)
+    dp.activate(pos, dip, box_l=L)
+    f, t = np.zeros((n, 3)), np.zeros((n, 3))
+    dp.add_long_range_forces(pos, dip, f, t)
+    ora = D.OracleDP3M(L, M, cao, 0.8, 3., 0.7)
+    fr, tr, _ = ora.solve(pos - L * np.floor(pos / L), dip)
+    assert S.rel_rms(f, fr) <= 1e-10 and S.rel_rms(t, tr) <= 1e-10
+    t_only, f_only = np.zeros((n, 3)), np.zeros((n, 3))
+    dp.add_long_range_forces(pos, dip, None, t_only)
+    dp.add_long_range_forces(pos, dip, f_only, None)
+    assert S.rel_rms(t_only, t) <= 1e-13 and S.rel_rms(f_only, f) <= 1e-13
 
 
 @pytest.mark.parametrize("subdiv", [0, 1, 2])
