@@ -1091,12 +1091,14 @@ void DipolarSolver::short_range(int64_t n, double const *pos, double const *dip,
   sp.prefactor = p.prefactor;
   sp.alpha = p.alpha;
   sp.r_cut = p.r_cut;
-  // cells of r_cut/2 (64-thread blocks) only when they hold enough dipoles
-  // to fill a wave; sparser systems run faster on cells of r_cut with
-  // 256-thread blocks (N = 1e5 at density 0.09: 1.30 against 2.10 ms)
+  // cells of r_cut/2 test 42 % fewer candidates and give 8 times more
+  // blocks; cells of r_cut cost fewer passes over neighbour tables when the
+  // cells are sparse.  Measured crossover (N = 2e5, r_cut 10): between 2.7
+  // dipoles per r_cut/2 cell (0.76 against 2.33 ms) and 12.5 (2.81 against
+  // 2.51 ms); at 48: 10.5 against 6.2 ms
   double const half = 0.5 * p.r_cut;
   double const per_cell = static_cast<double>(n) * half * half * half / (p.box * p.box * p.box);
-  sp.subdivision = m->subdivision ? m->subdivision : (per_cell >= 48. ? 2 : 1);
+  sp.subdivision = m->subdivision ? m->subdivision : (per_cell >= 8. ? 2 : 1);
   if (m->sr->compute_dipolar(st, sp, exact != 0, n, pos, dip, force, torque, energy)) {
     if (sync && !energy) hip_check(hipStreamSynchronize(st), "sync dp3m pairs");
     return;

@@ -410,72 +410,157 @@ __device__ __forceinline__ void dip_pair(double dx, double dy, double dz, double
   if constexpr (EV) e += prefactor * (mimj * B - mir * mjr * Cr);
 }
 
-/** Dipolar counterpart of k_sr_cells: the cell's particles against the
- *  (2 NSUB + 1)^3 neighbour cells staged through LDS (positions with the
- *  periodic shift, dipoles), both orders of a pair. */
-template <int NSUB, int NT, bool EV>
-__global__ __launch_bounds__(NT) void k_sr_cells_dip(CellGrid g, int const *__restrict__ offsets,
-                                                     double4 const *__restrict__ spos,
-                                                     double4 const *__restrict__ sdip,
-                                                     int const *__restrict__ sidx,
-                                                     double prefactor, double alpha, double rc2,
-                                                     int exact, double *__restrict__ force,
-                                                     double *__restrict__ torque,
-                                                     double *partials) {
-  __shared__ double4 tp[NT], tm[NT];
+/** Dipolar pair terms over cell lists with dense evaluation: one block of
+ *  256 threads per home cell.  The block enumerates the (home particle,
+ *  candidate) pairs of the (2 NSUB + 1)^3 neighbour cells 256 at a time,
+ *  appends the pairs inside r_cut to an LDS queue (wave ballots, one LDS
+ *  atomic per wave) and drains the queue with every lane evaluating one
+ *  pair, so the pair terms run without idle lanes however sparse the cells.
+ *  Force and torque on each home particle accumulate in LDS (FP64 LDS
+ *  atomics) and are added to the caller's arrays once. */
+template <int NSUB, bool EV>
+__global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__restrict__ offsets,
+                                                      double4 const *__restrict__ spos,
+                                                      double4 const *__restrict__ sdip,
+                                                      int const *__restrict__ sidx,
+                                                      double prefactor, double alpha, double rc2,
+                                                      int exact, double *__restrict__ force,
+                                                      double *__restrict__ torque,
+                                                      double *partials) {
+  constexpr int NT = 256, W = 2 * NSUB + 1, NN = W * W * W;
+  constexpr int TPT = 4;                  // candidate pairs tested per thread and chunk
+  constexpr int CH = NT * TPT;            // pairs per chunk
+  constexpr int QCAP = 2048, TCAP = 512;  // queue entries; candidates per table window
+  __shared__ double4 hp[NT], hm[NT];
+  __shared__ double acc[NT * 6];
+  __shared__ int2 queue[QCAP]; // (candidate index, home slot | neighbour << 16)
+  __shared__ int2 ctab[TCAP];  // (candidate index, neighbour)
+  __shared__ int nbeg[NN], npre[NN + 1];
+  __shared__ double nsh[NN * 3];
+  __shared__ int qn;
   __shared__ double red[16];
-  constexpr int W = 2 * NSUB + 1;
   int const cell = blockIdx.x;
   int const cz = cell % g.nc[2], cy = (cell / g.nc[2]) % g.nc[1], cx = cell / (g.nc[2] * g.nc[1]);
   int const beg = offsets[cell], end = offsets[cell + 1];
+  if (beg == end) {
+    if (EV && threadIdx.x == 0) partials[blockIdx.x * 8] = 0.;
+    return;
+  }
+  for (int o = threadIdx.x; o < NN; o += NT) {
+    int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
+    double sx = 0., sy = 0., sz = 0.;
+    if (nx < 0) { nx += g.nc[0]; sx = -g.L[0]; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = g.L[0]; }
+    if (ny < 0) { ny += g.nc[1]; sy = -g.L[1]; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = g.L[1]; }
+    if (nz < 0) { nz += g.nc[2]; sz = -g.L[2]; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = g.L[2]; }
+    int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
+    nbeg[o] = offsets[nb];
+    npre[o + 1] = offsets[nb + 1] - offsets[nb];
+    nsh[3 * o] = sx;
+    nsh[3 * o + 1] = sy;
+    nsh[3 * o + 2] = sz;
+  }
+  if (threadIdx.x == 0) qn = 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    npre[0] = 0;
+    for (int o = 0; o < NN; ++o) npre[o + 1] += npre[o];
+  }
+  __syncthreads();
+  int const ncand = npre[NN];
+  int const lane = static_cast<int>(threadIdx.x & 63);
   double e = 0.;
   for (int hb = beg; hb < end; hb += NT) {
-    int const i = hb + static_cast<int>(threadIdx.x);
-    bool const has = i < end;
-    double4 const pi = has ? spos[i] : double4{0., 0., 0., 0.};
-    double4 const mi = has ? sdip[i] : double4{0., 0., 0., 0.};
-    double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
-    for (int o = 0; o < W * W * W; ++o) {
-      int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
-      double sx = 0., sy = 0., sz = 0.;
-      if (nx < 0) { nx += g.nc[0]; sx = -g.L[0]; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = g.L[0]; }
-      if (ny < 0) { ny += g.nc[1]; sy = -g.L[1]; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = g.L[1]; }
-      if (nz < 0) { nz += g.nc[2]; sz = -g.L[2]; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = g.L[2]; }
-      int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
-      int const jb = offsets[nb], je = offsets[nb + 1];
-      for (int tb = jb; tb < je; tb += NT) {
-        __syncthreads();
-        int const j = tb + static_cast<int>(threadIdx.x);
-        if (j < je) {
-          double4 const pj = spos[j];
-          tp[threadIdx.x] = double4{pj.x + sx, pj.y + sy, pj.z + sz, 0.};
-          tm[threadIdx.x] = sdip[j];
+    int const ni = min(NT, end - hb);
+    if (static_cast<int>(threadIdx.x) < ni) {
+      hp[threadIdx.x] = spos[hb + threadIdx.x];
+      hm[threadIdx.x] = sdip[hb + threadIdx.x];
+    }
+    for (int x = threadIdx.x; x < ni * 6; x += NT) acc[x] = 0.;
+    for (int kw = 0; kw < ncand; kw += TCAP) {
+      int const nw = min(TCAP, ncand - kw);
+      __syncthreads(); // table readers of the previous window are done
+      for (int o = 0; o < NN; ++o) {
+        int const s0 = npre[o];
+        int const lo = max(s0, kw), hi = min(npre[o + 1], kw + nw);
+        for (int k = lo + static_cast<int>(threadIdx.x); k < hi; k += NT)
+          ctab[k - kw] = make_int2(nbeg[o] + (k - s0), o);
+      }
+      __syncthreads();
+      int const total = ni * nw;
+      bool const last_window = kw + TCAP >= ncand;
+      float const rni = 1.f / static_cast<float>(ni);
+      for (int c0 = 0; c0 < total; c0 += CH) {
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+          int const c = c0 + u * NT + static_cast<int>(threadIdx.x);
+          bool hit = false;
+          int2 ent = make_int2(0, 0);
+          if (c < total) {
+            // k = c / ni from a float reciprocal, corrected (c < 2^19)
+            int k = static_cast<int>(static_cast<float>(c) * rni);
+            if (k * ni > c) --k;
+            if ((k + 1) * ni <= c) ++k;
+            int const il = c - k * ni;
+            int2 const cj = ctab[k];
+            double4 const pj = spos[cj.x];
+            double4 const pi = hp[il];
+            double const dx = pi.x - (pj.x + nsh[3 * cj.y]);
+            double const dy = pi.y - (pj.y + nsh[3 * cj.y + 1]);
+            double const dz = pi.z - (pj.z + nsh[3 * cj.y + 2]);
+            double const r2 = dx * dx + dy * dy + dz * dz;
+            hit = r2 < rc2 && r2 > 0.;
+            ent = make_int2(cj.x, il | (cj.y << 16));
+          }
+          uint64_t const mask = __ballot(hit);
+          int base = 0;
+          if (lane == 0 && mask) base = atomicAdd(&qn, __popcll(mask));
+          base = __shfl(base, 0, 64);
+          if (hit) queue[base + __popcll(mask & ((1ull << lane) - 1ull))] = ent;
         }
         __syncthreads();
-        if (has) {
-          int const cnt = min(NT, je - tb);
-          for (int k = 0; k < cnt; ++k) {
-            double const dx = pi.x - tp[k].x, dy = pi.y - tp[k].y, dz = pi.z - tp[k].z;
+        int const qcount = qn;
+        __syncthreads(); // every wave has read the count before it changes
+        // at most CH entries join per chunk: drain while they still fit
+        if (qcount > QCAP - CH || (last_window && c0 + CH >= total)) {
+          for (int q = threadIdx.x; q < qcount; q += NT) {
+            int2 const en = queue[q];
+            int const il = en.y & 0xffff, o = en.y >> 16;
+            double4 const pj = spos[en.x];
+            double4 const pi = hp[il];
+            double const dx = pi.x - (pj.x + nsh[3 * o]);
+            double const dy = pi.y - (pj.y + nsh[3 * o + 1]);
+            double const dz = pi.z - (pj.z + nsh[3 * o + 2]);
             double const r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 < rc2 && r2 > 0.)
-              dip_pair<EV>(dx, dy, dz, r2, mi, tm[k], prefactor, alpha, exact, f, t, e);
+            double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+            dip_pair<EV>(dx, dy, dz, r2, hm[il], sdip[en.x], prefactor, alpha, exact, f, t, e);
+            atomicAdd(&acc[il * 6], f[0]);
+            atomicAdd(&acc[il * 6 + 1], f[1]);
+            atomicAdd(&acc[il * 6 + 2], f[2]);
+            atomicAdd(&acc[il * 6 + 3], t[0]);
+            atomicAdd(&acc[il * 6 + 4], t[1]);
+            atomicAdd(&acc[il * 6 + 5], t[2]);
           }
+          __syncthreads();
+          if (threadIdx.x == 0) qn = 0;
+          __syncthreads();
         }
       }
     }
-    if (has) {
-      int64_t const o = 3 * static_cast<int64_t>(sidx[i]);
+    if (static_cast<int>(threadIdx.x) < ni) {
+      int64_t const o = 3 * static_cast<int64_t>(sidx[hb + threadIdx.x]);
+      double const *a = acc + threadIdx.x * 6;
       if (force) {
-        force[o] += f[0];
-        force[o + 1] += f[1];
-        force[o + 2] += f[2];
+        force[o] += a[0];
+        force[o + 1] += a[1];
+        force[o + 2] += a[2];
       }
       if (torque) {
-        torque[o] += t[0];
-        torque[o + 1] += t[1];
-        torque[o + 2] += t[2];
+        torque[o] += a[3];
+        torque[o + 1] += a[4];
+        torque[o + 2] += a[5];
       }
     }
+    __syncthreads(); // before the next home chunk restages hp, hm and acc
   }
   if constexpr (EV) {
     double v[1] = {e};
@@ -670,13 +755,13 @@ bool ShortRange::compute_dipolar(hipStream_t st, ShortRangeParams const &p, bool
   auto launch = [&](auto EVc) {
     constexpr bool EV = decltype(EVc)::value;
     if (nsub == 2)
-      k_sr_cells_dip<2, 64, EV><<<ncells, 64, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
-                                                       p.prefactor, p.alpha, rc2, ex, force,
-                                                       torque, m->partials);
+      k_sr_dip_queue<2, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
+                                                    p.prefactor, p.alpha, rc2, ex, force, torque,
+                                                    m->partials);
     else
-      k_sr_cells_dip<1, 256, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip,
-                                                         m->sidx, p.prefactor, p.alpha, rc2, ex,
-                                                         force, torque, m->partials);
+      k_sr_dip_queue<1, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
+                                                    p.prefactor, p.alpha, rc2, ex, force, torque,
+                                                    m->partials);
   };
   if (energy)
     launch(std::true_type{});

@@ -98,6 +98,30 @@ def test_short_range_cell_lists_match_oracle(subdiv):
         dp.set_short_range_subdivision(3)
 
 
+@pytest.mark.parametrize("subdiv", [1, 2])
+def test_short_range_dense_cluster_matches_oracle(subdiv):
+    """400 dipoles packed into one cell among 3000 sparse ones: a home cell
+    of more than 256 dipoles (two home passes), candidate tables of more
+    than one window and many queue drains per window."""
+    from espresso_amd.magnetostatics import DipolarP3M
+    from oracle import dp3m as D
+    L = 20.
+    pos, dip = _system(3000, L, 12)
+    rng = np.random.default_rng(13)
+    pos = np.concatenate([pos, 7.3 + rng.uniform(0., 1., (400, 3))])
+    dip = np.concatenate([dip, rng.normal(size=(400, 3))])
+    n = len(pos)
+    dp = DipolarP3M(prefactor=1.1, accuracy=1e-4, mesh=16, cao=4, alpha=0.9, r_cut=3.5,
+                    tune=False)
+    dp.activate(pos, dip, box_l=L)
+    dp.set_short_range_subdivision(subdiv)
+    f, t = np.zeros((n, 3)), np.zeros((n, 3))
+    e = dp.add_short_range_forces(pos, dip, f, t, energy=True)
+    fr, tr, er = D.pair_terms(pos - L * np.floor(pos / L), dip, L, 1.1, 0.9, 3.5, 1e-4)
+    assert S.rel_rms(f, fr) <= 1e-12 and S.rel_rms(t, tr) <= 1e-12
+    assert abs(e - er) <= 1e-12 * abs(er)
+
+
 @pytest.mark.parametrize("accuracy", [1e-4, 1e-7])
 def test_short_range_matches_oracle(accuracy):
     """A&S erfc for accuracy > 5e-6, libm erfc below (dp3m.hpp:176-178)."""
