@@ -419,7 +419,7 @@ __device__ __forceinline__ void dip_pair(double dx, double dy, double dz, double
  *  Force and torque on each home particle accumulate in LDS (FP64 LDS
  *  atomics) and are added to the caller's arrays once. */
 template <int NSUB, bool EV>
-__global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__restrict__ offsets,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_sr_dip_queue(CellGrid g, int const *__restrict__ offsets,
                                                       double4 const *__restrict__ spos,
                                                       double4 const *__restrict__ sdip,
                                                       int const *__restrict__ sidx,
@@ -428,15 +428,15 @@ __global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__r
                                                       double *__restrict__ torque,
                                                       double *partials) {
   constexpr int NT = 256, W = 2 * NSUB + 1, NN = W * W * W;
-  constexpr int TPT = 4;                  // candidate pairs tested per thread and chunk
+  constexpr int TPT = 2;                  // candidate pairs tested per thread and chunk
   constexpr int CH = NT * TPT;            // pairs per chunk
-  constexpr int QCAP = 2048, TCAP = 512;  // queue entries; candidates per table window
+  constexpr int QCAP = 1024, TCAP = 256;  // queue entries; candidates per table window
   __shared__ double4 hp[NT], hm[NT];
   __shared__ double acc[NT * 6];
   __shared__ int2 queue[QCAP]; // (candidate index, home slot | neighbour << 16)
   __shared__ int2 ctab[TCAP];  // (candidate index, neighbour)
   __shared__ int nbeg[NN], npre[NN + 1];
-  __shared__ double nsh[NN * 3];
+  __shared__ signed char nsc[NN * 3]; // periodic shift of each neighbour in box lengths
   __shared__ int qn;
   __shared__ double red[16];
   int const cell = blockIdx.x;
@@ -448,16 +448,16 @@ __global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__r
   }
   for (int o = threadIdx.x; o < NN; o += NT) {
     int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
-    double sx = 0., sy = 0., sz = 0.;
-    if (nx < 0) { nx += g.nc[0]; sx = -g.L[0]; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = g.L[0]; }
-    if (ny < 0) { ny += g.nc[1]; sy = -g.L[1]; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = g.L[1]; }
-    if (nz < 0) { nz += g.nc[2]; sz = -g.L[2]; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = g.L[2]; }
+    int sx = 0, sy = 0, sz = 0;
+    if (nx < 0) { nx += g.nc[0]; sx = -1; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = 1; }
+    if (ny < 0) { ny += g.nc[1]; sy = -1; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = 1; }
+    if (nz < 0) { nz += g.nc[2]; sz = -1; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = 1; }
     int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
     nbeg[o] = offsets[nb];
     npre[o + 1] = offsets[nb + 1] - offsets[nb];
-    nsh[3 * o] = sx;
-    nsh[3 * o + 1] = sy;
-    nsh[3 * o + 2] = sz;
+    nsc[3 * o] = static_cast<signed char>(sx);
+    nsc[3 * o + 1] = static_cast<signed char>(sy);
+    nsc[3 * o + 2] = static_cast<signed char>(sz);
   }
   if (threadIdx.x == 0) qn = 0;
   __syncthreads();
@@ -504,9 +504,9 @@ __global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__r
             int2 const cj = ctab[k];
             double4 const pj = spos[cj.x];
             double4 const pi = hp[il];
-            double const dx = pi.x - (pj.x + nsh[3 * cj.y]);
-            double const dy = pi.y - (pj.y + nsh[3 * cj.y + 1]);
-            double const dz = pi.z - (pj.z + nsh[3 * cj.y + 2]);
+            double const dx = pi.x - (pj.x + nsc[3 * cj.y] * g.L[0]);
+            double const dy = pi.y - (pj.y + nsc[3 * cj.y + 1] * g.L[1]);
+            double const dz = pi.z - (pj.z + nsc[3 * cj.y + 2] * g.L[2]);
             double const r2 = dx * dx + dy * dy + dz * dz;
             hit = r2 < rc2 && r2 > 0.;
             ent = make_int2(cj.x, il | (cj.y << 16));
@@ -527,9 +527,9 @@ __global__ __launch_bounds__(256) void k_sr_dip_queue(CellGrid g, int const *__r
             int const il = en.y & 0xffff, o = en.y >> 16;
             double4 const pj = spos[en.x];
             double4 const pi = hp[il];
-            double const dx = pi.x - (pj.x + nsh[3 * o]);
-            double const dy = pi.y - (pj.y + nsh[3 * o + 1]);
-            double const dz = pi.z - (pj.z + nsh[3 * o + 2]);
+            double const dx = pi.x - (pj.x + nsc[3 * o] * g.L[0]);
+            double const dy = pi.y - (pj.y + nsc[3 * o + 1] * g.L[1]);
+            double const dz = pi.z - (pj.z + nsc[3 * o + 2] * g.L[2]);
             double const r2 = dx * dx + dy * dy + dz * dz;
             double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
             dip_pair<EV>(dx, dy, dz, r2, hm[il], sdip[en.x], prefactor, alpha, exact, f, t, e);
