@@ -98,6 +98,27 @@ def test_short_range_cell_lists_match_oracle(subdiv):
         dp.set_short_range_subdivision(3)
 
 
+def test_particle_count_beyond_int32_rejected():
+    """The device sort indexes particles with 32-bit integers: a count of
+    2^31 is rejected before any array is read."""
+    import ctypes as C
+    import torch
+    from espresso_amd import _lib
+    from espresso_amd.magnetostatics import DipolarP3M
+    pos, dip = _system(50, 10., 3)
+    dp = DipolarP3M(prefactor=1., accuracy=1e-4, mesh=8, cao=3, alpha=0.9, r_cut=3.,
+                    tune=False)
+    dp.activate(pos, dip, box_l=10.)
+    t = torch.zeros((4, 3), dtype=torch.float64, device="cuda")
+    e = C.c_double(0.)
+    L = _lib.load()
+    for fn in (L.p3m_dipolar_solve, L.p3m_dipolar_short_range):
+        rc = fn(dp._ctx, 2 ** 31, t.data_ptr(), t.data_ptr(), None, None,
+                _lib.P3M_ENERGY | _lib.P3M_DEVICE_POINTERS, C.byref(e))
+        with pytest.raises(ValueError, match="2\\^31"):
+            _lib.raise_for(rc, dipolar=dp._ctx)
+
+
 @pytest.mark.parametrize("subdiv", [1, 2])
 def test_short_range_dense_cluster_matches_oracle(subdiv):
     """400 dipoles packed into one cell among 3000 sparse ones: a home cell
