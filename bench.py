@@ -380,7 +380,7 @@ def timed_loop(p3m, tp, tq, tf, steps, warmup, torch):
     return (t1 - t0) / steps, {k: v / max(nprof, 1) for k, v in stages.items()}
 
 
-def bench_elc(steps, warmup, torch, dev, n=100_000, L=60., gap=10., seed=7001):
+def bench_elc(steps, warmup, torch, dev, n=100_000, L=60., gap=10., seed=7001, cpu=True):
     """Slab workload: n charges (+-1 alternating) in an L x L x (L + gap) box
     with the top gap empty; P3M (FP64, timed tuning at 1e-4) with and
     without ELC (maxPWerror 1e-4).  The ELC share is the difference of the
@@ -431,11 +431,26 @@ def bench_elc(steps, warmup, torch, dev, n=100_000, L=60., gap=10., seed=7001):
            "p3m_ms": round(t_p3m * 1e3, 4), "p3m_elc_ms": round(t_all * 1e3, 4),
            "elc_ms": round(t_elc * 1e3, 4),
            "elc_particle_modes_per_s": round(n * modes / t_elc, 1)}
+    if cpu:
+        # CPU baseline: the oracle's numpy restatement of the ELC terms
+        # (elc.cpp:830-940) on the same slab, one thread
+        from threadpoolctl import threadpool_limits
+        from oracle import elc as E
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            E.elc_forces_energy(pos, q, box, 1.0, gap, fc)
+            t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(n / t_cpu / 1e6, 4), "unit": "Mparticles/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "ELC forces and energy of the full slab by the "
+                                         "oracle's numpy restatement, single thread",
+                               "seconds": round(t_cpu, 3),
+                               "gpu_vs_cpu": round(t_cpu / t_elc, 1)}
     print(f"[bench] elc: {rec}", flush=True)
     return rec
 
 
-def bench_dp3m(steps, warmup, torch, dev, n=100_000, seed=7101):
+def bench_dp3m(steps, warmup, torch, dev, n=100_000, seed=7101, cpu=True, cpu_n=20_000):
     """Dipolar P3M at the density of the reference's dipolar test
     (0.09 per unit volume, dipolar_mdlc_p3m_scafacos_p2nfft.py): n random
     unit dipoles in a cubic box, mesh 64, cao 5, accuracy 1e-4 (alpha and
@@ -475,6 +490,23 @@ def bench_dp3m(steps, warmup, torch, dev, n=100_000, seed=7101):
            "kspace_mparticles_per_s": round(n / t_k / 1e6, 2),
            "real_space_ms": round(t_sr * 1e3, 4),
            "total_mparticles_per_s": round(n / (t_k + t_sr) / 1e6, 2)}
+    if cpu:
+        # CPU baseline: the oracle's numpy restatement of the dipolar k-space
+        # (dp3m.cpp:246-517) for the first cpu_n dipoles of the same box and
+        # mesh (the particle work dominates), one thread
+        from threadpoolctl import threadpool_limits
+        from oracle import dp3m as D
+        ora = D.OracleDP3M(L, 64, 5, prm["alpha"], prm["r_cut"], 1.0)
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            ora.solve(pos[:cpu_n], dip[:cpu_n])
+            t_cpu = time.perf_counter() - t0
+        rec["cpu_baseline"] = {"value": round(cpu_n / t_cpu / 1e6, 4),
+                               "unit": "Mparticles/s", "cores": 1, "kind": "port",
+                               "sample": f"k-space forces, torques and energy of the first "
+                                         f"{cpu_n} dipoles (same box, mesh 64, cao 5) by the "
+                                         f"oracle's numpy restatement, single thread",
+                               "seconds": round(t_cpu, 3)}
     print(f"[bench] dp3m: {rec}", flush=True)
     return rec
 
@@ -521,8 +553,10 @@ def main():
         dev = torch.device("cuda:0")
         runs = {"batched": lambda: bench_batched(args.steps, args.warmup, torch, dev),
                 "serving": lambda: bench_serving(torch, dev),
-                "elc": lambda: bench_elc(args.steps, args.warmup, torch, dev),
-                "dp3m": lambda: bench_dp3m(args.steps, args.warmup, torch, dev)}
+                "elc": lambda: bench_elc(args.steps, args.warmup, torch, dev,
+                                         cpu=not args.no_cpu_baseline),
+                "dp3m": lambda: bench_dp3m(args.steps, args.warmup, torch, dev,
+                                           cpu=not args.no_cpu_baseline)}
         res = runs[args.only]()
         print(json.dumps({"workload": args.only, "record": res}), flush=True)
         return
@@ -653,11 +687,11 @@ def main():
 
     elc = None
     if not args.no_elc:
-        elc = bench_elc(args.steps, args.warmup, torch, dev)
+        elc = bench_elc(args.steps, args.warmup, torch, dev, cpu=not args.no_cpu_baseline)
 
     dp3m = None
     if not args.no_dp3m:
-        dp3m = bench_dp3m(args.steps, args.warmup, torch, dev)
+        dp3m = bench_dp3m(args.steps, args.warmup, torch, dev, cpu=not args.no_cpu_baseline)
 
     batched = None
     if not args.no_batched:
