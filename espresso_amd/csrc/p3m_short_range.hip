@@ -488,19 +488,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       __syncthreads();
       int const total = ni * nw;
       bool const last_window = kw + TCAP >= ncand;
-      float const rni = 1.f / static_cast<float>(ni);
+      // pair c = k ni + il (candidate k, home slot il) of this thread:
+      // c = tid at the start, advanced by NT per test
+      int const dk = NT / ni, dil = NT - dk * ni;
+      int k = static_cast<int>(threadIdx.x) / ni;
+      int il = static_cast<int>(threadIdx.x) - k * ni;
       for (int c0 = 0; c0 < total; c0 += CH) {
 #pragma unroll
         for (int u = 0; u < TPT; ++u) {
-          int const c = c0 + u * NT + static_cast<int>(threadIdx.x);
           bool hit = false;
           int2 ent = make_int2(0, 0);
-          if (c < total) {
-            // k = c / ni from a float reciprocal, corrected (c < 2^19)
-            int k = static_cast<int>(static_cast<float>(c) * rni);
-            if (k * ni > c) --k;
-            if ((k + 1) * ni <= c) ++k;
-            int const il = c - k * ni;
+          if (k < nw) { // c < total
             int2 const cj = ctab[k];
             double4 const pj = spos[cj.x];
             double4 const pi = hp[il];
@@ -510,6 +508,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             double const r2 = dx * dx + dy * dy + dz * dz;
             hit = r2 < rc2 && r2 > 0.;
             ent = make_int2(cj.x, il | (cj.y << 16));
+          }
+          il += dil;
+          k += dk;
+          if (il >= ni) {
+            il -= ni;
+            ++k;
           }
           uint64_t const mask = __ballot(hit);
           int base = 0;
