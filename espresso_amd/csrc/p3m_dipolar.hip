@@ -24,16 +24,17 @@ constexpr int kDpGatherBlock = 512;
 // for the kernel's static LDS
 constexpr int kDpGatherLds = 160 * 1024 - 256;
 
-__global__ void k_dp_clear(int *flag) { *flag = 0; }
 
-__global__ __launch_bounds__(kDpBlock) void k_dp_izero(int *a, int n) {
-  int const i = blockIdx.x * kDpBlock + threadIdx.x;
-  if (i < n) a[i] = 0;
-}
 
-__global__ __launch_bounds__(kDpBlock) void k_dp_zero(C2 *a, int64_t n) {
+/** Start of a solve: the three dipole meshes (blockIdx.y), the bucket
+ *  counters and the error flag set to zero in one launch. */
+__global__ __launch_bounds__(kDpBlock) void k_dp_prep(C2 *a0, C2 *a1, C2 *a2, int64_t m3,
+                                                      int *counts, int nk, int *err) {
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  if (i < n) a[i] = C2{0., 0.};
+  C2 *a = blockIdx.y == 0 ? a0 : blockIdx.y == 1 ? a1 : a2;
+  if (i < m3) a[i] = C2{0., 0.};
+  if (blockIdx.y == 0 && i < nk) counts[i] = 0;
+  if (blockIdx.y == 0 && i == 0) *err = 0;
 }
 
 /** Weights and periodic index of the first stencil point of one particle
@@ -377,23 +378,37 @@ __global__ __launch_bounds__(kDpGatherBlock) void k_dp_gather_tile(
 
 /** Counting sort of the particles into nbk^3 buckets of mesh cells (the
  *  spread and gather take one bucket per block): bucket key and rank per
- *  particle. */
+ *  particle, and the block partials of sum |mu|^2 and of the total dipole
+ *  (energy self term and surface term). */
 __global__ __launch_bounds__(kDpBlock) void k_dp_key(int64_t n, double const *__restrict__ pos,
-                                                     double L, int nbk, int *__restrict__ counts,
-                                                     int *__restrict__ key,
-                                                     int *__restrict__ rank) {
+                                                     double const *__restrict__ dip, double L,
+                                                     int nbk, int *__restrict__ counts,
+                                                     int *__restrict__ key, int *__restrict__ rank,
+                                                     double *__restrict__ partials) {
+  __shared__ double red[4 * 16];
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  if (i >= n) return;
-  int c[3];
+  double v[4] = {0., 0., 0., 0.};
+  if (i < n) {
+    int c[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double x = pos[3 * i + d];
-    x -= L * floor(x / L);
-    c[d] = isfinite(x) ? min(max(static_cast<int>(x / L * nbk), 0), nbk - 1) : 0;
+    for (int d = 0; d < 3; ++d) {
+      double x = pos[3 * i + d];
+      x -= L * floor(x / L);
+      c[d] = isfinite(x) ? min(max(static_cast<int>(x / L * nbk), 0), nbk - 1) : 0;
+    }
+    int const k = (c[0] * nbk + c[1]) * nbk + c[2];
+    key[i] = k;
+    rank[i] = atomicAdd(&counts[k], 1);
+    double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
+    v[0] = m0 * m0 + m1 * m1 + m2 * m2;
+    v[1] = m0;
+    v[2] = m1;
+    v[3] = m2;
   }
-  int const k = (c[0] * nbk + c[1]) * nbk + c[2];
-  key[i] = k;
-  rank[i] = atomicAdd(&counts[k], 1);
+  // moment partials of the block: sum |mu|^2 and the total dipole
+  block_sum<4>(v, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = v[k];
 }
 
 /** Exclusive scan of the block counts (one block of 1024 threads). */
@@ -455,22 +470,6 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_reduce4(double const *__restric
 }
 
 /** Sums of |mu|^2 and of the dipole vector (block partials). */
-__global__ __launch_bounds__(kDpBlock) void k_dp_moments(int64_t n, double const *__restrict__ dip,
-                                                         double *partials) {
-  __shared__ double red[4 * 16];
-  int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  double v[4] = {0., 0., 0., 0.};
-  if (i < n) {
-    double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
-    v[0] = m0 * m0 + m1 * m1 + m2 * m2;
-    v[1] = m0;
-    v[2] = m1;
-    v[3] = m2;
-  }
-  block_sum<4>(v, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = v[k];
-}
 
 /** Surface term torques (calc_surface_term, dp3m.cpp:451-517):
  *  torque -= pref mu x M_total. */
@@ -937,9 +936,6 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     hip_check(hipMalloc(&m->part, sizeof(double) * need), "hipMalloc dp3m partials");
     m->cap_part = need;
   }
-  // moments (sum |mu|^2, total dipole) on the device, no host round trip
-  k_dp_moments<<<nb, kDpBlock, 0, st>>>(n, dip, m->part);
-  k_dp_reduce4<<<1, kDpBlock, 0, st>>>(m->part, nb, m->mom);
   // particles sorted into buckets of S^3 mesh cells for the spread and
   // gather; the spread's LDS tile of E = ceil(M / nbk) + cao points per edge
   // and component stays within 64 KiB
@@ -972,8 +968,12 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       m->cap_keys = nk;
     }
   }
-  k_dp_izero<<<(nk + kDpBlock - 1) / kDpBlock, kDpBlock, 0, st>>>(m->counts, nk);
-  k_dp_key<<<nb, kDpBlock, 0, st>>>(n, pos, p.box, nbk, m->counts, m->key, m->rank);
+  k_dp_prep<<<dim3(static_cast<unsigned>(std::max<int64_t>(kb, (nk + kDpBlock - 1) / kDpBlock)), 3),
+              kDpBlock, 0, st>>>(m->mesh[0], m->mesh[1], m->mesh[2], M3, m->counts, nk, m->err);
+  // bucket keys, and the moments (sum |mu|^2, total dipole) on the device
+  k_dp_key<<<nb, kDpBlock, 0, st>>>(n, pos, dip, p.box, nbk, m->counts, m->key, m->rank,
+                                    m->part);
+  k_dp_reduce4<<<1, kDpBlock, 0, st>>>(m->part, nb, m->mom);
   k_dp_scan<<<1, 1024, 0, st>>>(m->counts, nk, m->offsets);
   k_dp_scatter<<<nb, kDpBlock, 0, st>>>(n, pos, dip, m->key, m->rank, m->offsets, m->spos,
                                         m->sdip, m->perm);
@@ -983,8 +983,6 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
   double const wn = 2. * std::numbers::pi / p.box;
   double const dipole_prefac = p.prefactor / static_cast<double>(M3);
   {
-    for (int c = 0; c < 3; ++c) k_dp_zero<<<kb, kDpBlock, 0, st>>>(m->mesh[c], M3);
-    k_dp_clear<<<1, 1, 0, st>>>(m->err);
     switch (p.cao) {
     case 1: launch_spread<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
     case 2: launch_spread<2>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
@@ -1018,8 +1016,9 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     if (wf || wt) {
       k_dp_back_meshes<<<kb, kDpBlock, 0, st>>>(m->mesh[3], m->mesh[4], M, m->d_ptrs,
                                                 m->d_ptrs + 3, wf ? 1 : 0);
-      fft(m->d_ptrs, 3, true);
-      if (wf) fft(m->d_ptrs + 3, 6, true);
+      // the field meshes [0..2] and force meshes [3..8] are consecutive in
+      // the table: one launch per axis transforms all of them
+      fft(m->d_ptrs, wf ? 9 : 3, true);
       double const pt = dipole_prefac * wn, pf = dipole_prefac * wn * wn;
       switch (p.cao) {
       case 1: launch_gather<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
