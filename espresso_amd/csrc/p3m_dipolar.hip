@@ -186,15 +186,17 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_fft(C2 *const *meshes, int M, i
   }
 }
 
-/** k-space step (dp3m.cpp:270-311, 322-343, 369-389): z = sum_d D_d A_d
- *  with D the integer wave numbers (Nyquist 0); energy partials of
- *  G_e |z|^2; Kt = G_e z (torques), Kf = -i G_f z (forces). */
-__global__ __launch_bounds__(kDpBlock) void k_dp_kspace(C2 const *__restrict__ A0,
-                                                        C2 const *__restrict__ A1,
-                                                        C2 const *__restrict__ A2,
+/** k-space step (dp3m.cpp:270-311, 322-343, 369-389) and back-transform
+ *  meshes in one pass over the mesh (Nyquist wave number 0): with
+ *  the transformed dipole components A_d (table entries 0..2) and the
+ *  integer wave numbers D, z = sum_d D_d A_d gives the energy partials
+ *  G_e |z|^2.  With back != 0 the field meshes T_d = D_d G_e z overwrite
+ *  A_d, and with forces != 0 the six force meshes F_ab = D_a D_b (-i G_f z)
+ *  (xx xy xz yy yz zz, table entries 3..8) are written. */
+__global__ __launch_bounds__(kDpBlock) void k_dp_kspace(C2 *const *T, C2 *const *F,
                                                         double const *__restrict__ Ge,
                                                         double const *__restrict__ Gf, int M,
-                                                        C2 *__restrict__ Kt, C2 *__restrict__ Kf,
+                                                        int back, int forces,
                                                         double *partials) {
   __shared__ double red[16];
   int64_t const M3 = (int64_t)M * M * M;
@@ -204,49 +206,35 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_kspace(C2 const *__restrict__ A
     int const iz = static_cast<int>(k % M);
     int const iy = static_cast<int>((k / M) % M);
     int const ix = static_cast<int>(k / ((int64_t)M * M));
-    double const dx = dop_shift(ix, M), dy = dop_shift(iy, M), dz = dop_shift(iz, M);
-    C2 const a0 = A0[k], a1 = A1[k], a2 = A2[k];
-    double const re = dx * a0.x + dy * a1.x + dz * a2.x;
-    double const im = dx * a0.y + dy * a1.y + dz * a2.y;
-    double const ge = Ge[k], gf = Gf[k];
+    double const D[3] = {static_cast<double>(dop_shift(ix, M)),
+                         static_cast<double>(dop_shift(iy, M)),
+                         static_cast<double>(dop_shift(iz, M))};
+    C2 const a0 = T[0][k], a1 = T[1][k], a2 = T[2][k];
+    double const re = D[0] * a0.x + D[1] * a1.x + D[2] * a2.x;
+    double const im = D[0] * a0.y + D[1] * a1.y + D[2] * a2.y;
+    double const ge = Ge[k];
     e = ge * (re * re + im * im);
-    Kt[k] = C2{ge * re, ge * im};
-    Kf[k] = C2{gf * im, -gf * re};
+    if (back) {
+      C2 const kt = C2{ge * re, ge * im};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) T[d][k] = C2{D[d] * kt.x, D[d] * kt.y};
+      if (forces) {
+        double const gf = Gf[k];
+        C2 const kf = C2{gf * im, -gf * re};
+        int c = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = a; b < 3; ++b, ++c) {
+            double const sab = D[a] * D[b];
+            F[c][k] = C2{sab * kf.x, sab * kf.y};
+          }
+      }
+    }
   }
   double v[1] = {e};
   block_sum<1>(v, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
-}
-
-/** Torque field meshes T_d = D_d Kt (overwrite the component meshes) and
- *  the six force meshes F_ab = D_a D_b Kf (xx xy xz yy yz zz). */
-__global__ __launch_bounds__(kDpBlock) void k_dp_back_meshes(C2 const *__restrict__ Kt,
-                                                             C2 const *__restrict__ Kf, int M,
-                                                             C2 *const *T, C2 *const *F,
-                                                             int with_forces) {
-  int64_t const M3 = (int64_t)M * M * M;
-  int64_t const k = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
-  if (k >= M3) return;
-  int const iz = static_cast<int>(k % M);
-  int const iy = static_cast<int>((k / M) % M);
-  int const ix = static_cast<int>(k / ((int64_t)M * M));
-  double const D[3] = {static_cast<double>(dop_shift(ix, M)),
-                       static_cast<double>(dop_shift(iy, M)),
-                       static_cast<double>(dop_shift(iz, M))};
-  C2 const kt = Kt[k];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) T[d][k] = C2{D[d] * kt.x, D[d] * kt.y};
-  if (with_forces) {
-    C2 const kf = Kf[k];
-    int c = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = a; b < 3; ++b, ++c) {
-        double const s = D[a] * D[b];
-        F[c][k] = C2{s * kf.x, s * kf.y};
-      }
-  }
 }
 
 /** AssignTorques / AssignForces (dp3m.cpp:195-244): torque -= mu x (pt E),
@@ -701,7 +689,7 @@ struct DipolarSolver::Impl {
   FftPlanDev plan{};
   int M = 0;
   C2 *tw = nullptr, *twi = nullptr;
-  C2 *mesh[11] = {};      // A0..A2 (T0..T2), Kt, Kf, F0..F5
+  C2 *mesh[9] = {};       // A0..A2 (T0..T2), F0..F5
   C2 **d_ptrs = nullptr;  // device pointer table: A[3], F[6]
   double *Ge = nullptr, *Gf = nullptr;
   double *part = nullptr;
@@ -807,9 +795,8 @@ void DipolarSolver::configure(DipolarParams const &p) {
           C2{static_cast<double>(std::cos(ang)), static_cast<double>(std::sin(ang))};
     }
     hip_check(hipMemcpy(m->tw, tw.data(), sizeof(C2) * M, hipMemcpyHostToDevice), "copy tw");
-    C2 *tab[12] = {m->mesh[0], m->mesh[1], m->mesh[2], m->mesh[5], m->mesh[6],
-                   m->mesh[7], m->mesh[8], m->mesh[9], m->mesh[10], m->mesh[3],
-                   m->mesh[4], nullptr};
+    C2 *tab[12] = {m->mesh[0], m->mesh[1], m->mesh[2], m->mesh[3], m->mesh[4], m->mesh[5],
+                   m->mesh[6], m->mesh[7], m->mesh[8], nullptr, nullptr, nullptr};
     hip_check(hipMemcpy(m->d_ptrs, tab, sizeof(tab), hipMemcpyHostToDevice), "copy table");
     m->M = M;
   }
@@ -998,11 +985,11 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       for (int axis = 2; axis >= 0; --axis)
         k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, axis, m->plan, m->tw, inv);
     };
-    // device table: [0..2] A (torque meshes after the k-space step),
-    // [3..8] F, [9] Kt, [10] Kf
+    // device table: [0..2] A (field meshes after the k-space step), [3..8] F
     fft(m->d_ptrs, 3, false);
-    k_dp_kspace<<<kb, kDpBlock, 0, st>>>(m->mesh[0], m->mesh[1], m->mesh[2], m->Ge, m->Gf, M,
-                                         m->mesh[3], m->mesh[4], m->part);
+    bool const wf = force != nullptr, wt = torque != nullptr;
+    k_dp_kspace<<<kb, kDpBlock, 0, st>>>(m->d_ptrs, m->d_ptrs + 3, m->Ge, m->Gf, M,
+                                         (wf || wt) ? 1 : 0, wf ? 1 : 0, m->part);
     std::vector<double> ep(static_cast<size_t>(kb));
     double mom[4] = {0., 0., 0., 0.};
     if (energy) {
@@ -1012,10 +999,7 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       hip_check(hipMemcpyAsync(mom, m->mom, sizeof(mom), hipMemcpyDeviceToHost, st),
                 "copy dp3m moments");
     }
-    bool const wf = force != nullptr, wt = torque != nullptr;
     if (wf || wt) {
-      k_dp_back_meshes<<<kb, kDpBlock, 0, st>>>(m->mesh[3], m->mesh[4], M, m->d_ptrs,
-                                                m->d_ptrs + 3, wf ? 1 : 0);
       // the field meshes [0..2] and force meshes [3..8] are consecutive in
       // the table: one launch per axis transforms all of them
       fft(m->d_ptrs, wf ? 9 : 3, true);
