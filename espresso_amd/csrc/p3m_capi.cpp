@@ -1273,8 +1273,10 @@ struct DpArgs {
 DpArgs dp_stage(p3m_dipolar *c, int64_t n, double const *pos, double const *dip, double *f,
                 double *t, unsigned flags) {
   if (n > 0 && (!pos || !dip)) throw std::invalid_argument("DipolarP3M: null particle arrays");
-  // the device sorts index particles with 32-bit integers
-  if (n > INT32_MAX) throw std::invalid_argument("DipolarP3M: more than 2^31 - 1 particles");
+  // the device sorts index particles with 32-bit integers (ranks times the
+  // number of counter copies included)
+  if (n > INT32_MAX - 8191)
+    throw std::invalid_argument("DipolarP3M: more than 2^31 - 8192 particles");
   if (flags & P3M_DEVICE_POINTERS) return {pos, dip, f, t};
   if (n > c->cap) {
     hip_check(hipSetDevice(c->device), "hipSetDevice");

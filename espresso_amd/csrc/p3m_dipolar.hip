@@ -20,6 +20,9 @@ using C2 = double2;
 constexpr int kDpBlock = 256;
 constexpr int kDpLines = 8; // FFT lines per block
 constexpr int kDpGatherBlock = 512;
+// bucket counters per bucket: block b counts into copy b % kDpCopies, which
+// spreads the returning atomics on one bucket over several addresses
+constexpr int kDpCopies = 8;
 // dynamic LDS of the gather: gfx950's 160 KiB per workgroup less a margin
 // for the kernel's static LDS
 constexpr int kDpGatherLds = 160 * 1024 - 256;
@@ -33,7 +36,7 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_prep(C2 *a0, C2 *a1, C2 *a2, in
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
   C2 *a = blockIdx.y == 0 ? a0 : blockIdx.y == 1 ? a1 : a2;
   if (i < m3) a[i] = C2{0., 0.};
-  if (blockIdx.y == 0 && i < nk) counts[i] = 0;
+  if (blockIdx.y == 0 && i < (int64_t)nk * kDpCopies) counts[i] = 0;
   if (blockIdx.y == 0 && i == 0) *err = 0;
 }
 
@@ -385,8 +388,9 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_key(int64_t n, double const *__
       c[d] = isfinite(x) ? min(max(static_cast<int>(x / L * nbk), 0), nbk - 1) : 0;
     }
     int const k = (c[0] * nbk + c[1]) * nbk + c[2];
+    int const copy = static_cast<int>(blockIdx.x) & (kDpCopies - 1);
     key[i] = k;
-    rank[i] = atomicAdd(&counts[k], 1);
+    rank[i] = atomicAdd(&counts[k * kDpCopies + copy], 1) * kDpCopies + copy;
     double const m0 = dip[3 * i], m1 = dip[3 * i + 1], m2 = dip[3 * i + 2];
     v[0] = m0 * m0 + m1 * m1 + m2 * m2;
     v[1] = m0;
@@ -399,14 +403,18 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_key(int64_t n, double const *__
     for (int k = 0; k < 4; ++k) partials[blockIdx.x * 4 + k] = v[k];
 }
 
-/** Exclusive scan of the block counts (one block of 1024 threads). */
-__global__ __launch_bounds__(1024) void k_dp_scan(int const *__restrict__ counts, int nk,
-                                                  int *__restrict__ offsets) {
+/** Exclusive scan of the bucket counts (one block of 1024 threads): bucket
+ *  start offsets[k] and size total[k], and each counter copy turned into the
+ *  start of its run within the bucket. */
+__global__ __launch_bounds__(1024) void k_dp_scan(int *__restrict__ counts, int nk,
+                                                  int *__restrict__ offsets,
+                                                  int *__restrict__ total) {
   __shared__ int part[1024];
   int const per = (nk + 1023) / 1024;
   int const b = threadIdx.x * per;
   int s = 0;
-  for (int k = b; k < min(nk, b + per); ++k) s += counts[k];
+  for (int k = b; k < min(nk, b + per); ++k)
+    for (int c = 0; c < kDpCopies; ++c) s += counts[k * kDpCopies + c];
   part[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -421,7 +429,13 @@ __global__ __launch_bounds__(1024) void k_dp_scan(int const *__restrict__ counts
   int acc = part[threadIdx.x];
   for (int k = b; k < min(nk, b + per); ++k) {
     offsets[k] = acc;
-    acc += counts[k];
+    int const a0 = acc;
+    for (int c = 0; c < kDpCopies; ++c) {
+      int const v = counts[k * kDpCopies + c];
+      counts[k * kDpCopies + c] = acc;
+      acc += v;
+    }
+    total[k] = acc - a0;
   }
 }
 
@@ -429,13 +443,15 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_scatter(int64_t n, double const
                                                          double const *__restrict__ dip,
                                                          int const *__restrict__ key,
                                                          int const *__restrict__ rank,
-                                                         int const *__restrict__ offsets,
+                                                         int const *__restrict__ starts,
                                                          double *__restrict__ spos,
                                                          double *__restrict__ sdip,
                                                          int *__restrict__ perm) {
   int64_t const i = (int64_t)blockIdx.x * kDpBlock + threadIdx.x;
   if (i >= n) return;
-  int64_t const s = offsets[key[i]] + rank[i];
+  int const r = rank[i];
+  int64_t const s = static_cast<int64_t>(starts[key[i] * kDpCopies + (r & (kDpCopies - 1))]) +
+                    r / kDpCopies;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     spos[3 * s + d] = pos[3 * i + d];
@@ -701,7 +717,8 @@ struct DipolarSolver::Impl {
   double *mom = nullptr; // device: sum |mu|^2, total dipole
   // particles sorted by blocks of mesh cells
   double *spos = nullptr, *sdip = nullptr;
-  int *key = nullptr, *rank = nullptr, *perm = nullptr, *counts = nullptr, *offsets = nullptr;
+  int *key = nullptr, *rank = nullptr, *perm = nullptr, *counts = nullptr, *offsets = nullptr,
+      *total = nullptr;
   int64_t cap_sort = 0;
   int cap_keys = 0;
 };
@@ -729,7 +746,8 @@ DipolarSolver::~DipolarSolver() {
   for (void *x : {static_cast<void *>(m->mom), static_cast<void *>(m->spos),
                   static_cast<void *>(m->sdip), static_cast<void *>(m->key),
                   static_cast<void *>(m->rank), static_cast<void *>(m->perm),
-                  static_cast<void *>(m->counts), static_cast<void *>(m->offsets)})
+                  static_cast<void *>(m->counts), static_cast<void *>(m->offsets),
+                  static_cast<void *>(m->total)})
     dp_free(x);
   if (m->st) (void)hipStreamDestroy(m->st);
 }
@@ -950,19 +968,23 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     if (nk > m->cap_keys) {
       dp_free(m->counts);
       dp_free(m->offsets);
-      hip_check(hipMalloc(&m->counts, sizeof(int) * nk), "hipMalloc dp3m sort");
+      dp_free(m->total);
+      hip_check(hipMalloc(&m->counts, sizeof(int) * nk * kDpCopies), "hipMalloc dp3m sort");
       hip_check(hipMalloc(&m->offsets, sizeof(int) * nk), "hipMalloc dp3m sort");
+      hip_check(hipMalloc(&m->total, sizeof(int) * nk), "hipMalloc dp3m sort");
       m->cap_keys = nk;
     }
   }
-  k_dp_prep<<<dim3(static_cast<unsigned>(std::max<int64_t>(kb, (nk + kDpBlock - 1) / kDpBlock)), 3),
+  k_dp_prep<<<dim3(static_cast<unsigned>(std::max<int64_t>(
+                       kb, (static_cast<int64_t>(nk) * kDpCopies + kDpBlock - 1) / kDpBlock)),
+                   3),
               kDpBlock, 0, st>>>(m->mesh[0], m->mesh[1], m->mesh[2], M3, m->counts, nk, m->err);
   // bucket keys, and the moments (sum |mu|^2, total dipole) on the device
   k_dp_key<<<nb, kDpBlock, 0, st>>>(n, pos, dip, p.box, nbk, m->counts, m->key, m->rank,
                                     m->part);
   k_dp_reduce4<<<1, kDpBlock, 0, st>>>(m->part, nb, m->mom);
-  k_dp_scan<<<1, 1024, 0, st>>>(m->counts, nk, m->offsets);
-  k_dp_scatter<<<nb, kDpBlock, 0, st>>>(n, pos, dip, m->key, m->rank, m->offsets, m->spos,
+  k_dp_scan<<<1, 1024, 0, st>>>(m->counts, nk, m->offsets, m->total);
+  k_dp_scatter<<<nb, kDpBlock, 0, st>>>(n, pos, dip, m->key, m->rank, m->counts, m->spos,
                                         m->sdip, m->perm);
   double const *sp = m->spos, *sd = m->sdip;
   int const *perm = m->perm;
@@ -971,13 +993,13 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
   double const dipole_prefac = p.prefactor / static_cast<double>(M3);
   {
     switch (p.cao) {
-    case 1: launch_spread<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    case 2: launch_spread<2>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    case 3: launch_spread<3>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    case 4: launch_spread<4>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    case 5: launch_spread<5>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    case 6: launch_spread<6>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
-    default: launch_spread<7>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->mesh, m->err); break;
+    case 1: launch_spread<1>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    case 2: launch_spread<2>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    case 3: launch_spread<3>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    case 4: launch_spread<4>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    case 5: launch_spread<5>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    case 6: launch_spread<6>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
+    default: launch_spread<7>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->mesh, m->err); break;
     }
     size_t const lds = sizeof(C2) * (M + 2 * kDpLines * (M + 1));
     int const lines = static_cast<int>((static_cast<int64_t>(M) * M + kDpLines - 1) / kDpLines);
@@ -1005,13 +1027,13 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
       fft(m->d_ptrs, wf ? 9 : 3, true);
       double const pt = dipole_prefac * wn, pf = dipole_prefac * wn * wn;
       switch (p.cao) {
-      case 1: launch_gather<1>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 2: launch_gather<2>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 3: launch_gather<3>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 4: launch_gather<4>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 5: launch_gather<5>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 6: launch_gather<6>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      default: launch_gather<7>(st, nk, E, m->offsets, m->counts, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 1: launch_gather<1>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 2: launch_gather<2>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 3: launch_gather<3>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 4: launch_gather<4>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 5: launch_gather<5>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 6: launch_gather<6>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      default: launch_gather<7>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
       }
     }
     hip_check(hipGetLastError(), "dipolar P3M kernels");
