@@ -20,6 +20,7 @@ using C2 = double2;
 constexpr int kDpBlock = 256;
 constexpr int kDpLines = 8; // FFT lines per block
 constexpr int kDpGatherBlock = 512;
+constexpr int kDpPlaneBlock = 1024; // threads of the z/y plane transform
 // bucket counters per bucket: block b counts into copy b % kDpCopies, which
 // spreads the returning atomics on one bucket over several addresses
 constexpr int kDpCopies = 8;
@@ -186,6 +187,39 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_fft(C2 *const *meshes, int M, i
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
     int const l = e / M, j = e - l * M;
     data[base(l0 + l) + j * stride] = res[l * ls + j];
+  }
+}
+
+/** z and y transforms of one x plane (blockIdx.x) of nmesh cubic meshes
+ *  (blockIdx.y) in LDS: z lines, a transpose, then y lines, for meshes whose
+ *  plane fits twice in a workgroup's LDS.  Unnormalised, sign as k_dp_fft. */
+__global__ __launch_bounds__(kDpPlaneBlock) void k_dp_fft_zy(C2 *const *meshes, int M,
+                                                             FftPlanDev plan,
+                                                             C2 const *__restrict__ tw, bool inv) {
+  extern __shared__ double smem_d[];
+  C2 *twl = reinterpret_cast<C2 *>(smem_d);
+  int const ls = M + 1;
+  int const MM = M * M;
+  C2 *X = twl + M;
+  C2 *Y = X + M * ls;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) twl[m] = tw[m];
+  C2 *data = meshes[blockIdx.y] + (int64_t)blockIdx.x * MM;
+  for (int e = threadIdx.x; e < MM; e += blockDim.x) {
+    int const y = e / M, z = e - y * M;
+    X[y * ls + z] = data[e];
+  }
+  __syncthreads();
+  C2 *r1 = lds_fft(X, Y, M, ls, plan, twl, inv); // along z, line y
+  C2 *o = (r1 == X) ? Y : X;
+  for (int e = threadIdx.x; e < MM; e += blockDim.x) {
+    int const y = e / M, z = e - y * M;
+    o[z * ls + y] = r1[y * ls + z];
+  }
+  __syncthreads();
+  C2 *r2 = lds_fft(o, r1, M, ls, plan, twl, inv); // along y, line z
+  for (int e = threadIdx.x; e < MM; e += blockDim.x) {
+    int const y = e / M, z = e - y * M;
+    data[e] = r2[z * ls + y];
   }
 }
 
@@ -890,6 +924,12 @@ void DipolarSolver::configure(DipolarParams const &p) {
     hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k_dp_fft),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds),
               "hipFuncSetAttribute");
+  size_t const lds_plane = sizeof(C2) * (M + 2 * static_cast<size_t>(M) * (M + 1));
+  if (lds_plane > 64 * 1024 && lds_plane <= static_cast<size_t>(kDpGatherLds))
+    hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k_dp_fft_zy),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds_plane)),
+              "hipFuncSetAttribute");
   m->p = p;
   m->configured = true;
 }
@@ -1003,9 +1043,18 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     }
     size_t const lds = sizeof(C2) * (M + 2 * kDpLines * (M + 1));
     int const lines = static_cast<int>((static_cast<int64_t>(M) * M + kDpLines - 1) / kDpLines);
+    size_t const lds_plane = sizeof(C2) * (M + 2 * static_cast<size_t>(M) * (M + 1));
+    bool const plane = lds_plane <= static_cast<size_t>(kDpGatherLds);
     auto fft = [&](C2 *const *table, int nmesh, bool inv) {
-      for (int axis = 2; axis >= 0; --axis)
-        k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, axis, m->plan, m->tw, inv);
+      if (plane) {
+        k_dp_fft_zy<<<dim3(M, nmesh), kDpPlaneBlock, lds_plane, st>>>(table, M, m->plan, m->tw,
+                                                                       inv);
+        k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, 0, m->plan, m->tw, inv);
+      } else {
+        for (int axis = 2; axis >= 0; --axis)
+          k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, axis, m->plan, m->tw,
+                                                              inv);
+      }
     };
     // device table: [0..2] A (field meshes after the k-space step), [3..8] F
     fft(m->d_ptrs, 3, false);
