@@ -158,10 +158,11 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_spread_tile(
 
 /** In-place complex FFT of the lines along `axis` of nmesh cubic meshes
  *  (blockIdx.y: mesh), kDpLines lines per block, unnormalised, forward
- *  sign -1 (inv: +1). */
+ *  sign -1 (inv: +1).  With rout, the real parts go to the real meshes
+ *  rout[mesh] instead (the last pass of a back transform). */
 __global__ __launch_bounds__(kDpBlock) void k_dp_fft(C2 *const *meshes, int M, int axis,
                                                      FftPlanDev plan, C2 const *__restrict__ tw,
-                                                     bool inv) {
+                                                     bool inv, double *const *rout) {
   extern __shared__ double smem_d[];
   C2 *twl = reinterpret_cast<C2 *>(smem_d);
   int const ls = M + 1;
@@ -184,6 +185,14 @@ __global__ __launch_bounds__(kDpBlock) void k_dp_fft(C2 *const *meshes, int M, i
   }
   __syncthreads();
   C2 *res = lds_fft(X, Y, kDpLines, ls, plan, twl, inv);
+  if (rout) { // last pass of a back transform: real parts only, compact
+    double *r = rout[blockIdx.y];
+    for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
+      int const l = e / M, j = e - l * M;
+      r[base(l0 + l) + j * stride] = res[l * ls + j].x;
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < nl * M; e += blockDim.x) {
     int const l = e / M, j = e - l * M;
     data[base(l0 + l) + j * stride] = res[l * ls + j];
@@ -287,7 +296,7 @@ template <int P>
 __global__ __launch_bounds__(kDpGatherBlock) void k_dp_gather_tile(
     int const *__restrict__ offsets, int const *__restrict__ counts,
     double const *__restrict__ pos, double const *__restrict__ dip, DevGeo g, int E,
-    C2 *const *T, C2 *const *F, double pt, double pf, int const *__restrict__ perm,
+    double const *const *R, double pt, double pf, int const *__restrict__ perm,
     double *__restrict__ force, double *__restrict__ torque) {
   extern __shared__ double tile[]; // [nm][E][E][E]: E_x E_y E_z H_xx .. H_zz
   __shared__ int org[3];
@@ -331,7 +340,7 @@ __global__ __launch_bounds__(kDpGatherBlock) void k_dp_gather_tile(
     int const tz = q % E, ty = (q / E) % E, tx = q / (E * E);
     int64_t const idx =
         ((int64_t)((b0 + tx) % M) * M + (b1 + ty) % M) * M + (b2 + tz) % M;
-    tile[e] = (c < 3 ? T[c] : F[c - 3])[idx].x;
+    tile[e] = R[c][idx];
   }
   __syncthreads();
   int const a = static_cast<int>(threadIdx.x & 7);
@@ -369,10 +378,10 @@ __global__ __launch_bounds__(kDpGatherBlock) void k_dp_gather_tile(
               int64_t const idx = ((int64_t)ix * M + wrap_once(gi[1] + b, M)) * M + iz;
               double const wt = w[0][rr] * w[1][b] * wz;
 #pragma unroll
-              for (int d = 0; d < 3; ++d) v[d] += wt * T[d][idx].x;
+              for (int d = 0; d < 3; ++d) v[d] += wt * R[d][idx];
               if (wf)
 #pragma unroll
-                for (int d = 0; d < 6; ++d) v[3 + d] += wt * F[d][idx].x;
+                for (int d = 0; d < 6; ++d) v[3 + d] += wt * R[3 + d][idx];
             }
           }
         }
@@ -741,6 +750,8 @@ struct DipolarSolver::Impl {
   C2 *tw = nullptr, *twi = nullptr;
   C2 *mesh[9] = {};       // A0..A2 (T0..T2), F0..F5
   C2 **d_ptrs = nullptr;  // device pointer table: A[3], F[6]
+  double *rmesh[9] = {};  // real parts of the back-transformed meshes
+  double **d_rptrs = nullptr; // device pointer table of rmesh
   double *Ge = nullptr, *Gf = nullptr;
   double *part = nullptr;
   size_t cap_part = 0;
@@ -763,6 +774,7 @@ DipolarSolver::DipolarSolver(int device) : m(std::make_unique<Impl>()) {
   hip_check(hipStreamCreate(&m->st), "hipStreamCreate");
   hip_check(hipMalloc(&m->err, sizeof(int)), "hipMalloc dp3m flag");
   hip_check(hipMalloc(&m->d_ptrs, sizeof(C2 *) * 12), "hipMalloc dp3m table");
+  hip_check(hipMalloc(&m->d_rptrs, sizeof(double *) * 9), "hipMalloc dp3m table");
   hip_check(hipMalloc(&m->mom, sizeof(double) * 4), "hipMalloc dp3m moments");
 }
 
@@ -770,9 +782,11 @@ DipolarSolver::~DipolarSolver() {
   (void)hipSetDevice(m->device);
   if (m->st) (void)hipStreamSynchronize(m->st);
   for (auto *x : m->mesh) dp_free(x);
+  for (auto *x : m->rmesh) dp_free(x);
   dp_free(m->tw);
   dp_free(m->twi);
   dp_free(m->d_ptrs);
+  dp_free(m->d_rptrs);
   dp_free(m->Ge);
   dp_free(m->Gf);
   dp_free(m->part);
@@ -832,11 +846,19 @@ void DipolarSolver::configure(DipolarParams const &p) {
       dp_free(x);
       x = nullptr;
     }
+    for (auto *&x : m->rmesh) {
+      dp_free(x);
+      x = nullptr;
+    }
     dp_free(m->tw);
     dp_free(m->Ge);
     dp_free(m->Gf);
     size_t const M3 = static_cast<size_t>(M) * M * M;
     for (auto *&x : m->mesh) hip_check(hipMalloc(&x, sizeof(C2) * M3), "hipMalloc dp3m mesh");
+    for (auto *&x : m->rmesh)
+      hip_check(hipMalloc(&x, sizeof(double) * M3), "hipMalloc dp3m real mesh");
+    hip_check(hipMemcpy(m->d_rptrs, m->rmesh, sizeof(m->rmesh), hipMemcpyHostToDevice),
+              "copy table");
     hip_check(hipMalloc(&m->Ge, sizeof(double) * M3), "hipMalloc dp3m G");
     hip_check(hipMalloc(&m->Gf, sizeof(double) * M3), "hipMalloc dp3m G");
     hip_check(hipMalloc(&m->tw, sizeof(C2) * M), "hipMalloc dp3m twiddles");
@@ -945,7 +967,7 @@ template <int P> void launch_spread(hipStream_t st, int nk, int E, int const *of
 }
 template <int P> void launch_gather(hipStream_t st, int nk, int E, int const *offsets,
                                     int const *counts, double const *pos, double const *dip,
-                                    DevGeo const &g, C2 *const *T, C2 *const *F, double pt,
+                                    DevGeo const &g, double const *const *R, double pt,
                                     double pf, int const *perm, double *force, double *torque) {
   static bool const raised = [] {
     hip_check(hipFuncSetAttribute(reinterpret_cast<void const *>(k_dp_gather_tile<P>),
@@ -957,7 +979,7 @@ template <int P> void launch_gather(hipStream_t st, int nk, int E, int const *of
   size_t const lds = sizeof(double) * (force ? 9 : 3) * static_cast<size_t>(E) * E * E;
   if (lds > static_cast<size_t>(kDpGatherLds))
     throw std::runtime_error("DipolarP3M: gather tile exceeds the LDS of a workgroup");
-  k_dp_gather_tile<P><<<nk, kDpGatherBlock, lds, st>>>(offsets, counts, pos, dip, g, E, T, F,
+  k_dp_gather_tile<P><<<nk, kDpGatherBlock, lds, st>>>(offsets, counts, pos, dip, g, E, R,
                                                        pt, pf, perm, force, torque);
 }
 } // namespace
@@ -1045,19 +1067,21 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     int const lines = static_cast<int>((static_cast<int64_t>(M) * M + kDpLines - 1) / kDpLines);
     size_t const lds_plane = sizeof(C2) * (M + 2 * static_cast<size_t>(M) * (M + 1));
     bool const plane = lds_plane <= static_cast<size_t>(kDpGatherLds);
-    auto fft = [&](C2 *const *table, int nmesh, bool inv) {
+    // rout: real meshes for the real parts of the last (x) pass, or null
+    auto fft = [&](C2 *const *table, int nmesh, bool inv, double *const *rout) {
       if (plane) {
         k_dp_fft_zy<<<dim3(M, nmesh), kDpPlaneBlock, lds_plane, st>>>(table, M, m->plan, m->tw,
                                                                        inv);
-        k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, 0, m->plan, m->tw, inv);
+        k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, 0, m->plan, m->tw, inv,
+                                                            rout);
       } else {
         for (int axis = 2; axis >= 0; --axis)
           k_dp_fft<<<dim3(lines, nmesh), kDpBlock, lds, st>>>(table, M, axis, m->plan, m->tw,
-                                                              inv);
+                                                              inv, axis == 0 ? rout : nullptr);
       }
     };
     // device table: [0..2] A (field meshes after the k-space step), [3..8] F
-    fft(m->d_ptrs, 3, false);
+    fft(m->d_ptrs, 3, false, nullptr);
     bool const wf = force != nullptr, wt = torque != nullptr;
     k_dp_kspace<<<kb, kDpBlock, 0, st>>>(m->d_ptrs, m->d_ptrs + 3, m->Ge, m->Gf, M,
                                          (wf || wt) ? 1 : 0, wf ? 1 : 0, m->part);
@@ -1073,16 +1097,16 @@ void DipolarSolver::solve(int64_t n, double const *pos, double const *dip, doubl
     if (wf || wt) {
       // the field meshes [0..2] and force meshes [3..8] are consecutive in
       // the table: one launch per axis transforms all of them
-      fft(m->d_ptrs, wf ? 9 : 3, true);
+      fft(m->d_ptrs, wf ? 9 : 3, true, m->d_rptrs);
       double const pt = dipole_prefac * wn, pf = dipole_prefac * wn * wn;
       switch (p.cao) {
-      case 1: launch_gather<1>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 2: launch_gather<2>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 3: launch_gather<3>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 4: launch_gather<4>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 5: launch_gather<5>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      case 6: launch_gather<6>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
-      default: launch_gather<7>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_ptrs, m->d_ptrs + 3, pt, pf, perm, force, torque); break;
+      case 1: launch_gather<1>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      case 2: launch_gather<2>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      case 3: launch_gather<3>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      case 4: launch_gather<4>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      case 5: launch_gather<5>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      case 6: launch_gather<6>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
+      default: launch_gather<7>(st, nk, E, m->offsets, m->total, sp, sd, m->g, m->d_rptrs, pt, pf, perm, force, torque); break;
       }
     }
     hip_check(hipGetLastError(), "dipolar P3M kernels");
