@@ -29,9 +29,10 @@ def _system(n, L, seed, zero_frac=0.1):
 
 
 # (8, 7): the per-bucket LDS tile (11 points per edge) wraps around the mesh;
-# (10, 5) and (12, 3): meshes that the buckets of 8 cells do not divide
+# (10, 5) and (12, 3): meshes that the buckets of 8 cells do not divide;
+# (80, 3): above 64 the transforms run one pass per axis (no plane kernel)
 @pytest.mark.parametrize("M,cao,epsilon", [(16, 3, 0.), (24, 5, 0.), (32, 7, 2.), (20, 4, 0.),
-                                           (8, 7, 0.), (10, 5, 0.), (12, 3, 2.)])
+                                           (8, 7, 0.), (10, 5, 0.), (12, 3, 2.), (80, 3, 0.)])
 def test_kspace_matches_oracle(M, cao, epsilon):
     from espresso_amd.magnetostatics import DipolarP3M
     from oracle import dp3m as D
