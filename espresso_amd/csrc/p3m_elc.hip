@@ -49,15 +49,76 @@ __device__ __forceinline__ double fold_into(double x, double L) {
   return xf;
 }
 
+/** sin and cos of kx x and ky y of one particle over a sequence of modes:
+ *  p = 1 (q = 1) from one sincos per particle, p = p_prev + 1 by angle
+ *  addition, p = p_prev reused, any other p by sincos (kx x); p = 0 (q = 0)
+ *  is the missing direction (cos = 1, sin = 0).  Angle addition over p
+ *  steps rounds to about p ulps. */
+struct ElcTrig {
+  double x, y, s1x, c1x, s1y, c1y, sx, cx, sy, cy;
+  int px, qy;
+  __device__ __forceinline__ ElcTrig(double4 const &p, ElcGeo const &g) : x(p.x), y(p.y) {
+    double const two_pi = 2. * 3.14159265358979323846;
+    sincos(two_pi * (1. / g.L[0]) * x, &s1x, &c1x);
+    sincos(two_pi * (1. / g.L[1]) * y, &s1y, &c1y);
+    sx = sy = 0.;
+    cx = cy = 1.;
+    px = qy = 0;
+  }
+  __device__ __forceinline__ void get(ElcMode const &md, double &osx, double &ocx, double &osy,
+                                      double &ocy) {
+    if (md.p == 0) {
+      osx = 0.;
+      ocx = 1.;
+    } else {
+      if (md.p != px) {
+        if (md.p == 1) {
+          sx = s1x;
+          cx = c1x;
+        } else if (md.p == px + 1) {
+          double const s = sx * c1x + cx * s1x, c = cx * c1x - sx * s1x;
+          sx = s;
+          cx = c;
+        } else {
+          sincos(md.kx * x, &sx, &cx);
+        }
+        px = md.p;
+      }
+      osx = sx;
+      ocx = cx;
+    }
+    if (md.q == 0) {
+      osy = 0.;
+      ocy = 1.;
+    } else {
+      if (md.q != qy) {
+        if (md.q == 1) {
+          sy = s1y;
+          cy = c1y;
+        } else if (md.q == qy + 1) {
+          double const s = sy * c1y + cy * s1y, c = cy * c1y - sy * s1y;
+          sy = s;
+          cy = c;
+        } else {
+          sincos(md.ky * y, &sy, &cy);
+        }
+        qy = md.q;
+      }
+      osy = sy;
+      ocy = cy;
+    }
+  }
+};
+
 /** The eight values of particle (x, y, z, q) for mode md, in the order
  *  [ss, sc, cs, cc] x [e^{+omega z}, e^{-omega z}] (v[0..3] plus, v[4..7]
  *  minus).  kx = 0 or ky = 0 stands for the missing direction (cos = 1,
  *  sin = 0), which turns the pq-mode values into those of the p- and
  *  q-modes (elc.cpp:500-600, 641-770). */
-__device__ __forceinline__ void mode_values(ElcMode const &md, double4 const &p, double v[8]) {
-  double sx = 0., cx = 1., sy = 0., cy = 1.;
-  if (md.kx != 0.) sincos(md.kx * p.x, &sx, &cx);
-  if (md.ky != 0.) sincos(md.ky * p.y, &sy, &cy);
+__device__ __forceinline__ void mode_values(ElcMode const &md, double4 const &p, ElcTrig &tr,
+                                            double v[8]) {
+  double sx, cx, sy, cy;
+  tr.get(md, sx, cx, sy, cy);
   double const ep = exp(md.omega * p.z);
   double const qp = p.w * ep, qm = p.w / ep;
   double const ss = sx * sy, sc = sx * cy, cs = cx * sy, cc = cx * cy;
@@ -162,10 +223,9 @@ __global__ __launch_bounds__(kElcBlock) void k_elc_prep(int64_t n, double const 
  *  layer images) and pref_di (the image series, elc.cpp:520-597). */
 template <bool DIEL>
 __device__ __forceinline__ void mode_sums(ElcMode const &md, double4 const &p, ElcGeo const &g,
-                                          double Lz, double acc[8]) {
-  double sx = 0., cx = 1., sy = 0., cy = 1.;
-  if (md.kx != 0.) sincos(md.kx * p.x, &sx, &cx);
-  if (md.ky != 0.) sincos(md.ky * p.y, &sy, &cy);
+                                          double Lz, ElcTrig &tr, double acc[8]) {
+  double sx, cx, sy, cy;
+  tr.get(md, sx, cx, sy, cy);
   double const w = md.omega, z = p.z, q = p.w;
   double const ep = exp(w * z);
   double cp = md.pref * q * ep, cm = md.pref * q / ep;
@@ -228,9 +288,10 @@ __global__ __launch_bounds__(kElcBlock) void k_elc_sums(int64_t n, int64_t ppb,
   for (int64_t i = beg + threadIdx.x; i < end; i += kElcBlock) {
     double4 const p = spos[i];
     if (p.w == 0.) continue;
+    ElcTrig tr(p, g);
 #pragma unroll
     for (int t = 0; t < kModeTile; ++t)
-      if (t < nm) mode_sums<DIEL>(modes[m0 + t], p, g, g.L[2], acc[t]);
+      if (t < nm) mode_sums<DIEL>(modes[m0 + t], p, g, g.L[2], tr, acc[t]);
   }
   int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -301,15 +362,16 @@ __global__ __launch_bounds__(kElcBlock) void k_elc_forces(
     double4 const p = spos[i];
     if (p.w != 0.) {
       double fx = 0., fy = 0., fz = 0.;
+      ElcTrig tr(p, g);
       for (int m = 0; m < nmodes; ++m) {
         ElcMode const md = modes[m];
         double a[8];
-        mode_values(md, p, a);
+        mode_values(md, p, tr, a);
         double const *g = G + m * 8;
         // plus values against minus sums and minus values against plus sums
         double const gp0 = g[0], gp1 = g[1], gp2 = g[2], gp3 = g[3];
         double const gm0 = g[4], gm1 = g[5], gm2 = g[6], gm3 = g[7];
-        double const inv_w = 1. / md.omega;
+        double const inv_w = md.inv_omega;
         if (md.kx != 0.) {
           double const t = a[5] * gp3 + a[4] * gp2 - a[7] * gp1 - a[6] * gp0 +
                            a[1] * gm3 + a[0] * gm2 - a[3] * gm1 - a[2] * gm0;
@@ -471,16 +533,17 @@ std::vector<ElcMode> elc_modes(ElcParams const &p) {
   auto const ny = static_cast<std::size_t>(std::ceil(p.far_cut * p.box[1]) + 1.);
   double const area_pi = p.prefactor * std::numbers::pi * ux * uy;
   std::vector<ElcMode> out;
-  auto add = [&](double kx, double ky, double omega, double c) {
-    out.push_back(ElcMode{kx, ky, omega, -c * area_pi / std::expm1(omega * Lz), c * area_pi});
+  auto add = [&](double kx, double ky, double omega, double c, std::size_t i, std::size_t j) {
+    out.push_back(ElcMode{kx, ky, omega, -c * area_pi / std::expm1(omega * Lz), c * area_pi,
+                          1. / omega, static_cast<int>(i), static_cast<int>(j)});
   };
   for (std::size_t i = 1; ux * static_cast<double>(i - 1) < p.far_cut && i <= nx; ++i) {
     double const om = two_pi * ux * static_cast<double>(i);
-    add(om, 0., om, 4.);
+    add(om, 0., om, 4., i, 0);
   }
   for (std::size_t j = 1; uy * static_cast<double>(j - 1) < p.far_cut && j <= ny; ++j) {
     double const om = two_pi * uy * static_cast<double>(j);
-    add(0., om, om, 4.);
+    add(0., om, om, 4., 0, j);
   }
   double const fc2 = p.far_cut * p.far_cut;
   for (std::size_t i = 1; ux * static_cast<double>(i - 1) < p.far_cut && i <= nx; ++i)
@@ -495,7 +558,7 @@ std::vector<ElcMode> elc_modes(ElcParams const &p) {
       add(kx, ky,
           two_pi * std::sqrt((ux * static_cast<double>(i)) * (ux * static_cast<double>(i)) +
                              (uy * static_cast<double>(j)) * (uy * static_cast<double>(j))),
-          8.);
+          8., i, j);
     }
   return out;
 }

@@ -42,6 +42,8 @@ struct ElcParams {
  *  expm1(omega L_z). */
 struct ElcMode {
   double kx, ky, omega, pref, pref_di;
+  double inv_omega;
+  int p, q; // kx = 2 pi p / L_x, ky = 2 pi q / L_y
 };
 
 /** Reciprocal vectors in the reference's loop order (elc.cpp:842-883). */
