@@ -1170,13 +1170,13 @@ void DipolarSolver::short_range(int64_t n, double const *pos, double const *dip,
   sp.alpha = p.alpha;
   sp.r_cut = p.r_cut;
   // cells of r_cut/2 test 42 % fewer candidates and give 8 times more
-  // blocks; cells of r_cut cost fewer passes over neighbour tables when the
-  // cells are sparse.  Measured crossover (N = 2e5, r_cut 10): between 2.7
-  // dipoles per r_cut/2 cell (0.76 against 2.33 ms) and 12.5 (2.81 against
-  // 2.51 ms); at 48: 10.5 against 6.2 ms
+  // blocks; cells of r_cut cost fewer passes over neighbour tables.
+  // Measured (N = 2e5, r_cut 10), r_cut against r_cut/2 cells: 0.76 against
+  // 2.41 ms at 2.7 dipoles per r_cut/2 cell, 1.98 against 2.54 at 12.5,
+  // 6.6 against 6.1 at 48, 71.5 against 19.8 at 125
   double const half = 0.5 * p.r_cut;
   double const per_cell = static_cast<double>(n) * half * half * half / (p.box * p.box * p.box);
-  sp.subdivision = m->subdivision ? m->subdivision : (per_cell >= 8. ? 2 : 1);
+  sp.subdivision = m->subdivision ? m->subdivision : (per_cell >= 40. ? 2 : 1);
   if (m->sr->compute_dipolar(st, sp, exact != 0, n, pos, dip, force, torque, energy)) {
     if (sync && !energy) hip_check(hipStreamSynchronize(st), "sync dp3m pairs");
     return;

@@ -410,8 +410,8 @@ __device__ __forceinline__ void dip_pair(double dx, double dy, double dz, double
   if constexpr (EV) e += prefactor * (mimj * B - mir * mjr * Cr);
 }
 
-/** Dipolar pair terms over cell lists with dense evaluation: one block of
- *  256 threads per home cell.  The block enumerates the (home particle,
+/** Dipolar pair terms over cells of edge r_cut/2 (NSUB = 2, dense systems),
+ *  dense evaluation per block: one block of 256 threads per home cell.  The block enumerates the (home particle,
  *  candidate) pairs of the (2 NSUB + 1)^3 neighbour cells 256 at a time,
  *  appends the pairs inside r_cut to an LDS queue (wave ballots, one LDS
  *  atomic per wave) and drains the queue with every lane evaluating one
@@ -565,6 +565,136 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
     }
     __syncthreads(); // before the next home chunk restages hp, hm and acc
+  }
+  if constexpr (EV) {
+    double v[1] = {e};
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x * 8] = v[0];
+  }
+}
+
+/** Dipolar pair terms over cells of edge r_cut (NSUB = 1), dense evaluation
+ *  per wave: one block of 256 threads per home cell.  Wave w takes the candidates w, w + 4, ... of
+ *  each neighbour cell (one uniform load per candidate), its lanes the home
+ *  particles; the pairs inside r_cut go to the wave's own LDS queue (ballot
+ *  and a wave-uniform count, no atomics), which the wave drains itself with
+ *  every lane evaluating one pair, so the pair terms run without idle
+ *  lanes however sparse the cells and the loop needs no block barriers.
+ *  Force and torque on each home particle accumulate in LDS (FP64 LDS
+ *  atomics) and are added to the caller's arrays once. */
+template <int NSUB, bool EV>
+__global__ __launch_bounds__(256) void k_sr_dip_wave(
+    CellGrid g, int const *__restrict__ offsets, double4 const *__restrict__ spos,
+    double4 const *__restrict__ sdip, int const *__restrict__ sidx, double prefactor,
+    double alpha, double rc2, int exact, double *__restrict__ force, double *__restrict__ torque,
+    double *partials) {
+  constexpr int NT = 256, NW = NT / 64, W = 2 * NSUB + 1, NN = W * W * W;
+  constexpr int QW = 256;   // queue entries per wave
+  __shared__ double4 hp[NT], hm[NT];
+  __shared__ double acc[NT * 6];
+  __shared__ int2 queue[NW][QW]; // (candidate index, home slot | neighbour << 16)
+  __shared__ int nbeg[NN], ncnt[NN];
+  __shared__ signed char nsc[NN * 3]; // periodic shift of each neighbour in box lengths
+  __shared__ double red[16];
+  int const cell = blockIdx.x;
+  int const cz = cell % g.nc[2], cy = (cell / g.nc[2]) % g.nc[1], cx = cell / (g.nc[2] * g.nc[1]);
+  int const beg = offsets[cell], end = offsets[cell + 1];
+  if (beg == end) {
+    if (EV && threadIdx.x == 0) partials[blockIdx.x * 8] = 0.;
+    return;
+  }
+  for (int o = threadIdx.x; o < NN; o += NT) {
+    int nx = cx + o / (W * W) - NSUB, ny = cy + (o / W) % W - NSUB, nz = cz + o % W - NSUB;
+    int sx = 0, sy = 0, sz = 0;
+    if (nx < 0) { nx += g.nc[0]; sx = -1; } else if (nx >= g.nc[0]) { nx -= g.nc[0]; sx = 1; }
+    if (ny < 0) { ny += g.nc[1]; sy = -1; } else if (ny >= g.nc[1]) { ny -= g.nc[1]; sy = 1; }
+    if (nz < 0) { nz += g.nc[2]; sz = -1; } else if (nz >= g.nc[2]) { nz -= g.nc[2]; sz = 1; }
+    int const nb = (nx * g.nc[1] + ny) * g.nc[2] + nz;
+    nbeg[o] = offsets[nb];
+    ncnt[o] = offsets[nb + 1] - offsets[nb];
+    nsc[3 * o] = static_cast<signed char>(sx);
+    nsc[3 * o + 1] = static_cast<signed char>(sy);
+    nsc[3 * o + 2] = static_cast<signed char>(sz);
+  }
+  int const lane = static_cast<int>(threadIdx.x & 63);
+  int const wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  int2 *const wq = queue[wave];
+  double e = 0.;
+  for (int hb = beg; hb < end; hb += NT) {
+    int const ni = min(NT, end - hb);
+    __syncthreads(); // neighbour table written; previous chunk's readers done
+    if (static_cast<int>(threadIdx.x) < ni) {
+      hp[threadIdx.x] = spos[hb + threadIdx.x];
+      hm[threadIdx.x] = sdip[hb + threadIdx.x];
+    }
+    for (int x = threadIdx.x; x < ni * 6; x += NT) acc[x] = 0.;
+    __syncthreads();
+    int const nr = (ni + 63) >> 6;
+    int count = 0; // wave-uniform
+    auto drain = [&]() {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int q = lane; q < count; q += 64) {
+        int2 const en = wq[q];
+        int const il = en.y & 0xffff, o = en.y >> 16;
+        double4 const pj = spos[en.x];
+        double4 const pi = hp[il];
+        double const dx = pi.x - (pj.x + nsc[3 * o] * g.L[0]);
+        double const dy = pi.y - (pj.y + nsc[3 * o + 1] * g.L[1]);
+        double const dz = pi.z - (pj.z + nsc[3 * o + 2] * g.L[2]);
+        double const r2 = dx * dx + dy * dy + dz * dz;
+        double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+        dip_pair<EV>(dx, dy, dz, r2, hm[il], sdip[en.x], prefactor, alpha, exact, f, t, e);
+        atomicAdd(&acc[il * 6], f[0]);
+        atomicAdd(&acc[il * 6 + 1], f[1]);
+        atomicAdd(&acc[il * 6 + 2], f[2]);
+        atomicAdd(&acc[il * 6 + 3], t[0]);
+        atomicAdd(&acc[il * 6 + 4], t[1]);
+        atomicAdd(&acc[il * 6 + 5], t[2]);
+      }
+      __builtin_amdgcn_wave_barrier();
+      count = 0;
+    };
+    for (int o = 0; o < NN; ++o) {
+      int const jb = __builtin_amdgcn_readfirstlane(nbeg[o]);
+      int const cnt = __builtin_amdgcn_readfirstlane(ncnt[o]);
+      double const shx = nsc[3 * o] * g.L[0], shy = nsc[3 * o + 1] * g.L[1],
+                   shz = nsc[3 * o + 2] * g.L[2];
+      for (int t = wave; t < cnt; t += NW) {
+        double4 const pj = spos[jb + t];
+        double const qx = pj.x + shx, qy = pj.y + shy, qz = pj.z + shz;
+        for (int r = 0; r < nr; ++r) {
+          int const il = lane + 64 * r;
+          double4 const pi = hp[min(il, NT - 1)];
+          double const dx = pi.x - qx, dy = pi.y - qy, dz = pi.z - qz;
+          double const r2 = dx * dx + dy * dy + dz * dz;
+          bool const hit = il < ni && r2 < rc2 && r2 > 0.;
+          uint64_t const mask = __ballot(hit);
+          if (hit)
+            wq[count + __popcll(mask & ((1ull << lane) - 1ull))] = make_int2(jb + t, il | (o << 16));
+          count += __popcll(mask);
+          // at most 64 entries join per test: drain while they still fit
+          if (count > QW - 64) drain();
+        }
+      }
+    }
+    drain();
+    __syncthreads(); // every wave's pairs are in acc
+    if (static_cast<int>(threadIdx.x) < ni) {
+      int64_t const o = 3 * static_cast<int64_t>(sidx[hb + threadIdx.x]);
+      double const *a = acc + threadIdx.x * 6;
+      if (force) {
+        force[o] += a[0];
+        force[o + 1] += a[1];
+        force[o + 2] += a[2];
+      }
+      if (torque) {
+        torque[o] += a[3];
+        torque[o + 1] += a[4];
+        torque[o + 2] += a[5];
+      }
+    }
   }
   if constexpr (EV) {
     double v[1] = {e};
@@ -762,6 +892,12 @@ bool ShortRange::compute_dipolar(hipStream_t st, ShortRangeParams const &p, bool
       k_sr_dip_queue<2, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
                                                     p.prefactor, p.alpha, rc2, ex, force, torque,
                                                     m->partials);
+    // cells of r_cut: the per-wave kernel needs about half a wave of home
+    // particles per cell; sparser cells run faster on the block queue
+    else if (static_cast<double>(n) >= 32. * static_cast<double>(ncells))
+      k_sr_dip_wave<1, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
+                                                   p.prefactor, p.alpha, rc2, ex, force, torque,
+                                                   m->partials);
     else
       k_sr_dip_queue<1, EV><<<ncells, 256, 0, st>>>(g, m->offsets, m->spos, m->sdip, m->sidx,
                                                     p.prefactor, p.alpha, rc2, ex, force, torque,

@@ -270,7 +270,7 @@ int p3m_dipolar_solve(p3m_dipolar *ctx, int64_t n, const double *pos, const doub
 int p3m_dipolar_short_range(p3m_dipolar *ctx, int64_t n, const double *pos, const double *dip,
                             double *force, double *torque, unsigned flags, double *energy);
 /* Cells per r_cut of p3m_dipolar_short_range: 0 = by occupancy (r_cut/2
- * cells when they hold about 8 dipoles or more), 1 or 2. */
+ * cells when they hold about 40 dipoles or more), 1 or 2. */
 int p3m_dipolar_set_short_range_subdivision(p3m_dipolar *ctx, int cells_per_cutoff);
 int p3m_dipolar_get_stream(const p3m_dipolar *ctx, void **stream);
 int p3m_dipolar_synchronize(p3m_dipolar *ctx);

@@ -120,11 +120,13 @@ def test_particle_count_beyond_int32_rejected():
             _lib.raise_for(rc, dipolar=dp._ctx)
 
 
-@pytest.mark.parametrize("subdiv", [1, 2])
-def test_short_range_dense_cluster_matches_oracle(subdiv):
+# r_cut 3.5: 125 cells of about 27 dipoles (block queue); r_cut 6.5: 27 cells
+# of about 126 dipoles (per-wave queues, home passes of more than 64 lanes)
+@pytest.mark.parametrize("subdiv,r_cut", [(1, 3.5), (2, 3.5), (1, 6.5)])
+def test_short_range_dense_cluster_matches_oracle(subdiv, r_cut):
     """400 dipoles packed into one cell among 3000 sparse ones: a home cell
     of more than 256 dipoles (two home passes), candidate tables of more
-    than one window and many queue drains per window."""
+    than one window and many queue drains."""
     from espresso_amd.magnetostatics import DipolarP3M
     from oracle import dp3m as D
     L = 20.
@@ -133,13 +135,13 @@ def test_short_range_dense_cluster_matches_oracle(subdiv):
     pos = np.concatenate([pos, 7.3 + rng.uniform(0., 1., (400, 3))])
     dip = np.concatenate([dip, rng.normal(size=(400, 3))])
     n = len(pos)
-    dp = DipolarP3M(prefactor=1.1, accuracy=1e-4, mesh=16, cao=4, alpha=0.9, r_cut=3.5,
+    dp = DipolarP3M(prefactor=1.1, accuracy=1e-4, mesh=16, cao=4, alpha=0.9, r_cut=r_cut,
                     tune=False)
     dp.activate(pos, dip, box_l=L)
     dp.set_short_range_subdivision(subdiv)
     f, t = np.zeros((n, 3)), np.zeros((n, 3))
     e = dp.add_short_range_forces(pos, dip, f, t, energy=True)
-    fr, tr, er = D.pair_terms(pos - L * np.floor(pos / L), dip, L, 1.1, 0.9, 3.5, 1e-4)
+    fr, tr, er = D.pair_terms(pos - L * np.floor(pos / L), dip, L, 1.1, 0.9, r_cut, 1e-4)
     assert S.rel_rms(f, fr) <= 1e-12 and S.rel_rms(t, tr) <= 1e-12
     assert abs(e - er) <= 1e-12 * abs(er)
 
