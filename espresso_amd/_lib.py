@@ -39,6 +39,9 @@ P3M_TUNE_RS_DEVICE = 1
 EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
     "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
+    "p3m_prepare_fft_mesh", "p3m_assign_charges", "p3m_charge_assign",
+    "p3m_count_charged_particles", "p3m_count_charged_particles_elc",
+    "p3m_adapt_epsilon_elc", "p3m_assigned_count", "p3m_solve_assigned",
     "p3m_debug_charge_mesh", "p3m_debug_indices", "p3m_k_space_error",
     "p3m_real_space_error", "p3m_pair_forces_targets", "p3m_pair_force",
     "p3m_pair_energy",
@@ -144,6 +147,14 @@ def load():
     L.p3m_is_tuned.argtypes = [vp, C.POINTER(C.c_int)]
     L.p3m_activate.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]
     L.p3m_solve.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp, dp]
+    L.p3m_prepare_fft_mesh.argtypes = [vp, C.c_int]
+    L.p3m_assign_charges.argtypes = [vp, C.c_int64, vp, vp, C.c_int, C.c_uint]
+    L.p3m_charge_assign.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]
+    L.p3m_count_charged_particles.argtypes = [vp, C.c_int64, vp, C.c_uint]
+    L.p3m_count_charged_particles_elc.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+    L.p3m_adapt_epsilon_elc.argtypes = [vp]
+    L.p3m_assigned_count.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.p3m_solve_assigned.argtypes = [vp, vp, vp, C.c_uint, dp]
     L.p3m_debug_charge_mesh.argtypes = [vp, C.c_int64, vp, vp, vp]
     L.p3m_debug_indices.argtypes = [vp, C.c_int64, vp, vp, vp]
     L.p3m_k_space_error.argtypes = [C.c_double, C.POINTER(C.c_int), C.c_int,

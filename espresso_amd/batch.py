@@ -28,15 +28,11 @@ import numpy as np
 from . import _lib
 
 
-def _device_ptr(t, name, dtype_size, numel):
-    import torch
-    if not (isinstance(t, torch.Tensor) and t.is_cuda):
-        raise TypeError(f"'{name}' must be a device tensor")
-    if t.dtype != torch.float64 or not t.is_contiguous():
-        raise TypeError(f"'{name}' must be a contiguous float64 tensor")
-    if t.numel() != numel:
-        raise ValueError(f"'{name}' has {t.numel()} values, expected {numel}")
-    return t.data_ptr()
+def _device_ptr(t, name, numel, device):
+    """Data pointer of a contiguous float64 device tensor of ``numel`` values
+    on HIP device ``device`` (checked before anything is launched)."""
+    from .electrostatics import device_output
+    return device_output(t, name, numel, device).data_ptr()
 
 
 def shape_dict(shape, best_ms=None, default_ms=None):
@@ -62,6 +58,7 @@ class P3MBatch:
         self._b = C.c_void_p()
         prec = _lib.P3M_PREC_FP32 if single_precision else _lib.P3M_PREC_FP64
         _lib.raise_for(L.p3m_batch_create(int(device), prec, C.byref(self._b)))
+        self.device = int(device)
         self._counts = []
         self.single_precision = bool(single_precision)
 
@@ -138,9 +135,9 @@ class P3MBatch:
 
     def _arrays(self, pos, q, force):
         n = self.total_particles
-        p = _device_ptr(pos, "pos", 8, 3 * n)
-        qq = _device_ptr(q, "q", 8, n)
-        f = None if force is None else _device_ptr(force, "force", 8, 3 * n)
+        p = _device_ptr(pos, "pos", 3 * n, self.device)
+        qq = _device_ptr(q, "q", n, self.device)
+        f = None if force is None else _device_ptr(force, "force", 3 * n, self.device)
         _lib.wait_for_torch(self.stream())
         return p, qq, f
 
@@ -312,7 +309,7 @@ class P3MServer:
             counts = np.array([b.particle_count(int(s)) for s in ids], dtype=np.int64)
             off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(counts)[:-1]])
                                        .astype(np.int64))
-            out_ptr = _device_ptr(capture, "capture", 8, 3 * int(counts.sum()))
+            out_ptr = _device_ptr(capture, "capture", 3 * int(counts.sum()), b.device)
         lat = np.zeros(n)
         wall = C.c_double()
         nb, pl = C.c_int(), C.c_int()

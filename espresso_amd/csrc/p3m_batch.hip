@@ -2154,9 +2154,12 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa,
                                                 ln.p_brick, ln.p_rank, ln.counts, ln.spos,
                                                 ln.sidx);
-  k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
+  bool const no_spread = (flags & kSolveNoSpread) != 0;
+  if (!(flags & kSolveAccumulate) && !no_spread)
+    k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
   if (ev) (void)hipEventRecord(ev[1], st);
   for (auto const &g : p.groups) {
+    if (no_spread) break;
     dispatch_cao(g.cao, [&](auto P) {
       with_threads(shape.threads, [&](auto NT) {
         size_t const lds =
@@ -2176,7 +2179,7 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     });
   }
   if (ev) (void)hipEventRecord(ev[2], st);
-  if (stop_after_spread) return;
+  if (stop_after_spread || (flags & kSolveMeshOnly)) return;
   k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
   if (p.blocks_y > 0)
     k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);
@@ -2269,7 +2272,8 @@ void finalize_results(BatchSolver::Impl *m, Lane const &ln, std::vector<int> con
     // energy (p3m.cpp:551-589)
     double const volume = hs.box_l[0] * hs.box_l[1] * hs.box_l[2];
     double const alpha = hs.params.alpha;
-    double energy = h[8] / (2. * volume);
+    r.kspace = h[8] / (2. * volume);
+    double energy = r.kspace;
     energy -= r.sum_q2 * alpha * std::numbers::inv_sqrtpi;
     energy -= (r.sum_q * r.sum_q) * std::numbers::pi / (2. * volume * alpha * alpha);
     if (hs.params.epsilon != P3M_EPSILON_METALLIC) {

@@ -36,12 +36,22 @@ FftPlan1D make_fft_plan(int n);
 
 struct SolveResult {
   double energy = 0.;    // k-space energy incl. self/net/dipole terms * prefactor
+  double kspace = 0.;    // the mesh part sum G |rho_hat|^2 / (2 V), before the corrections
   double virial[9] = {}; // k-space virial tensor (p3m.cpp:393-449 return value)
   double n_q = 0.;       // number of charged particles
   double sum_q = 0.;
   double sum_q2 = 0.;
   double dipole[3] = {};
 };
+
+/** Solve flags beyond the C ABI's P3M_* bits, for the charge-assignment
+ *  protocol of CoulombP3M (prepare_fft_mesh / assign_charge, p3m.hpp:91-143):
+ *  kSolveAccumulate keeps the charge mesh (no zeroing) and adds the particles
+ *  to it; kSolveMeshOnly stops after the spread; kSolveNoSpread bins the
+ *  particles for the gather only and transforms the mesh as assigned. */
+constexpr unsigned kSolveAccumulate = 1u << 16;
+constexpr unsigned kSolveMeshOnly = 1u << 17;
+constexpr unsigned kSolveNoSpread = 1u << 18;
 
 /** Device particle arrays of all systems, concatenated in system order
  *  (positions AoS [n][3], charges [n], forces [n][3]; upos may be null). */

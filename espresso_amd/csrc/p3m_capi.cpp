@@ -53,6 +53,15 @@ struct p3m_context {
   // staging for host pointers
   int64_t cap = 0;
   double *d_pos = nullptr, *d_q = nullptr, *d_upos = nullptr, *d_force = nullptr;
+  // charge-assignment protocol of CoulombP3M (prepare_fft_mesh,
+  // assign_charge, count_charged_particles_elc; p3m.hpp:91-143) as ELC drives
+  // it: the interpolation cache keeps the positions and charges of the
+  // assignments made without skip_cache, in order; the charge sums feed the
+  // energy of p3m_solve_assigned (sum_qpart, sum_q2, square_sum_q)
+  bool mesh_assigned = false;
+  int64_t cache_n = 0, cache_cap = 0;
+  double *d_cache_pos = nullptr, *d_cache_q = nullptr;
+  double sum_qpart = 0., sum_q2 = 0., square_sum_q = 0.;
 };
 
 struct p3m_dipolar {
@@ -154,6 +163,29 @@ void free_staging(p3m_context *ctx) {
     *p = nullptr;
   }
   ctx->cap = 0;
+}
+
+void free_cache(p3m_context *ctx) {
+  for (double **p : {&ctx->d_cache_pos, &ctx->d_cache_q}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  ctx->cache_cap = 0;
+  ctx->cache_n = 0;
+}
+
+/** Charge sums of count_charged_particles (p3m.cpp:97-115). */
+void count_charges(p3m_context *ctx, std::vector<double> const &charges) {
+  ctx->sum_qpart = 0.;
+  ctx->sum_q2 = 0.;
+  double sum_q = 0.;
+  for (double const qq : charges)
+    if (qq != 0.) {
+      ctx->sum_qpart += 1.;
+      ctx->sum_q2 += qq * qq;
+      sum_q += qq;
+    }
+  ctx->square_sum_q = sum_q * sum_q;
 }
 
 void ensure_staging(p3m_context *ctx, int64_t n) {
@@ -270,6 +302,7 @@ void init_solver(p3m_context *ctx) {
   require_device(ctx);
   ctx->solver->configure(p, box, ctx->user.skin, ctx->user.prefactor, prec_of(ctx->user));
   ctx->initialized = true;
+  ctx->mesh_assigned = false; // new meshes
 }
 } // namespace
 
@@ -317,6 +350,7 @@ int p3m_destroy(p3m_context *ctx) {
   if (!ctx) return P3M_OK;
   if (ctx->solver) (void)hipSetDevice(ctx->device);
   free_staging(ctx);
+  free_cache(ctx);
   delete ctx;
   return P3M_OK;
 }
@@ -426,6 +460,8 @@ int p3m_activate(p3m_context *ctx, int64_t n, double const *pos, double const *q
     }
     auto const charges = host_charges(ctx, n, q, flags);
     check_charge_neutrality(charges, ctx->user.charge_neutrality_tolerance);
+    count_charges(ctx, charges);
+    ctx->mesh_assigned = false; // tuning solves overwrite the mesh
     bool const elc_diel =
         ctx->elc_on && (ctx->elc.delta_mid_top != 0. || ctx->elc.delta_mid_bot != 0.);
     if (ctx->elc_on) {
@@ -529,7 +565,11 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
       throw std::invalid_argument("P3M: P3M_DEVICE_RESULTS requires P3M_DEVICE_POINTERS");
     if (dev_res && ctx->elc_on)
       throw std::invalid_argument("P3M: P3M_DEVICE_RESULTS is not available with ELC");
+    if (flags & ~(P3M_FORCES | P3M_ENERGY | P3M_PRESSURE | P3M_DEVICE_POINTERS | P3M_ASYNC |
+                  P3M_DEVICE_RESULTS))
+      throw std::invalid_argument("P3M: unknown solve flags");
     DevArgs const d = stage_inputs(ctx, n, pos, q, pos_unfolded, force, flags);
+    ctx->mesh_assigned = false;
     SolveResult r;
     // one P3M pass over (nn, pp, qq): forces with the context's convention
     // (the GPU kernels' index rule makes a second pass for energy and
@@ -641,6 +681,185 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
   });
 }
 
+namespace {
+void require_solver_ready(p3m_context *ctx) {
+  require_params(ctx);
+  require_device(ctx);
+  if (!ctx->is_tuned) throw std::runtime_error("P3M: solver used before activation");
+  hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+  if (!ctx->initialized) init_solver(ctx);
+}
+
+void check_protocol_flags(unsigned flags, unsigned allowed) {
+  if (flags & ~allowed) throw std::invalid_argument("P3M: unsupported flags for this call");
+}
+} // namespace
+
+int p3m_prepare_fft_mesh(p3m_context *ctx, int reset_weights) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_solver_ready(ctx);
+    // zero the charge mesh: a spread of no particles into a cleared mesh
+    ctx->solver->set_convention(0);
+    ctx->solver->solve(0, ctx->d_pos, ctx->d_q, nullptr, nullptr, kSolveMeshOnly, nullptr, true);
+    if (reset_weights) ctx->cache_n = 0;
+    ctx->mesh_assigned = true;
+  });
+}
+
+int p3m_assign_charges(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                       int skip_cache, unsigned flags) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_solver_ready(ctx);
+    check_protocol_flags(flags, P3M_DEVICE_POINTERS);
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    if (!ctx->mesh_assigned)
+      throw std::runtime_error("P3M: assign_charge before prepare_fft_mesh");
+    if (n == 0) return;
+    DevArgs const d = stage_inputs(ctx, n, pos, q, nullptr, nullptr, flags);
+    hipStream_t const st = ctx->solver->stream();
+    ctx->solver->set_convention(0);
+    ctx->solver->solve(n, d.pos, d.q, nullptr, nullptr, kSolveAccumulate | kSolveMeshOnly,
+                       nullptr, true);
+    if (!skip_cache) {
+      int64_t const need = ctx->cache_n + n;
+      if (need > ctx->cache_cap) {
+        int64_t const cap = std::max<int64_t>(need + need / 2, 1024);
+        double *np = nullptr, *nq = nullptr;
+        hip_check(hipMalloc(&np, sizeof(double) * 3 * cap), "hipMalloc cache");
+        hip_check(hipMalloc(&nq, sizeof(double) * cap), "hipMalloc cache");
+        if (ctx->cache_n > 0) {
+          hip_check(hipMemcpyAsync(np, ctx->d_cache_pos, sizeof(double) * 3 * ctx->cache_n,
+                                   hipMemcpyDeviceToDevice, st),
+                    "copy cache");
+          hip_check(hipMemcpyAsync(nq, ctx->d_cache_q, sizeof(double) * ctx->cache_n,
+                                   hipMemcpyDeviceToDevice, st),
+                    "copy cache");
+        }
+        hip_check(hipStreamSynchronize(st), "sync");
+        int64_t const keep = ctx->cache_n;
+        free_cache(ctx);
+        ctx->d_cache_pos = np;
+        ctx->d_cache_q = nq;
+        ctx->cache_cap = cap;
+        ctx->cache_n = keep;
+      }
+      hip_check(hipMemcpyAsync(ctx->d_cache_pos + 3 * ctx->cache_n, d.pos, sizeof(double) * 3 * n,
+                               hipMemcpyDeviceToDevice, st),
+                "copy cache");
+      hip_check(hipMemcpyAsync(ctx->d_cache_q + ctx->cache_n, d.q, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, st),
+                "copy cache");
+      hip_check(hipStreamSynchronize(st), "sync");
+      ctx->cache_n = need;
+    }
+  });
+}
+
+int p3m_charge_assign(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                      unsigned flags) {
+  int const st = p3m_prepare_fft_mesh(ctx, 1);
+  if (st != P3M_OK) return st;
+  return p3m_assign_charges(ctx, n, pos, q, 0, flags);
+}
+
+int p3m_count_charged_particles(p3m_context *ctx, int64_t n, double const *q, unsigned flags) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    check_protocol_flags(flags, P3M_DEVICE_POINTERS);
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    if (n > 0 && !q) throw std::invalid_argument("P3M: charges required");
+    if ((flags & P3M_DEVICE_POINTERS) && ctx->solver)
+      hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    count_charges(ctx, host_charges(ctx, n, q, flags));
+  });
+}
+
+int p3m_count_charged_particles_elc(p3m_context *ctx, int n, double sum_q2, double square_sum_q) {
+  if (!ctx) return P3M_ERR_INVALID;
+  ctx->sum_qpart = n;
+  ctx->sum_q2 = sum_q2;
+  ctx->square_sum_q = square_sum_q;
+  return P3M_OK;
+}
+
+int p3m_adapt_epsilon_elc(p3m_context *ctx) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    ctx->params.epsilon = P3M_EPSILON_METALLIC;
+    ctx->user.epsilon = P3M_EPSILON_METALLIC;
+    ctx->initialized = false; // the dipole term of the descriptors changes
+  });
+}
+
+int p3m_assigned_count(p3m_context const *ctx, int64_t *n) {
+  if (!ctx || !n) return P3M_ERR_INVALID;
+  *n = ctx->cache_n;
+  return P3M_OK;
+}
+
+int p3m_solve_assigned(p3m_context *ctx, double const *pos_unfolded, double *force,
+                       unsigned flags, double *energy) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_solver_ready(ctx);
+    check_protocol_flags(flags, P3M_FORCES | P3M_ENERGY | P3M_DEVICE_POINTERS);
+    if (!ctx->mesh_assigned)
+      throw std::runtime_error("P3M: no charges assigned since the last solve");
+    int64_t const n = ctx->cache_n;
+    bool const want_f = (flags & P3M_FORCES) && n > 0;
+    if (want_f && !force) throw std::invalid_argument("P3M: force array required");
+    if (energy) *energy = 0.;
+    // long_range_kernel skips the mesh work without charges (p3m.cpp:463)
+    if (!(ctx->sum_q2 > 0.)) return;
+    hipStream_t const st = ctx->solver->stream();
+    bool const host = !(flags & P3M_DEVICE_POINTERS);
+    double *d_force = force;
+    double const *d_upos = pos_unfolded;
+    if (host && n > 0) {
+      ensure_staging(ctx, n);
+      if (want_f) {
+        hip_check(hipMemcpyAsync(ctx->d_force, force, sizeof(double) * 3 * n,
+                                 hipMemcpyHostToDevice, st),
+                  "copy force");
+        d_force = ctx->d_force;
+      }
+      if (pos_unfolded) {
+        hip_check(hipMemcpyAsync(ctx->d_upos, pos_unfolded, sizeof(double) * 3 * n,
+                                 hipMemcpyHostToDevice, st),
+                  "copy upos");
+        d_upos = ctx->d_upos;
+      }
+    }
+    SolveResult r;
+    ctx->solver->set_convention(0);
+    ctx->solver->solve(n, ctx->d_cache_pos, ctx->d_cache_q, d_upos, want_f ? d_force : nullptr,
+                       (flags & (P3M_FORCES | P3M_ENERGY)) | kSolveNoSpread, &r, true);
+    ctx->mesh_assigned = false;
+    if (want_f && host)
+      hip_check(hipMemcpy(force, d_force, sizeof(double) * 3 * n, hipMemcpyDeviceToHost),
+                "copy force");
+    if (energy && (flags & P3M_ENERGY)) {
+      // the energy corrections of long_range_kernel (p3m.cpp:551-589) with
+      // the charge sums of count_charged_particles(_elc)
+      auto const box = v3(ctx->user.box_l);
+      double const volume = box[0] * box[1] * box[2];
+      double const alpha = ctx->params.alpha;
+      double e = r.kspace;
+      e -= ctx->sum_q2 * alpha * std::numbers::inv_sqrtpi;
+      e -= ctx->square_sum_q * std::numbers::pi / (2. * volume * alpha * alpha);
+      if (ctx->params.epsilon != P3M_EPSILON_METALLIC) {
+        double const pref = 4. * std::numbers::pi / volume / (2. * ctx->params.epsilon + 1.);
+        e += pref * (r.dipole[0] * r.dipole[0] + r.dipole[1] * r.dipole[1] +
+                     r.dipole[2] * r.dipole[2]);
+      }
+      *energy = e * ctx->user.prefactor;
+    }
+  });
+}
+
 int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, double const *pos, double const *q,
                           double *mesh_out) {
   if (!ctx) return P3M_ERR_INVALID;
@@ -653,6 +872,7 @@ int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, double const *pos, double
     std::vector<double> out;
     ctx->solver->set_convention(ctx->force_conv);
     ctx->solver->debug_charge_mesh(n, d.pos, d.q, out);
+    ctx->mesh_assigned = false;
     std::memcpy(mesh_out, out.data(), sizeof(double) * out.size());
   });
 }
@@ -669,6 +889,7 @@ int p3m_debug_indices(p3m_context *ctx, int64_t n, double const *pos, double con
     std::vector<int32_t> out;
     ctx->solver->set_convention(ctx->force_conv);
     ctx->solver->debug_indices(n, d.pos, d.q, out);
+    ctx->mesh_assigned = false;
     std::memcpy(nmp, out.data(), sizeof(int32_t) * out.size());
   });
 }
@@ -911,6 +1132,7 @@ int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q
     if (!ctx->initialized) init_solver(ctx);
     ensure_staging(ctx, n);
     AutotuneResult const r = ctx->solver->autotune(n, pos, q, ctx->d_force, reps);
+    ctx->mesh_assigned = false;
     if (best) *best = to_abi(r.best);
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;

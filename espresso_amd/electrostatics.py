@@ -35,32 +35,71 @@ def _is_float(x):
     return isinstance(x, (numbers.Real, np.floating)) and not isinstance(x, bool)
 
 
-class _Arrays:
-    """Device or host view of pos/q/force for one call."""
+def check_device_tensor(t, name, numel, device, minimum=False):
+    """Validate a device tensor handed to the solver before anything is
+    launched: on HIP device ``device`` and holding ``numel`` elements (at
+    least ``numel`` with ``minimum``).  Returns True when it is a contiguous
+    float64 tensor that the kernels can use in place."""
+    if not getattr(t, "is_cuda", False):
+        raise TypeError(f"'{name}' must be a device tensor too")
+    dev = t.device.index if t.device.index is not None else 0
+    if dev != device:
+        raise ValueError(f"'{name}' is on device {dev}, the solver on device {device}")
+    size = int(t.numel())
+    if (size < numel) if minimum else (size != numel):
+        want = f"at least {numel}" if minimum else str(numel)
+        raise ValueError(f"'{name}' has {size} elements, expected {want}")
+    return str(t.dtype) == "torch.float64" and bool(t.is_contiguous())
 
-    def __init__(self, pos, q, force=None, pos_unfolded=None):
+
+def device_input(t, name, numel, device, asynchronous):
+    """Float64 contiguous version of a device input tensor.  A converted copy
+    is a temporary that the caching allocator may hand out again as soon as
+    this call returns, so an asynchronous call (the solver stream still
+    reading it) refuses inputs that need one."""
+    if check_device_tensor(t, name, numel, device):
+        return t
+    if asynchronous:
+        raise TypeError(f"'{name}' must be a contiguous float64 tensor for an "
+                        f"asynchronous call")
+    import torch
+    return t.contiguous().to(torch.float64)
+
+
+def device_output(t, name, numel, device, minimum=False):
+    """Device output tensor, written in place: it must already be a
+    contiguous float64 tensor of the right size on the solver's device."""
+    if not check_device_tensor(t, name, numel, device, minimum):
+        raise TypeError(f"'{name}' must be a contiguous float64 tensor")
+    return t
+
+
+class _Arrays:
+    """Device or host view of pos/q/force for one call.  Device tensors are
+    checked against the solver's device and the particle count before any
+    launch; with ``asynchronous`` set they must already be contiguous
+    float64 tensors."""
+
+    def __init__(self, pos, q, force=None, pos_unfolded=None, device=0,
+                 asynchronous=False):
         self.keep = []
         self.device = _is_device_tensor(pos)
         if self.device:
-            import torch
-            for t, name in ((q, "q"), (force, "force"),
-                            (pos_unfolded, "pos_unfolded")):
-                if t is not None and not _is_device_tensor(t):
-                    raise TypeError(f"'{name}' must be a device tensor too")
-            pos = pos.contiguous().to(torch.float64)
-            q = q.contiguous().to(torch.float64)
+            if not getattr(q, "is_cuda", False):
+                raise TypeError("'q' must be a device tensor too")
+            self.n = int(q.numel())
+            q = device_input(q, "q", self.n, device, asynchronous)
+            pos = device_input(pos, "pos", 3 * self.n, device, asynchronous)
             self.keep += [pos, q]
-            self.n = int(q.shape[0])
             self.pos = pos.data_ptr()
             self.q = q.data_ptr()
             self.force = None
             if force is not None:
-                if force.dtype != torch.float64 or not force.is_contiguous():
-                    raise TypeError("'force' must be a contiguous float64 tensor")
-                self.force = force.data_ptr()
+                self.force = device_output(force, "force", 3 * self.n, device).data_ptr()
             self.upos = None
             if pos_unfolded is not None:
-                up = pos_unfolded.contiguous().to(torch.float64)
+                up = device_input(pos_unfolded, "pos_unfolded", 3 * self.n, device,
+                                  asynchronous)
                 self.keep.append(up)
                 self.upos = up.data_ptr()
         else:
@@ -229,7 +268,7 @@ class _P3MBase:
             self._build_cparams(box_l=box_l, skin=skin)
             _lib.raise_for(L.p3m_set_params(self._ctx, C.byref(self._cp)),
                            self._ctx)
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         a.order_after_torch(self._ctx)
         _lib.raise_for(L.p3m_activate(self._ctx, a.n, a.pos, a.q, a.flags),
                        self._ctx)
@@ -249,7 +288,8 @@ class _P3MBase:
         With device tensors and ``sync=False`` the solve is only enqueued on
         the solver's stream (call ``synchronize()`` before reading)."""
         self._require_active()
-        a = _Arrays(pos, q, force, pos_unfolded)
+        a = _Arrays(pos, q, force, pos_unfolded, device=self._device,
+                    asynchronous=not sync)
         a.order_after_torch(self._ctx)
         flags = _lib.P3M_FORCES | a.flags
         if not sync:
@@ -287,7 +327,7 @@ class _P3MBase:
         these device tensors and keep it.  Returns the shape and the mean
         solve times before and after."""
         self._require_active()
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         if not a.device:
             raise TypeError("autotune takes device tensors")
         a.order_after_torch(self._ctx)
@@ -319,7 +359,7 @@ class _P3MBase:
     def long_range_energy(self, pos, q, pos_unfolded=None):
         """k-space energy incl. self/net-charge/dipole terms (p3m.cpp:551-589)."""
         self._require_active()
-        a = _Arrays(pos, q, None, pos_unfolded)
+        a = _Arrays(pos, q, None, pos_unfolded, device=self._device)
         a.order_after_torch(self._ctx)
         e = C.c_double(0.)
         _lib.raise_for(_lib.load().p3m_solve(
@@ -331,7 +371,7 @@ class _P3MBase:
         """k-space virial tensor as returned by long_range_pressure
         (p3m.cpp:393-449), i.e. before the division by the volume."""
         self._require_active()
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         a.order_after_torch(self._ctx)
         v = (C.c_double * 9)()
         _lib.raise_for(_lib.load().p3m_solve(
@@ -343,7 +383,7 @@ class _P3MBase:
               pos_unfolded=None):
         """Forces, energy and virial from one charge assignment and FFT."""
         self._require_active()
-        a = _Arrays(pos, q, force, pos_unfolded)
+        a = _Arrays(pos, q, force, pos_unfolded, device=self._device)
         a.order_after_torch(self._ctx)
         flags = a.flags
         if force is not None:
@@ -366,13 +406,13 @@ class _P3MBase:
         per-step energy or NpT virial (p3m.cpp:580-583).  Without sync the
         call only enqueues; ``synchronize()`` waits."""
         self._require_active()
-        a = _Arrays(pos, q, force)
-        if not a.device:
+        if not _is_device_tensor(pos):
             raise TypeError("solve_to_device takes device tensors")
-        for t, name in ((energy, "energy"), (virial, "virial")):
-            if t is not None and not (_is_device_tensor(t) and t.is_contiguous()
-                                      and str(t.dtype) == "torch.float64"):
-                raise TypeError(f"'{name}' must be a contiguous float64 device tensor")
+        a = _Arrays(pos, q, force, device=self._device, asynchronous=not sync)
+        if energy is not None:
+            device_output(energy, "energy", 1, self._device, minimum=True)
+        if virial is not None:
+            device_output(virial, "virial", 9, self._device, minimum=True)
         a.order_after_torch(self._ctx)
         flags = a.flags | _lib.P3M_DEVICE_RESULTS
         if force is not None:
@@ -399,7 +439,7 @@ class _P3MBase:
         ``pressure=True`` (a (energy, virial) pair with both).  With device
         tensors and ``sync=False`` the work is only enqueued."""
         self._require_active()
-        a = _Arrays(pos, q, force)
+        a = _Arrays(pos, q, force, device=self._device, asynchronous=not sync)
         a.order_after_torch(self._ctx)
         flags = a.flags
         if energy:
@@ -440,7 +480,7 @@ class _P3MBase:
         device tensors in, a (len(targets), 3) device tensor out."""
         import torch
         self._require_active()
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         if not a.device:
             raise TypeError("pair_forces_on takes device tensors")
         a.order_after_torch(self._ctx)
@@ -468,7 +508,7 @@ class _P3MBase:
     # -- stage outputs for parity tests -------------------------------------
     def charge_mesh(self, pos, q):
         self._require_active()
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         a.order_after_torch(self._ctx)
         if a.device:
             raise TypeError("charge_mesh takes host arrays")
@@ -480,7 +520,7 @@ class _P3MBase:
 
     def mesh_indices(self, pos, q):
         self._require_active()
-        a = _Arrays(pos, q)
+        a = _Arrays(pos, q, device=self._device)
         a.order_after_torch(self._ctx)
         out = np.zeros((a.n, 3), np.int32)
         _lib.raise_for(_lib.load().p3m_debug_indices(

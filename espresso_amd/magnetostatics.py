@@ -11,36 +11,34 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .electrostatics import _is_device_tensor, _is_float, _is_int
+from .electrostatics import (_is_device_tensor, _is_float, _is_int, device_input,
+                              device_output)
 
 _KEYS = ("prefactor", "accuracy", "cao", "r_cut", "alpha", "mesh", "epsilon",
          "mesh_off", "single_precision", "tune", "timings", "verbose")
 
 
 class _Vec:
-    """Host or device view of [n][3] arrays for one call."""
+    """Host or device view of [n][3] arrays for one call.  Device tensors are
+    checked against the solver's device and the particle count before any
+    launch; with ``asynchronous`` set they must already be contiguous float64
+    tensors (a converted copy could be reused while the stream reads it)."""
 
-    def __init__(self, pos, dip, force=None, torque=None):
+    def __init__(self, pos, dip, force=None, torque=None, device=0, asynchronous=False):
         self.keep = []
         self.device = _is_device_tensor(pos)
         outs = []
         if self.device:
-            import torch
-            for t, name in ((dip, "dip"), (force, "force"), (torque, "torque")):
-                if t is not None and not _is_device_tensor(t):
-                    raise TypeError(f"'{name}' must be a device tensor too")
-            pos = pos.contiguous().to(torch.float64)
-            dip = dip.contiguous().to(torch.float64)
+            if int(pos.numel()) % 3:
+                raise ValueError("'pos' must hold [n][3] values")
+            self.n = int(pos.numel()) // 3
+            pos = device_input(pos, "pos", 3 * self.n, device, asynchronous)
+            dip = device_input(dip, "dip", 3 * self.n, device, asynchronous)
             self.keep += [pos, dip]
-            self.n = int(pos.shape[0])
             self.pos, self.dip = pos.data_ptr(), dip.data_ptr()
             for t, name in ((force, "force"), (torque, "torque")):
-                if t is None:
-                    outs.append(None)
-                    continue
-                if t.dtype != torch.float64 or not t.is_contiguous():
-                    raise TypeError(f"'{name}' must be a contiguous float64 tensor")
-                outs.append(t.data_ptr())
+                outs.append(None if t is None else
+                            device_output(t, name, 3 * self.n, device).data_ptr())
         else:
             pos = np.ascontiguousarray(np.asarray(pos, np.float64).reshape(-1, 3))
             dip = np.ascontiguousarray(np.asarray(dip, np.float64).reshape(-1, 3))
@@ -111,6 +109,7 @@ class DipolarP3M:
         self._params = params
         self._ctx = C.c_void_p()
         _lib.raise_for(L.p3m_dipolar_create(int(device), C.byref(self._ctx)))
+        self._device = int(device)
         self._activated = False
         self._build(box_l=1.0, skin=0.4)
 
@@ -172,7 +171,7 @@ class DipolarP3M:
         if box.size == 3 and not (box[0] == box[1] == box[2]):
             raise RuntimeError("DipolarP3M: requires a cubic box")
         self._build(box_l=float(box[0]), skin=skin)
-        a = _Vec(pos, dip)
+        a = _Vec(pos, dip, device=self._device)
         a.order_after_torch(self._ctx)
         _lib.raise_for(_lib.load().p3m_dipolar_activate(self._ctx, a.n, a.pos, a.dip, a.flags),
                        dipolar=self._ctx)
@@ -196,7 +195,7 @@ class DipolarP3M:
     def add_long_range_forces(self, pos, dip, force, torque, sync=True):
         """k-space forces and torques, added in place."""
         self._require()
-        a = _Vec(pos, dip, force, torque)
+        a = _Vec(pos, dip, force, torque, device=self._device, asynchronous=not sync)
         a.order_after_torch(self._ctx)
         flags = _lib.P3M_FORCES | a.flags
         if not sync:
@@ -208,7 +207,7 @@ class DipolarP3M:
 
     def long_range_energy(self, pos, dip):
         self._require()
-        a = _Vec(pos, dip)
+        a = _Vec(pos, dip, device=self._device)
         a.order_after_torch(self._ctx)
         e = C.c_double(0.)
         _lib.raise_for(_lib.load().p3m_dipolar_solve(
@@ -221,7 +220,7 @@ class DipolarP3M:
         dp3m.hpp:160-211), added in place; returns the pair energy with
         energy=True."""
         self._require()
-        a = _Vec(pos, dip, force, torque)
+        a = _Vec(pos, dip, force, torque, device=self._device, asynchronous=not sync)
         a.order_after_torch(self._ctx)
         flags = a.flags | (_lib.P3M_ENERGY if energy else 0)
         if not sync:

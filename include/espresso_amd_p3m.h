@@ -129,6 +129,52 @@ int p3m_solve(p3m_context *ctx, int64_t n, const double *pos, const double *q,
               const double *pos_unfolded, double *force, unsigned flags, double *energy,
               double *virial);
 
+/* ------------------------------------------------------------------------
+ * Charge-assignment protocol of CoulombP3M, as the reference's
+ * ElectrostaticLayerCorrection drives it (elc.cpp:1014, 1110-1141, 1143-1187,
+ * 1189-1256): the caller fills the charge mesh itself (real charges, then
+ * image charges), sets the charge sums, and runs the k-space kernel on that
+ * mesh without a fresh assignment (long_range_kernel under ELC,
+ * p3m.cpp:463-468).  All calls are synchronous; arrays are host pointers, or
+ * device pointers with P3M_DEVICE_POINTERS.
+ * ------------------------------------------------------------------------ */
+/* prepare_fft_mesh(reset_weights) (p3m.impl.hpp:161-168): zero the charge
+ * mesh; with reset_weights also empty the interpolation cache. */
+int p3m_prepare_fft_mesh(p3m_context *ctx, int reset_weights);
+/* assign_charge(q, real_pos, skip_cache) (p3m.hpp:135-141, p3m.cpp:333-345)
+ * for n charges at once: each (q[i], pos[i]) with q[i] != 0 is added to the
+ * mesh with the CA-mesh weights of p3m_calculate_interpolation_weights;
+ * positions outside the box wrap periodically.  Without skip_cache every
+ * (pos[i], q[i]) is appended to the interpolation cache, in order. */
+int p3m_assign_charges(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                       int skip_cache, unsigned flags);
+/* charge_assign(particles) (p3m.cpp:321-331): p3m_prepare_fft_mesh(ctx, 1)
+ * then p3m_assign_charges(..., skip_cache = 0). */
+int p3m_charge_assign(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                      unsigned flags);
+/* count_charged_particles (p3m.cpp:97-115): sum_qpart, sum_q2 and
+ * square_sum_q of the charges q[n]; p3m_activate sets them too. */
+int p3m_count_charged_particles(p3m_context *ctx, int64_t n, const double *q, unsigned flags);
+/* count_charged_particles_elc(n, sum_q2, square_sum_q) (p3m.impl.hpp:105-110). */
+int p3m_count_charged_particles_elc(p3m_context *ctx, int n, double sum_q2,
+                                    double square_sum_q);
+/* adapt_epsilon_elc (p3m.impl.hpp:111-113): epsilon becomes metallic. */
+int p3m_adapt_epsilon_elc(p3m_context *ctx);
+/* Number of entries in the interpolation cache. */
+int p3m_assigned_count(const p3m_context *ctx, int64_t *n);
+/* long_range_kernel on the assigned mesh (p3m.cpp:452-592 with the charge
+ * assignment skipped, as under ELC): forward FFT, influence function and ik.
+ * P3M_FORCES: force has one [3] row per cache entry, in cache order, and
+ * row k -= q_k E(r_k) * prefactor / V (AssignForces with the cached weights,
+ * p3m.cpp:344-373), plus the dipole term when epsilon is not metallic (the
+ * moment of the cache entries; pos_unfolded, one row per entry or NULL, gives
+ * their unfolded positions).  P3M_ENERGY: *energy = the mesh energy with the
+ * self, net-charge and dipole corrections of the current charge sums
+ * (p3m.cpp:551-589).  Nothing is done when sum_q2 is zero.  The mesh is
+ * consumed: assign again before the next call. */
+int p3m_solve_assigned(p3m_context *ctx, const double *pos_unfolded, double *force,
+                       unsigned flags, double *energy);
+
 /* Stage outputs for parity checks (device-side meshes copied to host):
  * charge mesh after the periodic wrap, [Mx][My][Mz] real. */
 int p3m_debug_charge_mesh(p3m_context *ctx, int64_t n, const double *pos, const double *q,
@@ -189,14 +235,14 @@ typedef struct {
   double gap_size;      /* empty layer below L_z (required, > 0)           */
   double far_cut;       /* cutoff of the far-field sums, -1 = tune         */
   int neutralize;       /* keep P3M's neutralising background (default 1)  */
-  double delta_mid_top; /* dielectric contrasts: 0 (only 0 is supported)   */
-  double delta_mid_bot;
+  double delta_mid_top; /* dielectric contrast at the top boundary, [-1, 1] */
+  double delta_mid_bot; /* dielectric contrast at the bottom boundary       */
   int const_pot;        /* constant potential (with dielectric contrasts)  */
   double pot_diff;      /* applied potential difference (needs const_pot) */
 } p3m_elc_params;
 
-/* Defaults of the Python actor: far_cut -1, neutralize 1, no contrasts;
- * maxPWerror and gap_size -1 (must be set). */
+/* Defaults of the Python actor: far_cut -1, neutralize 1, no contrasts,
+ * no const_pot; maxPWerror and gap_size -1 (must be set). */
 int p3m_elc_params_default(p3m_elc_params *p);
 
 /* Put ELC on top of the context's P3M solver, or remove it (p == NULL).
@@ -208,7 +254,12 @@ int p3m_elc_params_default(p3m_elc_params *p);
  * 830-940); P3M_PRESSURE returns a zero virial, as the reference has no
  * ELC pressure (coulomb.cpp:125-131).  A charged particle in the gap is
  * reported by the solve (check_gap, elc.cpp:184-195).  Parameter errors
- * carry the reference's messages; dielectric contrasts are rejected. */
+ * carry the reference's messages.  Dielectric contrasts (delta_mid_top/bot)
+ * put image charges of the boundary layers into the mesh and add their pair
+ * and self terms (elc.cpp:1110-1256, elc.hpp:140-171, 250-296); const_pot
+ * with pot_diff adds the applied field (elc.cpp:201-342).  The reference's
+ * own ELC actor can also drive the solver through the charge-assignment
+ * protocol above (p3m_prepare_fft_mesh ... p3m_solve_assigned). */
 int p3m_set_elc(p3m_context *ctx, const p3m_elc_params *p);
 /* Current ELC parameters (far_cut as tuned by the last activation or
  * solve) and whether ELC is on; read-only getters of the script interface
