@@ -172,15 +172,6 @@ __global__ void k_influence(MT *G, int Mx, int My, int Mz, int Kz, double wx, do
 // ---------------------------------------------------------------------------
 // particle binning (one block per 256 particles of a system)
 // ---------------------------------------------------------------------------
-__global__ void k_zero_counts(SysDesc const *__restrict__ sys, int const *__restrict__ sel,
-                              int nsel, int *counts) {
-  int const s = sel[blockIdx.x];
-  SysDesc const &d = sys[s];
-  int const n = d.nbricks * kCountCopies;
-  int *c = counts + d.brick_off * kCountCopies;
-  for (int b = threadIdx.x; b < n; b += blockDim.x) c[b] = 0;
-}
-
 __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
                                                 int const *__restrict__ sel,
                                                 int const *__restrict__ wt, int nsel,
@@ -248,35 +239,56 @@ __global__ void k_reduce_sys(int const *__restrict__ sel, int const *__restrict_
   (void)nsel;
 }
 
-/** exclusive scan of one system's brick counts (one block per system) */
+/** One block per selected system: the per-system sums of the binning
+ *  partials (n_q, sum q, sum q^2, dipole; block order, deterministic), then
+ *  the exclusive scan of the system's brick counts, kScanItems bricks per
+ *  thread and pass.  The counters become the start of each copy's run
+ *  within its brick; offsets[b] is the first particle of brick b and
+ *  offsets[nb] the system's charged-particle count. */
+constexpr int kScanItems = 4;
 __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
-                                               int const *__restrict__ sel, int *counts,
-                                               int *offsets) {
+                                               int const *__restrict__ sel,
+                                               int const *__restrict__ wt_bin,
+                                               double const *__restrict__ bin_part,
+                                               double *sums, int *counts, int *offsets) {
   __shared__ int wsum[16];
   __shared__ int carry;
-  SysDesc const &d = sys[sel[blockIdx.x]];
+  __shared__ double red[16 * 8];
+  int const slot = blockIdx.x;
+  int const s = sel[slot];
+  SysDesc const &d = sys[s];
+  {
+    double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
+    for (int b = wt_bin[slot] + threadIdx.x; b < wt_bin[slot + 1]; b += blockDim.x)
+      for (int k = 0; k < 7; ++k) v[k] += bin_part[b * 8 + k];
+    block_sum<8>(v, red);
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 7; ++k) sums[s * 16 + k] = v[k];
+  }
   int const nb = d.nbricks;
   int *count = counts + d.brick_off * kCountCopies;
   int *offset = offsets + d.off_off;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   int const lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int base = 0; base < nb; base += blockDim.x) {
-    int const i = base + threadIdx.x;
-    int cc[kCountCopies];
+  for (int base = 0; base < nb; base += blockDim.x * kScanItems) {
+    int const i0 = base + threadIdx.x * kScanItems;
+    int cc[kScanItems][kCountCopies];
     int c = 0;
 #pragma unroll
-    for (int k = 0; k < kCountCopies; ++k) {
-      cc[k] = (i < nb) ? count[i * kCountCopies + k] : 0;
-      c += cc[k];
-    }
-    int s = c;
+    for (int it = 0; it < kScanItems; ++it)
+#pragma unroll
+      for (int k = 0; k < kCountCopies; ++k) {
+        cc[it][k] = (i0 + it < nb) ? count[(i0 + it) * kCountCopies + k] : 0;
+        c += cc[it][k];
+      }
+    int sc = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-      int const t = __shfl_up(s, off, 64);
-      if (lane >= off) s += t;
+      int const t = __shfl_up(sc, off, 64);
+      if (lane >= off) sc += t;
     }
-    if (lane == 63) wsum[wid] = s;
+    if (lane == 63) wsum[wid] = sc;
     __syncthreads();
     if (threadIdx.x == 0) {
       int acc = 0;
@@ -287,18 +299,20 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
       }
     }
     __syncthreads();
-    if (i < nb) {
-      int o = carry + wsum[wid] + s - c;
-      offset[i] = o;
-      // the counters become the start of each copy's run within the brick
+    int o = carry + wsum[wid] + sc - c;
 #pragma unroll
-      for (int k = 0; k < kCountCopies; ++k) {
-        count[i * kCountCopies + k] = o;
-        o += cc[k];
+    for (int it = 0; it < kScanItems; ++it) {
+      if (i0 + it < nb) {
+        offset[i0 + it] = o;
+#pragma unroll
+        for (int k = 0; k < kCountCopies; ++k) {
+          count[(i0 + it) * kCountCopies + k] = o;
+          o += cc[it][k];
+        }
       }
     }
     __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry += wsum[wid] + s;
+    if (threadIdx.x == blockDim.x - 1) carry += wsum[wid] + sc;
     __syncthreads();
   }
   if (threadIdx.x == 0) offset[nb] = carry;
@@ -327,12 +341,23 @@ __global__ __launch_bounds__(kBlock) void k_scatter(SysDesc const *__restrict__ 
   sidx[slot] = static_cast<int>(li);
 }
 
+/** Start of a solve, one launch over the systems' mesh blocks: zero the
+ *  charge mesh (unless it is kept), each block its share of the system's
+ *  brick counters, and the first block the error flag. */
 template <typename MT>
-__global__ __launch_bounds__(kBlock) void k_fill_rho(SysDesc const *__restrict__ sys,
-                                                     int const *__restrict__ sel,
-                                                     int const *__restrict__ wt, int nsel) {
+__global__ __launch_bounds__(kBlock) void k_clear(SysDesc const *__restrict__ sys,
+                                                  int const *__restrict__ sel,
+                                                  int const *__restrict__ wt, int nsel,
+                                                  int *counts, int *err_flag, int zero_rho) {
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
+  if (L.vb == 0 && threadIdx.x == 0) *err_flag = 0;
+  int const nblk = wt[L.slot + 1] - wt[L.slot];
+  int const nc = d.nbricks * kCountCopies;
+  int const per = (nc + nblk - 1) / nblk;
+  int *c = counts + d.brick_off * kCountCopies;
+  for (int e = L.lb * per + threadIdx.x; e < min(nc, (L.lb + 1) * per); e += blockDim.x) c[e] = 0;
+  if (!zero_rho) return;
   int64_t const R = (int64_t)d.g.M[0] * d.g.M[1] * d.g.M[2];
   MT *rho = static_cast<MT *>(d.rho);
   int64_t const beg = (int64_t)L.lb * kFill;
@@ -1356,9 +1381,6 @@ template <typename F> void for_threads(F &&f) {
   f(std::integral_constant<int, 512>{});
 }
 
-__global__ void k_fill_int(int *p, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
-}
 
 template <typename T> void dfree(T *&p) {
   if (p) (void)hipFree(p);
@@ -2138,25 +2160,26 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   Plan &p = ln.plan;
   hipStream_t const st = ln.stream;
   bool const want_force = (flags & 1u) && pa.force;
+  bool const want_sums = (flags & 6u) != 0; // energy or virial (P3M_ENERGY | P3M_PRESSURE)
   int const ns = p.nsel;
   C const *tw = static_cast<C const *>(m->d_tw);
   SysDesc const *sys = m->d_sys;
   if (ev) (void)hipEventRecord(ev[0], st);
-  k_zero_counts<<<ns, kBlock, 0, st>>>(sys, p.d_sel, ns, ln.counts);
+  bool const no_spread = (flags & kSolveNoSpread) != 0;
+  bool const zero_rho = !(flags & kSolveAccumulate) && !no_spread;
+  k_clear<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns, ln.counts,
+                                                ln.err_flag, zero_rho ? 1 : 0);
   if (p.blocks_bin > 0) {
     k_bin<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa, ln.counts,
                                             ln.p_brick, ln.p_rank, p_nmp, ln.bin_part,
                                             ln.err_flag);
   }
-  k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_bin, ns, ln.bin_part, 7, 0, ln.sums, 0);
-  k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, ln.counts, ln.offsets);
+  k_scan<<<ns, 1024, 0, st>>>(sys, p.d_sel, p.wt_bin, ln.bin_part, ln.sums, ln.counts,
+                              ln.offsets);
   if (p.blocks_bin > 0)
     k_scatter<<<p.blocks_bin, kBlock, 0, st>>>(sys, p.d_sel, p.wt_bin, ns, p.soff, pa,
                                                 ln.p_brick, ln.p_rank, ln.counts, ln.spos,
                                                 ln.sidx);
-  bool const no_spread = (flags & kSolveNoSpread) != 0;
-  if (!(flags & kSolveAccumulate) && !no_spread)
-    k_fill_rho<MT><<<p.blocks_fill, kBlock, 0, st>>>(sys, p.d_sel, p.wt_fill, ns);
   if (ev) (void)hipEventRecord(ev[1], st);
   for (auto const &g : p.groups) {
     if (no_spread) break;
@@ -2187,13 +2210,15 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (p.blocks_x > 0) {
     k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
                                                           want_force ? 1 : 0, ln.conv_part);
-    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, ln.conv_part, 8, 8, ln.sums, 1);
+    if (want_sums)
+      k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, ln.conv_part, 8, 8, ln.sums, 1);
   }
   if (p.blocks_f > 0) {
     double *fpart = ln.conv_part + 8 * static_cast<size_t>(p.blocks_x);
     k_fft_yx_conv<MT><<<p.blocks_f, kBlock, p.lds_f, st>>>(sys, p.d_sel, p.wt_f, ns, tw,
                                                            want_force ? 1 : 0, fpart);
-    k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, ln.sums, 1);
+    if (want_sums)
+      k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, ln.sums, 1);
   }
   if (ev) (void)hipEventRecord(ev[4], st);
   if (want_force) {
@@ -2312,7 +2337,6 @@ void BatchSolver::solve_on(int lane, std::vector<int> const &sel, ParticleArrays
   hipEvent_t *ev = ln.ev;
   if (ln.profiling && static_cast<size_t>(ln.ring_used + 1) * 8 <= ln.ring.size())
     ev = &ln.ring[8 * static_cast<size_t>(ln.ring_used++)];
-  k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
   if (m_prec == MeshPrecision::FP64)
     enqueue<double>(m, ln, pa, flags, m_shape, ev, nullptr, false);
   else
@@ -2400,8 +2424,7 @@ void BatchSolver::graph_solve(std::vector<int> const &sel, ParticleArrays const 
     p.graph = nullptr;
     hipGraph_t graph;
     hip_check(hipStreamBeginCapture(ln.stream, hipStreamCaptureModeRelaxed), "begin capture");
-    k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
-    if (m_prec == MeshPrecision::FP64)
+      if (m_prec == MeshPrecision::FP64)
       enqueue<double>(m, ln, pa, flags, m_shape, nullptr, nullptr, false);
     else
       enqueue<float>(m, ln, pa, flags, m_shape, nullptr, nullptr, false);
@@ -2455,7 +2478,6 @@ void BatchSolver::debug_charge_mesh(int s, ParticleArrays const &pa, std::vector
   upload_descs(m, ln.stream);
   build_plan(m, ln, {s}, m_prec, m_shape);
   ensure_scratch(m, ln, ln.plan.npart, nullptr, 0);
-  k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
   if (m_prec == MeshPrecision::FP64)
     enqueue<double>(m, ln, pa, 0, m_shape, ln.ev, nullptr, true);
   else
@@ -2489,8 +2511,7 @@ void BatchSolver::debug_indices(int s, ParticleArrays const &pa, std::vector<int
   int *d_nmp = nullptr;
   hip_check(hipMalloc(&d_nmp, sizeof(int) * 3 * std::max<int64_t>(n, 1)), "hipMalloc nmp");
   try {
-    k_fill_int<<<1, 64, 0, ln.stream>>>(ln.err_flag, 1);
-    if (m_prec == MeshPrecision::FP64)
+      if (m_prec == MeshPrecision::FP64)
       enqueue<double>(m, ln, pa, 0, m_shape, ln.ev, d_nmp, true);
     else
       enqueue<float>(m, ln, pa, 0, m_shape, ln.ev, d_nmp, true);

@@ -102,8 +102,19 @@ def lib():
     L.p3mo_real_space_subset.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double,
                                          C.c_double, C.c_double, C.c_int,
                                          C.c_int, _dp]
+    L.p3mo_set_threads.restype = None
+    L.p3mo_set_threads.argtypes = [C.c_int]
+    L.p3mo_get_threads.restype = C.c_int
+    L.p3mo_get_threads.argtypes = []
     _lib = L
     return L
+
+
+def set_threads(n):
+    """Threads of the oracle's solve: 1 (default) is the serial restatement
+    the parity checks use; more threads time the all-cores CPU baseline
+    (the charge mesh is then summed from per-thread meshes)."""
+    lib().p3mo_set_threads(int(n))
 
 
 def _v3(x, dtype=np.float64):

@@ -28,6 +28,7 @@
  * results differ only by FP64 rounding).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -43,6 +44,15 @@
 /* small helpers                                                            */
 /* ------------------------------------------------------------------------ */
 static double sqr(double x) { return x * x; }
+
+/* Threads of the solve (charge assignment, FFT lines, k-space loop, force
+ * interpolation).  1 (the default) keeps the serial loops that the parity
+ * checks use; more threads are for timing the CPU baseline: the charge mesh
+ * is then summed from per-thread meshes, which changes the order of the
+ * floating-point additions and nothing else. */
+static int g_threads = 1;
+void p3mo_set_threads(int t) { g_threads = t < 1 ? 1 : t; }
+int p3mo_get_threads(void) { return g_threads; }
 
 /* math::sinc, math.hpp:53-81: sin(pi x)/(pi x), 8th-order Taylor series for
  * |x| <= 0.1 (coefficients 1/1!, -1/3!, 1/5!, -1/7!, 1/9!). */
@@ -262,6 +272,21 @@ void p3mo_fft3d(int n0, int n1, int n2, double *data_interleaved, int sign) {
       tw[j].im = sin(ang);
     }
     int const o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+    if (g_threads > 1) {
+      int const nlines = n[o1] * n[o2];
+#pragma omp parallel num_threads(g_threads)
+      {
+        cplx *tl = malloc(sizeof(cplx) * 3 * (size_t)nmax);
+#pragma omp for schedule(static)
+        for (int ab = 0; ab < nlines; ++ab) {
+          int const a = ab / n[o2], b = ab % n[o2];
+          size_t const base = (size_t)a * strides[o1] + (size_t)b * strides[o2];
+          fft1d_strided(d + base, len, (int)strides[ax], sign, tl, tl + nmax, tl + 2 * nmax, tw);
+        }
+        free(tl);
+      }
+      continue;
+    }
     for (int a = 0; a < n[o1]; ++a)
       for (int b = 0; b < n[o2]; ++b) {
         size_t const base = (size_t)a * strides[o1] + (size_t)b * strides[o2];
@@ -665,6 +690,8 @@ void p3mo_count_charged(p3mo *s, int n, double const *q) {
 
 /* prepare_fft_mesh + charge_assign (AssignCharge<cao>), p3m.impl.hpp:161-168,
  * p3m.cpp:283-330 and the p3m_interpolate stencil walk, interpolation.hpp:191-208 */
+static int charge_assign_threaded(p3mo *s, int n, double const *pos, double const *q);
+
 static int charge_assign(p3mo *s, int n, double const *pos, double const *q) {
   int const cao = s->cao;
   for (int i = 0; i < s->size; i++) s->rs_scalar[i] = 0.;
@@ -674,6 +701,7 @@ static int charge_assign(p3mo *s, int n, double const *pos, double const *q) {
     s->ca_fmp = realloc(s->ca_fmp, sizeof(int) * (size_t)n);
     s->ca_frac = realloc(s->ca_frac, sizeof(double) * 3 * (size_t)cao * (size_t)n);
   }
+  if (g_threads > 1) return charge_assign_threaded(s, n, pos, q);
   for (int p = 0; p < n; ++p) {
     if (q[p] == 0.0) continue;
     int ind, nmp[3];
@@ -697,12 +725,66 @@ static int charge_assign(p3mo *s, int n, double const *pos, double const *q) {
   return 0;
 }
 
+/* charge_assign on g_threads threads: the cache slot of each charged
+ * particle from a serial prefix count, weights in parallel, each thread
+ * spreading its share of the particles into a private CA mesh, then the
+ * private meshes summed point by point. */
+static int charge_assign_threaded(p3mo *s, int n, double const *pos, double const *q) {
+  int const cao = s->cao;
+  int *slot = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int nc = 0;
+  for (int p = 0; p < n; ++p) slot[p] = (q[p] != 0.0) ? nc++ : -1;
+  s->n_cache = nc;
+  int const T = g_threads;
+  size_t const size = (size_t)s->size;
+  double *priv = calloc((size_t)T * size, sizeof(double));
+  int bad = 0;
+#pragma omp parallel num_threads(T) reduction(| : bad)
+  {
+    double *mesh = priv + (size_t)omp_get_thread_num() * size;
+#pragma omp for schedule(static)
+    for (int p = 0; p < n; ++p) {
+      if (slot[p] < 0) continue;
+      int ind, nmp[3];
+      double *w = s->ca_frac + 3 * (size_t)cao * (size_t)slot[p];
+      if (p3mo_weights(s, pos + 3 * (size_t)p, &ind, nmp, w) != 0) {
+        bad = 1;
+        continue;
+      }
+      s->ca_fmp[slot[p]] = ind;
+      int q_ind = ind;
+      for (int i0 = 0; i0 < cao; i0++) {
+        double const tmp0 = w[i0];
+        for (int i1 = 0; i1 < cao; i1++) {
+          double const tmp1 = tmp0 * w[cao + i1];
+          for (int i2 = 0; i2 < cao; i2++) {
+            mesh[q_ind] += (tmp1 * w[2 * cao + i2]) * q[p];
+            q_ind++;
+          }
+          q_ind += s->q_2_off;
+        }
+        q_ind += s->q_21_off;
+      }
+    }
+#pragma omp for schedule(static)
+    for (size_t i = 0; i < size; ++i) {
+      double acc = 0.;
+      for (int t = 0; t < T; ++t) acc += priv[(size_t)t * size + i];
+      s->rs_scalar[i] = acc;
+    }
+  }
+  free(priv);
+  free(slot);
+  return bad ? -1 : 0;
+}
+
 /* inner block of the CA mesh -> complex M^3 array and forward DFT
  * (FFTBackendLegacy::forward_fft -> fft.cpp:715-745) */
 static void forward_fft(p3mo *s) {
   int const *M = s->mesh;
-  size_t idx = 0;
-  for (int x = 0; x < M[0]; ++x)
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+  for (int x = 0; x < M[0]; ++x) {
+    size_t idx = (size_t)x * M[1] * M[2];
     for (int y = 0; y < M[1]; ++y)
       for (int z = 0; z < M[2]; ++z) {
         int const li = (s->in_ld[2] + z) + s->dim[2] * ((s->in_ld[1] + y) + s->dim[1] * (s->in_ld[0] + x));
@@ -710,6 +792,7 @@ static void forward_fft(p3mo *s) {
         s->kmesh[idx].im = 0.;
         ++idx;
       }
+  }
   p3mo_fft3d(M[0], M[1], M[2], (double *)s->kmesh, -1);
 }
 
@@ -718,14 +801,16 @@ static void forward_fft(p3mo *s) {
 static void backward_fft(p3mo *s, double *rs) {
   int const *M = s->mesh;
   p3mo_fft3d(M[0], M[1], M[2], (double *)s->kfield, +1);
-  size_t idx = 0;
-  for (int x = 0; x < M[0]; ++x)
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+  for (int x = 0; x < M[0]; ++x) {
+    size_t idx = (size_t)x * M[1] * M[2];
     for (int y = 0; y < M[1]; ++y)
       for (int z = 0; z < M[2]; ++z) {
         int const li = (s->in_ld[2] + z) + s->dim[2] * ((s->in_ld[1] + y) + s->dim[1] * (s->in_ld[0] + x));
         rs[li] = s->kfield[idx].re;
         ++idx;
       }
+  }
 }
 
 static void calc_dipole(int n, double const *q, double const *upos, double dip[3]) {
@@ -759,8 +844,9 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
     double wavevector[3];
     for (int d = 0; d < 3; ++d) wavevector[d] = (2. * P3MO_PI) * (1. / s->box_l[d]);
     for (int d_rs = 0; d_rs < 3; ++d_rs) {
-      size_t idx = 0;
-      for (int x = 0; x < M[0]; ++x)
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+      for (int x = 0; x < M[0]; ++x) {
+        size_t idx = (size_t)x * M[1] * M[2];
         for (int y = 0; y < M[1]; ++y)
           for (int z = 0; z < M[2]; ++z) {
             int const nn[3] = {x, y, z};
@@ -771,6 +857,7 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
             s->kfield[idx].im = +k * phr;
             ++idx;
           }
+      }
       backward_fft(s, s->rs_fields[d_rs]);
       (void)M3;
     }
@@ -779,9 +866,14 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
     /* AssignForces<cao>, p3m.cpp:344-373 */
     double const force_prefac = s->prefactor / volume;
     int const cao = s->cao;
-    int p_index = 0;
+    /* cache slot of each charged particle: its rank among the charged ones */
+    int *slot = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int nc = 0;
+    for (int p = 0; p < n; ++p) slot[p] = (q[p] != 0.0) ? nc++ : -1;
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
     for (int p = 0; p < n; ++p) {
       if (q[p] == 0.0) continue;
+      int const p_index = slot[p];
       double const pf = q[p] * force_prefac;
       double const *w = s->ca_frac + 3 * (size_t)cao * (size_t)p_index;
       int q_ind = s->ca_fmp[p_index];
@@ -803,8 +895,8 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
       }
       if (force)
         for (int d = 0; d < 3; ++d) force[3 * (size_t)p + d] -= pf * f[d];
-      ++p_index;
     }
+    free(slot);
     /* dipole correction, p3m.cpp:538-547 */
     if (has_dipole && force) {
       double dm[3];
@@ -816,8 +908,14 @@ static double long_range_kernel(p3mo *s, int force_flag, int energy_flag, int n,
 
   if (energy_flag) {
     double node_energy = 0.;
-    for (size_t i = 0; i < M3; i++)
-      node_energy += s->g_energy[i] * (sqr(s->kmesh[i].re) + sqr(s->kmesh[i].im));
+    if (g_threads > 1) {
+#pragma omp parallel for schedule(static) num_threads(g_threads) reduction(+ : node_energy)
+      for (size_t i = 0; i < M3; i++)
+        node_energy += s->g_energy[i] * (sqr(s->kmesh[i].re) + sqr(s->kmesh[i].im));
+    } else {
+      for (size_t i = 0; i < M3; i++)
+        node_energy += s->g_energy[i] * (sqr(s->kmesh[i].re) + sqr(s->kmesh[i].im));
+    }
     node_energy /= 2. * volume;
     double energy = node_energy;
     energy -= s->sum_q2 * s->alpha * P3MO_INV_SQRTPI;

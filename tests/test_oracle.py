@@ -370,3 +370,28 @@ def test_gpu_convention_even_cao_against_ewald():
     d = f_k + f_r - fx["force"]
     err = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
     assert err <= 1.5 * acc, (err, t[2])
+
+
+def test_threaded_solve_matches_serial():
+    """The threaded oracle (all-cores CPU baseline) changes only the order of
+    floating-point additions: forces and energy within 1e-12 of the serial
+    solve, for cao 7 on a non-cubic mesh with uncharged particles."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(77)
+    n = 6000
+    box = [14., 15., 16.]
+    pos = rng.uniform(0., 1., (n, 3)) * box
+    q = np.where(np.arange(n) % 2 == 0, 1., -1.)
+    q[::11] = 0.
+    ora = O.OracleP3M(box, [20, 24, 22], 7, 0.9, 4.0, 1.1)
+    f1 = ora.forces(pos, q)
+    e1 = ora.energy(pos, q)
+    try:
+        O.set_threads(4)
+        f4 = ora.forces(pos, q)
+        e4 = ora.energy(pos, q)
+    finally:
+        O.set_threads(1)
+    assert np.sqrt(np.sum((f4 - f1) ** 2) / np.sum(f1 ** 2)) <= 1e-12
+    assert abs(e4 - e1) <= 1e-12 * abs(e1)
+    assert O.lib().p3mo_get_threads() == 1
