@@ -2202,7 +2202,13 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     });
   }
   if (ev) (void)hipEventRecord(ev[2], st);
-  if (stop_after_spread || (flags & kSolveMeshOnly)) return;
+  if (stop_after_spread || (flags & kSolveMeshOnly)) {
+    // the stage events of a mesh-only solve still bracket it: an event that
+    // was never recorded makes hipEventElapsedTime fail
+    if (ev)
+      for (int k = 3; k <= 6; ++k) (void)hipEventRecord(ev[k], st);
+    return;
+  }
   k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
   if (p.blocks_y > 0)
     k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);

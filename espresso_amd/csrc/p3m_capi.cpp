@@ -77,12 +77,15 @@ struct p3m_dipolar {
 
 struct p3m_batch {
   int device = 0;
-  std::unique_ptr<BatchSolver> solver;
+  // shared with the servers built over the batch, which keep the solver
+  // alive when the batch handle is destroyed before them
+  std::shared_ptr<BatchSolver> solver;
   std::string last_error;
 };
 
 struct p3m_server {
-  p3m_batch *batch = nullptr;
+  int device = 0;
+  std::shared_ptr<BatchSolver> solver; // declared first: outlives `server`
   std::unique_ptr<Server> server;
   std::string last_error;
 };
@@ -1376,8 +1379,9 @@ int p3m_serve_create(p3m_batch *b, int lanes, int64_t budget_bytes, int max_batc
     cfg.max_batch = max_batch;
     cfg.sigma = sigma;
     auto s = std::make_unique<p3m_server>();
-    s->batch = b;
-    s->server = std::make_unique<Server>(*b->solver, cfg);
+    s->device = b->device;
+    s->solver = b->solver;
+    s->server = std::make_unique<Server>(*s->solver, cfg);
     *out = s.release();
   });
 }
@@ -1396,9 +1400,9 @@ int p3m_serve_run(p3m_server *s, int n_req, double const *arrival_s, int const *
     if (n_req < 0 || (n_req > 0 && (!arrival_s || !sys_ids || !seeds)))
       throw std::invalid_argument("P3M serve: request arrays required");
     if (force_out && !out_off) throw std::invalid_argument("P3M serve: out_off required");
-    if (s->batch->solver->total_particles() > 0 && (!pos || !q || !force))
+    if (s->solver->total_particles() > 0 && (!pos || !q || !force))
       throw std::invalid_argument("P3M serve: particle arrays required");
-    hip_check(hipSetDevice(s->batch->device), "hipSetDevice");
+    hip_check(hipSetDevice(s->device), "hipSetDevice");
     std::vector<ServeRequest> reqs(static_cast<size_t>(n_req));
     for (int r = 0; r < n_req; ++r) reqs[r] = ServeRequest{arrival_s[r], sys_ids[r], seeds[r]};
     ServeStats const st = s->server->run(reqs, pos, q, force, force_out, out_off);

@@ -97,6 +97,20 @@ ServeStats Server::run(std::vector<ServeRequest> const &reqs, double *pos, doubl
   int const nl = static_cast<int>(m_lanes.size());
   std::vector<InFlight> fl(nl);
   for (auto &f : fl) hip_check(hipEventCreateWithFlags(&f.done, hipEventDisableTiming), "event");
+  {
+    // the serving lanes start after the work enqueued on the batch's own
+    // stream (the caller fences that stream with its producers of pos/q/force)
+    hipEvent_t ready = nullptr;
+    hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "event");
+    hipError_t e = hipEventRecord(ready, m_batch.stream());
+    for (int l : m_lanes)
+      if (e == hipSuccess) e = hipStreamWaitEvent(m_batch.lane_stream(l), ready, 0);
+    (void)hipEventDestroy(ready);
+    if (e != hipSuccess) {
+      for (auto &f : fl) (void)hipEventDestroy(f.done);
+      hip_check(e, "P3M serve: fence on the batch stream");
+    }
+  }
   std::vector<char> busy_sys(nsys, 0), in_batch(nsys, 0), taken(nreq, 0);
   std::deque<int> pending;
   ArenaAllocator arena(m_cfg.budget_bytes);

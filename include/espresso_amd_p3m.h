@@ -452,7 +452,8 @@ const char *p3m_batch_last_error(const p3m_batch *b);
  * particle scratch of every batch in flight comes from one device arena of
  * budget_bytes.  Per request the system's positions get a Gaussian
  * displacement of width sigma (counter-based draws keyed by the seed, folded
- * into the box), its forces are cleared and recomputed.
+ * into the box), its forces are cleared and recomputed.  A server keeps the
+ * batch's solver alive: the batch handle may be destroyed before it.
  * ------------------------------------------------------------------------ */
 typedef struct p3m_server p3m_server;
 
@@ -461,6 +462,8 @@ int p3m_serve_create(p3m_batch *b, int lanes, int64_t budget_bytes, int max_batc
 int p3m_serve_destroy(p3m_server *s);
 /* Serve n_req requests sorted by arrival_s (seconds after the call starts)
  * to completion.  pos, q, force: device arrays of all systems of the batch.
+ * The serving streams first wait for the work enqueued on the batch's stream
+ * (p3m_batch_get_stream) before the call.
  * force_out (device, may be NULL): each request's forces are copied to
  * force_out + 3 * out_off[r] when its batch completes.  Outputs: latency_s
  * [n_req] (arrival to completion), wall_s, n_batches, peak_bytes (working

@@ -115,3 +115,16 @@ def rel_rms(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b ** 2)))
+
+
+def energy_scale(e_ref, q, alpha, prefactor):
+    """Magnitude of the sums a P3M energy is the difference of.
+
+    The k-space energy (p3m.cpp:551-589) is the mesh sum minus the self term
+    prefactor*alpha/sqrt(pi)*sum(q^2); on dense small systems the two are
+    100x larger than their difference, so a rounding error of the FP32 mesh
+    sum shows in the total energy amplified by that ratio.  FP32 energy
+    tolerances are stated relative to |E| + E_self.
+    """
+    q = np.asarray(q, dtype=np.float64)
+    return abs(e_ref) + prefactor * alpha / math.sqrt(math.pi) * float(np.sum(q * q))
