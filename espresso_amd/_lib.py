@@ -39,6 +39,7 @@ P3M_TUNE_RS_DEVICE = 1
 EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
     "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
+    "p3m_solve_noalloc", "p3m_noalloc_scratch_bytes",
     "p3m_prepare_fft_mesh", "p3m_assign_charges", "p3m_charge_assign",
     "p3m_count_charged_particles", "p3m_count_charged_particles_elc",
     "p3m_adapt_epsilon_elc", "p3m_assigned_count", "p3m_solve_assigned",
@@ -147,6 +148,8 @@ def load():
     L.p3m_is_tuned.argtypes = [vp, C.POINTER(C.c_int)]
     L.p3m_activate.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]
     L.p3m_solve.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp, dp]
+    L.p3m_solve_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, C.c_uint, dp, dp]
+    L.p3m_noalloc_scratch_bytes.argtypes = [C.POINTER(C.c_int64)]
     L.p3m_prepare_fft_mesh.argtypes = [vp, C.c_int]
     L.p3m_assign_charges.argtypes = [vp, C.c_int64, vp, vp, C.c_int, C.c_uint]
     L.p3m_charge_assign.argtypes = [vp, C.c_int64, vp, vp, C.c_uint]

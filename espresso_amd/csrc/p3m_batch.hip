@@ -5,10 +5,21 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numbers>
 #include <stdexcept>
 #include <string>
 #include <tuple>
+
+namespace {
+/** Working memory of the allocation-free solves (BatchSolver::solve_fixed):
+ *  one array per device, held until the process ends.  Host code never
+ *  passes `scratch` itself to a kernel: it uses the device address that
+ *  hipGetSymbolAddress returns for the current device. */
+constexpr size_t kFixedScratchBytes = size_t{48} << 20;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+} // namespace
 
 namespace espresso_amd {
 
@@ -1500,6 +1511,10 @@ struct Lane {
   int ring_used = 0;
   bool profiling = false;
   Plan plan;
+  // the fixed lane of allocation-free solves: lane 0's stream, every device
+  // buffer carved from the fixed scratch array, sums read back into h_fixed
+  bool fixed = false;
+  std::vector<double> h_fixed;
 };
 
 struct BatchSolver::Impl {
@@ -1514,6 +1529,9 @@ struct BatchSolver::Impl {
   size_t tw_len = 0;
   std::vector<std::unique_ptr<Lane>> lanes;
   bool kernels_ready = false;
+  std::unique_ptr<Lane> fixed; // see enable_fixed()
+  char *fixed_base = nullptr;  // this device's address of the fixed scratch array
+  size_t fixed_nsys = 0;       // systems when the fixed lane was set up
 };
 
 namespace {
@@ -1557,6 +1575,7 @@ void quiesce(BatchSolver::Impl *m) {
     hip_check(hipStreamSynchronize(ln->stream), "sync lane");
     free_plan(ln->plan);
   }
+  if (m->fixed) free_plan(m->fixed->plan); // on lane 0's stream, drained above
 }
 } // namespace
 
@@ -1572,6 +1591,11 @@ BatchSolver::BatchSolver(int device, MeshPrecision precision)
 
 BatchSolver::~BatchSolver() {
   (void)hipSetDevice(m_device);
+  if (m->fixed) { // borrowed stream and carved buffers: only the events are its own
+    free_plan(m->fixed->plan);
+    for (auto &e : m->fixed->ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   for (auto &ln : m->lanes) destroy_lane(*ln);
   for (auto &h : m->sys) {
     void *ptrs[] = {h.desc.rho, h.desc.Ey, h.desc.Ez, h.desc.khat, h.desc.A, h.desc.B, h.desc.G};
@@ -2000,11 +2024,50 @@ void ensure_scratch(BatchSolver::Impl *m, Lane &ln, int64_t npart, void *externa
   }
 }
 
+/** Every device buffer of a fixed-lane solve, carved from the fixed scratch
+ *  array in 256-byte aligned pieces: error flag, per-system sums, brick
+ *  counters and offsets of all systems, binning and convolution partials,
+ *  work tables, then the particle scratch of the selection. */
+void carve_fixed(BatchSolver::Impl *m, Lane &ln, size_t table_ints, int64_t npart,
+                 int blocks_bin, int conv_blocks) {
+  int nb = 0;
+  for (auto const &h : m->sys) nb += h.nbricks;
+  size_t const nsys = m->sys.size();
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *const p = m->fixed_base + off;
+    off += align256(std::max<size_t>(bytes, 1));
+    return p;
+  };
+  ln.err_flag = reinterpret_cast<int *>(take(sizeof(int)));
+  ln.sums = reinterpret_cast<double *>(take(sizeof(double) * 16 * nsys));
+  ln.counts = reinterpret_cast<int *>(take(sizeof(int) * kCountCopies * nb));
+  ln.offsets = reinterpret_cast<int *>(take(sizeof(int) * (nb + nsys)));
+  ln.bin_part = reinterpret_cast<double *>(take(sizeof(double) * 8 * blocks_bin));
+  ln.conv_part = reinterpret_cast<double *>(take(sizeof(double) * 8 * conv_blocks));
+  ln.tables = reinterpret_cast<int *>(take(sizeof(int) * table_ints));
+  char *const parts = take(scratch_bytes(npart));
+  if (off > kFixedScratchBytes)
+    throw std::invalid_argument(
+        "P3M: an allocation-free solve of " + std::to_string(npart) + " particles needs " +
+        std::to_string(off) + " bytes of scratch, more than the fixed " +
+        std::to_string(kFixedScratchBytes) + " (about 44 bytes per particle)");
+  carve(ln, parts, npart);
+  ln.counts_cap = nb;
+  ln.offsets_cap = nb + static_cast<int>(nsys);
+  ln.sums_cap = static_cast<int>(16 * nsys);
+  ln.bin_cap = blocks_bin;
+  ln.conv_cap = conv_blocks;
+  ln.tables_cap = table_ints;
+}
+
 void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, MeshPrecision prec,
                 LaunchShape const &shape) {
   Plan &p = ln.plan;
   hipStream_t const st = ln.stream;
-  if (p.version == m->version && p.sel == sel && p.dbuf) return;
+  // the fixed lane's tables live in the device-wide fixed scratch, which
+  // other contexts' fixed solves overwrite: rebuilt on every solve
+  if (!ln.fixed && p.version == m->version && p.sel == sel && p.dbuf) return;
   hip_check(hipStreamSynchronize(st), "sync before plan rebuild"); // queued work reads the tables
   free_plan(p);
   p.sel = sel;
@@ -2103,8 +2166,15 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
     npart += m->sys[s].n;
   }
   if (npart > INT32_MAX) throw std::runtime_error("P3M: too many particles in one selection");
+  if (ln.fixed) {
+    carve_fixed(m, ln, host.size(), npart, bin.second, fx.second + ff.second);
+    hip_check(hipMemcpyAsync(ln.tables, host.data(), sizeof(int) * host.size(),
+                             hipMemcpyHostToDevice, st),
+              "copy plan");
+    hip_check(hipStreamSynchronize(st), "copy plan"); // `host` is a local
+  }
   // the lane is idle (synchronised above): its table buffers can be reused
-  if (host.size() > ln.tables_cap) {
+  if (!ln.fixed && host.size() > ln.tables_cap) {
     dfree(ln.tables);
     if (ln.h_tables) (void)hipHostFree(ln.h_tables);
     ln.h_tables = nullptr;
@@ -2113,11 +2183,13 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
     hip_check(hipHostMalloc(&ln.h_tables, sizeof(int) * cap), "hipHostMalloc plan");
     ln.tables_cap = cap;
   }
-  std::memcpy(ln.h_tables, host.data(), sizeof(int) * host.size());
   p.dbuf = ln.tables;
-  hip_check(hipMemcpyAsync(p.dbuf, ln.h_tables, sizeof(int) * host.size(), hipMemcpyHostToDevice,
-                           st),
-            "copy plan");
+  if (!ln.fixed) {
+    std::memcpy(ln.h_tables, host.data(), sizeof(int) * host.size());
+    hip_check(hipMemcpyAsync(p.dbuf, ln.h_tables, sizeof(int) * host.size(),
+                             hipMemcpyHostToDevice, st),
+              "copy plan");
+  }
   p.d_sel = p.dbuf;
   p.wt_bin = p.dbuf + bin.first;
   p.blocks_bin = bin.second;
@@ -2138,6 +2210,7 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
   for (auto const &g : gh)
     p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks});
   p.version = m->version;
+  if (ln.fixed) return; // partial-sum buffers carved above
   if (p.blocks_bin > ln.bin_cap) {
     dfree(ln.bin_part); // the stream is idle (synchronised above)
     hip_check(hipMalloc(&ln.bin_part, sizeof(double) * 8 * std::max(p.blocks_bin, 1)),
@@ -2365,6 +2438,83 @@ void BatchSolver::solve_on(int lane, std::vector<int> const &sel, ParticleArrays
   last_ms[0] = t;
   for (int st = 1; st <= 6; ++st) {
     (void)hipEventElapsedTime(&t, ev[st - 1], ev[st]);
+    last_ms[st] = t;
+  }
+  if (out) finalize_results(m, ln, sel, out);
+}
+
+namespace {
+/** This device's address of the fixed scratch array, looked up once per
+ *  device (the device must be current). */
+char *fixed_scratch_address(int device) {
+  static std::mutex mu;
+  static std::map<int, char *> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto const it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void *p = nullptr;
+  hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!p) throw std::runtime_error("P3M: no device address for the fixed scratch array");
+  addr[device] = static_cast<char *>(p);
+  return static_cast<char *>(p);
+}
+} // namespace
+
+size_t BatchSolver::fixed_scratch_bytes() { return kFixedScratchBytes; }
+
+void BatchSolver::enable_fixed() {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  char *const base = fixed_scratch_address(m_device);
+  if (!m->fixed) {
+    auto ln = std::make_unique<Lane>();
+    ln->fixed = true;
+    ln->stream = m_stream;
+    for (auto &e : ln->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    m->fixed = std::move(ln);
+  }
+  m->fixed_base = base;
+  m->fixed->h_fixed.assign(16 * m->sys.size() + 8, 0.);
+  m->fixed->h_sums = m->fixed->h_fixed.data();
+  m->fixed_nsys = m->sys.size();
+  upload_descs(m, m_stream);
+}
+
+void BatchSolver::solve_fixed(std::vector<int> const &sel, ParticleArrays const &pa,
+                              unsigned flags, std::vector<SolveResult> *out) {
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  if (!m->fixed) throw std::runtime_error("P3M: allocation-free solve before its set-up");
+  if (m->sys.size() != m->fixed_nsys || m->sys.size() > m->d_sys_cap)
+    throw std::runtime_error("P3M: systems were added after the allocation-free set-up");
+  if (sel.empty()) {
+    if (out) out->clear();
+    return;
+  }
+  for (int s : sel)
+    if (s < 0 || s >= num_systems()) throw std::invalid_argument("P3M: invalid system id");
+  Lane &ln = *m->fixed;
+  upload_descs(m, ln.stream); // copies into the existing descriptor array
+  build_plan(m, ln, sel, m_prec, m_shape);
+  if (m_prec == MeshPrecision::FP64)
+    enqueue<double>(m, ln, pa, flags, m_shape, ln.ev, nullptr, false);
+  else
+    enqueue<float>(m, ln, pa, flags, m_shape, ln.ev, nullptr, false);
+  hip_check(hipGetLastError(), "P3M solve launch");
+  int const ns = num_systems() * 16;
+  hip_check(hipMemcpyAsync(ln.h_sums, ln.sums, sizeof(double) * ns, hipMemcpyDeviceToHost,
+                           ln.stream),
+            "copy sums");
+  hip_check(hipMemcpyAsync(ln.h_sums + ns, ln.err_flag, sizeof(int), hipMemcpyDeviceToHost,
+                           ln.stream),
+            "copy err");
+  hip_check(hipStreamSynchronize(ln.stream), "P3M solve");
+  int err = 0;
+  std::memcpy(&err, ln.h_sums + ns, sizeof(int));
+  if (err) throw std::runtime_error("P3M: non-finite particle position");
+  float t = 0.f;
+  (void)hipEventElapsedTime(&t, ln.ev[0], ln.ev[6]);
+  last_ms[0] = t;
+  for (int st = 1; st <= 6; ++st) {
+    (void)hipEventElapsedTime(&t, ln.ev[st - 1], ln.ev[st]);
     last_ms[st] = t;
   }
   if (out) finalize_results(m, ln, sel, out);

@@ -150,6 +150,25 @@ public:
    *  systems (particle scratch). */
   size_t working_bytes(std::vector<int> const &sel) const;
 
+  /** Set-up of allocation-free solves (solve_fixed): looks up this device's
+   *  copy of the library's fixed scratch array, uploads the descriptors and
+   *  creates the events of the fixed lane.  Allocates; call once after the
+   *  systems are added (Solver::configure does). */
+  void enable_fixed();
+  /** Synchronous solve of `sel` whose every device buffer beyond the
+   *  resident systems and the caller's arrays (particle scratch, brick
+   *  counters and offsets, partial and per-system sums, work tables, error
+   *  flag) is carved from the fixed scratch array of fixed_scratch_bytes():
+   *  no device or pinned-host memory is allocated or freed.  Throws when the
+   *  solve needs more than that array or enable_fixed() was not called (or
+   *  systems were added since).  The array is shared by every context on
+   *  the device: fixed solves that overlap on one device race. */
+  void solve_fixed(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags,
+                   std::vector<SolveResult> *out);
+  /** Bytes of the fixed scratch a solve of `sel` needs. */
+  size_t fixed_bytes(std::vector<int> const &sel) const;
+  static size_t fixed_scratch_bytes();
+
   /** Copy the charge mesh of system s after spreading ([Mx][My][Mz]). */
   void debug_charge_mesh(int s, ParticleArrays const &pa, std::vector<double> &out);
   /** CA-local first stencil point of every particle of system s. */

@@ -684,6 +684,55 @@ int p3m_solve(p3m_context *ctx, int64_t n, double const *pos, double const *q,
   });
 }
 
+int p3m_solve_noalloc(p3m_context *ctx, int64_t n, double const *pos, double const *q,
+                      double const *pos_unfolded, double *force, unsigned flags, double *energy,
+                      double *virial) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_params(ctx);
+    require_device(ctx);
+    if (!ctx->is_tuned || !ctx->initialized)
+      throw std::runtime_error("P3M: p3m_solve_noalloc needs a context set up by p3m_activate "
+                               "(or by an earlier p3m_solve)");
+    if (ctx->elc_on) throw std::invalid_argument("P3M: p3m_solve_noalloc does not support ELC");
+    if (!(flags & P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: p3m_solve_noalloc takes device pointers only "
+                                  "(P3M_DEVICE_POINTERS)");
+    if (flags & ~(P3M_FORCES | P3M_ENERGY | P3M_PRESSURE | P3M_DEVICE_POINTERS))
+      throw std::invalid_argument("P3M: p3m_solve_noalloc is synchronous: P3M_ASYNC and "
+                                  "P3M_DEVICE_RESULTS are not available");
+    if (n < 0) throw std::invalid_argument("P3M: negative particle count");
+    if ((flags & P3M_FORCES) && !force && n > 0)
+      throw std::invalid_argument("P3M: force array required");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->mesh_assigned = false;
+    SolveResult r;
+    unsigned const red = flags & (P3M_ENERGY | P3M_PRESSURE);
+    if (ctx->force_conv == P3M_CONV_GPU && (flags & P3M_FORCES)) {
+      // forces with the GPU index rule, energy and virial from the CPU mesh
+      ctx->solver->set_convention(1);
+      ctx->solver->solve_fixed(n, pos, q, pos_unfolded, force, flags & ~red, nullptr);
+      if (red) {
+        ctx->solver->set_convention(0);
+        ctx->solver->solve_fixed(n, pos, q, pos_unfolded, nullptr, flags & ~P3M_FORCES, &r);
+      }
+    } else {
+      ctx->solver->set_convention(0);
+      ctx->solver->solve_fixed(n, pos, q, pos_unfolded, (flags & P3M_FORCES) ? force : nullptr,
+                               flags, &r);
+    }
+    if (energy) *energy = (flags & P3M_ENERGY) ? r.energy : 0.;
+    if (virial)
+      for (int k = 0; k < 9; ++k) virial[k] = (flags & P3M_PRESSURE) ? r.virial[k] : 0.;
+  });
+}
+
+int p3m_noalloc_scratch_bytes(int64_t *bytes) {
+  if (!bytes) return P3M_ERR_INVALID;
+  *bytes = static_cast<int64_t>(BatchSolver::fixed_scratch_bytes());
+  return P3M_OK;
+}
+
 namespace {
 void require_solver_ready(p3m_context *ctx) {
   require_params(ctx);

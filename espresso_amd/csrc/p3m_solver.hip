@@ -50,6 +50,7 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
   if (m_shape_set) b->set_launch_shape(m_shape);
   b->add_system(params, box_l, skin, prefactor, 0);
   b->set_convention(0, m_conv);
+  b->enable_fixed(); // set-up of the allocation-free solves
   m_batch = std::move(b);
 }
 
@@ -72,6 +73,18 @@ void Solver::solve(int64_t n, double const *d_pos, double const *d_q, double con
   std::vector<SolveResult> out;
   m_batch->solve({0}, pa, flags, sync, res ? &out : nullptr);
   if (!sync) return;
+  for (int k = 0; k < 8; ++k) last_ms[k] = m_batch->last_ms[k];
+  if (res) *res = out.at(0);
+}
+
+void Solver::solve_fixed(int64_t n, double const *d_pos, double const *d_q,
+                         double const *d_upos, double *d_force, unsigned flags,
+                         SolveResult *res) {
+  if (!m_batch) throw std::runtime_error("P3M solver used before configure()");
+  m_batch->set_particle_count(0, n);
+  ParticleArrays pa{d_pos, d_q, d_upos, d_force};
+  std::vector<SolveResult> out;
+  m_batch->solve_fixed({0}, pa, flags, res ? &out : nullptr);
   for (int k = 0; k < 8; ++k) last_ms[k] = m_batch->last_ms[k];
   if (res) *res = out.at(0);
 }

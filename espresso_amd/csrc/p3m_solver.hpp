@@ -43,6 +43,12 @@ public:
   void solve(int64_t n, double const *d_pos, double const *d_q, double const *d_upos,
              double *d_force, unsigned flags, SolveResult *res, bool sync = true);
 
+  /** solve() on device pointers without any allocation: every working
+   *  buffer comes from the library's fixed scratch array
+   *  (BatchSolver::solve_fixed); synchronous. */
+  void solve_fixed(int64_t n, double const *d_pos, double const *d_q, double const *d_upos,
+                   double *d_force, unsigned flags, SolveResult *res);
+
   /** Energy and virial of the last solve written to device memory in
    *  stream order (BatchSolver::device_results). */
   void device_results(double *d_energy, double *d_virial);

@@ -129,6 +129,26 @@ int p3m_solve(p3m_context *ctx, int64_t n, const double *pos, const double *q,
               const double *pos_unfolded, double *force, unsigned flags, double *energy,
               double *virial);
 
+/* p3m_solve without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code that runs under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every working buffer of the solve (particle scratch, brick counters and
+ * offsets, partial and per-system sums, work tables, error flag) is carved
+ * from one fixed device array of p3m_noalloc_scratch_bytes() (48 MiB) that
+ * the library holds on each device until the process ends; the meshes are
+ * the context's own, allocated when p3m_activate set it up.  Requirements:
+ * an activated context, P3M_DEVICE_POINTERS, no ELC; synchronous (no
+ * P3M_ASYNC or P3M_DEVICE_RESULTS).  A solve needing more scratch than the
+ * array (about 44 bytes per particle: some 1.1 million particles) fails
+ * with P3M_ERR_INVALID.  The array is shared by every context on a device
+ * and keeps the last call's contents: calls that overlap on one device
+ * race. */
+int p3m_solve_noalloc(p3m_context *ctx, int64_t n, const double *pos, const double *q,
+                      const double *pos_unfolded, double *force, unsigned flags, double *energy,
+                      double *virial);
+/* Size of the fixed scratch array of p3m_solve_noalloc in bytes. */
+int p3m_noalloc_scratch_bytes(int64_t *bytes);
+
 /* ------------------------------------------------------------------------
  * Charge-assignment protocol of CoulombP3M, as the reference's
  * ElectrostaticLayerCorrection drives it (elc.cpp:1014, 1110-1141, 1143-1187,
