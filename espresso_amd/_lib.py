@@ -39,7 +39,8 @@ P3M_TUNE_RS_DEVICE = 1
 EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
     "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
-    "p3m_solve_noalloc", "p3m_noalloc_scratch_bytes",
+    "p3m_solve_noalloc", "p3m_noalloc_scratch_bytes", "p3m_set_launch_shape",
+    "p3m_get_launch_shape",
     "p3m_prepare_fft_mesh", "p3m_assign_charges", "p3m_charge_assign",
     "p3m_count_charged_particles", "p3m_count_charged_particles_elc",
     "p3m_adapt_epsilon_elc", "p3m_assigned_count", "p3m_solve_assigned",
@@ -95,7 +96,8 @@ class p3m_params(C.Structure):
 
 class p3m_launch_shape(C.Structure):
     _fields_ = [(name, C.c_int) for name in
-                ("gather", "ncol", "zlines", "fused_lds", "spread", "threads", "chunk")]
+                ("gather", "ncol", "zlines", "fused_lds", "spread", "threads", "chunk",
+                 "column", "seg", "col_lanes")]
 
 
 class p3m_elc_params(C.Structure):
@@ -211,6 +213,8 @@ def load():
     L.p3m_batch_working_bytes.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
+    L.p3m_set_launch_shape.argtypes = [vp, shp]
+    L.p3m_get_launch_shape.argtypes = [vp, shp]
     L.p3m_batch_set_launch_shape.argtypes = [vp, shp]
     L.p3m_batch_get_launch_shape.argtypes = [vp, shp]
     L.p3m_batch_autotune.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_int, shp, dp, dp]

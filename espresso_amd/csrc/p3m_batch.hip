@@ -76,6 +76,7 @@ struct SysDesc {
   int nbricks;
   void *rho, *Ey, *Ez, *khat, *A, *B, *G;
   int tw_off[3];
+  int lz, nseg; // column blocks: x layers per block and blocks per column
   int kzmajor; // 1: half spectra stored [Kz][Mx][My] and the fused y/x pass
   FftPlanDev plan[3];
   double wv[3];
@@ -204,7 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
       int nmp[3], gi[3];
       double dist[3];
       if (!particle_coordinates_rt(d.g, d.cao, x, nmp, gi, dist)) atomicOr(err_flag, 1);
-      b = ((gi[0] / kBrick) * d.g.nb[1] + gi[1] / kBrick) * d.g.nb[2] + gi[2] / kBrick;
+      b = d.g.col ? ((gi[1] / d.g.col) * d.g.nb[2] + gi[2] / d.g.col) * d.g.M[0] + gi[0]
+                  : ((gi[0] / kBrick) * d.g.nb[1] + gi[1] / kBrick) * d.g.nb[2] + gi[2] / kBrick;
       // brick counters [brick][copy]: blocks take copy blockIdx.x % 8, which
       // spreads the returning atomics of one brick over eight addresses
       int const copy = static_cast<int>(blockIdx.x) & (kCountCopies - 1);
@@ -1106,6 +1108,7 @@ __global__ __launch_bounds__(NTH) void k_gather_plane(
 
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
+  int const tyz = d.g.T[1] * d.g.T[2]; // == TYZ (bricks), read at run time
   int const b = L.lb;
   int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
   if (beg == end) return;
@@ -1143,6 +1146,9 @@ __global__ __launch_bounds__(NTH) void k_gather_plane(
       if (p < cnt) {
         double const *w = s.w + p * 3 * P;
         int const lo = s.lo[p];
+        double wx[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) wx[i] = w[i];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           int const r = r0 + t * S;
@@ -1150,13 +1156,20 @@ __global__ __launch_bounds__(NTH) void k_gather_plane(
             int const j = r / P, k = r - (r / P) * P;
             double const wyz = w[P + j] * w[2 * P + k];
             int const base = lo + j * T + k;
+            // the stencil column of (j, k) summed over i, then weighted; the
+            // layer pitch comes from the descriptor, so the compiler keeps
+            // single 64-bit LDS reads (paired ds_read2 reads at this pitch
+            // measured about 30% slower)
+            double ax = 0., ay = 0., az = 0.;
 #pragma unroll
             for (int i = 0; i < P; ++i) {
-              double const ww = w[i] * wyz;
-              fx += ww * static_cast<double>(tx[base + i * TYZ]);
-              fy += ww * static_cast<double>(ty[base + i * TYZ]);
-              fz += ww * static_cast<double>(tz[base + i * TYZ]);
+              ax += wx[i] * static_cast<double>(tx[base + i * tyz]);
+              ay += wx[i] * static_cast<double>(ty[base + i * tyz]);
+              az += wx[i] * static_cast<double>(tz[base + i * tyz]);
             }
+            fx += wyz * ax;
+            fy += wyz * ay;
+            fz += wyz * az;
           }
         }
       }
@@ -1267,6 +1280,333 @@ __global__ __launch_bounds__(NTH) void k_gather_layer(
         apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Column march (spread and gather).  Particles are binned into columns of
+// W x W first stencil points in (y, z) and, within a column, into x layers.
+// A block takes a run of x layers of one column and keeps the P mesh planes
+// x = X .. X+P-1 that layer X's stencils touch in an LDS ring (slot X mod P),
+// T x T points per plane (T = W + P - 1, the column plus the stencil reach).
+// Ring layout of a field: point (jy, kz), slot q at jy IP + kz P + q with
+// IP >= T P and IP = P^2 (mod 64): the P^2 values (kz0..kz0+P-1, every slot)
+// of one stencil row jy are consecutive, and row jy + 1 continues the bank
+// sequence, so the lanes of a particle read or add consecutive LDS words.
+// Moving to the next layer brings in one plane (gather) or flushes one
+// completed plane into the mesh (spread) instead of a whole brick tile.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ring_pitch(int T, int P) {
+  return T * P + (((P * P) - T * P) & 63);
+}
+
+template <int P, int W> struct ColGeo {
+  static constexpr int T = W + P - 1;
+  static constexpr int IP = ring_pitch(T, P);
+  static constexpr int RING = T * IP; // elements of one field's ring
+  static constexpr int RL = (T <= 16) ? 16 : 32; // lanes per tile row in plane copies
+  static constexpr int RPP = 256 / RL;           // rows per pass of a 256-thread block
+  static constexpr int NPASS = (T + RPP - 1) / RPP;
+  static_assert(T <= RL, "tile row wider than a plane-copy row");
+};
+
+/** Column block: column c of the system, x layers [x0, x1), tile origin
+ *  (oy, oz) in y and z. */
+struct ColBlock {
+  int c, x0, x1, oy, oz;
+};
+__device__ __forceinline__ ColBlock col_block(SysDesc const &d, int lb) {
+  ColBlock b;
+  b.c = lb / d.nseg;
+  int const sg = lb - b.c * d.nseg;
+  b.x0 = sg * d.lz;
+  b.x1 = min(b.x0 + d.lz, d.g.M[0]);
+  int const c1 = b.c / d.g.nb[2], c2 = b.c - c1 * d.g.nb[2];
+  b.oy = c1 * d.g.col;
+  b.oz = c2 * d.g.col;
+  return b;
+}
+
+/** Weights of a chunk of one layer's particles, one thread per (particle,
+ *  dimension), as stage_chunk; the particle's ring offset is
+ *  (y - oy) IP + (z - oz) P. */
+template <int P, int IP>
+__device__ __forceinline__ void stage_layer(DevGeo const &g, double4 const *spos, int64_t c0,
+                                            int cnt, int oy, int oz, ChunkSlots s) {
+  int const t = threadIdx.x;
+  if (t < 3 * cnt) {
+    int const p = t / 3;
+    int const dd = t - 3 * p;
+    double4 const pp = spos[c0 + p];
+    double const x[3] = {pp.x, pp.y, pp.z};
+    int nmp, gi;
+    double dist;
+    coordinate_1d(g, P, x[dd], dd, nmp, gi, dist);
+    bspline_weights<P>(dist, s.w + p * 3 * P + dd * P);
+    if (dd == 0) {
+      int n1, g1, n2, g2;
+      double d1, d2;
+      coordinate_1d(g, P, x[1], 1, n1, g1, d1);
+      coordinate_1d(g, P, x[2], 2, n2, g2, d2);
+      s.qq[p] = pp.w;
+      s.lo[p] = (g1 - oy) * IP + (g2 - oz) * P;
+    }
+  }
+}
+
+/** Row mapping of one ring plane over a 256-thread block: lane c of a row
+ *  covers kz = c (active for c < T), row jy = threadIdx.x / RL + k RPP. */
+template <int P, int W> struct PlaneRows {
+  using G = ColGeo<P, W>;
+  int kz, row0, zz;
+  bool lane_ok;
+  __device__ PlaneRows(int oz, int M2) {
+    kz = threadIdx.x % G::RL;
+    row0 = threadIdx.x / G::RL;
+    lane_ok = kz < G::T;
+    zz = wrap_once(oz + kz, M2);
+  }
+  __device__ bool valid(int k) const { return lane_ok && row0 + k * G::RPP < G::T; }
+  __device__ int row(int k) const { return row0 + k * G::RPP; }
+  __device__ int64_t mesh_index(int k, int x, int oy, int M1, int M2) const {
+    return ((int64_t)x * M1 + wrap_once(oy + row(k), M1)) * M2 + zz;
+  }
+  __device__ int lds_index(int k, int q) const { return row(k) * G::IP + kz * P + q; }
+};
+
+template <int P, typename MT, int W, int S>
+__global__ __launch_bounds__(256) void k_gather_col(
+    SysDesc const *__restrict__ sys, int const *__restrict__ sel, int const *__restrict__ wt,
+    int nsel, int const *__restrict__ soff, int const *__restrict__ offsets,
+    double4 const *__restrict__ spos, int const *__restrict__ sidx, double *__restrict__ force,
+    double const *__restrict__ sums, int chunk) {
+  using G = ColGeo<P, W>;
+  constexpr int IP = G::IP;
+  constexpr int PPW = 64 / S;                // particles per wave
+  constexpr int NM = (P * P + S - 1) / S;    // stencil-row elements per lane
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + chunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + chunk);
+  MT *rx = reinterpret_cast<MT *>(reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
+  MT *ry = rx + G::RING;
+  MT *rz = ry + G::RING;
+
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  ColBlock const cb = col_block(d, L.lb);
+  int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
+  int const *off = offsets + d.off_off + cb.c * M0;
+  if (off[cb.x0] == off[cb.x1]) return; // no particles in these layers
+  double dm[3] = {0., 0., 0.};
+  // the GPU index rule follows the reference's GPU force kernel, which has
+  // no dipole term (p3m.cpp:867-888)
+  if (d.dip_pref != 0. && !d.g.conv)
+    for (int k = 0; k < 3; ++k) dm[k] = d.dip_pref * sums[L.s * 16 + 3 + k];
+  double4 const *sp = spos + soff[L.s];
+  int const *si = sidx + soff[L.s];
+  double *fbase = force + 3 * d.part_off;
+  MT const *Ex = static_cast<MT const *>(d.rho);
+  MT const *Ey = static_cast<MT const *>(d.Ey);
+  MT const *Ez = static_cast<MT const *>(d.Ez);
+  PlaneRows<P, W> const pr(cb.oz, M2);
+  // prologue: planes x0 .. x0 + P - 2 straight into their slots, plane
+  // x0 + P - 1 into registers
+  for (int X = cb.x0; X < cb.x0 + P - 1; ++X) {
+    int const xm = wrap_once(X, M0), q = X % P;
+#pragma unroll
+    for (int k = 0; k < G::NPASS; ++k)
+      if (pr.valid(k)) {
+        int64_t const gidx = pr.mesh_index(k, xm, cb.oy, M1, M2);
+        int const e = pr.lds_index(k, q);
+        rx[e] = Ex[gidx];
+        ry[e] = Ey[gidx];
+        rz[e] = Ez[gidx];
+      }
+  }
+  MT vx[G::NPASS], vy[G::NPASS], vz[G::NPASS];
+  auto fetch = [&](int X) {
+    int const xm = wrap_once(X, M0);
+#pragma unroll
+    for (int k = 0; k < G::NPASS; ++k) {
+      vx[k] = vy[k] = vz[k] = MT(0);
+      if (pr.valid(k)) {
+        int64_t const gidx = pr.mesh_index(k, xm, cb.oy, M1, M2);
+        vx[k] = Ex[gidx];
+        vy[k] = Ey[gidx];
+        vz[k] = Ez[gidx];
+      }
+    }
+  };
+  fetch(cb.x0 + P - 1);
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const hl = lane - sub * S;
+  for (int X = cb.x0; X < cb.x1; ++X) {
+    {
+      // plane X + P - 1 into the slot of plane X - 1 (last read by an
+      // earlier layer, before that layer's closing barrier)
+      int const q = (X + P - 1) % P;
+#pragma unroll
+      for (int k = 0; k < G::NPASS; ++k)
+        if (pr.valid(k)) {
+          int const e = pr.lds_index(k, q);
+          rx[e] = vx[k];
+          ry[e] = vy[k];
+          rz[e] = vz[k];
+        }
+    }
+    if (X + 1 < cb.x1) fetch(X + P); // in flight during this layer
+    int const beg = off[X], end = off[X + 1];
+    if (beg == end) continue;
+    int const lq = X % P; // slot of plane X: weight index of slot q is (q - lq) mod P
+    for (int c0 = beg; c0 < end; c0 += chunk) {
+      int const cnt = min(chunk, end - c0);
+      if (c0 != beg) __syncthreads(); // the previous chunk's slots are read
+      stage_layer<P, IP>(d.g, sp, c0, cnt, cb.oy, cb.oz, s);
+      __syncthreads();
+      int const ngroups = (cnt + PPW - 1) / PPW;
+      for (int grp = wave; grp < ngroups; grp += nwaves) {
+        int const p = grp * PPW + sub;
+        bool const valid = p < cnt;
+        double fx = 0., fy = 0., fz = 0.;
+        if (valid) {
+          double const *w = s.w + p * 3 * P;
+          double wy[P];
+#pragma unroll
+          for (int r = 0; r < P; ++r) wy[r] = w[P + r];
+          int const lo = s.lo[p];
+          // one stencil-row element at a time: P loads per field in flight
+          // per wave, and registers for more waves
+#pragma unroll 1
+          for (int mi = 0; mi < NM; ++mi) {
+            int const m = hl + mi * S;
+            if (m < P * P) {
+              int const kk = m / P, q = m - (m / P) * P;
+              int k = q - lq;
+              if (k < 0) k += P;
+              double const wzx = w[2 * P + kk] * w[k];
+              int const base = lo + m;
+              MT ax = MT(0), ay = MT(0), az = MT(0);
+#pragma unroll
+              for (int r = 0; r < P; ++r) {
+                MT const wr = static_cast<MT>(wy[r]);
+                ax += wr * rx[base + r * IP];
+                ay += wr * ry[base + r * IP];
+                az += wr * rz[base + r * IP];
+              }
+              fx += wzx * static_cast<double>(ax);
+              fy += wzx * static_cast<double>(ay);
+              fz += wzx * static_cast<double>(az);
+            }
+          }
+        }
+        fx = segment_total<S>(fx);
+        fy = segment_total<S>(fy);
+        fz = segment_total<S>(fz);
+        if (hl == 0 && valid)
+          apply_force(fbase, si[c0 + p], s.qq[p], d.force_prefac, fx, fy, fz, dm);
+      }
+    }
+    __syncthreads(); // this layer's reads of the ring and of the slots are done
+  }
+}
+
+template <int P, typename MT, int W, int S>
+__global__ __launch_bounds__(256) void k_spread_col(SysDesc const *__restrict__ sys,
+                                                   int const *__restrict__ sel,
+                                                   int const *__restrict__ wt, int nsel,
+                                                   int const *__restrict__ soff,
+                                                   int const *__restrict__ offsets,
+                                                   double4 const *__restrict__ spos, int chunk) {
+  using G = ColGeo<P, W>;
+  constexpr int IP = G::IP;
+  constexpr int PPW = 64 / S;
+  constexpr int NM = (P * P + S - 1) / S;
+  extern __shared__ double smem_d[];
+  ChunkSlots s;
+  s.w = smem_d;
+  s.qq = s.w + chunk * 3 * P;
+  s.lo = reinterpret_cast<int *>(s.qq + chunk);
+  double *ring = reinterpret_cast<double *>(reinterpret_cast<char *>(smem_d) +
+                                            chunk_slot_bytes(P, chunk));
+
+  Slot const L = locate(wt, sel, nsel);
+  SysDesc const &d = sys[L.s];
+  ColBlock const cb = col_block(d, L.lb);
+  int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
+  int const *off = offsets + d.off_off + cb.c * M0;
+  if (off[cb.x0] == off[cb.x1]) return;
+  double4 const *sp = spos + soff[L.s];
+  MT *rho = static_cast<MT *>(d.rho);
+  PlaneRows<P, W> const pr(cb.oz, M2);
+  for (int e = threadIdx.x; e < G::RING; e += blockDim.x) ring[e] = 0.;
+  __syncthreads(); // flushes of leading empty layers read other threads' zeros
+  // flush the completed plane X (slot X mod P) into the mesh and clear it
+  auto flush = [&](int X) {
+    int const xm = wrap_once(X, M0), q = X % P;
+#pragma unroll
+    for (int k = 0; k < G::NPASS; ++k)
+      if (pr.valid(k)) {
+        int const e = pr.lds_index(k, q);
+        double const v = ring[e];
+        if (v != 0.) {
+          unsafeAtomicAdd(rho + pr.mesh_index(k, xm, cb.oy, M1, M2), static_cast<MT>(v));
+          ring[e] = 0.;
+        }
+      }
+  };
+  int const lane = threadIdx.x & 63;
+  int const wave = threadIdx.x >> 6;
+  int const nwaves = blockDim.x >> 6;
+  int const sub = lane / S;
+  int const hl = lane - sub * S;
+  for (int X = cb.x0; X < cb.x1; ++X) {
+    int const beg = off[X], end = off[X + 1];
+    if (beg != end) {
+      int const lq = X % P;
+      for (int c0 = beg; c0 < end; c0 += chunk) {
+        int const cnt = min(chunk, end - c0);
+        if (c0 != beg) __syncthreads();
+        // the barrier after staging also orders the clearing of the slot
+        // flushed last (plane X - 1, now the slot of plane X + P - 1) before
+        // the additions
+        stage_layer<P, IP>(d.g, sp, c0, cnt, cb.oy, cb.oz, s);
+        __syncthreads();
+        int const ngroups = (cnt + PPW - 1) / PPW;
+        for (int grp = wave; grp < ngroups; grp += nwaves) {
+          int const p = grp * PPW + sub;
+          if (p < cnt) {
+            double const *w = s.w + p * 3 * P;
+            double const qp = s.qq[p];
+            double wy[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r) wy[r] = w[P + r];
+            int const lo = s.lo[p];
+#pragma unroll
+            for (int mi = 0; mi < NM; ++mi) {
+              int const m = hl + mi * S;
+              if (m < P * P) {
+                int const kk = m / P, q = m - (m / P) * P;
+                int k = q - lq;
+                if (k < 0) k += P;
+                double const wzxq = (w[2 * P + kk] * w[k]) * qp;
+                double *dst = ring + lo + m;
+#pragma unroll
+                for (int r = 0; r < P; ++r) atomicAdd(dst + r * IP, wy[r] * wzxq);
+              }
+            }
+          }
+        }
+      }
+      __syncthreads(); // every addition of layer X is in the ring
+    }
+    flush(X); // plane X receives nothing from later layers of this block
+  }
+  // planes x1 .. x1 + P - 2 (the last additions are behind the last barrier)
+  for (int X = cb.x1; X < cb.x1 + P - 1; ++X) flush(X);
 }
 
 // ---------------------------------------------------------------------------
@@ -1391,6 +1731,27 @@ template <typename F> void for_threads(F &&f) {
   f(std::integral_constant<int, 256>{});
   f(std::integral_constant<int, 512>{});
 }
+/** Column width and lanes per particle of the column-march kernels. */
+template <typename F> void with_column(int w, int lanes, F &&f) {
+  auto by_lanes = [&](auto W) {
+    if (lanes == 8)
+      f(W, std::integral_constant<int, 8>{});
+    else if (lanes == 16)
+      f(W, std::integral_constant<int, 16>{});
+    else
+      throw std::runtime_error("column march: lanes per particle must be 8 or 16");
+  };
+  if (w == 8)
+    by_lanes(std::integral_constant<int, 8>{});
+  else if (w == 16)
+    by_lanes(std::integral_constant<int, 16>{});
+  else
+    throw std::runtime_error("column march: column width must be 8 or 16");
+}
+template <typename F> void for_columns(F &&f) {
+  for (int w : {8, 16})
+    for (int l : {8, 16}) with_column(w, l, f);
+}
 
 
 template <typename T> void dfree(T *&p) {
@@ -1412,6 +1773,8 @@ template <typename Kern> void allow_lds(Kern k) {
 }
 
 size_t chunk_bytes(int cao, int chunk) { return chunk_slot_bytes(cao, chunk); }
+/** Ring elements of one field of a column block. */
+int ring_points(int cao, int w) { return (w + cao - 1) * ring_pitch(w + cao - 1, cao); }
 int tile_points(int cao) {
   int const t = kBrick + cao - 1;
   return t * t * t;
@@ -1655,6 +2018,8 @@ size_t BatchSolver::working_bytes(std::vector<int> const &sel) const {
 
 namespace {
 void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st);
+void set_bins(HostSys &h, LaunchShape const &shape);
+void recompute_offsets(BatchSolver::Impl *m);
 } // namespace
 
 void BatchSolver::set_launch_shape(LaunchShape const &s) {
@@ -1666,12 +2031,20 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   if ((s.threads != 256 && s.threads != 512) || s.chunk < 1 || s.chunk > kChunkMax ||
       3 * s.chunk > s.threads)
     throw std::invalid_argument("P3M: invalid particle-mesh block shape");
+  if ((s.column != 0 && s.column != 8 && s.column != 16) || s.seg < 1 || s.seg > 1024 ||
+      (s.col_lanes != 8 && s.col_lanes != 16))
+    throw std::invalid_argument("P3M: invalid column-march shape");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
+  bool const rebin = s.column != m_shape.column || s.seg != m_shape.seg;
   m_shape = s;
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
   m_shape.fused_lds = std::max(0, std::min(128 * 1024, s.fused_lds));
   quiesce(m);
+  if (rebin) { // particle bins and block counts of every system
+    for (auto &h : m->sys) set_bins(h, m_shape);
+    recompute_offsets(m);
+  }
   // systems whose y/x pass changes switch spectrum layout: recompute G
   bool changed = false;
   for (auto &h : m->sys) {
@@ -1753,6 +2126,24 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     consider(c);
   }
   try_blocks();
+  // column march: width and lanes per particle, then x layers per block
+  {
+    LaunchShape const base = res.best;
+    for (int w : {8, 16})
+      for (int l : {8, 16}) {
+        if (w == base.column && l == base.col_lanes) continue;
+        LaunchShape c = base;
+        c.column = w;
+        c.col_lanes = l;
+        consider(c);
+      }
+    if (res.best.column)
+      for (int sg : {16, 64}) {
+        LaunchShape c = res.best;
+        c.seg = sg;
+        consider(c);
+      }
+  }
   for (int v : {2, 4, 8, 16}) {
     if (v == res.best.ncol) continue;
     LaunchShape c = res.best;
@@ -1812,6 +2203,10 @@ template <typename MT> void set_kernel_attrs() {
         allow_lds(k_gather_plane<P.value, MT, 8, NT.value>);
         allow_lds(k_gather_layer<P.value, MT, NT.value>);
       });
+      for_columns([&](auto W, auto S) {
+        allow_lds(k_spread_col<P.value, MT, W.value, S.value>);
+        allow_lds(k_gather_col<P.value, MT, W.value, S.value>);
+      });
     });
 }
 
@@ -1830,6 +2225,44 @@ void compute_influence(HostSys const &h, MeshPrecision prec, hipStream_t st) {
         static_cast<float *>(d.G), g.M[0], g.M[1], g.M[2], g.Kz, d.wv[0], d.wv[1], d.wv[2],
         h.params.a[0], h.params.a[1], h.params.a[2], h.params.alpha, d.cao, d.kzmajor);
   hip_check(hipGetLastError(), "k_influence");
+}
+
+/** Particle bins of a system for the launch shape: 8^3 bricks, or W x W
+ *  columns in (y, z) with one bin per x layer and blocks of `seg` layers. */
+void set_bins(HostSys &h, LaunchShape const &shape) {
+  DevGeo &g = h.desc.g;
+  int const cao = h.desc.cao;
+  g.col = shape.column;
+  if (shape.column == 0) {
+    for (int k = 0; k < 3; ++k) {
+      g.B[k] = kBrick;
+      g.nb[k] = (g.M[k] + kBrick - 1) / kBrick;
+      g.T[k] = kBrick + cao - 1;
+    }
+    h.desc.lz = 0;
+    h.desc.nseg = 0;
+  } else {
+    int const w = shape.column;
+    g.B[0] = 1;
+    g.nb[0] = g.M[0];
+    g.T[0] = cao;
+    for (int k = 1; k < 3; ++k) {
+      g.B[k] = w;
+      g.nb[k] = (g.M[k] + w - 1) / w;
+      g.T[k] = w + cao - 1;
+    }
+    h.desc.lz = std::max(1, std::min(shape.seg, g.M[0]));
+    h.desc.nseg = (g.M[0] + h.desc.lz - 1) / h.desc.lz;
+  }
+  g.tz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[2]) + 1;
+  g.tyz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[1] * g.T[2]) + 1;
+  h.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+}
+
+/** Blocks of a system in the spread and gather launches. */
+int particle_mesh_blocks(HostSys const &h) {
+  SysDesc const &d = h.desc;
+  return d.g.col ? d.g.nb[1] * d.g.nb[2] * d.nseg : h.nbricks;
 }
 
 void recompute_offsets(BatchSolver::Impl *m) {
@@ -1882,23 +2315,18 @@ int BatchSolver::add_system(P3MParams const &params_in, Vec3d const &box_l, doub
     g.M[k] = h.params.mesh[k];
     g.dim[k] = h.lm.dim[k];
     g.marg[k] = h.lm.margin[2 * k];
-    g.B[k] = kBrick;
-    g.nb[k] = (g.M[k] + kBrick - 1) / kBrick;
-    g.T[k] = kBrick + cao - 1;
     g.ld_pos[k] = h.lm.ld_pos[k];
     g.ai[k] = h.params.ai[k];
     g.box_l[k] = box_l[k];
   }
   g.Kz = g.M[2] / 2 + 1;
-  g.tz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[2]) + 1;
-  g.tyz_magic = (1ull << 32) / static_cast<unsigned long long>(g.T[1] * g.T[2]) + 1;
   g.pos_shift = std::floor((cao - 1) / 2.0) - (cao % 2) / 2.0;
   g.conv = 0;
   for (int k = 0; k < 3; ++k)
     g.hi_f[k] = static_cast<float>(g.M[k]) / static_cast<float>(box_l[k]);
   g.ps_f = static_cast<float>((cao - 1) / 2);
   d.cao = cao;
-  h.nbricks = g.nb[0] * g.nb[1] * g.nb[2];
+  set_bins(h, m_shape);
   double const two_pi = 2. * std::numbers::pi;
   double const volume = box_l[0] * box_l[1] * box_l[2];
   for (int k = 0; k < 3; ++k) d.wv[k] = two_pi * (1. / box_l[k]);
@@ -2151,7 +2579,7 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
     int acc = 0;
     host.push_back(0);
     for (int s : gs) {
-      acc += m->sys[s].nbricks;
+      acc += particle_mesh_blocks(m->sys[s]);
       host.push_back(acc);
     }
     g.blocks = acc;
@@ -2256,6 +2684,18 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (ev) (void)hipEventRecord(ev[1], st);
   for (auto const &g : p.groups) {
     if (no_spread) break;
+    if (shape.column) {
+      dispatch_cao(g.cao, [&](auto P) {
+        with_column(shape.column, shape.col_lanes, [&](auto W, auto S) {
+          size_t const lds = chunk_bytes(P.value, shape.chunk) +
+                             ring_points(P.value, W.value) * sizeof(double);
+          k_spread_col<P.value, MT, W.value, S.value><<<g.blocks, 256, lds, st>>>(
+              sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+              static_cast<double4 const *>(ln.spos), shape.chunk);
+        });
+      });
+      continue;
+    }
     dispatch_cao(g.cao, [&](auto P) {
       with_threads(shape.threads, [&](auto NT) {
         size_t const lds =
@@ -2310,6 +2750,19 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (ev) (void)hipEventRecord(ev[5], st);
   if (want_force) {
     for (auto const &g : p.groups) {
+      if (shape.column) {
+        dispatch_cao(g.cao, [&](auto P) {
+          with_column(shape.column, shape.col_lanes, [&](auto W, auto S) {
+            size_t const lds = chunk_bytes(P.value, shape.chunk) +
+                               3 * ring_points(P.value, W.value) * sizeof(MT);
+            k_gather_col<P.value, MT, W.value, S.value><<<g.blocks, 256, lds, st>>>(
+                sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+                static_cast<double4 const *>(ln.spos), static_cast<int const *>(ln.sidx),
+                pa.force, static_cast<double const *>(ln.sums), shape.chunk);
+          });
+        });
+        continue;
+      }
       dispatch_cao(g.cao, [&](auto P) {
         with_threads(shape.threads, [&](auto NT) {
           size_t const lds =

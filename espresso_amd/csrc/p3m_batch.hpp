@@ -80,6 +80,9 @@ struct LaunchShape {
   int spread = kSpreadSeg64; // spread mapping
   int threads = 512;         // block size of the spread and gather kernels (256, 512)
   int chunk = 64;            // particles staged per brick pass (1..64, <= threads / 3)
+  int column = 0;            // 0: 8^3 bricks; 8 or 16: column march over W x W columns
+  int seg = 32;              // column march: x layers per block (1..1024)
+  int col_lanes = 8;         // column march: lanes per particle (8 or 16)
 };
 
 struct AutotuneResult {

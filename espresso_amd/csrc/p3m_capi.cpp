@@ -93,7 +93,8 @@ struct p3m_server {
 namespace {
 
 p3m_launch_shape to_abi(LaunchShape const &s) {
-  return p3m_launch_shape{s.gather, s.ncol, s.zlines, s.fused_lds, s.spread, s.threads, s.chunk};
+  return p3m_launch_shape{s.gather, s.ncol,    s.zlines, s.fused_lds, s.spread,
+                          s.threads, s.chunk, s.column, s.seg,       s.col_lanes};
 }
 
 LaunchShape from_abi(p3m_launch_shape const &a) {
@@ -110,6 +111,12 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
     throw std::invalid_argument("P3M batch: threads must be 256 or 512");
   if (a.chunk < 1 || a.chunk > 64 || 3 * a.chunk > a.threads)
     throw std::invalid_argument("P3M batch: chunk must be in [1, 64] and at most threads / 3");
+  if (a.column != 0 && a.column != 8 && a.column != 16)
+    throw std::invalid_argument("P3M batch: column must be 0 (bricks), 8 or 16");
+  if (a.seg < 1 || a.seg > 1024)
+    throw std::invalid_argument("P3M batch: seg must be in [1, 1024]");
+  if (a.col_lanes != 8 && a.col_lanes != 16)
+    throw std::invalid_argument("P3M batch: col_lanes must be 8 or 16");
   LaunchShape s;
   s.gather = a.gather;
   s.ncol = a.ncol;
@@ -118,6 +125,9 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
   s.spread = a.spread;
   s.threads = a.threads;
   s.chunk = a.chunk;
+  s.column = a.column;
+  s.seg = a.seg;
+  s.col_lanes = a.col_lanes;
   return s;
 }
 std::mutex g_err_mutex;
@@ -1189,6 +1199,23 @@ int p3m_autotune(p3m_context *ctx, int64_t n, double const *pos, double const *q
     if (best_ms) *best_ms = r.best_ms;
     if (default_ms) *default_ms = r.default_ms;
   });
+}
+
+int p3m_set_launch_shape(p3m_context *ctx, p3m_launch_shape const *shape) {
+  if (!ctx || !shape) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_device(ctx);
+    LaunchShape const s = from_abi(*shape);
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->solver->set_launch_shape(s);
+    ctx->mesh_assigned = false;
+  });
+}
+
+int p3m_get_launch_shape(p3m_context const *ctx, p3m_launch_shape *shape) {
+  if (!ctx || !shape || !ctx->solver) return P3M_ERR_INVALID;
+  *shape = to_abi(ctx->solver->launch_shape());
+  return P3M_OK;
 }
 
 int p3m_synchronize(p3m_context *ctx) {

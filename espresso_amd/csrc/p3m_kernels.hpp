@@ -144,6 +144,10 @@ struct DevGeo {
   int conv;
   float hi_f[3]; // float(M) / float(L)
   float ps_f;    // float((cao - 1) / 2), integer division
+  // particle bins: 0 = bricks of B[0] x B[1] x B[2] first stencil points;
+  // W > 0 = columns of W x W points in (y, z), one bin per x layer (the bins
+  // of a column are consecutive, x fastest)
+  int col;
 };
 
 /** floor(e / d) for 0 <= e < 2^16 and 1 <= d < 2^16 from the magic number

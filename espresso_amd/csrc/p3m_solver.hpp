@@ -71,6 +71,9 @@ public:
    *  fft_fwd, convolve, fft_bwd, gather. */
   float last_ms[8] = {};
   void set_launch_shape(LaunchShape const &s);
+  LaunchShape launch_shape() const {
+    return m_batch ? m_batch->launch_shape() : (m_shape_set ? m_shape : LaunchShape{});
+  }
   /** Index rule of the solves (BatchSolver::set_convention); kept across
    *  later configure() calls. */
   void set_convention(int conv);

@@ -320,6 +320,34 @@ class _P3MBase:
                        self._ctx)
         return "gpu" if c.value == _lib.P3M_CONV_GPU else "cpu"
 
+    def set_launch_shape(self, **shape):
+        """Launch shape of later solves: the keyword arguments of
+        ``P3MBatch.set_launch_shape`` (gather, ncol, zlines, fused_lds,
+        spread, threads, chunk, column, seg, col_lanes); unnamed fields keep
+        their current values."""
+        from .batch import P3MBatch
+        cur = _lib.p3m_launch_shape()
+        _lib.raise_for(_lib.load().p3m_get_launch_shape(self._ctx, C.byref(cur)), self._ctx)
+        for name, _ in _lib.p3m_launch_shape._fields_:
+            if name not in shape:
+                continue
+            v = shape.pop(name)
+            if name == "gather" and isinstance(v, str):
+                v = P3MBatch.GATHER[v]
+            elif name == "spread" and isinstance(v, str):
+                v = P3MBatch.SPREAD[v]
+            setattr(cur, name, int(v))
+        if shape:
+            raise TypeError(f"unknown launch-shape fields: {sorted(shape)}")
+        _lib.raise_for(_lib.load().p3m_set_launch_shape(self._ctx, C.byref(cur)), self._ctx)
+
+    def launch_shape(self):
+        """The launch shape of later solves (dict with mapping names)."""
+        from .batch import shape_dict
+        cur = _lib.p3m_launch_shape()
+        _lib.raise_for(_lib.load().p3m_get_launch_shape(self._ctx, C.byref(cur)), self._ctx)
+        return shape_dict(cur)
+
     def autotune(self, pos, q, reps=3):
         """Pick the fastest launch shape of the device kernels (fused y/x
         pass, particle-mesh blocks, gather and spread mappings, FFT columns,

@@ -398,6 +398,12 @@ typedef struct p3m_launch_shape {
                     particle over the (j, k) plane */
   int threads;   /* block size of the spread and gather kernels: 256 or 512 */
   int chunk;     /* particles staged per brick pass: 1..64, at most threads/3 */
+  int column;    /* particle bins of spread and gather: 0 = 8^3 bricks of first
+                    stencil points; 8 or 16 = column march over W x W columns in
+                    (y, z), one bin per x layer, a ring of the cao mesh planes a
+                    layer touches in LDS (256-thread blocks) */
+  int seg;       /* column march: x layers per block (1..1024) */
+  int col_lanes; /* column march: lanes per particle (8 or 16) */
 } p3m_launch_shape;
 
 /* Launch-shape autotuner of the context's device kernels (no counterpart in
@@ -406,6 +412,11 @@ typedef struct p3m_launch_shape {
  * the fastest for later solves.  Outputs may be NULL. */
 int p3m_autotune(p3m_context *ctx, int64_t n, const double *pos, const double *q, int reps,
                  p3m_launch_shape *best, double *best_ms, double *default_ms);
+
+/* Set (validated as documented in p3m_launch_shape) or read the launch
+ * shape of the context's later solves (no counterpart in the reference). */
+int p3m_set_launch_shape(p3m_context *ctx, const p3m_launch_shape *shape);
+int p3m_get_launch_shape(const p3m_context *ctx, p3m_launch_shape *shape);
 
 /* Block until all work enqueued on the context stream has finished. */
 int p3m_synchronize(p3m_context *ctx);

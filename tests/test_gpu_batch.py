@@ -1,6 +1,9 @@
 """Batched solver (many systems per launch) on HIP device 0.
 
-Every system of a batch must give what the FP64 oracle gives for it alone:
+Every system of a batch must give what the FP64 oracle gives for it alone,
+with 8^3 bricks and with the column march (column widths 8 and 16, blocks of
+1, 5 and 32 x layers; the specs below include meshes smaller than a column
+and not multiples of it, every order and an empty system):
 FP64 batch forces rel-rms <= 1e-6, energy rel <= 1e-8; FP32-mesh batch forces
 rel-rms <= 2e-5, energy rel <= 2e-6.  A system solved alone and solved in a
 mix with others gives the same result up to the order of floating-point
@@ -70,7 +73,10 @@ def _oracle(s):
     None,                                   # default: every mesh here takes the fused y/x pass
     dict(fused_lds=0),                      # split y, x and inverse-y passes
     dict(spread="plane8", gather="layer", threads=256, chunk=32),  # small blocks
-], ids=["fused", "split", "small_blocks"])
+    dict(column=8, seg=32, col_lanes=8),    # column march, 8 x 8 columns
+    dict(column=16, seg=5, col_lanes=16, fused_lds=0),  # 16 x 16, 5 layers per block
+    dict(column=8, seg=1, col_lanes=16, threads=256, chunk=7),  # one layer per block
+], ids=["fused", "split", "small_blocks", "col8", "col16_seg5", "col8_seg1"])
 @pytest.mark.parametrize("single,f_tol,e_tol", [(False, 1e-6, 1e-8),
                                                 (True, 2e-5, 2e-6)])
 def test_batch_matches_oracle_per_system(single, f_tol, e_tol, shape):

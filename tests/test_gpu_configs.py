@@ -1,7 +1,8 @@
 """Every workload at its real size on HIP device 0.
 
 C2  N = 10^6 (900k charged), L = 125.99, mesh 128, cao 7, r_cut 2.5 and the
-    closed-form alpha at accuracy 1e-5 (1.36793), tune=False: FP64 forces
+    closed-form alpha at accuracy 1e-5 (1.36793), tune=False, with 8^3 bricks
+    and with the column march (FP64 8 x 8, FP32 16 x 16 columns): FP64 forces
     rel-rms <= 1e-6 and energy rel <= 1e-8 against the FP64 oracle
     (restatement of CoulombP3MImpl<double, CPU>, p3m.cpp:452-592); the FP32
     mesh within 2e-5 / 2e-6 (FP32 energy relative to |E| + E_self, the size
@@ -45,17 +46,21 @@ def test_c2_full_size_parity():
     e_ref = ora.energy(pos, q)
     tp = torch.tensor(pos, device="cuda")
     tq = torch.tensor(q, device="cuda")
-    for single, f_tol, e_tol in ((False, 1e-6, 1e-8), (True, 2e-5, 2e-6)):
-        _say(f"C2 N=1e6 mesh 128 cao 7 single={single}")
+    for single, f_tol, e_tol, column in ((False, 1e-6, 1e-8, 0), (True, 2e-5, 2e-6, 0),
+                                         (False, 1e-6, 1e-8, 8), (True, 2e-5, 2e-6, 16)):
+        _say(f"C2 N=1e6 mesh 128 cao 7 single={single} column={column}")
         p3m = P3M(tune=False, single_precision=single, **params)
         p3m.activate(tp, tq, box_l=box)
+        if column:
+            p3m.set_launch_shape(column=column, seg=32, col_lanes=8)
         f = torch.zeros_like(tp)
         p3m.add_long_range_forces(tp, tq, f)
         e = p3m.long_range_energy(tp, tq)
         err_f = S.rel_rms(f.cpu().numpy(), f_ref)
         scale = S.energy_scale(e_ref, q, params["alpha"], 1.0) if single else abs(e_ref)
         err_e = abs(e - e_ref) / scale
-        _say(f"C2 single={single}: force rel-rms {err_f:.3e}, energy rel {err_e:.3e}")
+        _say(f"C2 single={single} column={column}: force rel-rms {err_f:.3e}, "
+             f"energy rel {err_e:.3e}")
         assert err_f <= f_tol and err_e <= e_tol
 
 
