@@ -87,29 +87,44 @@ def make_c4(n_sys=512, seed0=4000):
     return out
 
 
-def tune_systems(systems, accuracy, mesh_range, cao_range, workers=8):
+# device real-space cost per pair inside r_cut: p3m_short_range on the
+# N = 10^6 workload (r_cut 8.558, 5.3e8 pairs) took 11.6 ms on one MI355X;
+# bench.py measures it again in its short-range leg and tunes with that rate
+DEVICE_PAIR_NS = 0.022
+
+
+def model_pairs(n_q, L, r_cut):
+    """Pairs within r_cut of a uniform system (the tuning model's count)."""
+    return 0.5 * n_q * (n_q / L ** 3) * (4. / 3.) * math.pi * r_cut ** 3
+
+
+def tune_systems(systems, accuracy, mesh_range, cao_range, workers=8, pair_ns=None):
     """Per-system tuning with the reference algorithm and the device cost
-    model (real-space pairs at 1 ns each: the short-range loop stays on the
-    host as with the reference's GPU P3M)."""
+    model: k-space stage rates of the solver plus the device real-space pair
+    loop (p3m_short_range) at pair_ns per pair (DEVICE_PAIR_NS by default)."""
     from concurrent.futures import ThreadPoolExecutor
     from espresso_amd import tune_model
+    pns = DEVICE_PAIR_NS if pair_ns is None else pair_ns
 
     def one(s):
         return tune_model([s["L"]] * 3, s["n"], float(np.sum(s["q"] ** 2)), 1.0,
-                          accuracy, mesh_range=mesh_range, cao_range=cao_range)
+                          accuracy, mesh_range=mesh_range, cao_range=cao_range,
+                          pair_ns=pns)
     with ThreadPoolExecutor(workers) as ex:
         return list(ex.map(one, systems))
 
 
-def bench_batched(steps, warmup, torch, dev):
+def bench_batched(steps, warmup, torch, dev, pair_ns=None):
     """512 systems per launch (FP32 mesh, FP64 reductions and forces),
     autotuned launch shape, hipGraph replay; against the plain FP64 solver
-    applied to the same systems one context per system."""
+    applied to the same systems one context per system.  The total force
+    (batched k-space + device real space) of 16 systems spanning N is
+    checked against exact Ewald forces (tests/golden/ewald_c4_sample.npz)."""
     from espresso_amd import P3MBatch
     from espresso_amd.electrostatics import P3M
     systems = make_c4()
     t0 = time.perf_counter()
-    tuned = tune_systems(systems, 1e-4, (16, 48), (3, 7))
+    tuned = tune_systems(systems, 1e-4, (16, 48), (3, 7), pair_ns=pair_ns)
     t_tune = time.perf_counter() - t0
     n_tot = sum(s["n"] for s in systems)
     print(f"[bench] batched: {len(systems)} systems, {n_tot} particles, tuned "
@@ -134,6 +149,16 @@ def bench_batched(steps, warmup, torch, dev):
         b.graph_solve(tp, tq, tf)
     b.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
+    # per-batch latency: each replay bracketed by events on the batch stream
+    ext = torch.cuda.ExternalStream(b.stream())
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for e0, e1 in evs:
+        e0.record(ext)
+        b.graph_solve(tp, tq, tf)
+        e1.record(ext)
+    b.synchronize()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
     # stage split of the same selection from eager solves
     b.profile_begin(steps)
     for _ in range(steps):
@@ -185,6 +210,28 @@ def bench_batched(steps, warmup, torch, dev):
     d = (f32 - f64).cpu().numpy()
     ref = f64.cpu().numpy()
     rel = float(np.sqrt(np.sum(d * d) / np.sum(ref * ref)))
+    # total force error against exact Ewald on the fixture's systems:
+    # batched FP32 k-space + the device real-space pair forces
+    ewald = None
+    fx_path = os.path.join(GOLDEN, "ewald_c4_sample.npz")
+    if os.path.exists(fx_path):
+        fx = np.load(fx_path)
+        f32n = f32.cpu().numpy()
+        errs, start = [], 0
+        for sid, cnt in zip(fx["sys"], fx["count"]):
+            sid, cnt = int(sid), int(cnt)
+            vp, vq, _ = views[sid]
+            f_r = plain[sid].pair_forces_on(vp, vq, np.arange(cnt)).cpu().numpy()
+            dd = f32n[offs[sid]:offs[sid] + cnt] + f_r - fx["force"][start:start + cnt]
+            errs.append(float(np.sqrt(np.mean(np.sum(dd * dd, axis=1)))))
+            start += cnt
+        ewald = {"systems": [int(x) for x in fx["sys"]],
+                 "rms_force_error": [round(e, 8) for e in errs],
+                 "max_rms_force_error": max(errs),
+                 "estimate_max": max(tuned[int(i)]["accuracy"] for i in fx["sys"]),
+                 "reference": "exact FP64 Ewald, first min(N, 500) particles of 16 "
+                              "systems spanning N (tests/golden/ewald_c4_sample.npz); "
+                              "total force = batched FP32 k-space + device real space"}
     meshes = [t["mesh"][0] for t in tuned]
     caos = [t["cao"] for t in tuned]
     res = {
@@ -194,6 +241,9 @@ def bench_batched(steps, warmup, torch, dev):
         "mesh_range": [min(meshes), max(meshes)], "cao_range": [min(caos), max(caos)],
         "max_estimated_rms_force_error": max(t["accuracy"] for t in tuned),
         "ms_per_batch": round(t_batch * 1e3, 4),
+        "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                             "p99": round(float(np.percentile(lat, 99)), 4),
+                             "replays": len(lat)},
         "mparticles_per_s": round(n_tot / t_batch / 1e6, 2),
         "solves_per_s": round(len(systems) / t_batch, 1),
         "stages_ms_eager": st,
@@ -202,12 +252,16 @@ def bench_batched(steps, warmup, torch, dev):
         "plain_fp64_mparticles_per_s": round(n_tot / t_plain / 1e6, 2),
         "speedup_vs_plain_fp64": round(t_plain / t_batch, 2),
         "force_rel_rms_fp32_batch_vs_fp64_plain": rel,
+        "ewald_check": ewald,
+        "tuning_pair_ns": DEVICE_PAIR_NS if pair_ns is None else round(pair_ns, 5),
         "tuning_s": round(t_tune, 2),
     }
     print(f"[bench] batched: {res['ms_per_batch']} ms/batch, "
           f"{res['mparticles_per_s']} Mparticles/s, {res['solves_per_s']} solves/s; "
           f"plain FP64 {res['plain_fp64_mparticles_per_s']} Mparticles/s "
-          f"(x{res['speedup_vs_plain_fp64']}); rel-rms {rel:.2e}", flush=True)
+          f"(x{res['speedup_vs_plain_fp64']}); rel-rms {rel:.2e}; latency "
+          f"{res['batch_latency_ms']}; Ewald max rms err "
+          f"{ewald['max_rms_force_error'] if ewald else None}", flush=True)
     del plain
     return res
 
@@ -673,6 +727,10 @@ def main():
                 "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": ("stored counter figure: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                                   "per dispatch, calibrated by a device copy "
+                                   "(profiles/hbm_traffic_c2_n1e6_mesh128_cao7.json)"
+                                   if traffic is not None else None),
                 "algorithmic_bytes": ab[dom],
                 "avg_launch_ms": round(single[dom], 5)}
     solve_achieved = ab["solve"] / t_step / 1e9
@@ -725,6 +783,7 @@ def main():
     # device real-space pair forces of the same system (cell lists), and the
     # total force (k-space + real space) per step
     short_range = None
+    pair_ns = None
     if not args.no_short_range:
         fs = torch.zeros_like(tp)
         for _ in range(max(1, args.warmup)):
@@ -736,8 +795,10 @@ def main():
             p3m.add_short_range_forces(tp, tq, fs, sync=False)
         p3m.synchronize()
         t_sr = (time.perf_counter() - t0) / reps
+        pair_ns = t_sr * 1e9 / model_pairs(n_q, L, prm["r_cut"])
         short_range = {
             "ms": round(t_sr * 1e3, 3),
+            "pair_ns": round(pair_ns, 5),
             "cells": p3m.short_range_cells(),
             "r_cut": round(prm["r_cut"], 5),
             "total_force_ms_fp64": round((t_sr + t_step) * 1e3, 3),
@@ -754,7 +815,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        batched = bench_batched(args.steps, args.warmup, torch, dev)
+        batched = bench_batched(args.steps, args.warmup, torch, dev, pair_ns=pair_ns)
     serving = None
     if not args.no_serving:
         serving = bench_serving(torch, dev)

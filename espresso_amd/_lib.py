@@ -40,7 +40,7 @@ EXPORTED_SYMBOLS = (
     "p3m_params_default", "p3m_create", "p3m_destroy", "p3m_set_params",
     "p3m_get_params", "p3m_is_tuned", "p3m_activate", "p3m_solve",
     "p3m_solve_noalloc", "p3m_noalloc_scratch_bytes", "p3m_set_launch_shape",
-    "p3m_get_launch_shape",
+    "p3m_get_launch_shape", "p3m_set_deterministic", "p3m_batch_set_deterministic",
     "p3m_prepare_fft_mesh", "p3m_assign_charges", "p3m_charge_assign",
     "p3m_count_charged_particles", "p3m_count_charged_particles_elc",
     "p3m_adapt_epsilon_elc", "p3m_assigned_count", "p3m_solve_assigned",
@@ -213,6 +213,8 @@ def load():
     L.p3m_batch_working_bytes.argtypes = [vp, C.c_int, ip, C.POINTER(C.c_int64)]
     L.p3m_batch_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp, C.c_uint, dp, dp]
     L.p3m_batch_graph_solve.argtypes = [vp, C.c_int, ip, vp, vp, vp]
+    L.p3m_set_deterministic.argtypes = [vp, C.c_int]
+    L.p3m_batch_set_deterministic.argtypes = [vp, C.c_int]
     L.p3m_set_launch_shape.argtypes = [vp, shp]
     L.p3m_get_launch_shape.argtypes = [vp, shp]
     L.p3m_batch_set_launch_shape.argtypes = [vp, shp]

@@ -199,6 +199,12 @@ class P3MBatch:
                                       int(col_lanes))
         self._check(_lib.load().p3m_batch_set_launch_shape(self._b, C.byref(shape)))
 
+    def set_deterministic(self, on=True):
+        """Bit-reproducible solves of an FP64 batch (fixed-point charge
+        assignment with integer atomics; the default for FP64 batches).
+        Column-march launch shapes need it off."""
+        self._check(_lib.load().p3m_batch_set_deterministic(self._b, 1 if on else 0))
+
     def launch_shape(self):
         """The launch shape of later solves (dict with mapping names)."""
         shape = _lib.p3m_launch_shape()

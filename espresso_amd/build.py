@@ -36,10 +36,20 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, jobs=None):
-    """Compile every source (always) and link the shared library."""
-    os.makedirs(LIBDIR, exist_ok=True)
-    objdir = os.path.join(HERE, "_build")
+# host-code sanitizers (the device code is built as usual): every
+# -fsanitize goes to the host pass only
+SANITIZE_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
+SANITIZE_LIB = os.path.join(LIBDIR, "sanitize", LIBNAME)
+
+
+def build(verbose=False, jobs=None, sanitize=False):
+    """Compile every source (always) and link the shared library.  With
+    sanitize, the host code is instrumented with AddressSanitizer and
+    UndefinedBehaviorSanitizer and the library goes to lib/sanitize/."""
+    target = SANITIZE_LIB if sanitize else LIB_PATH
+    os.makedirs(os.path.dirname(target), exist_ok=True)
+    objdir = os.path.join(HERE, "_build_sanitize" if sanitize else "_build")
     os.makedirs(objdir, exist_ok=True)
     cc = hipcc()
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
@@ -52,7 +62,8 @@ def build(verbose=False, jobs=None):
         obj = os.path.join(objdir, src + ".o")
         objs.append(obj)
         lang = ["-x", "hip"]
-        cmd = [cc] + lang + HIPFLAGS + inc + ["-c", path, "-o", obj]
+        cmd = [cc] + lang + HIPFLAGS + (SANITIZE_FLAGS if sanitize else []) + inc + [
+            "-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE,
@@ -69,15 +80,16 @@ def build(verbose=False, jobs=None):
         for src, text in failed:
             sys.stderr.write(f"--- {src} ---\n{text}\n")
         raise RuntimeError("hipcc failed for: " + ", ".join(s for s, _ in failed))
-    tmp = LIB_PATH + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    tmp = target + ".tmp"
+    link_san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host",
+                "-fsanitize=undefined", "-shared-libsan"] if sanitize else []
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC"] + link_san + ["-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
-    print(LIB_PATH)
+    print(build(verbose="-v" in sys.argv, sanitize="--sanitize" in sys.argv))

@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <type_traits>
 
 namespace {
 /** Working memory of the allocation-free solves (BatchSolver::solve_fixed):
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
       v[3] = qi * u[0];
       v[4] = qi * u[1];
       v[5] = qi * u[2];
+      v[6] = fabs(qi); // bounds the fixed-point charge mesh of deterministic solves
     } else if (p_nmp) {
       p_nmp[3 * li] = p_nmp[3 * li + 1] = p_nmp[3 * li + 2] = -1;
     }
@@ -487,13 +489,33 @@ template <int P> __device__ __forceinline__ void brick_origin(SysDesc const &d, 
   o2 = bz * kBrick;
 }
 
+/** Scale 2^s of the 64-bit fixed-point charge mesh of a system whose charges
+ *  sum to sum_abs_q in magnitude: no mesh point can exceed sum_abs_q (each
+ *  particle's weights are at most 1), so |value| 2^s < 2^61 for every point
+ *  and every partial sum.  s = 61 - e with sum_abs_q < 2^e. */
+__host__ __device__ inline double fixed_scale(double sum_abs_q) {
+  int e = 0;
+  (void)frexp(sum_abs_q > 1. ? sum_abs_q : 1., &e);
+  return ldexp(1., 61 - e);
+}
+
+/** Charge assignment of one brick: contributions are rounded once to 64-bit
+ *  fixed point (scale fixed_scale of the system's sum |q|) and summed in the
+ *  LDS tile with integer atomics (ds_add_u64 measured faster than ds_add_f64
+ *  on this pattern, and integer sums do not depend on the order).  The tile
+ *  is flushed into the periodic mesh with device atomics: as 64-bit integers
+ *  into a fixed-point mesh (int_mesh, FP64 meshes: the whole charge
+ *  assignment is then independent of the order of particles, bricks and
+ *  atomics), or converted to the mesh type.  Lanes over the (j, k) stencil
+ *  plane in segments of up to SEGMAX lanes per particle. */
 template <int P, typename MT, int SEGMAX, int NTH>
 __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
                                                int const *__restrict__ sel,
                                                int const *__restrict__ wt, int nsel,
                                                int const *__restrict__ soff,
                                                int const *__restrict__ offsets,
-                                               double4 const *__restrict__ spos, int chunk) {
+                                               double4 const *__restrict__ spos, int chunk,
+                                               double const *__restrict__ sums, int int_mesh) {
   constexpr int T = kBrick + P - 1;
   constexpr int TYZ = T * T;
   constexpr int tile_n = T * TYZ;
@@ -502,19 +524,18 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
   s.w = smem_d;
   s.qq = s.w + chunk * 3 * P;
   s.lo = reinterpret_cast<int *>(s.qq + chunk);
-  double *tile = reinterpret_cast<double *>(reinterpret_cast<char *>(smem_d) +
-                                              chunk_slot_bytes(P, chunk));
+  unsigned long long *tile = reinterpret_cast<unsigned long long *>(
+      reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
 
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int const b = L.lb;
   int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
   if (beg == end) return;
+  double const scale = fixed_scale(sums[L.s * 16 + 6]);
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
   double4 const *sp = spos + soff[L.s];
-  // lane segments of S lanes per particle over the (j, k) stencil plane,
-  // each lane sweeping pairs r0, r0 + S, ... and the stencil layers i
   constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
   constexpr int ppw = 64 / S;
   constexpr int NPAIR = P * P;
@@ -524,7 +545,7 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
   int const nwaves = blockDim.x >> 6;
   int const sub = lane / S;
   int const r0 = lane - sub * S;
-  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0.;
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0ull;
   for (int c0 = beg; c0 < end; c0 += chunk) {
     int const cnt = min(chunk, end - c0);
     if (c0 != beg) __syncthreads(); // the tile zeroing and the slots are disjoint
@@ -535,17 +556,18 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
       int const p = grp * ppw + sub;
       if (p < cnt) {
         double const *w = s.w + p * 3 * P;
-        double const qp = s.qq[p];
+        double const qs = s.qq[p] * scale;
         int const lo = s.lo[p];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           int const r = r0 + t * S;
           if (r < NPAIR) {
             int const jj = r / P, kk = r - (r / P) * P;
-            double const wyzq = (w[P + jj] * w[2 * P + kk]) * qp;
-            double *dst = tile + lo + jj * T + kk;
+            double const wyzq = (w[P + jj] * w[2 * P + kk]) * qs;
+            unsigned long long *dst = tile + lo + jj * T + kk;
 #pragma unroll
-            for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, w[i] * wyzq);
+            for (int i = 0; i < P; ++i)
+              atomicAdd(dst + i * TYZ, static_cast<unsigned long long>(llrint(w[i] * wyzq)));
           }
         }
       }
@@ -553,16 +575,24 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
   }
   __syncthreads();
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
-  MT *rho = static_cast<MT *>(d.rho);
+  double const inv_scale = 1. / scale;
   TileRows<P, NTH> const tr(o2, M2);
 #pragma unroll
   for (int k = 0; k < TileRows<P, NTH>::NPASS; ++k) {
     if (tr.valid(k)) {
-      double const v = tile[tr.row(k) * T + tr.c];
-      if (v != 0.) unsafeAtomicAdd(rho + tr.mesh_index(k, o0, o1, M0, M1, M2), static_cast<MT>(v));
+      unsigned long long const v = tile[tr.row(k) * T + tr.c];
+      if (v == 0ull) continue;
+      int64_t const g = tr.mesh_index(k, o0, o1, M0, M1, M2);
+      if (int_mesh)
+        atomicAdd(static_cast<unsigned long long *>(d.rho) + g, v);
+      else
+        unsafeAtomicAdd(static_cast<MT *>(d.rho) + g,
+                        static_cast<MT>(static_cast<double>(static_cast<long long>(v)) *
+                                        inv_scale));
     }
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // FFT passes.  Real mesh [Mx][My][Mz]; half spectrum [Mx][My][Kz].
@@ -578,11 +608,15 @@ template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict__ sys,
                                                       int const *__restrict__ sel,
                                                       int const *__restrict__ wt, int nsel,
-                                                      cx_t<MT> const *__restrict__ tw, int nlp) {
+                                                      cx_t<MT> const *__restrict__ tw, int nlp,
+                                                      double const *__restrict__ sums, int det) {
   using C = cx_t<MT>;
   extern __shared__ double smem_d[];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
+  // deterministic solves: the charge mesh holds 64-bit fixed-point sums
+  double const inv_scale = det ? 1. / fixed_scale(sums[L.s * 16 + 6]) : 0.;
+  long long const *rho_fx = static_cast<long long const *>(d.rho);
   FftPlanDev const &plan = d.plan[2];
   int const n = plan.n;
   int const ls = n + 1;
@@ -599,8 +633,14 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
     int const l = fdiv(e, fn), j = e - l * n;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
-    MT const re = (r0 < rows) ? rho[r0 * n + j] : MT(0);
-    MT const im = (r1 < rows) ? rho[r1 * n + j] : MT(0);
+    MT re, im;
+    if (det) {
+      re = (r0 < rows) ? static_cast<MT>(static_cast<double>(rho_fx[r0 * n + j]) * inv_scale) : MT(0);
+      im = (r1 < rows) ? static_cast<MT>(static_cast<double>(rho_fx[r1 * n + j]) * inv_scale) : MT(0);
+    } else {
+      re = (r0 < rows) ? rho[r0 * n + j] : MT(0);
+      im = (r1 < rows) ? rho[r1 * n + j] : MT(0);
+    }
     b0[l * ls + j] = C{re, im};
   }
   __syncthreads();
@@ -1943,7 +1983,8 @@ void quiesce(BatchSolver::Impl *m) {
 } // namespace
 
 BatchSolver::BatchSolver(int device, MeshPrecision precision)
-    : m(new Impl), m_device(device), m_prec(precision) {
+    : m(new Impl), m_device(device), m_prec(precision),
+      m_det(precision == MeshPrecision::FP64) {
   hip_check(hipSetDevice(device), "hipSetDevice");
   m->lanes.push_back(std::make_unique<Lane>());
   create_lane(*m->lanes[0]);
@@ -2034,6 +2075,8 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   if ((s.column != 0 && s.column != 8 && s.column != 16) || s.seg < 1 || s.seg > 1024 ||
       (s.col_lanes != 8 && s.col_lanes != 16))
     throw std::invalid_argument("P3M: invalid column-march shape");
+  if (m_det && s.column)
+    throw std::invalid_argument("P3M: deterministic solves need 8^3 bricks (column = 0)");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   bool const rebin = s.column != m_shape.column || s.seg != m_shape.seg;
   m_shape = s;
@@ -2059,6 +2102,16 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
     hip_check(hipStreamSynchronize(m_stream), "sync influence");
     m->descs_dirty = true;
   }
+}
+
+void BatchSolver::set_deterministic(bool on) {
+  if (on && m_prec != MeshPrecision::FP64)
+    throw std::invalid_argument("P3M: deterministic solves need the FP64 mesh");
+  if (on && m_shape.column)
+    throw std::invalid_argument("P3M: deterministic solves need 8^3 bricks (column = 0)");
+  hip_check(hipSetDevice(m_device), "hipSetDevice");
+  quiesce(m); // captured graphs hold the previous spread
+  m_det = on;
 }
 
 AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays const &pa,
@@ -2127,7 +2180,7 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
   }
   try_blocks();
   // column march: width and lanes per particle, then x layers per block
-  {
+  if (!m_det) {
     LaunchShape const base = res.best;
     for (int w : {8, 16})
       for (int l : {8, 16}) {
@@ -2682,6 +2735,12 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
                                                 ln.p_brick, ln.p_rank, ln.counts, ln.spos,
                                                 ln.sidx);
   if (ev) (void)hipEventRecord(ev[1], st);
+  bool const det = (flags & kSolveDeterministic) && !no_spread && !stop_after_spread &&
+                   !(flags & (kSolveAccumulate | kSolveMeshOnly));
+  if (det && (shape.column || !std::is_same_v<MT, double>))
+    throw std::runtime_error("P3M: deterministic solves need FP64 meshes and 8^3 bricks");
+  // the fixed-point mesh of deterministic solves: the brick spread flushes
+  // integers, the r2c pass converts them
   for (auto const &g : p.groups) {
     if (no_spread) break;
     if (shape.column) {
@@ -2699,9 +2758,10 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     dispatch_cao(g.cao, [&](auto P) {
       with_threads(shape.threads, [&](auto NT) {
         size_t const lds =
-            chunk_bytes(P.value, shape.chunk) + tile_points(P.value) * sizeof(double);
+            chunk_bytes(P.value, shape.chunk) + tile_points(P.value) * sizeof(unsigned long long);
         auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
-                                          static_cast<double4 const *>(ln.spos), shape.chunk);
+                                          static_cast<double4 const *>(ln.spos), shape.chunk,
+                                          static_cast<double const *>(ln.sums), det ? 1 : 0);
         auto launch = [&](auto kern) {
           std::apply([&](auto... a) { kern<<<g.blocks, NT.value, lds, st>>>(a...); }, args);
         };
@@ -2722,7 +2782,8 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
       for (int k = 3; k <= 6; ++k) (void)hipEventRecord(ev[k], st);
     return;
   }
-  k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
+  k_fft_z_r2c<MT><<<p.blocks_z, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z,
+                                                       ln.sums, det ? 1 : 0);
   if (p.blocks_y > 0)
     k_fft_y<MT><<<p.blocks_y, kBlock, p.lds_y, st>>>(sys, p.d_sel, p.wt_y, ns, tw, p.ncol_y);
   if (ev) (void)hipEventRecord(ev[3], st);
@@ -2869,6 +2930,7 @@ void BatchSolver::solve_on(int lane, std::vector<int> const &sel, ParticleArrays
   hipEvent_t *ev = ln.ev;
   if (ln.profiling && static_cast<size_t>(ln.ring_used + 1) * 8 <= ln.ring.size())
     ev = &ln.ring[8 * static_cast<size_t>(ln.ring_used++)];
+  if (m_det) flags |= kSolveDeterministic;
   if (m_prec == MeshPrecision::FP64)
     enqueue<double>(m, ln, pa, flags, m_shape, ev, nullptr, false);
   else
@@ -2947,6 +3009,7 @@ void BatchSolver::solve_fixed(std::vector<int> const &sel, ParticleArrays const 
   Lane &ln = *m->fixed;
   upload_descs(m, ln.stream); // copies into the existing descriptor array
   build_plan(m, ln, sel, m_prec, m_shape);
+  if (m_det) flags |= kSolveDeterministic;
   if (m_prec == MeshPrecision::FP64)
     enqueue<double>(m, ln, pa, flags, m_shape, ln.ev, nullptr, false);
   else
@@ -3024,6 +3087,7 @@ void BatchSolver::graph_solve(std::vector<int> const &sel, ParticleArrays const 
   upload_descs(m, ln.stream);
   build_plan(m, ln, sel, m_prec, m_shape);
   ensure_scratch(m, ln, ln.plan.npart, nullptr, 0);
+  if (m_det) flags |= kSolveDeterministic;
   Plan &p = ln.plan;
   bool const same = p.graph && p.graph_flags == flags && p.graph_pa.pos == pa.pos &&
                     p.graph_pa.q == pa.q && p.graph_pa.upos == pa.upos &&

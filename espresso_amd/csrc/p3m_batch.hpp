@@ -52,6 +52,8 @@ struct SolveResult {
 constexpr unsigned kSolveAccumulate = 1u << 16;
 constexpr unsigned kSolveMeshOnly = 1u << 17;
 constexpr unsigned kSolveNoSpread = 1u << 18;
+/** Set by the solver itself for deterministic solves (set_deterministic). */
+constexpr unsigned kSolveDeterministic = 1u << 19;
 
 /** Device particle arrays of all systems, concatenated in system order
  *  (positions AoS [n][3], charges [n], forces [n][3]; upos may be null). */
@@ -187,6 +189,17 @@ public:
   void graph_solve(std::vector<int> const &sel, ParticleArrays const &pa, unsigned flags);
 
   void set_launch_shape(LaunchShape const &s);
+  /** Deterministic solves (FP64 meshes, 8^3 bricks; on by default for FP64
+   *  batches, which cannot then take column-march shapes until it is turned
+   *  off): the charge assignment
+   *  sums 64-bit fixed-point contributions with integer atomics, so every
+   *  solve of the same input gives the same bits, whatever the order of
+   *  particles or of the device's atomics.  The mesh values are rounded to
+   *  2^-s, s = 61 - ceil(log2(sum |q|)) (about 4e-13 at 10^6 unit charges).
+   *  Solves that assign charges in several calls (the charge-assignment
+   *  protocol) keep the floating-point mesh. */
+  void set_deterministic(bool on);
+  bool deterministic() const { return m_det; }
   /** Time force solves of `sel` for every candidate launch shape (gather
    *  mapping x FFT columns x z lines), `reps` solves each, and keep the
    *  fastest.  Forces are accumulated into pa.force on every solve. */
@@ -204,6 +217,7 @@ private:
   MeshPrecision m_prec;
   hipStream_t m_stream = nullptr;
   LaunchShape m_shape;
+  bool m_det = false;
 };
 
 void hip_check(hipError_t err, char const *what);

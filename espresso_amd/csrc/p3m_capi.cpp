@@ -1212,6 +1212,17 @@ int p3m_set_launch_shape(p3m_context *ctx, p3m_launch_shape const *shape) {
   });
 }
 
+int p3m_set_deterministic(p3m_context *ctx, int on) {
+  if (!ctx) return P3M_ERR_INVALID;
+  return guarded(ctx, [&] {
+    require_device(ctx);
+    if (on && ctx->user.precision != P3M_PREC_FP64)
+      throw std::invalid_argument("P3M: deterministic solves need precision P3M_PREC_FP64");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    ctx->solver->set_deterministic(on != 0);
+  });
+}
+
 int p3m_get_launch_shape(p3m_context const *ctx, p3m_launch_shape *shape) {
   if (!ctx || !shape || !ctx->solver) return P3M_ERR_INVALID;
   *shape = to_abi(ctx->solver->launch_shape());
@@ -1392,6 +1403,14 @@ int p3m_batch_get_launch_shape(p3m_batch const *b, p3m_launch_shape *shape) {
   if (!b || !shape) return P3M_ERR_INVALID;
   *shape = to_abi(b->solver->launch_shape());
   return P3M_OK;
+}
+
+int p3m_batch_set_deterministic(p3m_batch *b, int on) {
+  if (!b) return P3M_ERR_INVALID;
+  return guarded(&b->last_error, [&] {
+    hip_check(hipSetDevice(b->device), "hipSetDevice");
+    b->solver->set_deterministic(on != 0);
+  });
 }
 
 int p3m_batch_autotune(p3m_batch *b, int nsel, int const *sys_ids, double const *pos,

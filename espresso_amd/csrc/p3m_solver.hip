@@ -25,6 +25,11 @@ LocalMesh const &Solver::local_mesh() const {
   return m_batch->local_mesh(0);
 }
 
+void Solver::set_deterministic(bool on) {
+  if (m_batch) m_batch->set_deterministic(on);
+  m_det = on ? 1 : 0;
+}
+
 void Solver::set_launch_shape(LaunchShape const &s) {
   m_shape = s;
   m_shape_set = true;
@@ -47,6 +52,8 @@ void Solver::configure(P3MParams const &params, Vec3d const &box_l, double skin,
   hip_check(hipStreamSynchronize(m_idle_stream), "sync");
   m_batch.reset();
   auto b = std::make_unique<BatchSolver>(m_device, precision);
+  if (m_det >= 0 && (m_det == 1 || precision == MeshPrecision::FP64))
+    b->set_deterministic(m_det == 1); // before a column shape of a non-deterministic solver
   if (m_shape_set) b->set_launch_shape(m_shape);
   b->add_system(params, box_l, skin, prefactor, 0);
   b->set_convention(0, m_conv);

@@ -71,6 +71,10 @@ public:
    *  fft_fwd, convolve, fft_bwd, gather. */
   float last_ms[8] = {};
   void set_launch_shape(LaunchShape const &s);
+  /** Deterministic solves (BatchSolver::set_deterministic; the default of
+   *  FP64 solvers); an explicit choice is kept across later configure()
+   *  calls. */
+  void set_deterministic(bool on);
   LaunchShape launch_shape() const {
     return m_batch ? m_batch->launch_shape() : (m_shape_set ? m_shape : LaunchShape{});
   }
@@ -92,6 +96,7 @@ private:
   LaunchShape m_shape;
   bool m_shape_set = false;
   int m_conv = 0;
+  int m_det = -1; // -1: the batch solver's default
 };
 
 } // namespace espresso_amd

@@ -320,6 +320,14 @@ class _P3MBase:
                        self._ctx)
         return "gpu" if c.value == _lib.P3M_CONV_GPU else "cpu"
 
+    def set_deterministic(self, on=True):
+        """Bit-reproducible solves (FP64 actors, the default there): the
+        charge assignment sums 64-bit fixed-point contributions with integer
+        atomics, so repeated solves of the same input give identical forces
+        and energies.  Column-march launch shapes need it off."""
+        _lib.raise_for(_lib.load().p3m_set_deterministic(self._ctx, 1 if on else 0),
+                       self._ctx)
+
     def set_launch_shape(self, **shape):
         """Launch shape of later solves: the keyword arguments of
         ``P3MBatch.set_launch_shape`` (gather, ncol, zlines, fused_lds,

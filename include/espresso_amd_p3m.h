@@ -418,6 +418,16 @@ int p3m_autotune(p3m_context *ctx, int64_t n, const double *pos, const double *q
 int p3m_set_launch_shape(p3m_context *ctx, const p3m_launch_shape *shape);
 int p3m_get_launch_shape(const p3m_context *ctx, p3m_launch_shape *shape);
 
+/* Deterministic solves (no counterpart in the reference, whose CPU solver
+ * is deterministic by construction; on by default for FP64 contexts): the
+ * charge assignment rounds every contribution once to a 64-bit fixed-point
+ * value (resolution 2^-s with s = 61 - ceil(log2(sum |q|)), about 4e-13 for
+ * 10^6 unit charges) and sums with integer atomics into a fixed-point mesh,
+ * so repeated solves of the same input give the same bits.  FP64 precision
+ * and 8^3 bricks only: a column-march launch shape needs it off; the
+ * charge-assignment protocol calls keep the floating-point mesh. */
+int p3m_set_deterministic(p3m_context *ctx, int on);
+
 /* Block until all work enqueued on the context stream has finished. */
 int p3m_synchronize(p3m_context *ctx);
 
@@ -461,6 +471,9 @@ int p3m_batch_graph_solve(p3m_batch *b, int nsel, const int *sys_ids, const doub
  * of later solves. */
 int p3m_batch_set_launch_shape(p3m_batch *b, const p3m_launch_shape *shape);
 int p3m_batch_get_launch_shape(const p3m_batch *b, p3m_launch_shape *shape);
+/* Deterministic solves of an FP64 batch (see p3m_set_deterministic; on by
+ * default for FP64 batches). */
+int p3m_batch_set_deterministic(p3m_batch *b, int on);
 /* Autotuner: time `reps` force solves of the selection for candidate launch
  * shapes (coordinate descent over fused pass, particle-mesh block size and
  * chunk, gather and spread mappings, FFT columns and z lines) and keep the

@@ -13,7 +13,12 @@ which puts both truncations near exp(-31) relative.
                        shuffled): the first 2000 charged particles
   ewald_c3_subset.npz  N = 250000, L = 79.37 (seed 1003, +1/-1 alternating):
                        the first 2000 particles
-Prefactor 1.  Arrays: idx (subset indices), force [2000, 3].
+  ewald_c4_sample.npz  16 of the 512 batched systems (bench.make_c4: seeds
+                       4000 + s, N log-uniform in [500, 20000], density
+                       U[0.1, 0.8]) spanning N: the first min(N, 500)
+                       particles of each
+Prefactor 1.  Arrays: idx (subset indices), force [2000, 3]; for the C4
+sample sys (system ids), count (particles per system), force [sum, 3].
 """
 import os
 import sys
@@ -42,7 +47,37 @@ def exact_forces(pos, q, L, idx, alpha_e):
     return f_r + f_k, dict(alpha_e=alpha_e, r_e=r_e, kmax=kmax)
 
 
+def c4_sample_ids(ns, k=16):
+    """k systems spread over the sorted N (both ends included)."""
+    order = np.argsort(ns, kind="stable")
+    return sorted({int(order[int(round(j * (len(ns) - 1) / (k - 1)))]) for j in range(k)})
+
+
+def c4_sample():
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    import bench
+    systems = bench.make_c4()
+    ids = c4_sample_ids(np.array([s["n"] for s in systems]))
+    forces, counts = [], []
+    t = time.time()
+    for i in ids:
+        s = systems[i]
+        L = s["L"]
+        m = min(s["n"], 500)
+        # alpha_e r_e = 5.6 with r_e just inside the minimum image
+        alpha_e = PRODUCT / (0.49 * L)
+        f, _ = exact_forces(s["pos"], s["q"], L, np.arange(m), alpha_e)
+        forces.append(f)
+        counts.append(m)
+    np.savez(os.path.join(HERE, "ewald_c4_sample.npz"), sys=np.array(ids),
+             count=np.array(counts), force=np.concatenate(forces))
+    print(f"C4 sample ({len(ids)} systems) done in {time.time() - t:.0f} s")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "c4":
+        c4_sample()
+        return
     pos, q, box, _ = S.config_c2()
     idx = np.nonzero(q)[0][:2000]
     t = time.time()

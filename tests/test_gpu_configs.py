@@ -52,6 +52,7 @@ def test_c2_full_size_parity():
         p3m = P3M(tune=False, single_precision=single, **params)
         p3m.activate(tp, tq, box_l=box)
         if column:
+            p3m.set_deterministic(False)
             p3m.set_launch_shape(column=column, seg=32, col_lanes=8)
         f = torch.zeros_like(tp)
         p3m.add_long_range_forces(tp, tq, f)

@@ -333,6 +333,35 @@ def test_ewald_fixtures_present():
         assert float(d["alpha_e"]) * float(d["r_e"]) == pytest.approx(5.6)
 
 
+def test_c4_ewald_sample_fixture_is_pinned():
+    """The batched-workload fixture: 16 systems spanning N, exact forces on
+    the first min(N, 500) particles.  The smallest system is recomputed with
+    another splitting (real-space cutoff 0.35 L instead of 0.49 L): the
+    exact total force must not change."""
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    sys.path.insert(0, os.path.dirname(here))
+    sys.path.insert(0, os.path.dirname(os.path.dirname(here)))
+    import bench
+    d = np.load(os.path.join(here, "ewald_c4_sample.npz"))
+    assert d["sys"].shape == (16,) and d["count"].sum() == d["force"].shape[0]
+    assert np.all(np.isfinite(d["force"]))
+    systems = bench.make_c4()
+    ns = np.array([s["n"] for s in systems])
+    assert ns[d["sys"]].min() == ns.min() and ns[d["sys"]].max() == ns.max()
+    k = int(np.argmin(ns[d["sys"]]))
+    s = systems[int(d["sys"][k])]
+    cnt = int(d["count"][k])
+    start = int(d["count"][:k].sum())
+    alpha = 5.6 / (0.35 * s["L"])
+    f_r = O.real_space_subset(s["pos"], s["q"], [s["L"]] * 3, 1.0, alpha, 0.35 * s["L"],
+                              cnt, exact_erfc=True)
+    _, f_k = O.ewald_kspace(s["pos"], s["q"], [s["L"]] * 3, 1.0, alpha, 11.2 * alpha,
+                            n_out=cnt)
+    assert S.rel_rms(f_r + f_k, d["force"][start:start + cnt]) <= 1e-9
+
+
 @pytest.mark.parametrize("cao", [5, 7])
 def test_gpu_convention_equals_cpu_convention_with_zero_mesh_offset(cao):
     """For odd cao the reference GPU index rule (nmp = floor(x M/L -
