@@ -98,10 +98,12 @@ def model_pairs(n_q, L, r_cut):
     return 0.5 * n_q * (n_q / L ** 3) * (4. / 3.) * math.pi * r_cut ** 3
 
 
-def tune_systems(systems, accuracy, mesh_range, cao_range, workers=8, pair_ns=None):
+def tune_systems(systems, accuracy, mesh_range, cao_range, workers=8, pair_ns=None,
+                 single=True):
     """Per-system tuning with the reference algorithm and the device cost
-    model: k-space stage rates of the solver plus the device real-space pair
-    loop (p3m_short_range) at pair_ns per pair (DEVICE_PAIR_NS by default)."""
+    model: k-space stage rates of the solver (FP32 mesh if single) plus the
+    device real-space pair loop (p3m_short_range) at pair_ns per pair
+    (DEVICE_PAIR_NS by default)."""
     from concurrent.futures import ThreadPoolExecutor
     from espresso_amd import tune_model
     pns = DEVICE_PAIR_NS if pair_ns is None else pair_ns
@@ -109,7 +111,7 @@ def tune_systems(systems, accuracy, mesh_range, cao_range, workers=8, pair_ns=No
     def one(s):
         return tune_model([s["L"]] * 3, s["n"], float(np.sum(s["q"] ** 2)), 1.0,
                           accuracy, mesh_range=mesh_range, cao_range=cao_range,
-                          pair_ns=pns)
+                          pair_ns=pns, single_precision=single)
     with ThreadPoolExecutor(workers) as ex:
         return list(ex.map(one, systems))
 

@@ -228,7 +228,7 @@ def load():
     L.p3m_batch_last_error.restype = C.c_char_p
     L.p3m_tune_model.argtypes = [C.POINTER(p3m_params), C.c_int, C.c_double,
                                  C.c_int, C.c_int, C.c_int, C.c_int,
-                                 C.c_double]
+                                 C.c_double, C.c_int]
     i64p = C.POINTER(C.c_int64)
     L.p3m_serve_create.argtypes = [vp, C.c_int, C.c_int64, C.c_int, C.c_double,
                                    C.POINTER(vp)]

@@ -260,10 +260,11 @@ class P3MBatch:
 
 def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
                mesh_range=(1, 1 << 20), cao_range=(1, 7), skin=0.4,
-               epsilon=0., verbose=False, pair_ns=1.0):
+               epsilon=0., verbose=False, pair_ns=1.0, single_precision=False):
     """Tuned parameters of one system from the reference's tuning algorithm
-    with a modelled cost per trial (device k-space stage rates plus a
-    real-space loop at ``pair_ns`` ns per pair).  Returns a dict with mesh,
+    with a modelled cost per trial (device k-space stage rates of an FP64
+    mesh, or an FP32 mesh if ``single_precision``, plus a real-space loop at
+    ``pair_ns`` ns per pair).  Returns a dict with mesh,
     cao, alpha, r_cut and the estimated accuracy; raises RuntimeError
     ("CoulombP3M: failed to reach requested accuracy") like the reference."""
     L = _lib.load()
@@ -279,7 +280,7 @@ def tune_model(box_l, n_charged, sum_q2, prefactor, accuracy,
     _lib.raise_for(L.p3m_tune_model(C.byref(cp), int(n_charged), float(sum_q2),
                                     int(mesh_range[0]), int(mesh_range[1]),
                                     int(cao_range[0]), int(cao_range[1]),
-                                    float(pair_ns)))
+                                    float(pair_ns), 1 if single_precision else 0))
     return {"mesh": [cp.mesh[d] for d in range(3)], "cao": cp.cao,
             "alpha": cp.alpha, "r_cut": cp.r_cut, "accuracy": cp.accuracy}
 

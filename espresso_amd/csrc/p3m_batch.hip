@@ -231,9 +231,13 @@ __global__ __launch_bounds__(kBlock) void k_bin(SysDesc const *__restrict__ sys,
     }
     p_brick[si] = b;
   }
+  double qmax = v[6]; // |q| of this particle (0 if uncharged)
   block_sum<7>(v, red);
-  if (threadIdx.x == 0)
+  qmax = block_max(qmax, red);
+  if (threadIdx.x == 0) {
     for (int k = 0; k < 7; ++k) partials[L.vb * 8 + k] = v[k];
+    partials[L.vb * 8 + 7] = qmax;
+  }
 }
 
 /** per-system deterministic reduction of block partials: one block per
@@ -255,7 +259,8 @@ __global__ void k_reduce_sys(int const *__restrict__ sel, int const *__restrict_
 }
 
 /** One block per selected system: the per-system sums of the binning
- *  partials (n_q, sum q, sum q^2, dipole; block order, deterministic), then
+ *  partials (n_q, sum q, sum q^2, dipole, sum |q|; block order,
+ *  deterministic) and max |q| (slot 7), then
  *  the exclusive scan of the system's brick counts, kScanItems bricks per
  *  thread and pass.  The counters become the start of each copy's run
  *  within its brick; offsets[b] is the first particle of brick b and
@@ -273,12 +278,18 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
   int const s = sel[slot];
   SysDesc const &d = sys[s];
   {
-    double v[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
-    for (int b = wt_bin[slot] + threadIdx.x; b < wt_bin[slot + 1]; b += blockDim.x)
+    double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+    double qmax = 0.;
+    for (int b = wt_bin[slot] + threadIdx.x; b < wt_bin[slot + 1]; b += blockDim.x) {
       for (int k = 0; k < 7; ++k) v[k] += bin_part[b * 8 + k];
-    block_sum<8>(v, red);
-    if (threadIdx.x == 0)
+      qmax = fmax(qmax, bin_part[b * 8 + 7]);
+    }
+    block_sum<7>(v, red);
+    qmax = block_max(qmax, red);
+    if (threadIdx.x == 0) {
       for (int k = 0; k < 7; ++k) sums[s * 16 + k] = v[k];
+      sums[s * 16 + 7] = qmax;
+    }
   }
   int const nb = d.nbricks;
   int *count = counts + d.brick_off * kCountCopies;
@@ -384,10 +395,9 @@ __global__ __launch_bounds__(kBlock) void k_clear(SysDesc const *__restrict__ sy
 // charge assignment into LDS brick tiles (AssignCharge<cao>, p3m.cpp:283-330;
 // p3m_interpolate, interpolation.hpp:191-208).  One block per brick of
 // 8x8x8 first-stencil points: the tile holds the brick plus cao-1 planes on
-// the high side; contributions are added with LDS FP64 atomics (for both
-// mesh precisions: ds_add_f64 is several times faster than ds_add_f32 for
-// this scattered pattern on gfx950, and the sum is rounded once) and the
-// tile is flushed into the periodic mesh with device-scope atomics.
+// the high side; contributions are added in fixed point with LDS integer
+// atomics and the tile is flushed into the periodic mesh with device-scope
+// atomics.
 // ---------------------------------------------------------------------------
 /** Bytes of the chunk slots ([chunk][3P] weights, [chunk] charges, [chunk]
  *  tile indices) rounded up to 16, so that the brick tile after them is
@@ -489,27 +499,67 @@ template <int P> __device__ __forceinline__ void brick_origin(SysDesc const &d, 
   o2 = bz * kBrick;
 }
 
-/** Scale 2^s of the 64-bit fixed-point charge mesh of a system whose charges
- *  sum to sum_abs_q in magnitude: no mesh point can exceed sum_abs_q (each
- *  particle's weights are at most 1), so |value| 2^s < 2^61 for every point
- *  and every partial sum.  s = 61 - e with sum_abs_q < 2^e. */
-__host__ __device__ inline double fixed_scale(double sum_abs_q) {
-  int e = 0;
-  (void)frexp(sum_abs_q > 1. ? sum_abs_q : 1., &e);
-  return ldexp(1., 61 - e);
+/** Scale 2^s of the 64-bit fixed-point charge mesh of a system with charges
+ *  of magnitude below 2^a (max |q|) summing to below 2^b (sum |q|).  A
+ *  contribution q w_x w_y w_z 2^s is below 2^(a+s) (weights <= 1) and must
+ *  stay below 2^51 for fixed_round; a mesh point or partial sum is below
+ *  2^(b+s), kept below 2^62.  s = min(50 - a, 62 - b). */
+__host__ __device__ inline double fixed_scale(double sum_abs_q, double max_abs_q) {
+  int a = 0, b = 0;
+  (void)frexp(max_abs_q, &a); // |q| < 2^a (a = 0 for no charges)
+  (void)frexp(sum_abs_q, &b);
+  return ldexp(1., (50 - a < 62 - b) ? 50 - a : 62 - b);
 }
 
-/** Charge assignment of one brick: contributions are rounded once to 64-bit
- *  fixed point (scale fixed_scale of the system's sum |q|) and summed in the
- *  LDS tile with integer atomics (ds_add_u64 measured faster than ds_add_f64
- *  on this pattern, and integer sums do not depend on the order).  The tile
- *  is flushed into the periodic mesh with device atomics: as 64-bit integers
- *  into a fixed-point mesh (int_mesh, FP64 meshes: the whole charge
- *  assignment is then independent of the order of particles, bricks and
- *  atomics), or converted to the mesh type.  Lanes over the (j, k) stencil
- *  plane in segments of up to SEGMAX lanes per particle. */
+/** a b rounded to the nearest integer (ties to even) for |a b| < 2^51, as a
+ *  two's-complement 64-bit value: a b + 1.5 2^52 lies in [2^52, 2^53), where
+ *  the spacing of doubles is 1, so the fused multiply-add rounds the exact
+ *  product once, and the bits of the sum are those of 1.5 2^52 plus the
+ *  rounded integer. */
+template <typename TT = unsigned long long>
+__device__ __forceinline__ TT fixed_round(double a, double b) {
+  double const y = fma(a, b, 6755399441055744.0); // 1.5 2^52
+  // the low 32 bits of 1.5 2^52 are zero: a 32-bit result (|a b| < 2^31) is
+  // the low word of the sum
+  return static_cast<TT>(static_cast<unsigned long long>(__double_as_longlong(y)) -
+                         0x4338000000000000ull);
+}
+
+/** LDS tile word of the charge assignment: 64-bit fixed point for FP64
+ *  meshes; 32-bit for FP32 meshes, at a per-brick scale (brick_scale32),
+ *  which halves the tile and its atomic traffic (ds_add_u32 runs at about
+ *  1.6x the rate of ds_add_u64 on the stencil pattern, ds_add_f32 at a
+ *  twentieth). */
+template <typename MT> struct SpreadWord {
+  using type = unsigned long long;
+};
+template <> struct SpreadWord<float> {
+  using type = unsigned int;
+};
+
+/** Scale 2^s of a brick's 32-bit fixed-point tile: the brick's n particles
+ *  of |q| <= max_abs_q sum to below 2^e, so every tile point and every
+ *  contribution stays below 2^(e+s) = 2^30.  The resolution 2^-s (2^-24 for
+ *  a brick of 49 unit charges) is far below the FP32 mesh's own rounding
+ *  once the tile is flushed. */
+__device__ __forceinline__ double brick_scale32(int n, double max_abs_q) {
+  int e = 0;
+  (void)frexp(static_cast<double>(n) * max_abs_q, &e);
+  return ldexp(1., 30 - e);
+}
+
+/** Charge assignment of one brick: contributions are rounded once to fixed
+ *  point (fixed_round; 64-bit at fixed_scale of the system's charges for
+ *  FP64 meshes, 32-bit at brick_scale32 for FP32 meshes) and summed in the
+ *  LDS tile with integer atomics (faster than FP atomics on this pattern,
+ *  and integer sums do not depend on the order).  The tile is flushed into
+ *  the periodic mesh with device atomics: as 64-bit integers into a
+ *  fixed-point mesh (int_mesh, FP64 meshes: the whole charge assignment is
+ *  then independent of the order of particles, bricks and atomics), or
+ *  converted to the mesh type.  Lanes over the (j, k) stencil plane in
+ *  segments of up to SEGMAX lanes per particle. */
 template <int P, typename MT, int SEGMAX, int NTH>
-__global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
+__global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc const *__restrict__ sys,
                                                int const *__restrict__ sel,
                                                int const *__restrict__ wt, int nsel,
                                                int const *__restrict__ soff,
@@ -524,15 +574,17 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
   s.w = smem_d;
   s.qq = s.w + chunk * 3 * P;
   s.lo = reinterpret_cast<int *>(s.qq + chunk);
-  unsigned long long *tile = reinterpret_cast<unsigned long long *>(
-      reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
+  using TT = typename SpreadWord<MT>::type;
+  TT *tile = reinterpret_cast<TT *>(reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
 
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   int const b = L.lb;
   int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
   if (beg == end) return;
-  double const scale = fixed_scale(sums[L.s * 16 + 6]);
+  double const scale = (sizeof(TT) == 8)
+                           ? fixed_scale(sums[L.s * 16 + 6], sums[L.s * 16 + 7])
+                           : brick_scale32(end - beg, sums[L.s * 16 + 7]);
   int o0, o1, o2;
   brick_origin<P>(d, b, o0, o1, o2);
   double4 const *sp = spos + soff[L.s];
@@ -545,7 +597,7 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
   int const nwaves = blockDim.x >> 6;
   int const sub = lane / S;
   int const r0 = lane - sub * S;
-  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = 0ull;
+  for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = TT(0);
   for (int c0 = beg; c0 < end; c0 += chunk) {
     int const cnt = min(chunk, end - c0);
     if (c0 != beg) __syncthreads(); // the tile zeroing and the slots are disjoint
@@ -558,16 +610,20 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
         double const *w = s.w + p * 3 * P;
         double const qs = s.qq[p] * scale;
         int const lo = s.lo[p];
+        // x weights in registers: the LDS atomics below would otherwise make
+        // the compiler re-read them from LDS after every atomic
+        double wx[P];
+#pragma unroll
+        for (int i = 0; i < P; ++i) wx[i] = w[i];
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           int const r = r0 + t * S;
           if (r < NPAIR) {
             int const jj = r / P, kk = r - (r / P) * P;
             double const wyzq = (w[P + jj] * w[2 * P + kk]) * qs;
-            unsigned long long *dst = tile + lo + jj * T + kk;
+            TT *dst = tile + lo + jj * T + kk;
 #pragma unroll
-            for (int i = 0; i < P; ++i)
-              atomicAdd(dst + i * TYZ, static_cast<unsigned long long>(llrint(w[i] * wyzq)));
+            for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, fixed_round<TT>(wx[i], wyzq));
           }
         }
       }
@@ -580,15 +636,16 @@ __global__ __launch_bounds__(NTH) void k_spread(SysDesc const *__restrict__ sys,
 #pragma unroll
   for (int k = 0; k < TileRows<P, NTH>::NPASS; ++k) {
     if (tr.valid(k)) {
-      unsigned long long const v = tile[tr.row(k) * T + tr.c];
-      if (v == 0ull) continue;
+      TT const v = tile[tr.row(k) * T + tr.c];
+      if (v == TT(0)) continue;
       int64_t const g = tr.mesh_index(k, o0, o1, M0, M1, M2);
-      if (int_mesh)
-        atomicAdd(static_cast<unsigned long long *>(d.rho) + g, v);
+      double const sv = (sizeof(TT) == 8) ? static_cast<double>(static_cast<long long>(v))
+                                          : static_cast<double>(static_cast<int>(v));
+      if (sizeof(TT) == 8 && int_mesh)
+        atomicAdd(static_cast<unsigned long long *>(d.rho) + g,
+                  static_cast<unsigned long long>(v));
       else
-        unsafeAtomicAdd(static_cast<MT *>(d.rho) + g,
-                        static_cast<MT>(static_cast<double>(static_cast<long long>(v)) *
-                                        inv_scale));
+        unsafeAtomicAdd(static_cast<MT *>(d.rho) + g, static_cast<MT>(sv * inv_scale));
     }
   }
 }
@@ -615,7 +672,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_r2c(SysDesc const *__restrict_
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
   // deterministic solves: the charge mesh holds 64-bit fixed-point sums
-  double const inv_scale = det ? 1. / fixed_scale(sums[L.s * 16 + 6]) : 0.;
+  double const inv_scale =
+      det ? 1. / fixed_scale(sums[L.s * 16 + 6], sums[L.s * 16 + 7]) : 0.;
   long long const *rho_fx = static_cast<long long const *>(d.rho);
   FftPlanDev const &plan = d.plan[2];
   int const n = plan.n;
@@ -708,7 +766,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
                                                        int const *__restrict__ sel,
                                                        int const *__restrict__ wt, int nsel,
                                                        cx_t<MT> const *__restrict__ tw, int ncol,
-                                                       int want_fields, double *partials) {
+                                                       int want_fields, int want_energy,
+                                                       double *partials) {
   using C = cx_t<MT>;
   extern __shared__ double smem_d[];
   __shared__ double red[8 * 16];
@@ -751,19 +810,21 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
     double const wgt = (kz == 0 || 2 * kz == Mz) ? 1. : 2.;
     double const ek = static_cast<double>(gv) *
                       (static_cast<double>(rh.x) * rh.x + static_cast<double>(rh.y) * rh.y);
-    double const kx = dop_shift(j, Mx) * wvx;
-    double const kzv = dop_shift(kz, Mz) * wvz;
-    double const norm_sq = kx * kx + ky * ky + kzv * kzv;
-    acc[0] += wgt * ek;
-    if (norm_sq != 0.) {
-      double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
-      acc[1] += pref * kx * kx;
-      acc[2] += pref * kx * ky;
-      acc[3] += pref * kx * kzv;
-      acc[4] += pref * ky * ky;
-      acc[5] += pref * ky * kzv;
-      acc[6] += pref * kzv * kzv;
-      acc[7] += wgt * ek;
+    if (want_energy) {
+      double const kx = dop_shift(j, Mx) * wvx;
+      double const kzv = dop_shift(kz, Mz) * wvz;
+      double const norm_sq = kx * kx + ky * ky + kzv * kzv;
+      acc[0] += wgt * ek;
+      if (norm_sq != 0.) {
+        double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
+        acc[1] += pref * kx * kx;
+        acc[2] += pref * kx * ky;
+        acc[3] += pref * kx * kzv;
+        acc[4] += pref * ky * ky;
+        acc[5] += pref * ky * kzv;
+        acc[6] += pref * kzv * kzv;
+        acc[7] += wgt * ek;
+      }
     }
     if (want_fields) {
       C const phi = C{gv * rh.x, gv * rh.y};
@@ -772,9 +833,13 @@ __global__ __launch_bounds__(kBlock) void k_fft_x_conv(SysDesc const *__restrict
       oth[(ncol + c) * ls + j] = C{-kxf * phi.y, kxf * phi.x}; // B = i kx phi
     }
   }
-  block_sum<8>(acc, red);
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
+  if (want_energy) {
+    block_sum<8>(acc, red); // ends with a barrier
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
+  } else {
+    __syncthreads();
+  }
   if (!want_fields) return;
   C *ires = lds_fft(oth, res, 2 * ncol, ls, plan, twl, true);
   for (int e = threadIdx.x; e < n * ncol; e += blockDim.x) {
@@ -875,7 +940,8 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
                                                         int const *__restrict__ sel,
                                                         int const *__restrict__ wt, int nsel,
                                                         cx_t<MT> const *__restrict__ tw,
-                                                        int want_fields, double *partials) {
+                                                        int want_fields, int want_energy,
+                                                        double *partials) {
   using C = cx_t<MT>;
   extern __shared__ double smem_d[];
   __shared__ double red[8 * 16];
@@ -918,27 +984,33 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
     int const x = fdiv(e, fMy), y = e - x * My;
     C const rh = q[y * lsx + x];
     MT const gv = G[e];
-    double const ek = static_cast<double>(gv) *
-                      (static_cast<double>(rh.x) * rh.x + static_cast<double>(rh.y) * rh.y);
-    double const kx = dop_shift(x, Mx) * wvx;
-    double const ky = dop_shift(y, My) * wvy;
-    double const norm_sq = kx * kx + ky * ky + kzv * kzv;
-    acc[0] += wgt * ek;
-    if (norm_sq != 0.) {
-      double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
-      acc[1] += pref * kx * kx;
-      acc[2] += pref * kx * ky;
-      acc[3] += pref * kx * kzv;
-      acc[4] += pref * ky * ky;
-      acc[5] += pref * ky * kzv;
-      acc[6] += pref * kzv * kzv;
-      acc[7] += wgt * ek;
+    if (want_energy) {
+      double const ek = static_cast<double>(gv) *
+                        (static_cast<double>(rh.x) * rh.x + static_cast<double>(rh.y) * rh.y);
+      double const kx = dop_shift(x, Mx) * wvx;
+      double const ky = dop_shift(y, My) * wvy;
+      double const norm_sq = kx * kx + ky * ky + kzv * kzv;
+      acc[0] += wgt * ek;
+      if (norm_sq != 0.) {
+        double const pref = wgt * ek * (-2. * (1. / norm_sq + d.hais));
+        acc[1] += pref * kx * kx;
+        acc[2] += pref * kx * ky;
+        acc[3] += pref * kx * kzv;
+        acc[4] += pref * ky * ky;
+        acc[5] += pref * ky * kzv;
+        acc[6] += pref * kzv * kzv;
+        acc[7] += wgt * ek;
+      }
     }
     q[y * lsx + x] = C{gv * rh.x, gv * rh.y};
   }
-  block_sum<8>(acc, red); // ends with a barrier: q is complete
-  if (threadIdx.x == 0)
-    for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
+  if (want_energy) {
+    block_sum<8>(acc, red); // ends with a barrier: q is complete
+    if (threadIdx.x == 0)
+      for (int k = 0; k < 8; ++k) partials[L.vb * 8 + k] = acc[k];
+  } else {
+    __syncthreads(); // q is complete
+  }
   if (!want_fields) return;
   C *const Bp = static_cast<C *>(d.B) + (int64_t)kz * pl;
   C *const Ap = static_cast<C *>(d.A) + (int64_t)kz * pl;
@@ -1002,10 +1074,14 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   }
 }
 
-/** inverse c2r along z of the three field components (blockIdx.y: 0 = x
- *  from B into rho, 1 = y from khat into Ey, 2 = z from A times i k_z into
- *  Ez); two half-spectrum rows per complex line. */
-template <typename MT>
+/** inverse c2r along z of the three field components (x from B into rho,
+ *  y from khat into Ey, z from A times i k_z into Ez), two half-spectrum
+ *  rows per complex line.  NC = 3: a block takes the same nlp line pairs of
+ *  all three components (shared index arithmetic, six loads per element in
+ *  flight, one batched LDS transform of 3 nlp lines); NC = 1: blockIdx.y
+ *  selects the component (long lines, whose three-component blocks would
+ *  not fit the LDS budget with enough lines). */
+template <typename MT, int NC>
 __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict__ sys,
                                                       int const *__restrict__ sel,
                                                       int const *__restrict__ wt, int nsel,
@@ -1014,23 +1090,24 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
   extern __shared__ double smem_d[];
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  int const comp = blockIdx.y;
   FftPlanDev const &plan = d.plan[2];
   int const n = plan.n; // == Mz
   int const ls = n + 1;
   int const rows = d.g.M[0] * d.g.M[1];
   int const Kz = d.g.Kz;
   FastDiv const fn = make_fastdiv(n), fnl = make_fastdiv(nlp);
-  C const *S = static_cast<C const *>(comp == 0 ? d.B : comp == 1 ? d.khat : d.A);
-  MT *E = static_cast<MT *>(comp == 0 ? d.rho : comp == 1 ? d.Ey : d.Ez);
-  bool const mul_ikz = (comp == 2);
+  int const c0 = (NC == 3) ? 0 : static_cast<int>(blockIdx.y);
+  C const *S[3] = {static_cast<C const *>(d.B), static_cast<C const *>(d.khat),
+                   static_cast<C const *>(d.A)};
+  MT *E[3] = {static_cast<MT *>(d.rho), static_cast<MT *>(d.Ey), static_cast<MT *>(d.Ez)};
   double const wvz = d.wv[2];
   C *twl = reinterpret_cast<C *>(smem_d);
   C *b0 = twl + n;
-  C *b1 = b0 + nlp * ls;
+  C *b1 = b0 + NC * nlp * ls;
   load_twiddles(twl, tw + d.tw_off[2], n);
   int const pair0 = L.lb * nlp;
   bool const kzm = d.kzmajor != 0;
+  int const cstride = nlp * ls; // component c0 + c occupies lines [c nlp, (c + 1) nlp)
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
     int const q0 = fdiv(e, kzm ? fnl : fn);
     int const l = kzm ? e - q0 * nlp : q0;
@@ -1040,31 +1117,45 @@ __global__ __launch_bounds__(kBlock) void k_fft_z_c2r(SysDesc const *__restrict_
     int const ks = upper ? n - k : k;
     int64_t const i0 = kzm ? (int64_t)ks * rows + r0 : r0 * Kz + ks;
     int64_t const i1 = kzm ? i0 + 1 : i0 + Kz;
-    C x1 = (r0 < rows) ? S[i0] : C{MT(0), MT(0)};
-    C x2 = (r1 < rows) ? S[i1] : C{MT(0), MT(0)};
-    if (mul_ikz) {
-      MT const kz = static_cast<MT>(ik_shift(ks, n, d.g.conv)) * static_cast<MT>(wvz);
-      x1 = C{-kz * x1.y, kz * x1.x};
-      x2 = C{-kz * x2.y, kz * x2.x};
+    bool const v0 = r0 < rows, v1 = r1 < rows;
+    C const zero{MT(0), MT(0)};
+    C x[NC][2];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      x[c][0] = v0 ? S[c0 + c][i0] : zero;
+      x[c][1] = v1 ? S[c0 + c][i1] : zero;
     }
-    if (ks == 0 || 2 * ks == n) {
-      x1.y = MT(0);
-      x2.y = MT(0);
+    MT const kz = static_cast<MT>(ik_shift(ks, n, d.g.conv)) * static_cast<MT>(wvz);
+    bool const real_row = (ks == 0 || 2 * ks == n);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      C x1 = x[c][0], x2 = x[c][1];
+      if (c0 + c == 2) { // z component: times i k_z
+        x1 = C{-kz * x1.y, kz * x1.x};
+        x2 = C{-kz * x2.y, kz * x2.x};
+      }
+      if (real_row) {
+        x1.y = MT(0);
+        x2.y = MT(0);
+      }
+      if (upper) {
+        x1 = cconj(x1);
+        x2 = cconj(x2);
+      }
+      b0[c * cstride + l * ls + k] = C{x1.x - x2.y, x1.y + x2.x};
     }
-    if (upper) {
-      x1 = cconj(x1);
-      x2 = cconj(x2);
-    }
-    b0[l * ls + k] = C{x1.x - x2.y, x1.y + x2.x};
   }
   __syncthreads();
-  C *res = lds_fft(b0, b1, nlp, ls, plan, twl, true);
+  C *res = lds_fft(b0, b1, NC * nlp, ls, plan, twl, true);
   for (int e = threadIdx.x; e < nlp * n; e += blockDim.x) {
     int const l = fdiv(e, fn), j = e - l * n;
     int64_t const r0 = 2 * (int64_t)(pair0 + l), r1 = r0 + 1;
-    C const v = res[l * ls + j];
-    if (r0 < rows) E[r0 * n + j] = v.x;
-    if (r1 < rows) E[r1 * n + j] = v.y;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      C const v = res[c * cstride + l * ls + j];
+      if (r0 < rows) E[c0 + c][r0 * n + j] = v.x;
+      if (r1 < rows) E[c0 + c][r1 * n + j] = v.y;
+    }
   }
 }
 
@@ -1300,17 +1391,22 @@ __global__ __launch_bounds__(NTH) void k_gather_layer(
         for (int k = 0; k < P; ++k) wz[k] = static_cast<MT>(w[2 * P + k]);
         double const wxi = w[li];
         int const base = s.lo[p] + li * TYZ;
+        // each stencil row summed with the z weights, then weighted by
+        // w_x w_y: three FMAs per stencil point
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-          MT const wj = static_cast<MT>(wxi * w[P + j]);
           int const row = base + j * T;
+          MT bx = wz[0] * tx[row], by = wz[0] * ty[row], bz = wz[0] * tz[row];
 #pragma unroll
-          for (int k = 0; k < P; ++k) {
-            MT const ww = wj * wz[k];
-            ax += ww * tx[row + k];
-            ay += ww * ty[row + k];
-            az += ww * tz[row + k];
+          for (int k = 1; k < P; ++k) {
+            bx += wz[k] * tx[row + k];
+            by += wz[k] * ty[row + k];
+            bz += wz[k] * tz[row + k];
           }
+          MT const wj = static_cast<MT>(wxi * w[P + j]);
+          ax += wj * bx;
+          ay += wj * by;
+          az += wj * bz;
         }
       }
       double const fx = small_segment_total<S>(static_cast<double>(ax));
@@ -1862,10 +1958,10 @@ struct Plan {
   std::vector<int> sel;
   int nsel = 0;
   int *dbuf = nullptr; // device copy of all tables below (owned by the lane)
-  int *d_sel = nullptr, *wt_bin = nullptr, *wt_fill = nullptr, *wt_z = nullptr, *wt_y = nullptr,
-      *wt_x = nullptr, *wt_yi = nullptr, *wt_f = nullptr;
-  int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_y = 0, blocks_x = 0, blocks_yi = 0,
-      blocks_f = 0;
+  int *d_sel = nullptr, *wt_bin = nullptr, *wt_fill = nullptr, *wt_z = nullptr, *wt_zc = nullptr,
+      *wt_y = nullptr, *wt_x = nullptr, *wt_yi = nullptr, *wt_f = nullptr;
+  int blocks_bin = 0, blocks_fill = 0, blocks_z = 0, blocks_zc = 0, blocks_y = 0, blocks_x = 0,
+      blocks_yi = 0, blocks_f = 0;
   int *soff = nullptr; // per system id: first particle of the system in the lane scratch
   int64_t npart = 0;   // particles of the selection
   struct Group {
@@ -1875,8 +1971,8 @@ struct Plan {
     int blocks;
   };
   std::vector<Group> groups;
-  int nlp_z = 1, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
-  size_t lds_z = 0, lds_y = 0, lds_x = 0, lds_yi = 0, lds_f = 0;
+  int nlp_z = 1, nlp_zc = 1, c2r_comps = 3, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
+  size_t lds_z = 0, lds_zc = 0, lds_y = 0, lds_x = 0, lds_yi = 0, lds_f = 0;
   bool any_dipole = false;
   // graph
   hipGraphExec_t graph = nullptr;
@@ -2240,7 +2336,8 @@ template <typename MT> void append_twiddles(BatchSolver::Impl *m, int n, hipStre
 
 template <typename MT> void set_kernel_attrs() {
   allow_lds(k_fft_z_r2c<MT>);
-  allow_lds(k_fft_z_c2r<MT>);
+  allow_lds(k_fft_z_c2r<MT, 1>);
+  allow_lds(k_fft_z_c2r<MT, 3>);
   allow_lds(k_fft_y<MT>);
   allow_lds(k_fft_x_conv<MT>);
   allow_lds(k_fft_y_inv<MT>);
@@ -2569,6 +2666,11 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
   }
   size_t const budget = 64 * 1024;
   p.nlp_z = pick_lines(max_mz, csz, 2, shape.zlines, budget);
+  // c2r: three components per block when as many lines fit as the other
+  // z pass takes, else one component per block (blockIdx.y)
+  p.nlp_zc = pick_lines(max_mz, csz, 6, shape.zlines, budget);
+  p.c2r_comps = (p.nlp_zc >= p.nlp_z) ? 3 : 1;
+  if (p.c2r_comps == 1) p.nlp_zc = p.nlp_z;
   p.ncol_y = pick_lines(max_my, csz, 2, shape.ncol, budget);
   p.ncol_x = pick_lines(max_mx, csz, 4, shape.ncol, budget);
   p.ncol_yi = pick_lines(max_my, csz, 3, shape.ncol, budget);
@@ -2576,10 +2678,11 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
     return static_cast<size_t>(n) * csz + static_cast<size_t>(lines) * (n + 1) * csz;
   };
   p.lds_z = lds_lines(max_mz, 2 * p.nlp_z);
+  p.lds_zc = lds_lines(max_mz, 2 * p.c2r_comps * p.nlp_zc);
   p.lds_y = lds_lines(max_my, 2 * p.ncol_y);
   p.lds_x = lds_lines(max_mx, 4 * p.ncol_x);
   p.lds_yi = lds_lines(max_my, 3 * p.ncol_yi);
-  for (size_t b : {p.lds_z, p.lds_y, p.lds_x, p.lds_yi, p.lds_f})
+  for (size_t b : {p.lds_z, p.lds_zc, p.lds_y, p.lds_x, p.lds_yi, p.lds_f})
     if (b > 128 * 1024) throw std::runtime_error("P3M: mesh too large for the LDS budget");
 
   // tables: sel, then prefix arrays (each nsel+1), then per-cao groups
@@ -2602,10 +2705,14 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
     int64_t const R = (int64_t)h.desc.g.M[0] * h.desc.g.M[1] * h.desc.g.M[2];
     return static_cast<int>((R + kFill - 1) / kFill);
   });
-  int const nlp = p.nlp_z, ncy = p.ncol_y, ncx = p.ncol_x, ncyi = p.ncol_yi;
+  int const nlp = p.nlp_z, nlpc = p.nlp_zc, ncy = p.ncol_y, ncx = p.ncol_x, ncyi = p.ncol_yi;
   auto const fz = push_prefix([nlp](HostSys const &h) {
     int const pairs = (h.desc.g.M[0] * h.desc.g.M[1] + 1) / 2;
     return (pairs + nlp - 1) / nlp;
+  });
+  auto const fzc = push_prefix([nlpc](HostSys const &h) {
+    int const pairs = (h.desc.g.M[0] * h.desc.g.M[1] + 1) / 2;
+    return (pairs + nlpc - 1) / nlpc;
   });
   auto const fy = push_prefix([ncy](HostSys const &h) {
     return h.desc.kzmajor ? 0 : ((h.desc.g.Kz + ncy - 1) / ncy) * h.desc.g.M[0];
@@ -2678,6 +2785,8 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
   p.blocks_fill = fill.second;
   p.wt_z = p.dbuf + fz.first;
   p.blocks_z = fz.second;
+  p.wt_zc = p.dbuf + fzc.first;
+  p.blocks_zc = fzc.second;
   p.wt_y = p.dbuf + fy.first;
   p.blocks_y = fy.second;
   p.wt_x = p.dbuf + fx.first;
@@ -2757,8 +2866,8 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     }
     dispatch_cao(g.cao, [&](auto P) {
       with_threads(shape.threads, [&](auto NT) {
-        size_t const lds =
-            chunk_bytes(P.value, shape.chunk) + tile_points(P.value) * sizeof(unsigned long long);
+        size_t const lds = chunk_bytes(P.value, shape.chunk) +
+                           tile_points(P.value) * sizeof(typename SpreadWord<MT>::type);
         auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
                                           static_cast<double4 const *>(ln.spos), shape.chunk,
                                           static_cast<double const *>(ln.sums), det ? 1 : 0);
@@ -2789,14 +2898,16 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
   if (ev) (void)hipEventRecord(ev[3], st);
   if (p.blocks_x > 0) {
     k_fft_x_conv<MT><<<p.blocks_x, kBlock, p.lds_x, st>>>(sys, p.d_sel, p.wt_x, ns, tw, p.ncol_x,
-                                                          want_force ? 1 : 0, ln.conv_part);
+                                                          want_force ? 1 : 0, want_sums ? 1 : 0,
+                                                          ln.conv_part);
     if (want_sums)
       k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_x, ns, ln.conv_part, 8, 8, ln.sums, 1);
   }
   if (p.blocks_f > 0) {
     double *fpart = ln.conv_part + 8 * static_cast<size_t>(p.blocks_x);
     k_fft_yx_conv<MT><<<p.blocks_f, kBlock, p.lds_f, st>>>(sys, p.d_sel, p.wt_f, ns, tw,
-                                                           want_force ? 1 : 0, fpart);
+                                                           want_force ? 1 : 0, want_sums ? 1 : 0,
+                                                           fpart);
     if (want_sums)
       k_reduce_sys<<<ns, 1024, 0, st>>>(p.d_sel, p.wt_f, ns, fpart, 8, 8, ln.sums, 1);
   }
@@ -2805,8 +2916,13 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     if (p.blocks_yi > 0)
       k_fft_y_inv<MT><<<p.blocks_yi, kBlock, p.lds_yi, st>>>(sys, p.d_sel, p.wt_yi, ns, tw,
                                                               p.ncol_yi);
-    dim3 const gz(p.blocks_z, 3);
-    k_fft_z_c2r<MT><<<gz, kBlock, p.lds_z, st>>>(sys, p.d_sel, p.wt_z, ns, tw, p.nlp_z);
+    if (p.c2r_comps == 3) {
+      k_fft_z_c2r<MT, 3><<<p.blocks_zc, kBlock, p.lds_zc, st>>>(sys, p.d_sel, p.wt_zc, ns, tw,
+                                                                p.nlp_zc);
+    } else {
+      dim3 const gz(p.blocks_zc, 3);
+      k_fft_z_c2r<MT, 1><<<gz, kBlock, p.lds_zc, st>>>(sys, p.d_sel, p.wt_zc, ns, tw, p.nlp_zc);
+    }
   }
   if (ev) (void)hipEventRecord(ev[5], st);
   if (want_force) {

@@ -1515,7 +1515,8 @@ char const *p3m_serve_last_error(p3m_server const *s) {
 }
 
 int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
-                   int mesh_max, int cao_min, int cao_max, double pair_ns) {
+                   int mesh_max, int cao_min, int cao_max, double pair_ns,
+                   int single_precision) {
   if (!inout) return P3M_ERR_INVALID;
   return guarded(static_cast<std::string *>(nullptr), [&] {
     P3MParams params(true, inout->epsilon, inout->r_cut, v3i(inout->mesh), v3(inout->mesh_off),
@@ -1556,7 +1557,8 @@ int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min
       } catch (std::runtime_error const &) {
         return -1.;
       }
-      return model_step_ms(mesh, cao, trial.r_cut, n_charged, box, pair_ns > 0. ? pair_ns : 1.);
+      return model_step_ms(mesh, cao, trial.r_cut, n_charged, box, pair_ns > 0. ? pair_ns : 1.,
+                           single_precision != 0);
     };
     tune_p3m(params, in, timer);
     for (int d = 0; d < 3; ++d) {

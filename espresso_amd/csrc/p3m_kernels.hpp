@@ -171,10 +171,13 @@ __device__ __forceinline__ int fdiv(int e, FastDiv f) {
   return f.d == 1 ? e : static_cast<int>(__umulhi(static_cast<unsigned>(e), f.m));
 }
 
-/** Periodic mesh index of tile row/column v = o + a >= 0 (tiles may be
- *  wider than the mesh for tiny meshes, so wrap as often as needed). */
+/** Periodic mesh index of tile row/column v = o + a >= 0: one conditional
+ *  subtraction (v < 2M, every mesh at least as wide as a tile), a remainder
+ *  for the rare lanes of tiny meshes whose tiles wrap more than once. */
 __device__ __forceinline__ int wrap_once(int v, int M) {
-  while (v >= M) v -= M;
+  int const w = v - M;
+  v = (w < 0) ? v : w;
+  if (v >= M) v %= M;
   return v;
 }
 
@@ -358,6 +361,21 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *scratch) {
     }
   }
   __syncthreads();
+}
+
+/** Maximum of v over the block (blockDim.x multiple of 64, <= 1024); the
+ *  result is valid in thread 0.  `scratch` needs 16 doubles. */
+__device__ __forceinline__ double block_max(double v, double *scratch) {
+  int const lane = threadIdx.x & 63;
+  int const wid = threadIdx.x >> 6;
+  int const nw = (blockDim.x + 63) >> 6;
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if (lane == 0) scratch[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; ++w) v = fmax(v, scratch[w]);
+  __syncthreads();
+  return v;
 }
 
 // ---------------------------------------------------------------------------

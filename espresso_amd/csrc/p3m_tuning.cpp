@@ -305,17 +305,20 @@ double fft_stage_cost(int n) {
 }
 
 double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l,
-                     double pair_ns) {
-  // device k-space cost from the measured stage rates of the solver: about
-  // 1.3 ns per particle stencil point (spread + gather) and 0.12 ns per mesh
-  // point (FFTs and convolution) for three-stage lines (n = 128), half of it
-  // scaling with the FFT stage cost of the mesh, plus a fixed launch cost
+                     double pair_ns, bool single_precision) {
+  // device k-space cost from the measured stage rates of the solver, per
+  // particle stencil point (spread + gather: 1.5 ps FP64 mesh, 1.8 ps FP32
+  // mesh, whose systems are small and sparse) and per mesh point (FFTs and
+  // convolution: 74 ps FP64, 40 ps FP32) for three-stage lines (n = 128),
+  // half of it scaling with the FFT stage cost of the mesh, plus a fixed
+  // launch cost
   double const R = static_cast<double>(mesh[0]) * mesh[1] * mesh[2];
   double const stencil = static_cast<double>(n_q) * cao * cao * cao;
   double const stages =
       (fft_stage_cost(mesh[0]) + fft_stage_cost(mesh[1]) + fft_stage_cost(mesh[2])) / 3.;
-  double const mesh_ns = 0.12 * (0.5 + 0.5 * stages / 3.);
-  return 1.3e-9 * stencil + 1e-6 * mesh_ns * R + 0.01 +
+  double const stencil_ps = single_precision ? 1.8 : 1.5;
+  double const mesh_ps = (single_precision ? 40. : 74.) * (0.5 + 0.5 * stages / 3.);
+  return 1e-9 * (stencil_ps * stencil + mesh_ps * R) + 0.01 +
          model_real_space_ms(n_q, r_cut, box_l, pair_ns);
 }
 

@@ -32,11 +32,11 @@ struct TuningInput {
 /** Relative cost per point of the device FFT along a line of length n. */
 double fft_stage_cost(int n);
 
-/** Modelled cost (ms) of one k-space solve on the device plus the real-space
- *  pair loop at `pair_ns` nanoseconds per pair, for tuning without timing
- *  runs. */
+/** Modelled cost (ms) of one k-space solve on the device (FP64 or FP32 mesh)
+ *  plus the real-space pair loop at `pair_ns` nanoseconds per pair, for
+ *  tuning without timing runs. */
 double model_step_ms(Vec3i const &mesh, int cao, double r_cut, int n_q, Vec3d const &box_l,
-                     double pair_ns = 1.);
+                     double pair_ns = 1., bool single_precision = false);
 
 /** Returns the cost of one step in ms for a candidate (mesh, cao, r_cut_iL,
  *  alpha_L), or a negative value when the candidate cannot be timed. */

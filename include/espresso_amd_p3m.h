@@ -520,12 +520,14 @@ const char *p3m_serve_last_error(const p3m_server *s);
 
 /* Parameter tuning without a device: the reference's tuning algorithm
  * (p3m.cpp:594-791, TuningAlgorithm.cpp:57-328) with a modelled cost in place
- * of timed integration steps (device k-space stage rates plus a real-space
- * loop at pair_ns nanoseconds per pair; pair_ns <= 0 selects 1 ns), meshes
- * restricted to [mesh_min, mesh_max] and orders to [cao_min, cao_max].
+ * of timed integration steps (device k-space stage rates of an FP64 mesh, or
+ * of an FP32 mesh if single_precision, plus a real-space loop at pair_ns
+ * nanoseconds per pair; pair_ns <= 0 selects 1 ns), meshes restricted to
+ * [mesh_min, mesh_max] and orders to [cao_min, cao_max].
  * Fills mesh, cao, r_cut, alpha, accuracy. */
 int p3m_tune_model(p3m_params *inout, int n_charged, double sum_q2, int mesh_min,
-                   int mesh_max, int cao_min, int cao_max, double pair_ns);
+                   int mesh_max, int cao_min, int cao_max, double pair_ns,
+                   int single_precision);
 
 #ifdef __cplusplus
 }

@@ -6,8 +6,9 @@ FFTs and the reductions have a fixed order.  Repeated solves of the same
 input must then give identical bits (np.array_equal), also when the
 particles arrive in another order (forces, metallic boundary) and when a
 system is solved alone or in a mix of systems.  The rounding of the mesh
-values (2^-s, s = 61 - ceil(log2 sum|q|)) keeps the results at the FP64 bar
-against the oracle (forces rel-rms <= 1e-6, energy rel <= 1e-8).
+values (2^-s, s = min(50 - a, 62 - b) for max|q| < 2^a and sum|q| < 2^b)
+keeps the results at the FP64 bar against the oracle (forces rel-rms <= 1e-6,
+energy rel <= 1e-8).
 """
 import numpy as np
 import pytest
