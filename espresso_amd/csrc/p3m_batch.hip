@@ -62,7 +62,8 @@ using namespace dev;
 
 constexpr int kBlock = 256;  // block size of the mesh and particle-list kernels
 constexpr int kChunkMax = 64; // most particles staged per brick pass (3 x chunk <= threads)
-constexpr int kCountCopies = 8; // brick counter copies (one per XCD-sized share of the blocks)
+constexpr int kCountCopies = 8; // brick counter copies (one per XCD-sized share of the blocks;
+                                // k_scan reads a brick's copies as two 16-byte words)
 constexpr int kBrick = 8;    // first-stencil points per brick edge
 constexpr int kFill = 1024;  // mesh elements zeroed per block
 
@@ -299,15 +300,29 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
   int const lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int base = 0; base < nb; base += blockDim.x * kScanItems) {
     int const i0 = base + threadIdx.x * kScanItems;
+    // a brick's kCountCopies = 8 counters are 32 contiguous, 32-byte
+    // aligned bytes: two 16-byte loads
     int cc[kScanItems][kCountCopies];
     int c = 0;
 #pragma unroll
-    for (int it = 0; it < kScanItems; ++it)
-#pragma unroll
-      for (int k = 0; k < kCountCopies; ++k) {
-        cc[it][k] = (i0 + it < nb) ? count[(i0 + it) * kCountCopies + k] : 0;
-        c += cc[it][k];
+    for (int it = 0; it < kScanItems; ++it) {
+      int4 lo4 = {0, 0, 0, 0}, hi4 = {0, 0, 0, 0};
+      if (i0 + it < nb) {
+        int4 const *src = reinterpret_cast<int4 const *>(count + (i0 + it) * kCountCopies);
+        lo4 = src[0];
+        hi4 = src[1];
       }
+      cc[it][0] = lo4.x;
+      cc[it][1] = lo4.y;
+      cc[it][2] = lo4.z;
+      cc[it][3] = lo4.w;
+      cc[it][4] = hi4.x;
+      cc[it][5] = hi4.y;
+      cc[it][6] = hi4.z;
+      cc[it][7] = hi4.w;
+#pragma unroll
+      for (int k = 0; k < kCountCopies; ++k) c += cc[it][k];
+    }
     int sc = c;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -330,11 +345,15 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
     for (int it = 0; it < kScanItems; ++it) {
       if (i0 + it < nb) {
         offset[i0 + it] = o;
+        int st[kCountCopies];
 #pragma unroll
         for (int k = 0; k < kCountCopies; ++k) {
-          count[(i0 + it) * kCountCopies + k] = o;
+          st[k] = o;
           o += cc[it][k];
         }
+        int4 *dst = reinterpret_cast<int4 *>(count + (i0 + it) * kCountCopies);
+        dst[0] = int4{st[0], st[1], st[2], st[3]};
+        dst[1] = int4{st[4], st[5], st[6], st[7]};
       }
     }
     __syncthreads();
