@@ -91,6 +91,10 @@ def make_c4(n_sys=512, seed0=4000):
 # N = 10^6 workload (r_cut 8.558, 5.3e8 pairs) took 11.6 ms on one MI355X;
 # bench.py measures it again in its short-range leg and tunes with that rate
 DEVICE_PAIR_NS = 0.022
+# the 512-system batch is tuned with a margin below its 1e-4 rms force error
+# target: the P3M error estimate runs 3-8% below the error measured against
+# exact Ewald sums on these systems
+C4_TUNE_ACCURACY = 0.9e-4
 
 
 def model_pairs(n_q, L, r_cut):
@@ -126,7 +130,7 @@ def bench_batched(steps, warmup, torch, dev, pair_ns=None):
     from espresso_amd.electrostatics import P3M
     systems = make_c4()
     t0 = time.perf_counter()
-    tuned = tune_systems(systems, 1e-4, (16, 48), (3, 7), pair_ns=pair_ns)
+    tuned = tune_systems(systems, C4_TUNE_ACCURACY, (16, 48), (3, 7), pair_ns=pair_ns)
     t_tune = time.perf_counter() - t0
     n_tot = sum(s["n"] for s in systems)
     print(f"[bench] batched: {len(systems)} systems, {n_tot} particles, tuned "
@@ -238,7 +242,7 @@ def bench_batched(steps, warmup, torch, dev, pair_ns=None):
     caos = [t["cao"] for t in tuned]
     res = {
         "workload": "512 systems, N log-uniform [500, 20000], density U[0.1, 0.8], "
-                    "+-1 alternating, per-system tuning at 1e-4 (mesh 16-48, cao 3-7)",
+                    "+-1 alternating, per-system tuning at 0.9e-4 for a measured rms force error <= 1e-4 (mesh 16-48, cao 3-7)",
         "n_systems": len(systems), "n_particles": n_tot,
         "mesh_range": [min(meshes), max(meshes)], "cao_range": [min(caos), max(caos)],
         "max_estimated_rms_force_error": max(t["accuracy"] for t in tuned),

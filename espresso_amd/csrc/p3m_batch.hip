@@ -556,28 +556,72 @@ template <> struct SpreadWord<float> {
   using type = unsigned int;
 };
 
-/** Scale 2^s of a brick's 32-bit fixed-point tile: the brick's n particles
- *  of |q| <= max_abs_q sum to below 2^e, so every tile point and every
- *  contribution stays below 2^(e+s) = 2^30.  The resolution 2^-s (2^-24 for
- *  a brick of 49 unit charges) is far below the FP32 mesh's own rounding
+/** Upper bounds of w^3 for the weights w of an order-P charge assignment:
+ *  the cube of the peak of the cardinal B-spline of order P (1, 1, 3/4, 2/3,
+ *  115/192, 11/20, 5887/11520 for P = 1..7), rounded up. */
+__constant__ double kPeakCube[8] = {1., 1., 1., 0.4219, 0.2963, 0.2149, 0.1664, 0.1335};
+
+/** Scale 2^s of a brick's 32-bit fixed-point tile: each of the brick's n
+ *  particles (|q| <= max_abs_q) adds at most |q| kPeakCube[P] to a tile
+ *  point, so every tile point and contribution stays below 2^(e+s) = 2^30
+ *  for n max_abs_q kPeakCube[P] < 2^e.  The resolution 2^-s (2^-27 for a
+ *  brick of 49 unit charges at P = 7) is below the FP32 mesh's own rounding
  *  once the tile is flushed. */
-__device__ __forceinline__ double brick_scale32(int n, double max_abs_q) {
+__device__ __forceinline__ double brick_scale32(int n, double max_abs_q, int P) {
   int e = 0;
-  (void)frexp(static_cast<double>(n) * max_abs_q, &e);
+  (void)frexp(static_cast<double>(n) * max_abs_q * kPeakCube[P], &e);
   return ldexp(1., 30 - e);
 }
 
-/** Charge assignment of one brick: contributions are rounded once to fixed
- *  point (fixed_round; 64-bit at fixed_scale of the system's charges for
- *  FP64 meshes, 32-bit at brick_scale32 for FP32 meshes) and summed in the
- *  LDS tile with integer atomics (faster than FP atomics on this pattern,
- *  and integer sums do not depend on the order).  The tile is flushed into
- *  the periodic mesh with device atomics: as 64-bit integers into a
- *  fixed-point mesh (int_mesh, FP64 meshes: the whole charge assignment is
- *  then independent of the order of particles, bricks and atomics), or
- *  converted to the mesh type.  Lanes over the (j, k) stencil plane in
- *  segments of up to SEGMAX lanes per particle. */
-template <int P, typename MT, int SEGMAX, int NTH>
+/** Tile rows of an SX x SY x SZ group of bricks over a block of NTH threads:
+ *  LPR lanes per row of TZ points (z), rows r = a TY + b.  One periodic
+ *  wrap per coordinate. */
+template <int P, int NTH, int SX, int SY, int SZ> struct GroupRows {
+  static constexpr int TX = SX * kBrick + P - 1, TY = SY * kBrick + P - 1,
+                       TZ = SZ * kBrick + P - 1;
+  static constexpr int LPR = (TZ <= 16) ? 16 : 32;
+  static_assert(TZ <= 32, "tile rows wider than 32 lanes");
+  static constexpr int RPP = NTH / LPR;
+  static constexpr int ROWS = TX * TY;
+  static constexpr int NPASS = (ROWS + RPP - 1) / RPP;
+  int c, row0, zz;
+  bool lane_ok;
+  __device__ GroupRows(int o2, int M2) {
+    c = threadIdx.x % LPR;
+    row0 = threadIdx.x / LPR;
+    lane_ok = c < TZ;
+    zz = wrap_once(o2 + c, M2);
+  }
+  __device__ bool valid(int k) const { return lane_ok && row0 + k * RPP < ROWS; }
+  __device__ int row(int k) const { return row0 + k * RPP; }
+  __device__ int64_t mesh_index(int k, int o0, int o1, int M0, int M1, int M2) const {
+    int const r = row(k);
+    int const a = r / TY;
+    int const b = r - a * TY;
+    return ((int64_t)wrap_once(o0 + a, M0) * M1 + wrap_once(o1 + b, M1)) * M2 + zz;
+  }
+};
+
+/** Blocks of the spread of a system with bricks nb[3] grouped SX x SY x SZ. */
+__host__ __device__ inline int brick_groups(int const nb[3], int sx, int sy, int sz) {
+  return ((nb[0] + sx - 1) / sx) * ((nb[1] + sy - 1) / sy) * ((nb[2] + sz - 1) / sz);
+}
+
+/** Charge assignment of one group of SX x SY x SZ bricks (one brick by
+ *  default; a group shares one tile and its halo, which pays for sparse
+ *  bricks; a 32-bit tile's resolution falls with the group's particle
+ *  count):
+ *  contributions are rounded once to fixed point (fixed_round; 64-bit at
+ *  fixed_scale of the system's charges for FP64 meshes, 32-bit at
+ *  brick_scale32 for FP32 meshes) and summed in the LDS tile with integer
+ *  atomics (faster than FP atomics on this pattern, and integer sums do not
+ *  depend on the order).  The tile is flushed into the periodic mesh with
+ *  device atomics: as 64-bit integers into a fixed-point mesh (int_mesh,
+ *  FP64 meshes: the whole charge assignment is then independent of the order
+ *  of particles, bricks and atomics), or converted to the mesh type.  Lanes
+ *  over the (j, k) stencil plane in segments of up to SEGMAX lanes per
+ *  particle. */
+template <int P, typename MT, int SEGMAX, int NTH, int SX = 1, int SY = 1, int SZ = 1>
 __global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc const *__restrict__ sys,
                                                int const *__restrict__ sel,
                                                int const *__restrict__ wt, int nsel,
@@ -585,9 +629,11 @@ __global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc co
                                                int const *__restrict__ offsets,
                                                double4 const *__restrict__ spos, int chunk,
                                                double const *__restrict__ sums, int int_mesh) {
-  constexpr int T = kBrick + P - 1;
-  constexpr int TYZ = T * T;
-  constexpr int tile_n = T * TYZ;
+  using GR = GroupRows<P, NTH, SX, SY, SZ>;
+  constexpr int TY = GR::TY, TZ = GR::TZ;
+  constexpr int TYZ = TY * TZ;
+  constexpr int tile_n = GR::TX * TYZ;
+  constexpr int NSUB = SX * SY * SZ;
   extern __shared__ double smem_d[];
   ChunkSlots s;
   s.w = smem_d;
@@ -596,16 +642,30 @@ __global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc co
   using TT = typename SpreadWord<MT>::type;
   TT *tile = reinterpret_cast<TT *>(reinterpret_cast<char *>(smem_d) + chunk_slot_bytes(P, chunk));
 
+  // the block's group of SX x SY x SZ bricks and their particle runs
   Slot const L = locate(wt, sel, nsel);
   SysDesc const &d = sys[L.s];
-  int const b = L.lb;
-  int const beg = offsets[d.off_off + b], end = offsets[d.off_off + b + 1];
-  if (beg == end) return;
+  int const nb0 = d.g.nb[0], nb1 = d.g.nb[1], nb2 = d.g.nb[2];
+  int const ng2 = (nb2 + SZ - 1) / SZ, ng1 = (nb1 + SY - 1) / SY;
+  int const gz = L.lb % ng2, gy = (L.lb / ng2) % ng1, gx = L.lb / (ng2 * ng1);
+  int const o0 = gx * SX * kBrick, o1 = gy * SY * kBrick, o2 = gz * SZ * kBrick;
+  int rb[NSUB], re[NSUB];
+  int ntot = 0;
+#pragma unroll
+  for (int u = 0; u < NSUB; ++u) {
+    int const bx = gx * SX + u / (SY * SZ), by = gy * SY + (u / SZ) % SY, bz = gz * SZ + u % SZ;
+    rb[u] = re[u] = 0;
+    if (bx < nb0 && by < nb1 && bz < nb2) {
+      int const b = (bx * nb1 + by) * nb2 + bz;
+      rb[u] = offsets[d.off_off + b];
+      re[u] = offsets[d.off_off + b + 1];
+    }
+    ntot += re[u] - rb[u];
+  }
+  if (ntot == 0) return;
   double const scale = (sizeof(TT) == 8)
                            ? fixed_scale(sums[L.s * 16 + 6], sums[L.s * 16 + 7])
-                           : brick_scale32(end - beg, sums[L.s * 16 + 7]);
-  int o0, o1, o2;
-  brick_origin<P>(d, b, o0, o1, o2);
+                           : brick_scale32(ntot, sums[L.s * 16 + 7], P);
   double4 const *sp = spos + soff[L.s];
   constexpr int S = (Seg<P>::S < SEGMAX) ? Seg<P>::S : SEGMAX;
   constexpr int ppw = 64 / S;
@@ -617,32 +677,37 @@ __global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc co
   int const sub = lane / S;
   int const r0 = lane - sub * S;
   for (int e = threadIdx.x; e < tile_n; e += blockDim.x) tile[e] = TT(0);
-  for (int c0 = beg; c0 < end; c0 += chunk) {
-    int const cnt = min(chunk, end - c0);
-    if (c0 != beg) __syncthreads(); // the tile zeroing and the slots are disjoint
-    stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, T, T, s);
-    __syncthreads();
-    int const ngroups = (cnt + ppw - 1) / ppw;
-    for (int grp = wave; grp < ngroups; grp += nwaves) {
-      int const p = grp * ppw + sub;
-      if (p < cnt) {
-        double const *w = s.w + p * 3 * P;
-        double const qs = s.qq[p] * scale;
-        int const lo = s.lo[p];
-        // x weights in registers: the LDS atomics below would otherwise make
-        // the compiler re-read them from LDS after every atomic
-        double wx[P];
+  bool first = true;
+#pragma unroll 1
+  for (int u = 0; u < NSUB; ++u) {
+    for (int c0 = rb[u]; c0 < re[u]; c0 += chunk) {
+      int const cnt = min(chunk, re[u] - c0);
+      if (!first) __syncthreads(); // the tile zeroing and the slots are disjoint
+      first = false;
+      stage_chunk<P>(d.g, sp, c0, cnt, o0, o1, o2, TY, TZ, s);
+      __syncthreads();
+      int const ngroups = (cnt + ppw - 1) / ppw;
+      for (int grp = wave; grp < ngroups; grp += nwaves) {
+        int const p = grp * ppw + sub;
+        if (p < cnt) {
+          double const *w = s.w + p * 3 * P;
+          double const qs = s.qq[p] * scale;
+          int const lo = s.lo[p];
+          // x weights in registers: the LDS atomics below would otherwise
+          // make the compiler re-read them from LDS after every atomic
+          double wx[P];
 #pragma unroll
-        for (int i = 0; i < P; ++i) wx[i] = w[i];
+          for (int i = 0; i < P; ++i) wx[i] = w[i];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          int const r = r0 + t * S;
-          if (r < NPAIR) {
-            int const jj = r / P, kk = r - (r / P) * P;
-            double const wyzq = (w[P + jj] * w[2 * P + kk]) * qs;
-            TT *dst = tile + lo + jj * T + kk;
+          for (int t = 0; t < NT; ++t) {
+            int const r = r0 + t * S;
+            if (r < NPAIR) {
+              int const jj = r / P, kk = r - (r / P) * P;
+              double const wyzq = (w[P + jj] * w[2 * P + kk]) * qs;
+              TT *dst = tile + lo + jj * TZ + kk;
 #pragma unroll
-            for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, fixed_round<TT>(wx[i], wyzq));
+              for (int i = 0; i < P; ++i) atomicAdd(dst + i * TYZ, fixed_round<TT>(wx[i], wyzq));
+            }
           }
         }
       }
@@ -651,11 +716,11 @@ __global__ __launch_bounds__(NTH, SEGMAX == 64 ? 8 : 1) void k_spread(SysDesc co
   __syncthreads();
   int const M0 = d.g.M[0], M1 = d.g.M[1], M2 = d.g.M[2];
   double const inv_scale = 1. / scale;
-  TileRows<P, NTH> const tr(o2, M2);
+  GR const tr(o2, M2);
 #pragma unroll
-  for (int k = 0; k < TileRows<P, NTH>::NPASS; ++k) {
+  for (int k = 0; k < GR::NPASS; ++k) {
     if (tr.valid(k)) {
-      TT const v = tile[tr.row(k) * T + tr.c];
+      TT const v = tile[tr.row(k) * TZ + tr.c];
       if (v == TT(0)) continue;
       int64_t const g = tr.mesh_index(k, o0, o1, M0, M1, M2);
       double const sv = (sizeof(TT) == 8) ? static_cast<double>(static_cast<long long>(v))
@@ -1930,9 +1995,9 @@ template <typename Kern> void allow_lds(Kern k) {
 size_t chunk_bytes(int cao, int chunk) { return chunk_slot_bytes(cao, chunk); }
 /** Ring elements of one field of a column block. */
 int ring_points(int cao, int w) { return (w + cao - 1) * ring_pitch(w + cao - 1, cao); }
-int tile_points(int cao) {
-  int const t = kBrick + cao - 1;
-  return t * t * t;
+int tile_points(int cao, int group = 1) {
+  int const t = kBrick + cao - 1, t2 = 2 * kBrick + cao - 1;
+  return group == 4 ? t2 * t2 * t : t * t * t;
 }
 
 /** LDS of the fused y/x pass: both twiddle tables and two x-y planes. */
@@ -1988,6 +2053,8 @@ struct Plan {
     int nsel;
     int *d_sel, *wt;
     int blocks;
+    int *wt_sp;    // spread work table (brick groups of LaunchShape::group)
+    int blocks_sp;
   };
   std::vector<Group> groups;
   int nlp_z = 1, nlp_zc = 1, c2r_comps = 3, ncol_y = 1, ncol_x = 1, ncol_yi = 1;
@@ -2192,8 +2259,14 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
     throw std::invalid_argument("P3M: invalid column-march shape");
   if (m_det && s.column)
     throw std::invalid_argument("P3M: deterministic solves need 8^3 bricks (column = 0)");
+  if ((s.group != 1 && s.group != 4) ||
+      (s.group == 4 && (s.column != 0 || s.spread != kSpreadSeg64)))
+    throw std::invalid_argument(
+        "P3M: spread brick groups are 1 or 4, the latter with 8^3 bricks and the plane spread "
+        "mapping");
   hip_check(hipSetDevice(m_device), "hipSetDevice");
   bool const rebin = s.column != m_shape.column || s.seg != m_shape.seg;
+  bool const regroup = s.group != m_shape.group;
   m_shape = s;
   m_shape.ncol = std::max(1, std::min(64, s.ncol));
   m_shape.zlines = std::max(1, std::min(64, s.zlines));
@@ -2202,6 +2275,8 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   if (rebin) { // particle bins and block counts of every system
     for (auto &h : m->sys) set_bins(h, m_shape);
     recompute_offsets(m);
+  } else if (regroup) {
+    m->version++; // spread work tables change
   }
   // systems whose y/x pass changes switch spectrum layout: recompute G
   bool changed = false;
@@ -2291,8 +2366,17 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     if (v == res.best.spread) continue;
     LaunchShape c = res.best;
     c.spread = v;
+    c.group = 1;
     consider(c);
   }
+  // spread brick groups sharing one tile (sparse bricks)
+  if (res.best.spread == kSpreadSeg64 && res.best.column == 0)
+    for (int v : {1, 4}) {
+      if (v == res.best.group) continue;
+      LaunchShape c = res.best;
+      c.group = v;
+      consider(c);
+    }
   try_blocks();
   // column march: width and lanes per particle, then x layers per block
   if (!m_det) {
@@ -2303,6 +2387,7 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
         LaunchShape c = base;
         c.column = w;
         c.col_lanes = l;
+        c.group = 1;
         consider(c);
       }
     if (res.best.column)
@@ -2365,6 +2450,7 @@ template <typename MT> void set_kernel_attrs() {
     dispatch_cao(cao, [&](auto P) {
       for_threads([&](auto NT) {
         allow_lds(k_spread<P.value, MT, 64, NT.value>);
+        allow_lds(k_spread<P.value, MT, 64, NT.value, 2, 2, 1>);
         allow_lds(k_spread<P.value, MT, 16, NT.value>);
         allow_lds(k_spread<P.value, MT, 8, NT.value>);
         allow_lds(k_gather_plane<P.value, MT, 64, NT.value>);
@@ -2744,15 +2830,16 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
   });
   auto const ff = push_prefix([](HostSys const &h) { return h.desc.kzmajor ? h.desc.g.Kz : 0; });
   struct GH {
-    int cao, sel_start, nsel, wt_start, blocks;
+    int cao, sel_start, nsel, wt_start, blocks, wsp_start, blocks_sp;
   };
+  int const gsx = 2, gsy = 2, gsz = 1; // brick groups of the spread (LaunchShape::group 4)
   std::vector<GH> gh;
   for (int cao = 1; cao <= 7; ++cao) {
     std::vector<int> gs;
     for (int s : sel)
       if (m->sys[s].desc.cao == cao) gs.push_back(s);
     if (gs.empty()) continue;
-    GH g{cao, static_cast<int>(host.size()), static_cast<int>(gs.size()), 0, 0};
+    GH g{cao, static_cast<int>(host.size()), static_cast<int>(gs.size()), 0, 0, 0, 0};
     host.insert(host.end(), gs.begin(), gs.end());
     g.wt_start = static_cast<int>(host.size());
     int acc = 0;
@@ -2762,6 +2849,18 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
       host.push_back(acc);
     }
     g.blocks = acc;
+    g.wsp_start = g.wt_start;
+    g.blocks_sp = g.blocks;
+    if (shape.group > 1 && shape.column == 0) {
+      g.wsp_start = static_cast<int>(host.size());
+      acc = 0;
+      host.push_back(0);
+      for (int s : gs) {
+        acc += brick_groups(m->sys[s].desc.g.nb, gsx, gsy, gsz);
+        host.push_back(acc);
+      }
+      g.blocks_sp = acc;
+    }
     gh.push_back(g);
   }
   // per-system offsets of the selection's particles in the lane scratch
@@ -2817,7 +2916,8 @@ void build_plan(BatchSolver::Impl *m, Lane &ln, std::vector<int> const &sel, Mes
   p.soff = p.dbuf + soff_start;
   p.npart = npart;
   for (auto const &g : gh)
-    p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks});
+    p.groups.push_back({g.cao, g.nsel, p.dbuf + g.sel_start, p.dbuf + g.wt_start, g.blocks,
+                        p.dbuf + g.wsp_start, g.blocks_sp});
   p.version = m->version;
   if (ln.fixed) return; // partial-sum buffers carved above
   if (p.blocks_bin > ln.bin_cap) {
@@ -2886,14 +2986,17 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
     dispatch_cao(g.cao, [&](auto P) {
       with_threads(shape.threads, [&](auto NT) {
         size_t const lds = chunk_bytes(P.value, shape.chunk) +
-                           tile_points(P.value) * sizeof(typename SpreadWord<MT>::type);
-        auto const args = std::make_tuple(sys, g.d_sel, g.wt, g.nsel, p.soff, ln.offsets,
+                           tile_points(P.value, shape.group) *
+                               sizeof(typename SpreadWord<MT>::type);
+        auto const args = std::make_tuple(sys, g.d_sel, g.wt_sp, g.nsel, p.soff, ln.offsets,
                                           static_cast<double4 const *>(ln.spos), shape.chunk,
                                           static_cast<double const *>(ln.sums), det ? 1 : 0);
         auto launch = [&](auto kern) {
-          std::apply([&](auto... a) { kern<<<g.blocks, NT.value, lds, st>>>(a...); }, args);
+          std::apply([&](auto... a) { kern<<<g.blocks_sp, NT.value, lds, st>>>(a...); }, args);
         };
-        if (shape.spread == kSpreadSeg8)
+        if (shape.group == 4)
+          launch(k_spread<P.value, MT, 64, NT.value, 2, 2, 1>);
+        else if (shape.spread == kSpreadSeg8)
           launch(k_spread<P.value, MT, 8, NT.value>);
         else if (shape.spread == kSpreadSeg16)
           launch(k_spread<P.value, MT, 16, NT.value>);

@@ -85,6 +85,8 @@ struct LaunchShape {
   int column = 0;            // 0: 8^3 bricks; 8 or 16: column march over W x W columns
   int seg = 32;              // column march: x layers per block (1..1024)
   int col_lanes = 8;         // column march: lanes per particle (8 or 16)
+  int group = 1;             // spread: bricks per block, 1 or 4 (2x2x1, sharing one tile;
+                             // 8^3 bricks and the plane spread mapping)
 };
 
 struct AutotuneResult {

@@ -94,7 +94,8 @@ namespace {
 
 p3m_launch_shape to_abi(LaunchShape const &s) {
   return p3m_launch_shape{s.gather, s.ncol,    s.zlines, s.fused_lds, s.spread,
-                          s.threads, s.chunk, s.column, s.seg,       s.col_lanes};
+                          s.threads, s.chunk, s.column, s.seg,       s.col_lanes,
+                          s.group};
 }
 
 LaunchShape from_abi(p3m_launch_shape const &a) {
@@ -117,6 +118,8 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
     throw std::invalid_argument("P3M batch: seg must be in [1, 1024]");
   if (a.col_lanes != 8 && a.col_lanes != 16)
     throw std::invalid_argument("P3M batch: col_lanes must be 8 or 16");
+  if (a.group != 1 && a.group != 4)
+    throw std::invalid_argument("P3M batch: group must be 1 or 4");
   LaunchShape s;
   s.gather = a.gather;
   s.ncol = a.ncol;
@@ -128,6 +131,7 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
   s.column = a.column;
   s.seg = a.seg;
   s.col_lanes = a.col_lanes;
+  s.group = a.group;
   return s;
 }
 std::mutex g_err_mutex;

@@ -331,7 +331,7 @@ class _P3MBase:
     def set_launch_shape(self, **shape):
         """Launch shape of later solves: the keyword arguments of
         ``P3MBatch.set_launch_shape`` (gather, ncol, zlines, fused_lds,
-        spread, threads, chunk, column, seg, col_lanes); unnamed fields keep
+        spread, threads, chunk, column, seg, col_lanes, group); unnamed fields keep
         their current values."""
         from .batch import P3MBatch
         cur = _lib.p3m_launch_shape()

@@ -404,6 +404,9 @@ typedef struct p3m_launch_shape {
                     layer touches in LDS (256-thread blocks) */
   int seg;       /* column march: x layers per block (1..1024) */
   int col_lanes; /* column march: lanes per particle (8 or 16) */
+  int group;     /* spread: bricks per block sharing one tile and its halo: 1
+                    or 4 (2x2x1, for sparse bricks; needs column 0 and the
+                    plane spread mapping, spread 0) */
 } p3m_launch_shape;
 
 /* Launch-shape autotuner of the context's device kernels (no counterpart in

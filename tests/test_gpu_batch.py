@@ -76,7 +76,10 @@ def _oracle(s):
     dict(column=8, seg=32, col_lanes=8),    # column march, 8 x 8 columns
     dict(column=16, seg=5, col_lanes=16, fused_lds=0),  # 16 x 16, 5 layers per block
     dict(column=8, seg=1, col_lanes=16, threads=256, chunk=7),  # one layer per block
-], ids=["fused", "split", "small_blocks", "col8", "col16_seg5", "col8_seg1"])
+    dict(group=4),                          # spread: 2 x 2 x 1 bricks per tile
+    dict(group=4, threads=256, chunk=7),    # the same with small chunks
+], ids=["fused", "split", "small_blocks", "col8", "col16_seg5", "col8_seg1", "group4",
+        "group4_chunk7"])
 @pytest.mark.parametrize("single,f_tol,e_tol", [(False, 1e-6, 1e-8),
                                                 (True, 2e-5, 2e-6)])
 def test_batch_matches_oracle_per_system(single, f_tol, e_tol, shape):

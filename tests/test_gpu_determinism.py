@@ -58,6 +58,12 @@ def test_repeated_solves_are_bit_identical_and_match_oracle():
     e_ref = ora.energy(pos, q)
     assert S.rel_rms(runs[0][0], f_ref) <= 1e-6
     assert abs(runs[0][1] - e_ref) <= 1e-8 * abs(e_ref)
+    # grouped spread tiles: the same integer sums, so the same bits
+    p.set_launch_shape(group=4, spread="plane")
+    f = torch.zeros_like(tp)
+    p.add_long_range_forces(tp, tq, f)
+    assert np.array_equal(f.cpu().numpy(), runs[0][0])
+    p.set_launch_shape(group=1)
     # the floating-point spread agrees to rounding
     p.set_deterministic(False)
     f = torch.zeros_like(tp)
@@ -136,3 +142,9 @@ def test_deterministic_argument_errors():
     b.set_launch_shape(column=8)
     with pytest.raises(ValueError, match="8\\^3 bricks"):
         b.set_deterministic(True)
+    with pytest.raises(ValueError, match="brick groups"):
+        b.set_launch_shape(column=8, group=4)
+    with pytest.raises(ValueError, match="brick groups"):
+        b.set_launch_shape(group=4, spread="plane16")
+    with pytest.raises(ValueError, match="group must be"):
+        b.set_launch_shape(group=8)

@@ -95,6 +95,34 @@ def test_c1_fp32_mesh():
     _compare(pos, q, box, params, single=True, f_tol=2e-5, e_tol=2e-6)
 
 
+@pytest.mark.parametrize("cao", [2, 5, 7])
+def test_fp32_tile_scale_worst_case(cao):
+    """FP32 meshes assign charges into 32-bit fixed-point brick tiles whose
+    scale bounds a tile point by n max|q| (peak of the order-cao B-spline)^3
+    for a brick of n particles.  2000 equal charges on one point where the
+    weights peak reach that bound (the oracle's mesh value there is
+    2000 peak^3); the mesh and forces must still match the oracle."""
+    L, M = 10., 16
+    h = L / M
+    rng = np.random.default_rng(400 + cao)
+    x0 = 4.5 * h  # all three weights at their peak (checked on the oracle)
+    pos = np.concatenate([np.full((2000, 3), x0), rng.uniform(0., L, (2000, 3))])
+    q = np.concatenate([np.ones(2000), -np.ones(2000)])
+    params = dict(prefactor=1.0, mesh=M, cao=cao, r_cut=2.0, alpha=1.0, accuracy=1e-3)
+    p3m = _p3m([L] * 3, params, single_precision=True)
+    p3m.activate(pos, q, box_l=[L] * 3)
+    m_gpu = p3m.charge_mesh(pos, q)
+    m_ref = _oracle([L] * 3, params).charge_mesh(pos, q)
+    peak3 = {2: 1., 5: 0.21487695199471937, 7: 0.13345187197856614}[cao]
+    assert np.max(m_ref) >= 0.99 * 2000 * peak3  # the bound is (nearly) reached
+    # coincident particles round identically, so their errors add: at most
+    # n 2^-(s+1) at a point of value ~n peak^3 with 2^(30-s) < 2 n_tile peak^3,
+    # i.e. a relative error below n_tile 2^-30 (about 2.1e-6 for the ~2250
+    # particles of this tile)
+    assert np.max(np.abs(m_gpu - m_ref)) <= 4e-6 * np.max(np.abs(m_ref))
+    _compare(pos, q, [L] * 3, params, single=True, f_tol=2e-5, e_tol=2e-6)
+
+
 @pytest.mark.parametrize("cao", [1, 2, 3, 4, 5, 6, 7])
 def test_all_orders(cao):
     pos, q = S.uniform_system(3000, 10., 2000 + cao)
