@@ -23,8 +23,39 @@ import numpy as np
 from . import _lib
 
 _KEYS = ("prefactor", "accuracy", "cao", "r_cut", "alpha", "mesh", "epsilon",
-         "mesh_off", "check_neutrality", "check_complex_residuals", "tune",
-         "timings", "verbose", "single_precision")
+         "mesh_off", "check_neutrality", "charge_neutrality_tolerance",
+         "check_complex_residuals", "tune", "timings", "verbose",
+         "single_precision", "is_tuned")
+
+# default of Actor::charge_neutrality_tolerance (electrostatics/actor.hpp:38)
+NEUTRALITY_TOLERANCE = 2e-12
+
+
+def neutrality_tolerance(params, given):
+    """Charge-neutrality tolerance of an actor, -1 when unchecked, as
+    Actor::set_charge_neutrality_tolerance applies the two parameters
+    (script_interface/electrostatics/Actor.hpp:71-79, Actor.impl.hpp:44-80):
+    an explicit tolerance first (None switches the check off), then
+    check_neutrality restores the default or switches it off."""
+    tol = NEUTRALITY_TOLERANCE
+    if given:
+        v = params["charge_neutrality_tolerance"]
+        if v is None:
+            tol = -1.
+        else:
+            if not (_is_float(v) or _is_int(v)):
+                raise ValueError(
+                    "Parameter 'charge_neutrality_tolerance' has to be a float")
+            if float(v) < 0.:
+                raise ValueError(
+                    "Parameter 'charge_neutrality_tolerance' must be >= 0")
+            tol = float(v)
+    if params["check_neutrality"]:
+        if tol == -1.:
+            tol = NEUTRALITY_TOLERANCE
+    else:
+        tol = -1.
+    return tol
 
 
 def _is_int(x):
@@ -191,6 +222,9 @@ class _P3MBase:
         params = self.default_params()
         params.update(kwargs)
         self.validate_params(params)
+        params["charge_neutrality_tolerance"] = neutrality_tolerance(
+            params, "charge_neutrality_tolerance" in kwargs)
+        params["is_tuned"] = bool(params.get("is_tuned", not params["tune"]))
         self._params = params
         self._ctx = C.c_void_p()
         _lib.raise_for(L.p3m_create(int(device), C.byref(self._ctx)))
@@ -253,10 +287,13 @@ class _P3MBase:
         cp.skin = float(skin)
         cp.precision = (_lib.P3M_PREC_FP32 if p["single_precision"]
                         else _lib.P3M_PREC_FP64)
-        cp.tune = 1 if p["tune"] else 0
+        # a restored tuned actor keeps its parameters and is not tuned again:
+        # tuning = !is_tuned (CoulombP3M.hpp:114)
+        cp.tune = 0 if p["is_tuned"] else 1
         cp.timings = int(p["timings"])
         cp.verbose = 1 if p["verbose"] else 0
-        cp.charge_neutrality_tolerance = 2e-12 if p["check_neutrality"] else -1.
+        cp.charge_neutrality_tolerance = (p["charge_neutrality_tolerance"]
+                                          if p["check_neutrality"] else -1.)
         self._cp = cp
 
     # -- activation ---------------------------------------------------------
@@ -598,8 +635,29 @@ class _P3MBase:
             "tune": self._params["tune"],
             "verbose": self._params["verbose"],
             "check_neutrality": self._params["check_neutrality"],
+            "charge_neutrality_tolerance": (
+                self._params["charge_neutrality_tolerance"]
+                if self._params["check_neutrality"] else None),
             "check_complex_residuals": self._params["check_complex_residuals"],
         }
+
+    # -- checkpointing (test_checkpoint.py:846-866) -------------------------
+    _CHECKPOINT_KEYS = ("prefactor", "accuracy", "mesh", "mesh_off", "cao", "alpha",
+                        "r_cut", "epsilon", "tune", "timings", "verbose",
+                        "check_neutrality", "charge_neutrality_tolerance",
+                        "check_complex_residuals", "single_precision", "is_tuned")
+
+    def __reduce__(self):
+        """Pickling keeps the constructor parameters and the tuned ones: a
+        restored actor built from them with ``is_tuned`` set is not tuned
+        again on activation (CoulombP3M.hpp:102-128 constructs it with
+        tuning = !is_tuned).  Particles are not part of the state; the
+        restored actor is activated on the caller's particles."""
+        p = self.get_params()
+        if not p["is_tuned"]:  # still to be tuned: the parameters as given
+            p = dict(self._params, is_tuned=False)
+        kw = {k: p[k] for k in self._CHECKPOINT_KEYS}
+        return (_restore_actor, (type(self), self._device, kw))
 
     def __getattr__(self, name):
         if name.startswith("_"):
@@ -617,6 +675,14 @@ class _P3MBase:
             except Exception:  # pragma: no cover
                 pass
             self._ctx = None
+
+
+def _restore_actor(cls, device, kwargs):
+    return cls(device=device, **kwargs)
+
+
+def _restore_elc(kwargs):
+    return ELC(**kwargs)
 
 
 class P3M(_P3MBase):
@@ -656,7 +722,7 @@ class ELC:
 
     _KEYS = ("actor", "maxPWerror", "gap_size", "far_cut", "delta_mid_top",
              "delta_mid_bot", "const_pot", "pot_diff", "neutralize",
-             "check_neutrality")
+             "check_neutrality", "charge_neutrality_tolerance")
 
     def required_keys(self):
         return {"actor", "maxPWerror", "gap_size"}
@@ -685,6 +751,8 @@ class ELC:
         params = self.default_params()
         params.update(kwargs)
         self.validate_params(params)
+        params["charge_neutrality_tolerance"] = neutrality_tolerance(
+            params, "charge_neutrality_tolerance" in kwargs)
         actor = params["actor"]
         if not isinstance(actor, _P3MBase):
             raise ValueError(f"Parameter 'actor' of type {type(actor).__name__} "
@@ -703,8 +771,10 @@ class ELC:
         ep.const_pot = 1 if params["const_pot"] else 0
         ep.pot_diff = float(params["pot_diff"])
         _lib.raise_for(L.p3m_set_elc(actor._ctx, C.byref(ep)), actor._ctx)
-        if not params["check_neutrality"]:
-            actor._params["check_neutrality"] = False
+        # the ELC actor's neutrality check governs the solve it wraps
+        if (not params["check_neutrality"] or "charge_neutrality_tolerance" in kwargs):
+            actor._params["check_neutrality"] = bool(params["check_neutrality"])
+            actor._params["charge_neutrality_tolerance"] = params["charge_neutrality_tolerance"]
             if actor._activated:
                 actor._build_cparams(box_l=actor._box_l, skin=actor._cp.skin)
                 _lib.raise_for(L.p3m_set_params(actor._ctx, C.byref(actor._cp)),
@@ -740,7 +810,17 @@ class ELC:
                 "far_cut": ep.far_cut, "neutralize": bool(ep.neutralize),
                 "delta_mid_top": ep.delta_mid_top, "delta_mid_bot": ep.delta_mid_bot,
                 "const_pot": bool(ep.const_pot), "pot_diff": ep.pot_diff,
-                "check_neutrality": bool(self._params["check_neutrality"])}
+                "check_neutrality": bool(self._params["check_neutrality"]),
+                "charge_neutrality_tolerance": (
+                    self._params["charge_neutrality_tolerance"]
+                    if self._params["check_neutrality"] else None)}
+
+    def __reduce__(self):
+        """Pickling (test_checkpoint.py:868-888): the ELC parameters as given
+        (far_cut -1 is tuned again on activation) and the wrapped actor with
+        its own checkpoint state."""
+        kw = {k: self._params[k] for k in self._KEYS if k in self._params}
+        return (_restore_elc, (kw,))
 
     def activate(self, pos, q, box_l, skin=0.4):
         """ElectrostaticLayerCorrection::on_activation (elc.hpp:181-200):

@@ -15,7 +15,7 @@ from .electrostatics import (_is_device_tensor, _is_float, _is_int, device_input
                               device_output)
 
 _KEYS = ("prefactor", "accuracy", "cao", "r_cut", "alpha", "mesh", "epsilon",
-         "mesh_off", "single_precision", "tune", "timings", "verbose")
+         "mesh_off", "single_precision", "tune", "timings", "verbose", "is_tuned")
 
 
 class _Vec:
@@ -106,6 +106,7 @@ class DipolarP3M:
         params = self.default_params()
         params.update(kwargs)
         self.validate_params(params)
+        params["is_tuned"] = bool(params.get("is_tuned", not params["tune"]))
         self._params = params
         self._ctx = C.c_void_p()
         _lib.raise_for(L.p3m_dipolar_create(int(device), C.byref(self._ctx)))
@@ -158,7 +159,9 @@ class DipolarP3M:
             cp.mesh_off[d] = float(p["mesh_off"][d])
         cp.box_l = float(box_l)
         cp.skin = float(skin)
-        cp.tune = 1 if p["tune"] else 0
+        # tuning = !is_tuned (DipolarP3M.hpp:98-104): a tuned actor keeps its
+        # parameters when it is activated again or restored from a checkpoint
+        cp.tune = 0 if p["is_tuned"] else 1
         cp.timings = int(p["timings"])
         cp.verbose = 1 if p["verbose"] else 0
         _lib.raise_for(L.p3m_dipolar_set_params(self._ctx, C.byref(cp)), dipolar=self._ctx)
@@ -185,8 +188,18 @@ class DipolarP3M:
         return {"prefactor": cp.prefactor, "accuracy": cp.accuracy, "cao": cp.cao,
                 "r_cut": cp.r_cut, "alpha": cp.alpha, "mesh": [cp.mesh] * 3,
                 "epsilon": cp.epsilon, "mesh_off": list(cp.mesh_off),
-                "tune": bool(cp.tune), "timings": cp.timings,
-                "single_precision": False, "verbose": bool(cp.verbose)}
+                "tune": bool(self._params["tune"]), "is_tuned": not bool(cp.tune),
+                "timings": cp.timings, "single_precision": False,
+                "verbose": bool(cp.verbose)}
+
+    def __reduce__(self):
+        """Pickling keeps the constructor and tuned parameters; the restored
+        actor is built with ``is_tuned`` and is not tuned again."""
+        p = self.get_params()
+        if not p["is_tuned"]:  # still to be tuned: the parameters as given
+            p = dict(self._params, is_tuned=False)
+        kw = {k: p[k] for k in _KEYS}
+        return (_restore_dipolar, (self._device, kw))
 
     def _require(self):
         if not self._activated:
@@ -249,3 +262,7 @@ class DipolarP3M:
             except Exception:  # pragma: no cover
                 pass
             self._ctx = None
+
+
+def _restore_dipolar(device, kwargs):
+    return DipolarP3M(device=device, **kwargs)

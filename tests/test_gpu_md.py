@@ -99,3 +99,31 @@ def test_nve_energy_conservation_with_resident_particles():
             drift = max(drift, abs(total(ew) - e0))
     print(f"NVE 400 steps: max |E - E0| = {drift:.3e} (kinetic {ke0:.3f}, E0 {e0:.3f})")
     assert drift <= 1e-3 * ke0
+
+
+def test_checkpointed_tuned_actor_gives_the_same_forces():
+    """A tuned actor restored from a pickle keeps its tuned parameters, is
+    not tuned again on activation, and gives bit-identical forces and energy
+    (the reference restores actors from their parameters and is_tuned,
+    CoulombP3M.hpp:102-128, test_checkpoint.py:846-866)."""
+    import pickle
+    from espresso_amd.electrostatics import P3M
+    pos, q = _lattice(12, 14., 7)
+    p3m = P3M(prefactor=1.0, accuracy=1e-4, timings=2, verbose=False)
+    p3m.activate(pos, q, box_l=[14.] * 3)
+    g = p3m.get_params()
+    assert g["is_tuned"] and g["tune"]
+    f1 = np.zeros_like(pos)
+    p3m.add_long_range_forces(pos, q, f1)
+    e1 = p3m.long_range_energy(pos, q)
+    blob = pickle.dumps(p3m)
+    del p3m
+    r = pickle.loads(blob)
+    r.activate(pos, q, box_l=[14.] * 3)
+    g2 = r.get_params()
+    for k in ("mesh", "cao", "alpha", "r_cut", "accuracy", "is_tuned", "tune"):
+        assert g2[k] == g[k], k
+    f2 = np.zeros_like(pos)
+    r.add_long_range_forces(pos, q, f2)
+    assert np.array_equal(f1, f2)
+    assert r.long_range_energy(pos, q) == e1
