@@ -40,10 +40,26 @@ def stage_kernels(shape):
     mesh 128 (split FFT passes) for the chosen launch shape."""
     t = shape["threads"]
     g = shape["gather"]
+    if shape.get("column"):
+        w, lanes = shape["column"], shape["col_lanes"]
+        return {"spread": f"k_spread_col<7, double, {w}, {lanes}>",
+                "convolve": "k_fft_x_conv<double>",
+                "gather": f"k_gather_col<7, double, {w}, {lanes}>"}
     gather = (f"k_gather_layer<7, double, {t}>" if g == "layer"
               else f"k_gather_plane<7, double, {SEGMENT[g]}, {t}>")
-    return {"spread": f"k_spread<7, double, {SEGMENT[shape['spread']]}, {t}>",
+    grp = "2, 2, 1" if shape.get("group", 1) == 4 else "1, 1, 1"
+    return {"spread": f"k_spread<7, double, {SEGMENT[shape['spread']]}, {t}, {grp}>",
             "convolve": "k_fft_x_conv<double>", "gather": gather}
+
+
+# LDS of one MI355X: 256 CUs x 128 bytes per clock x 2.4 GHz
+LDS_PEAK_GBS = 256 * 128 * 2.4
+
+
+def lds_bytes_gather(n_q, cao, s_m):
+    """LDS bytes the gather reads per launch: the three field values of each
+    of the cao^3 stencil points of every charged particle."""
+    return 3 * cao ** 3 * s_m * n_q
 
 
 def make_system(n=1_000_000, seed=1002):
@@ -272,6 +288,108 @@ def bench_batched(steps, warmup, torch, dev, pair_ns=None):
     return res
 
 
+def bench_c3_batched(steps, warmup, torch, dev, pair_ns=None, n_sys=8):
+    """The batched form on systems of the precision-sweep shape: n_sys
+    systems of N = 250,000 charges +-1 uniform in L = 79.37 (density 0.5),
+    tuned at 1e-4 within mesh 48-96 and cao 3-7 (all systems share N and L,
+    so one tuning serves all).  The FP32 batch (hipGraph replay, autotuned
+    launch shape) against the plain FP64 solver applied per system, with the
+    batch latency p50/p99."""
+    from espresso_amd import P3MBatch, _lib
+    from espresso_amd.electrostatics import P3M
+    L, n = 79.37, 250_000
+    systems = []
+    for k in range(n_sys):
+        rng = np.random.default_rng(3000 + k)
+        systems.append({"pos": rng.uniform(0., L, (n, 3)), "L": L, "n": n,
+                        "q": np.where(np.arange(n) % 2 == 0, 1., -1.)})
+    t = tune_systems(systems[:1], 1e-4, (48, 96), (3, 7), pair_ns=pair_ns)[0]
+    tp = torch.tensor(np.concatenate([s["pos"] for s in systems]), device=dev)
+    tq = torch.tensor(np.concatenate([s["q"] for s in systems]), device=dev)
+    n_tot = n * n_sys
+    b = P3MBatch(single_precision=True)
+    for _ in systems:
+        b.add_system(box_l=[L] * 3, n=n, prefactor=1.0, mesh=t["mesh"], cao=t["cao"],
+                     alpha=t["alpha"], r_cut=t["r_cut"], accuracy=t["accuracy"])
+    f32 = torch.zeros((n_tot, 3), dtype=torch.float64, device=dev)
+    shape = b.autotune(tp, tq, f32, reps=3)
+    for _ in range(warmup):
+        b.graph_solve(tp, tq, f32)
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.graph_solve(tp, tq, f32)
+    b.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    ext = torch.cuda.ExternalStream(b.stream())
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    for e0, e1 in evs:
+        e0.record(ext)
+        b.graph_solve(tp, tq, f32)
+        e1.record(ext)
+    b.synchronize()
+    lat = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    plain = []
+    for s in systems:
+        p = P3M(prefactor=1.0, accuracy=t["accuracy"], mesh=t["mesh"], cao=t["cao"],
+                r_cut=t["r_cut"], alpha=t["alpha"], tune=False, verbose=False)
+        p.activate(s["pos"], s["q"], box_l=[L] * 3)
+        plain.append(p)
+    f64 = torch.zeros_like(f32)
+    views = [(tp[k * n:(k + 1) * n], tq[k * n:(k + 1) * n], f64[k * n:(k + 1) * n])
+             for k in range(n_sys)]
+    lib = _lib.load()
+    flags = _lib.P3M_FORCES | _lib.P3M_DEVICE_POINTERS | _lib.P3M_ASYNC
+    calls = [(p._ctx, n, vp.data_ptr(), vq.data_ptr(), vf.data_ptr())
+             for p, (vp, vq, vf) in zip(plain, views)]
+    torch.cuda.synchronize()
+
+    def plain_pass():
+        for ctx, nn, pp, qp, fp in calls:
+            _lib.raise_for(lib.p3m_solve(ctx, nn, pp, qp, None, fp, flags, None, None), ctx)
+
+    plain_pass()
+    for p in plain:
+        p.synchronize()
+    reps = max(3, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    for p in plain:
+        p.synchronize()
+    t_plain = (time.perf_counter() - t0) / reps
+    f32.zero_()
+    b.solve(tp, tq, f32)
+    f64.zero_()
+    torch.cuda.synchronize()
+    for p, (vp, vq, vf) in zip(plain, views):
+        p.add_long_range_forces(vp, vq, vf)
+    d = (f32 - f64).cpu().numpy()
+    ref = f64.cpu().numpy()
+    res = {"workload": f"{n_sys} systems of N=250000 +-1 in L=79.37 (density 0.5), tuned at "
+                       "1e-4 within mesh 48-96, cao 3-7; FP32 batch vs plain FP64 per system",
+           "mesh": t["mesh"][0], "cao": t["cao"], "r_cut": round(t["r_cut"], 4),
+           "estimated_rms_force_error": t["accuracy"],
+           "ms_per_batch": round(t_batch * 1e3, 4),
+           "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                "p99": round(float(np.percentile(lat, 99)), 4),
+                                "replays": len(lat)},
+           "mparticles_per_s": round(n_tot / t_batch / 1e6, 2),
+           "plain_fp64_ms_per_pass": round(t_plain * 1e3, 3),
+           "plain_fp64_mparticles_per_s": round(n_tot / t_plain / 1e6, 2),
+           "speedup_vs_plain_fp64": round(t_plain / t_batch, 2),
+           "force_rel_rms_fp32_batch_vs_fp64_plain":
+               float(np.sqrt(np.sum(d * d) / np.sum(ref * ref))),
+           "launch_shape": shape}
+    print(f"[bench] C3-shaped batch: {res['ms_per_batch']} ms, {res['mparticles_per_s']} "
+          f"Mparticles/s; plain FP64 {res['plain_fp64_mparticles_per_s']} (x"
+          f"{res['speedup_vs_plain_fp64']}); latency {res['batch_latency_ms']}", flush=True)
+    del plain
+    return res
+
+
 def make_c5_specs(n_sys=2048, seed0=5000):
     """2048 resident systems: N log-uniform in [1000, 200000] (even), density
     uniform in [0.2, 0.8]."""
@@ -284,7 +402,8 @@ def make_c5_specs(n_sys=2048, seed0=5000):
     return out
 
 
-def bench_serving(torch, dev, rates=(50, 100, 200, 500, 1000, 2000), seconds=1.5,
+def bench_serving(torch, dev, rates=(50, 100, 200, 500, 1000, 2000, 5000, 10000, 15000),
+                  seconds=1.5,
                   n_sys=2048, lanes=4, budget_bytes=512 << 20, max_batch=64):
     """Poisson request streams over resident systems: continuous batching,
     admission by working memory, p50/p99 latency and solves/s per offered
@@ -638,6 +757,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 512-system batched workload")
+    ap.add_argument("--no-c3-batched", action="store_true",
+                    help="skip the batched comparison on precision-sweep-shaped systems")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the 2048-system serving workload")
     ap.add_argument("--no-short-range", action="store_true",
@@ -651,7 +772,7 @@ def main():
     ap.add_argument("--no-accuracy-table", action="store_true",
                     help="skip the 27-cell precision sweep (N=250,000) against "
                          "exact Ewald forces")
-    ap.add_argument("--only", choices=("batched", "serving", "elc", "dp3m"),
+    ap.add_argument("--only", choices=("batched", "c3batched", "serving", "elc", "dp3m"),
                     help="run only this workload and print its record "
                          "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -671,6 +792,7 @@ def main():
     if args.only:
         dev = torch.device("cuda:0")
         runs = {"batched": lambda: bench_batched(args.steps, args.warmup, torch, dev),
+                "c3batched": lambda: bench_c3_batched(args.steps, args.warmup, torch, dev),
                 "serving": lambda: bench_serving(torch, dev),
                 "elc": lambda: bench_elc(args.steps, args.warmup, torch, dev,
                                          cpu=not args.no_cpu_baseline),
@@ -739,6 +861,16 @@ def main():
                                    if traffic is not None else None),
                 "algorithmic_bytes": ab[dom],
                 "avg_launch_ms": round(single[dom], 5)}
+    if dom == "gather":
+        # the gather's binding resource is LDS, not HBM (SQ counters in
+        # profiles/sq_counters_c2_n1e6_mesh128_cao7.md)
+        lb = lds_bytes_gather(n_q, 7, 8)
+        roofline["lds"] = {"achieved": round(lb / t_dom / 1e9, 1), "peak": LDS_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(lb / t_dom / 1e9 / LDS_PEAK_GBS, 4),
+                           "bytes": lb,
+                           "note": "algorithmic LDS reads: 3 fields x cao^3 stencil points "
+                                   "x 8 bytes per charged particle; peak 256 CUs x 128 B/clk "
+                                   "x 2.4 GHz"}
     solve_achieved = ab["solve"] / t_step / 1e9
 
     # FP32 mesh form on the same input
@@ -822,6 +954,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.steps, args.warmup, torch, dev, pair_ns=pair_ns)
+    c3_batched = None
+    if not args.no_c3_batched and not args.no_batched:
+        c3_batched = bench_c3_batched(args.steps, args.warmup, torch, dev, pair_ns=pair_ns)
     serving = None
     if not args.no_serving:
         serving = bench_serving(torch, dev)
@@ -862,6 +997,7 @@ def main():
         "elc": elc,
         "dp3m": dp3m,
         "batched": batched,
+        "batched_c3_shape": c3_batched,
         "serving": serving,
     }
     if table is not None:
