@@ -838,6 +838,25 @@ def main():
           f"stages {json.dumps({k: round(v, 4) for k, v in stages.items()})}",
           flush=True)
 
+    # the same particles stored in spatial cell order, as ESPResSo's cell
+    # system stores them (cells of 8 mesh spacings, x-major): the gather's
+    # per-particle force updates and the binning's reads then touch
+    # neighbouring memory; reported beside value, never as value
+    cell = np.floor(pos / (8. * L / mesh[0])).astype(np.int64) % (mesh[0] // 8)
+    order = np.lexsort((cell[:, 2], cell[:, 1], cell[:, 0]))
+    tp_c = torch.tensor(pos[order], device=dev)
+    tq_c = torch.tensor(q[order], device=dev)
+    tf_c = torch.zeros_like(tp_c)
+    t_cell, st_cell = timed_loop(p3m, tp_c, tq_c, tf_c, args.steps, args.warmup, torch)
+    cell_ordered = {"ms_per_step": round(t_cell * 1e3, 4),
+                    "mparticles_per_s": round(n / t_cell / 1e6, 2),
+                    "stages_ms": {k: round(v, 4) for k, v in st_cell.items()},
+                    "note": "same system and launch shape, particles sorted by cells of "
+                            "8 mesh spacings (ESPResSo stores particles cell by cell)"}
+    print(f"[bench] FP64 cell-ordered particles: {t_cell * 1e3:.3f} ms/step; stages "
+          f"{json.dumps(cell_ordered['stages_ms'])}", flush=True)
+    del tp_c, tq_c, tf_c
+
     # host-pointer (PCIe-inclusive) rate for the record
     f_host = np.zeros((n, 3))
     t0 = time.perf_counter()
@@ -994,6 +1013,7 @@ def main():
         "launch_shape": shape64,
         "solve_gbs_algorithmic": round(solve_achieved, 1),
         "host_pointer_ms": round(t_host * 1e3, 3),
+        "cell_ordered_input": cell_ordered,
         "fp32_mesh": fp32,
         "accuracy": accuracy,
         "short_range": short_range,
