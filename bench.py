@@ -39,14 +39,15 @@ def stage_kernels(shape):
     """Kernel behind each single-kernel stage of the FP64 solve at cao 7 and
     mesh 128 (split FFT passes) for the chosen launch shape."""
     t = shape["threads"]
+    tg = shape.get("gather_threads") or t
     g = shape["gather"]
     if shape.get("column"):
         w, lanes = shape["column"], shape["col_lanes"]
         return {"spread": f"k_spread_col<7, double, {w}, {lanes}>",
                 "convolve": "k_fft_x_conv<double>",
                 "gather": f"k_gather_col<7, double, {w}, {lanes}>"}
-    gather = (f"k_gather_layer<7, double, {t}>" if g == "layer"
-              else f"k_gather_plane<7, double, {SEGMENT[g]}, {t}>")
+    gather = (f"k_gather_layer<7, double, {tg}>" if g == "layer"
+              else f"k_gather_plane<7, double, {SEGMENT[g]}, {tg}>")
     grp = "2, 2, 1" if shape.get("group", 1) == 4 else "1, 1, 1"
     return {"spread": f"k_spread<7, double, {SEGMENT[shape['spread']]}, {t}, {grp}>",
             "convolve": "k_fft_x_conv<double>", "gather": gather}

@@ -97,7 +97,7 @@ class p3m_params(C.Structure):
 class p3m_launch_shape(C.Structure):
     _fields_ = [(name, C.c_int) for name in
                 ("gather", "ncol", "zlines", "fused_lds", "spread", "threads", "chunk",
-                 "column", "seg", "col_lanes", "group")]
+                 "column", "seg", "col_lanes", "group", "gather_threads")]
 
 
 class p3m_elc_params(C.Structure):

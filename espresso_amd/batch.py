@@ -44,7 +44,8 @@ def shape_dict(shape, best_ms=None, default_ms=None):
          "zlines": shape.zlines, "fused_lds": shape.fused_lds,
          "spread": snames.get(shape.spread, shape.spread), "threads": shape.threads,
          "chunk": shape.chunk, "column": shape.column, "seg": shape.seg,
-         "col_lanes": shape.col_lanes, "group": shape.group}
+         "col_lanes": shape.col_lanes, "group": shape.group,
+         "gather_threads": shape.gather_threads}
     if best_ms is not None:
         d.update(best_ms=best_ms, default_ms=default_ms)
     return d
@@ -182,7 +183,7 @@ class P3MBatch:
 
     def set_launch_shape(self, gather="plane16", ncol=8, zlines=16,
                          fused_lds=96 * 1024, spread="plane", threads=512, chunk=64,
-                         column=0, seg=32, col_lanes=8, group=1):
+                         column=0, seg=32, col_lanes=8, group=1, gather_threads=0):
         """Gather mapping ("layer", "plane", "plane16", "plane8" or 0-3),
         FFT columns per block (split y/x passes), z-line pairs per block, the
         LDS bytes up to which a system's x-y planes are transformed by the
@@ -193,12 +194,13 @@ class P3MBatch:
         (W x W columns, ``seg`` x layers per block, ``col_lanes`` lanes per
         particle); 0 keeps 8^3 bricks.  ``group`` 4 lets one spread block
         assign 2 x 2 x 1 bricks into one shared tile (for sparse bricks;
-        needs column 0 and spread "plane")."""
+        needs column 0 and spread "plane").  ``gather_threads`` 256 or 512
+        gives the brick gather its own block size (0: ``threads``)."""
         g = self.GATHER[gather] if isinstance(gather, str) else int(gather)
         sp = self.SPREAD[spread] if isinstance(spread, str) else int(spread)
         shape = _lib.p3m_launch_shape(g, int(ncol), int(zlines), int(fused_lds), sp,
                                       int(threads), int(chunk), int(column), int(seg),
-                                      int(col_lanes), int(group))
+                                      int(col_lanes), int(group), int(gather_threads))
         self._check(_lib.load().p3m_batch_set_launch_shape(self._b, C.byref(shape)))
 
     def set_deterministic(self, on=True):

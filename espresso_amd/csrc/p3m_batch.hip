@@ -2252,7 +2252,9 @@ void BatchSolver::set_launch_shape(LaunchShape const &s) {
   if (s.spread < kSpreadSeg64 || s.spread > kSpreadSeg8)
     throw std::invalid_argument("P3M: unknown spread mapping");
   if ((s.threads != 256 && s.threads != 512) || s.chunk < 1 || s.chunk > kChunkMax ||
-      3 * s.chunk > s.threads)
+      3 * s.chunk > s.threads ||
+      (s.gather_threads != 0 && s.gather_threads != 256 && s.gather_threads != 512) ||
+      3 * s.chunk > s.gather_block())
     throw std::invalid_argument("P3M: invalid particle-mesh block shape");
   if ((s.column != 0 && s.column != 8 && s.column != 16) || s.seg < 1 || s.seg > 1024 ||
       (s.col_lanes != 8 && s.col_lanes != 16))
@@ -2378,6 +2380,14 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
       consider(c);
     }
   try_blocks();
+  // gather block size apart from the spread's (brick gather only)
+  if (res.best.column == 0)
+    for (int v : {0, 256, 512}) {
+      if (v == res.best.gather_threads || (v != 0 && 3 * res.best.chunk > v)) continue;
+      LaunchShape c = res.best;
+      c.gather_threads = v;
+      consider(c);
+    }
   // column march: width and lanes per particle, then x layers per block
   if (!m_det) {
     LaunchShape const base = res.best;
@@ -3063,7 +3073,7 @@ void enqueue(BatchSolver::Impl *m, Lane &ln, ParticleArrays const &pa, unsigned 
         continue;
       }
       dispatch_cao(g.cao, [&](auto P) {
-        with_threads(shape.threads, [&](auto NT) {
+        with_threads(shape.gather_block(), [&](auto NT) {
           size_t const lds =
               chunk_bytes(P.value, shape.chunk) + 3 * tile_points(P.value) * sizeof(MT);
           auto const args =

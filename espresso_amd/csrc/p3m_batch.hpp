@@ -87,6 +87,8 @@ struct LaunchShape {
   int col_lanes = 8;         // column march: lanes per particle (8 or 16)
   int group = 1;             // spread: bricks per block, 1 or 4 (2x2x1, sharing one tile;
                              // 8^3 bricks and the plane spread mapping)
+  int gather_threads = 0;    // block size of the brick gather (0: threads; 256, 512)
+  int gather_block() const { return gather_threads ? gather_threads : threads; }
 };
 
 struct AutotuneResult {

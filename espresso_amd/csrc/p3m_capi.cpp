@@ -95,7 +95,7 @@ namespace {
 p3m_launch_shape to_abi(LaunchShape const &s) {
   return p3m_launch_shape{s.gather, s.ncol,    s.zlines, s.fused_lds, s.spread,
                           s.threads, s.chunk, s.column, s.seg,       s.col_lanes,
-                          s.group};
+                          s.group,   s.gather_threads};
 }
 
 LaunchShape from_abi(p3m_launch_shape const &a) {
@@ -120,6 +120,10 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
     throw std::invalid_argument("P3M batch: col_lanes must be 8 or 16");
   if (a.group != 1 && a.group != 4)
     throw std::invalid_argument("P3M batch: group must be 1 or 4");
+  if (a.gather_threads != 0 && a.gather_threads != 256 && a.gather_threads != 512)
+    throw std::invalid_argument("P3M batch: gather_threads must be 0, 256 or 512");
+  if (a.gather_threads != 0 && 3 * a.chunk > a.gather_threads)
+    throw std::invalid_argument("P3M batch: chunk must be at most gather_threads / 3");
   LaunchShape s;
   s.gather = a.gather;
   s.ncol = a.ncol;
@@ -132,6 +136,7 @@ LaunchShape from_abi(p3m_launch_shape const &a) {
   s.seg = a.seg;
   s.col_lanes = a.col_lanes;
   s.group = a.group;
+  s.gather_threads = a.gather_threads;
   return s;
 }
 std::mutex g_err_mutex;

@@ -368,8 +368,8 @@ class _P3MBase:
     def set_launch_shape(self, **shape):
         """Launch shape of later solves: the keyword arguments of
         ``P3MBatch.set_launch_shape`` (gather, ncol, zlines, fused_lds,
-        spread, threads, chunk, column, seg, col_lanes, group); unnamed fields keep
-        their current values."""
+        spread, threads, chunk, column, seg, col_lanes, group, gather_threads);
+        unnamed fields keep their current values."""
         from .batch import P3MBatch
         cur = _lib.p3m_launch_shape()
         _lib.raise_for(_lib.load().p3m_get_launch_shape(self._ctx, C.byref(cur)), self._ctx)

@@ -407,6 +407,10 @@ typedef struct p3m_launch_shape {
   int group;     /* spread: bricks per block sharing one tile and its halo: 1
                     or 4 (2x2x1, for sparse bricks; needs column 0 and the
                     plane spread mapping, spread 0) */
+  int gather_threads; /* block size of the brick gather: 0 = threads, or 256
+                    or 512 (chunk at most gather_threads/3); dense FP64 bricks
+                    gather fastest in 512-thread blocks while their spread
+                    prefers 256 */
 } p3m_launch_shape;
 
 /* Launch-shape autotuner of the context's device kernels (no counterpart in

@@ -78,8 +78,10 @@ def _oracle(s):
     dict(column=8, seg=1, col_lanes=16, threads=256, chunk=7),  # one layer per block
     dict(group=4),                          # spread: 2 x 2 x 1 bricks per tile
     dict(group=4, threads=256, chunk=7),    # the same with small chunks
+    dict(threads=256, gather_threads=512),  # gather in larger blocks than the spread
+    dict(threads=512, gather_threads=256, gather="plane8"),
 ], ids=["fused", "split", "small_blocks", "col8", "col16_seg5", "col8_seg1", "group4",
-        "group4_chunk7"])
+        "group4_chunk7", "gather512", "gather256"])
 @pytest.mark.parametrize("single,f_tol,e_tol", [(False, 1e-6, 1e-8),
                                                 (True, 2e-5, 2e-6)])
 def test_batch_matches_oracle_per_system(single, f_tol, e_tol, shape):
