@@ -267,3 +267,22 @@ def test_launch_shapes_and_autotune_keep_results():
     torch.cuda.synchronize()
     b.solve(pos, q, f)
     assert S.rel_rms(f.cpu().numpy(), ref_np) <= 1e-5
+
+
+def test_launch_shape_validation():
+    """Invalid launch shapes are refused before anything changes, with a
+    message naming the field; a valid split of the block sizes is kept."""
+    b = _batch(_systems()[:2], False)
+    before = b.launch_shape()
+    for kw, msg in [(dict(gather_threads=128), "gather_threads must be 0, 256 or 512"),
+                    (dict(threads=512, chunk=64, gather_threads=0, group=2),
+                     "group must be 1 or 4"),
+                    (dict(threads=256, chunk=64, gather_threads=512, column=12),
+                     "column must be 0"),
+                    (dict(threads=128), "threads must be 256 or 512")]:
+        with pytest.raises(ValueError, match=msg):
+            b.set_launch_shape(**kw)
+        assert b.launch_shape() == before
+    b.set_launch_shape(threads=256, chunk=64, gather_threads=512)
+    got = b.launch_shape()
+    assert got["threads"] == 256 and got["gather_threads"] == 512
