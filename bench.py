@@ -162,7 +162,7 @@ def bench_batched(steps, warmup, torch, dev, pair_ns=None):
                      cao=t["cao"], alpha=t["alpha"], r_cut=t["r_cut"],
                      accuracy=t["accuracy"])
     scratch = torch.zeros((n_tot, 3), dtype=torch.float64, device=dev)
-    shape = b.autotune(tp, tq, scratch, reps=3)
+    shape = b.autotune(tp, tq, scratch, reps=5)
     print(f"[bench] batched autotune: {shape}", flush=True)
     tf = torch.zeros_like(scratch)
     for _ in range(warmup):
@@ -315,7 +315,7 @@ def bench_c3_batched(steps, warmup, torch, dev, pair_ns=None, n_sys=8):
         b.add_system(box_l=[L] * 3, n=n, prefactor=1.0, mesh=t["mesh"], cao=t["cao"],
                      alpha=t["alpha"], r_cut=t["r_cut"], accuracy=t["accuracy"])
     f32 = torch.zeros((n_tot, 3), dtype=torch.float64, device=dev)
-    shape = b.autotune(tp, tq, f32, reps=3)
+    shape = b.autotune(tp, tq, f32, reps=5)
     for _ in range(warmup):
         b.graph_solve(tp, tq, f32)
     b.synchronize()
@@ -831,7 +831,7 @@ def main():
     tq = torch.tensor(q, device=dev)
     tf = torch.zeros((n, 3), dtype=torch.float64, device=dev)
 
-    shape64 = p3m.autotune(tp, tq, reps=3)
+    shape64 = p3m.autotune(tp, tq, reps=5)
     print(f"[bench] FP64 launch shape: {shape64}", flush=True)
     t_step, stages = timed_loop(p3m, tp, tq, tf, args.steps, args.warmup, torch)
     value = n / t_step / 1e6
@@ -903,7 +903,7 @@ def main():
                   r_cut=prm["r_cut"], alpha=prm["alpha"], single_precision=True)
         p32.activate(pos, q, box_l=box)
         tf32 = torch.zeros((n, 3), dtype=torch.float64, device=dev)
-        shape32 = p32.autotune(tp, tq, reps=3)
+        shape32 = p32.autotune(tp, tq, reps=5)
         print(f"[bench] FP32 launch shape: {shape32}", flush=True)
         t32, st32 = timed_loop(p32, tp, tq, tf32, args.steps, args.warmup, torch)
         f64 = torch.zeros((n, 3), dtype=torch.float64, device=dev)

@@ -2347,6 +2347,8 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
   }
   // particle-mesh block size and chunk: large blocks for dense bricks, small
   // ones (more bricks in flight per CU) for sparse bricks
+  // (the gather keeps its block size, so that a move of the spread's is
+  // judged on its own; gather_threads is a coordinate of its own below)
   auto try_blocks = [&] {
     LaunchShape const base = res.best;
     for (auto [t, c] : {std::pair{512, 64}, std::pair{256, 64}, std::pair{256, 32}}) {
@@ -2354,6 +2356,7 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
       LaunchShape cand = base;
       cand.threads = t;
       cand.chunk = c;
+      cand.gather_threads = base.gather_block();
       consider(cand);
     }
   };
