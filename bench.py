@@ -107,9 +107,9 @@ def make_c4(n_sys=512, seed0=4000):
 
 
 # device real-space cost per pair inside r_cut: p3m_short_range on the
-# N = 10^6 workload (r_cut 8.558, 5.3e8 pairs) took 11.6 ms on one MI355X;
+# N = 10^6 workload (r_cut 8.558, 5.3e8 pairs) took 9.6 ms on one MI355X;
 # bench.py measures it again in its short-range leg and tunes with that rate
-DEVICE_PAIR_NS = 0.022
+DEVICE_PAIR_NS = 0.018
 # the 512-system batch is tuned with a margin below its 1e-4 rms force error
 # target: the P3M error estimate runs 3-8% below the error measured against
 # exact Ewald sums on these systems

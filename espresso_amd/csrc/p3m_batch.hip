@@ -2367,21 +2367,17 @@ AutotuneResult BatchSolver::autotune(std::vector<int> const &sel, ParticleArrays
     c.gather = v;
     consider(c);
   }
-  for (int v : {kSpreadSeg64, kSpreadSeg16, kSpreadSeg8}) {
-    if (v == res.best.spread) continue;
+  // spread mappings, and brick groups sharing one tile (sparse bricks; the
+  // 64-lane mapping only) as candidates of the same coordinate
+  for (auto [v, gr] : {std::pair{kSpreadSeg64, 1}, std::pair{kSpreadSeg64, 4},
+                       std::pair{kSpreadSeg16, 1}, std::pair{kSpreadSeg8, 1}}) {
+    if (v == res.best.spread && gr == res.best.group) continue;
+    if (gr != 1 && res.best.column != 0) continue;
     LaunchShape c = res.best;
     c.spread = v;
-    c.group = 1;
+    c.group = gr;
     consider(c);
   }
-  // spread brick groups sharing one tile (sparse bricks)
-  if (res.best.spread == kSpreadSeg64 && res.best.column == 0)
-    for (int v : {1, 4}) {
-      if (v == res.best.group) continue;
-      LaunchShape c = res.best;
-      c.group = v;
-      consider(c);
-    }
   try_blocks();
   // gather block size apart from the spread's (brick gather only)
   if (res.best.column == 0)
