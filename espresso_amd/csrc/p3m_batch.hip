@@ -65,7 +65,7 @@ constexpr int kChunkMax = 64; // most particles staged per brick pass (3 x chunk
 constexpr int kCountCopies = 8; // brick counter copies (one per XCD-sized share of the blocks;
                                 // k_scan reads a brick's copies as two 16-byte words)
 constexpr int kBrick = 8;    // first-stencil points per brick edge
-constexpr int kFill = 1024;  // mesh elements zeroed per block
+constexpr int kFill = 4096;  // mesh elements zeroed per block
 
 /** Per-system state read by every kernel. */
 struct SysDesc {
@@ -281,9 +281,29 @@ __global__ __launch_bounds__(1024) void k_scan(SysDesc const *__restrict__ sys,
   {
     double v[7] = {0., 0., 0., 0., 0., 0., 0.};
     double qmax = 0.;
-    for (int b = wt_bin[slot] + threadIdx.x; b < wt_bin[slot + 1]; b += blockDim.x) {
-      for (int k = 0; k < 7; ++k) v[k] += bin_part[b * 8 + k];
-      qmax = fmax(qmax, bin_part[b * 8 + 7]);
+    // four partials per thread and pass, loaded before any is summed (the
+    // sums keep the order b, b + blockDim.x, ...)
+    int const bend = wt_bin[slot + 1];
+    for (int b = wt_bin[slot] + threadIdx.x; b < bend; b += 4 * blockDim.x) {
+      double4 pv[4][2];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int const bb = b + u * static_cast<int>(blockDim.x);
+        double4 const *src = reinterpret_cast<double4 const *>(bin_part + bb * 8);
+        pv[u][0] = bb < bend ? src[0] : double4{0., 0., 0., 0.};
+        pv[u][1] = bb < bend ? src[1] : double4{0., 0., 0., 0.};
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        v[0] += pv[u][0].x;
+        v[1] += pv[u][0].y;
+        v[2] += pv[u][0].z;
+        v[3] += pv[u][0].w;
+        v[4] += pv[u][1].x;
+        v[5] += pv[u][1].y;
+        v[6] += pv[u][1].z;
+        qmax = fmax(qmax, pv[u][1].w);
+      }
     }
     block_sum<7>(v, red);
     qmax = block_max(qmax, red);
@@ -407,7 +427,14 @@ __global__ __launch_bounds__(kBlock) void k_clear(SysDesc const *__restrict__ sy
   MT *rho = static_cast<MT *>(d.rho);
   int64_t const beg = (int64_t)L.lb * kFill;
   int64_t const end = min(beg + kFill, R);
-  for (int64_t e = beg + threadIdx.x; e < end; e += blockDim.x) rho[e] = MT(0);
+  // 16-byte stores over the block's whole words, element stores for the
+  // rest (and for a mesh that is not 16-byte aligned)
+  constexpr int kPerWord = 16 / static_cast<int>(sizeof(MT));
+  int64_t const words =
+      (reinterpret_cast<uintptr_t>(rho + beg) & 15) ? 0 : (end - beg) / kPerWord;
+  int4 *w = reinterpret_cast<int4 *>(rho + beg);
+  for (int64_t e = threadIdx.x; e < words; e += blockDim.x) w[e] = int4{0, 0, 0, 0};
+  for (int64_t e = beg + words * kPerWord + threadIdx.x; e < end; e += blockDim.x) rho[e] = MT(0);
 }
 
 // ---------------------------------------------------------------------------
