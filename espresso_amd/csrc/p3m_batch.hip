@@ -1076,15 +1076,10 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
     P0[x * lsy + y] = col[e];
   }
   __syncthreads();
-  C *r = lds_fft(P0, P1, Mx, lsy, d.plan[1], twy, false); // along y, layout [x][y]
+  C *r = lds_fft(P0, P1, Mx, lsy, d.plan[1], twy, false); // along y
   C *o = (r == P0) ? P1 : P0;
-  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const x = fdiv(e, fMy), y = e - x * My;
-    o[y * lsx + x] = r[x * lsy + y];
-  }
-  __syncthreads();
-  C *q = lds_fft(o, r, My, lsx, d.plan[0], twx, false); // along x, layout [y][x]
-  C *w = (q == o) ? r : o;
+  C *q = lds_fft_cols(r, o, My, lsy, d.plan[0], twx, false); // along x
+  C *w = (q == r) ? o : r;
 
   double acc[8] = {0., 0., 0., 0., 0., 0., 0., 0.};
   double const wvx = d.wv[0], wvy = d.wv[1], wvz = d.wv[2];
@@ -1093,7 +1088,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   MT const *G = static_cast<MT const *>(d.G) + (int64_t)kz * pl;
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
     int const x = fdiv(e, fMy), y = e - x * My;
-    C const rh = q[y * lsx + x];
+    C const rh = q[x * lsy + y];
     MT const gv = G[e];
     if (want_energy) {
       double const ek = static_cast<double>(gv) *
@@ -1113,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
         acc[7] += wgt * ek;
       }
     }
-    q[y * lsx + x] = C{gv * rh.x, gv * rh.y};
+    q[x * lsy + y] = C{gv * rh.x, gv * rh.y};
   }
   if (want_energy) {
     block_sum<8>(acc, red); // ends with a barrier: q is complete
@@ -1125,24 +1120,19 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   if (!want_fields) return;
   C *const Bp = static_cast<C *>(d.B) + (int64_t)kz * pl;
   C *const Ap = static_cast<C *>(d.A) + (int64_t)kz * pl;
-  // phi = G rho^ to the column ([y][x] order) and i k_x phi into w
+  // phi = G rho^ to the column and i k_x phi into w
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const y = fdiv(e, fMx), x = e - y * Mx;
-    C const v = q[y * lsx + x];
+    int const x = fdiv(e, fMy), y = e - x * My;
+    C const v = q[x * lsy + y];
     col[e] = v;
     MT const kf = static_cast<MT>(ik_shift(x, Mx, d.g.conv)) * static_cast<MT>(wvx);
-    w[y * lsx + x] = C{-kf * v.y, kf * v.x};
+    w[x * lsy + y] = C{-kf * v.y, kf * v.x};
   }
   __syncthreads();
   // x component: inverse along x, then along y, into B
-  C *a = lds_fft(w, q, My, lsx, d.plan[0], twx, true);
+  C *a = lds_fft_cols(w, q, My, lsy, d.plan[0], twx, true);
   C *b = (a == w) ? q : w;
-  for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const x = fdiv(e, fMy), y = e - x * My;
-    b[x * lsy + y] = a[y * lsx + x];
-  }
-  __syncthreads();
-  C *c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true);
+  C *c = lds_fft(a, b, Mx, lsy, d.plan[1], twy, true);
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
     int const x = fdiv(e, fMy), y = e - x * My;
     Bp[e] = c[x * lsy + y];
@@ -1150,21 +1140,20 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   __syncthreads(); // c may be P0
   // a = IFFT_x(phi), shared by the y and z components
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const y = fdiv(e, fMx), x = e - y * Mx;
-    P0[y * lsx + x] = col[e];
+    int const x = fdiv(e, fMy), y = e - x * My;
+    P0[x * lsy + y] = col[e];
   }
   __syncthreads();
-  a = lds_fft(P0, P1, My, lsx, d.plan[0], twx, true);
+  a = lds_fft_cols(P0, P1, My, lsy, d.plan[0], twx, true);
   b = (a == P0) ? P1 : P0;
+  // a to the column (the first stage below reads a and writes b only;
+  // its barrier precedes any write to a)
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const y = fdiv(e, fMx), x = e - y * Mx;
-    C const v = a[y * lsx + x];
-    col[e] = v;
-    b[x * lsy + y] = v;
+    int const x = fdiv(e, fMy), y = e - x * My;
+    col[e] = a[x * lsy + y];
   }
-  __syncthreads();
   // z component (i k_z is applied by the c2r pass): inverse along y into A
-  c = lds_fft(b, a, Mx, lsy, d.plan[1], twy, true);
+  c = lds_fft(a, b, Mx, lsy, d.plan[1], twy, true);
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
     int const x = fdiv(e, fMy), y = e - x * My;
     Ap[e] = c[x * lsy + y];
@@ -1172,7 +1161,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   __syncthreads(); // c may be P0
   // y component: i k_y a, inverse along y into this column (khat)
   for (int e = threadIdx.x; e < pl; e += blockDim.x) {
-    int const y = fdiv(e, fMx), x = e - y * Mx;
+    int const x = fdiv(e, fMy), y = e - x * My;
     C const v = col[e];
     MT const kf = static_cast<MT>(ik_shift(y, My, d.g.conv)) * static_cast<MT>(wvy);
     P0[x * lsy + y] = C{-kf * v.y, kf * v.x};

@@ -554,6 +554,129 @@ __device__ __forceinline__ void stockham_bfly(C const *src, C *dst, int j, int n
   for (int r = 0; r < R; ++r) dst[idxD + r * Ns] = v[r];
 }
 
+/** stockham_bfly for lines whose consecutive elements lie `es` apart */
+template <typename C, int R>
+__device__ __forceinline__ void stockham_bfly_es(C const *src, C *dst, int j, int nb, int Ns, int n,
+                                                 FastDiv fNs, int tstride, C const *tw, bool inv,
+                                                 int es) {
+  C v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = src[(j + r * nb) * es];
+  int const jq = fdiv(j, fNs);
+  int const jm = j - jq * Ns;
+  if (Ns > 1) {
+    int const step = tstride * jm;
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw_get(tw, r * step, inv));
+  }
+  dft_reg<C, R>(v, tw, n, inv);
+  int const idxD = jq * Ns * R + jm;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[(idxD + r * Ns) * es] = v[r];
+}
+
+/** Transform the `nlines` columns of an LDS plane: column l holds its
+ *  plan.n elements at l + m * es (m = 0 .. n-1), i.e. the first index of a
+ *  [n][es] layout, so a plane FFTed along its second index is transformed
+ *  along its first without a transpose.  Consecutive threads take
+ *  consecutive columns (unit-stride LDS accesses).  Returns the buffer
+ *  holding the result (buf0 or buf1); called by the whole block; ends with
+ *  a barrier. */
+template <typename C>
+__device__ __forceinline__ C *lds_fft_cols(C *buf0, C *buf1, int nlines, int es,
+                                           FftPlanDev const &plan, C const *tw, bool inv) {
+  int const n = __builtin_amdgcn_readfirstlane(plan.n);
+  int const nstages = __builtin_amdgcn_readfirstlane(plan.nstages);
+  FastDiv const fnl = make_fastdiv(nlines);
+  C *in = buf0;
+  C *out = buf1;
+  int Ns = 1;
+  for (int s = 0; s < nstages; ++s) {
+    int const R = __builtin_amdgcn_readfirstlane(plan.radix[s]);
+    int const nb = __builtin_amdgcn_readfirstlane(n / R);
+    int const tstride = __builtin_amdgcn_readfirstlane(nb / Ns); // n / (Ns R)
+    FastDiv const fNs = make_fastdiv(Ns);
+    if (R <= 8) {
+      int const items = nlines * nb;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const j = fdiv(it, fnl);
+        int const l = it - j * nlines;
+        C const *src = in + l;
+        C *dst = out + l;
+        switch (R) {
+        case 2: stockham_bfly_es<C, 2>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        case 3: stockham_bfly_es<C, 3>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        case 4: stockham_bfly_es<C, 4>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        case 5: stockham_bfly_es<C, 5>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        case 6: stockham_bfly_es<C, 6>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        case 7: stockham_bfly_es<C, 7>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        default: stockham_bfly_es<C, 8>(src, dst, j, nb, Ns, n, fNs, tstride, tw, inv, es); break;
+        }
+      }
+    } else if (Ns == 1) {
+      // generic radix as the first stage, as in lds_fft
+      int const half = (R + 1) >> 1;
+      int const items = nlines * nb * half;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const jk = fdiv(it, fnl); // (j, kp) pair
+        int const l = it - jk * nlines;
+        int const j = jk / half;
+        int const kp = jk - j * half;
+        C const *src = in + l;
+        C *dst = out + l;
+        C const v0 = src[j * es];
+        if (kp == 0) {
+          C sum = v0;
+          for (int r = 1; r < R; ++r) sum = cadd(sum, src[(j + r * nb) * es]);
+          dst[j * R * es] = sum;
+        } else {
+          C sa = v0, sb = C{0, 0};
+          int idx = 0;
+          for (int mm = 1; mm < half; ++mm) {
+            idx += kp; // (mm kp) mod R
+            if (idx >= R) idx -= R;
+            C const w = tw[idx * nb]; // exp(-2 pi i idx / R) = cos - i sin
+            C const u = src[(j + mm * nb) * es], z = src[(j + (R - mm) * nb) * es];
+            sa = caxpy(sa, w.x, cadd(u, z));
+            sb = caxpy(sb, -w.y, csub(u, z));
+          }
+          C const isb = cmul_mi(sb, inv);
+          dst[(j * R + kp) * es] = cadd(sa, isb);
+          dst[(j * R + R - kp) * es] = csub(sa, isb);
+        }
+      }
+    } else {
+      // generic radix: one output element per item, as in lds_fft
+      int const items = nlines * n;
+      for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        int const e = fdiv(it, fnl);
+        int const l = it - e * nlines;
+        int const j = e / R; // butterfly
+        int const k = e - j * R;
+        int const jq = fdiv(j, fNs);
+        int const jm = j - jq * Ns;
+        C const *src = in + l;
+        C *dst = out + l;
+        int const base = tstride * (jm + k * Ns); // < n
+        C sum = src[j * es];
+        int m = 0;
+        for (int r = 1; r < R; ++r) {
+          m += base; // (r * base) mod n
+          if (m >= n) m -= n;
+          sum = cadd(sum, cmul(src[(j + r * nb) * es], tw_get(tw, m, inv)));
+        }
+        dst[(jq * Ns * R + jm + k * Ns) * es] = sum;
+      }
+    }
+    __syncthreads();
+    C *t = in;
+    in = out;
+    out = t;
+    Ns *= R;
+  }
+  return in;
+}
+
 /** Transform `nlines` lines of length plan.n held in LDS (line l starts at
  *  l * ls).  Returns the buffer holding the result (buf0 or buf1).  Must be
  *  called by the whole block; ends with a barrier. */
