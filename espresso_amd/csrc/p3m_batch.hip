@@ -1043,7 +1043,9 @@ __global__ __launch_bounds__(kBlock) void k_fft_y_inv(SysDesc const *__restrict_
  *  constant along an x line, so the y and z components share one inverse x
  *  transform a = IFFT_x(phi): five inverse line passes instead of six.  The
  *  block's own khat column keeps phi, then a, between the passes (two LDS
- *  planes instead of three).  Outputs: B <- x field, khat <- y field,
+ *  planes instead of three).  The plane stays in [x][y] order throughout:
+ *  y transforms run along its rows, x transforms down its columns
+ *  (lds_fft_cols), so no pass transposes it.  Outputs: B <- x field, khat <- y field,
  *  A <- z field before i k_z (which the c2r z pass applies), all still
  *  transformed along z. */
 template <typename MT>
