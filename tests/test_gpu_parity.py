@@ -140,6 +140,11 @@ def test_all_orders(cao):
     ([64, 32, 48], [20., 10., 15.]),
     ([46, 34, 38], [15., 11., 12.]),      # 2 x 23, 2 x 17, 2 x 19 (paired first stage)
     ([40, 36, 44], [13., 12., 14.]),      # 8 x 5, 4 x 3 x 3, 11 x 4
+    # two radices above 8 in one direction (a generic radix after the first
+    # stage), in x (column transforms of the fused pass), y and z
+    ([242, 8, 10], [40., 8., 7.]),        # 11 x 11 x 2
+    ([8, 242, 10], [8., 40., 7.]),
+    ([10, 8, 242], [7., 8., 40.]),
 ])
 def test_meshes(mesh, box):
     pos, q = S.uniform_system(2000, 1., 31)
