@@ -846,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_y(SysDesc const *__restrict__ sy
   int const ls = n + 1;
   int const Kz = d.g.Kz;
   int const ncb = (Kz + ncol - 1) / ncol;
-  FastDiv const fnc = make_fastdiv(ncol), fn = make_fastdiv(n);
+  FastDiv const fnc = make_fastdiv(ncol);
   int const cb = L.lb % ncb;
   int64_t const x = L.lb / ncb;
   int const My = d.g.M[1];
@@ -1065,7 +1065,7 @@ __global__ __launch_bounds__(kBlock) void k_fft_yx_conv(SysDesc const *__restric
   int const lsy = My + 1, lsx = Mx + 1; // line strides of [x][y] and [y][x] planes
   int const plane = max(Mx * lsy, My * lsx);
   int const pl = Mx * My;
-  FastDiv const fMy = make_fastdiv(My), fMx = make_fastdiv(Mx);
+  FastDiv const fMy = make_fastdiv(My);
   C *twx = reinterpret_cast<C *>(smem_d);
   C *twy = twx + Mx;
   C *P0 = twy + My;

@@ -1730,6 +1730,7 @@ int p3m_dipolar_activate(p3m_dipolar *c, int64_t n, double const *pos, double co
     if (!c->have_params) throw std::runtime_error("DipolarP3M: parameters not set");
     if (!c->solver) throw std::runtime_error("DipolarP3M: host-only context cannot solve");
     if (n < 0) throw std::invalid_argument("DipolarP3M: negative particle count");
+    (void)pos; // the cost-model tuning needs the moments only
     // count_magnetic_particles (dp3m.cpp:83-97)
     std::vector<double> h(3 * static_cast<size_t>(n));
     if (n > 0) {

@@ -326,7 +326,9 @@ int p3m_dipolar_get_params(const p3m_dipolar *ctx, p3m_dipolar_params *out);
 /* count_magnetic_particles and tune (dp3m.cpp:83-97, 646-682; error
  * estimates :684-820): for each (mesh, cao) the smallest r_cut meeting the
  * accuracy, alpha from the real-space error; (mesh, cao) by a cost model.
- * Then init (sanity_checks_boxl, :822-846; influence functions). */
+ * Then init (sanity_checks_boxl, :822-846; influence functions).  The
+ * tuning is by cost model, so only the dipole moments are read; pos is
+ * accepted (and may be null) to keep the signature of p3m_activate. */
 int p3m_dipolar_activate(p3m_dipolar *ctx, int64_t n, const double *pos, const double *dip,
                          unsigned flags);
 /* k-space part: P3M_FORCES adds forces and torques
